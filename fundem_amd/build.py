@@ -1,0 +1,76 @@
+"""Build recipe for the native libraries (in-tree, gfx950).
+
+  fundem_amd/lib/libfundem_amd.so     HIP kernels + the C-ABI of include/fundem_amd.h
+  fundem_amd/lib/libfundem_solver.so  C binding of the C++ host API (LSSolver)
+
+Device code is compiled without floating-point contraction (each FP64
+operation rounds once, in source order) and without packed FP32 arithmetic.
+"""
+from __future__ import annotations
+
+import glob
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_DIR = os.path.join(PKG, "lib")
+CORE_LIB = os.path.join(LIB_DIR, "libfundem_amd.so")
+SOLVER_LIB = os.path.join(LIB_DIR, "libfundem_solver.so")
+
+ARCH = os.environ.get("FDM_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+
+
+def build_core() -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")))
+    tmp = CORE_LIB + ".tmp"
+    _run([_hipcc(), *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    os.replace(tmp, CORE_LIB)
+    return CORE_LIB
+
+
+def build_solver() -> str:
+    """The host API is plain C++ over the HIP runtime: built by the host
+    compiler, linked against the core library."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = SOLVER_LIB + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared",
+          "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-Wno-unused-result",
+          "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+          os.path.join(PKG, "cpp", "solver_capi.cpp"), "-L", LIB_DIR, "-lfundem_amd",
+          "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"), "-o", tmp])
+    os.replace(tmp, SOLVER_LIB)
+    return SOLVER_LIB
+
+
+def build_all() -> None:
+    build_core()
+    build_solver()
+
+
+if __name__ == "__main__":
+    build_all()
+    print(CORE_LIB)
+    print(SOLVER_LIB)

@@ -1,0 +1,284 @@
+// C binding of the C++ host API (LSSolver), for callers without C++: the
+// Python package, tests and benchmark bind it with ctypes. Every entry returns
+// 0 or a negative code; fdmx_last_error() gives the message.
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../include/fundem/LSSolver.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+// Exposes the protected density path of addParticleGeometry.
+class BoundSolver : public fundem::LSSolver {
+ public:
+  explicit BoundSolver(int device) : fundem::LSSolver("fundem", 0, 256, device) {}
+  void addGeometryDensity(double3& off, double density, double3 origin, double h, int3 size,
+                          const std::vector<double>& sdf, const std::vector<double3>& nodes) {
+    addParticleGeometry(off, density, origin, h, size, sdf, nodes, {});
+  }
+};
+
+template <class F>
+int guarded(F f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  } catch (...) {
+    g_error = "unknown error";
+    return -1;
+  }
+}
+
+double3 d3(const double* p) { return make_double3(p[0], p[1], p[2]); }
+std::vector<double3> d3v(const double* p, int n) {
+  std::vector<double3> v(n);
+  for (int i = 0; i < n; ++i) v[i] = d3(p + 3 * i);
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdmx_last_error() { return g_error.c_str(); }
+
+void* fdmx_create(int device) {
+  try {
+    return new BoundSolver(device);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+void fdmx_destroy(void* s) { delete static_cast<BoundSolver*>(s); }
+
+int fdmx_add_material(void* s, double kn, double ks, double mu, double e) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->addParticleMaterial(kn, ks, mu, e); });
+}
+int fdmx_add_wall_material(void* s, double mu, double e) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->addWallMaterial(mu, e); });
+}
+int fdmx_add_geometry(void* s, double mass, const double* I6, const double* o3, double h,
+                      const int* sz, const double* sdf, const double* nodes, int nn) {
+  return guarded([&] {
+    const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
+    static_cast<BoundSolver*>(s)->addParticleGeometry(
+        mass, symMatrix(I6[0], I6[1], I6[2], I6[3], I6[4], I6[5]), d3(o3), h,
+        make_int3(sz[0], sz[1], sz[2]), std::vector<double>(sdf, sdf + ns), d3v(nodes, nn), {});
+  });
+}
+int fdmx_add_geometry_density(void* s, double density, const double* o3, double h, const int* sz,
+                              const double* sdf, const double* nodes, int nn, double* off3) {
+  return guarded([&] {
+    const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
+    double3 off;
+    static_cast<BoundSolver*>(s)->addGeometryDensity(off, density, d3(o3), h,
+                                                     make_int3(sz[0], sz[1], sz[2]),
+                                                     std::vector<double>(sdf, sdf + ns), d3v(nodes, nn));
+    off3[0] = off.x;
+    off3[1] = off.y;
+    off3[2] = off.z;
+  });
+}
+int fdmx_add_wall_geometry(void* s, const double* o3, double h, const int* sz, const double* sdf,
+                           const double* nodes, int nn) {
+  return guarded([&] {
+    const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
+    static_cast<BoundSolver*>(s)->addWallGeometry(d3(o3), h, make_int3(sz[0], sz[1], sz[2]),
+                                                  std::vector<double>(sdf, sdf + ns), d3v(nodes, nn),
+                                                  {});
+  });
+}
+int fdmx_add_particles(void* s, int n, const double* x, const double* v, const double* w,
+                       const double* q, const int* mat, const int* geo) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    for (int i = 0; i < n; ++i)
+      S->addParticle(d3(x + 3 * i), d3(v + 3 * i), d3(w + 3 * i),
+                     make_quaternion(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]), mat[i],
+                     geo[i]);
+  });
+}
+int fdmx_add_wall(void* s, const double* x, const double* v, const double* w, const double* q,
+                  int mat, int geo) {
+  return guarded([&] {
+    static_cast<BoundSolver*>(s)->addWall(d3(x), d3(v), d3(w),
+                                          make_quaternion(q[0], q[1], q[2], q[3]), mat, geo);
+  });
+}
+int fdmx_add_bond(void* s, int i, int j, const double* pt, const double* nrm, double R, double L,
+                  double E, double nu, double st, double coh, double mub) {
+  return guarded([&] {
+    static_cast<BoundSolver*>(s)->addBondedInteraction(i, j, d3(pt), d3(nrm), R, L, E, nu, st, coh, mub);
+  });
+}
+int fdmx_set_periodic(void* s, int x, int y) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    S->removePeriodicBoundary();
+    if (x) S->addPeriodicBoundaryXD();
+    if (y) S->addPeriodicBoundaryYD();
+  });
+}
+int fdmx_set_cell_size(void* s, double cs) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setCellSize(cs); });
+}
+int fdmx_set_capacity(void* s, int pairs, int records) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setContactCapacity(pairs, records); });
+}
+int fdmx_use_graph(void* s, int on) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->useGraph(on != 0); });
+}
+int fdmx_set_timing(void* s, int on) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    S->setStageTiming(on != 0);
+    S->resetStageTiming();
+  });
+}
+int fdmx_initialize(void* s, const double* mn, const double* mx, const double* g, double dt) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->initialize(d3(mn), d3(mx), d3(g), dt); });
+}
+int fdmx_advance(void* s, long long n) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->advance((size_t)n); });
+}
+int fdmx_sync(void* s) {
+  return guarded([&] {
+    fundem::hipCheck(hipStreamSynchronize(static_cast<BoundSolver*>(s)->stream()), "sync");
+  });
+}
+int fdmx_status(void* s, unsigned* out) {
+  return guarded([&] { *out = static_cast<BoundSolver*>(s)->status(); });
+}
+int fdmx_get_state(void* s, double* x, double* v, double* w, double* q, double* f, double* t) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    S->download();
+    auto& P = S->getLSParticle();
+    const size_t n = P.num();
+    for (size_t i = 0; i < n; ++i) {
+      const double3 a = P.positionHostRef()[i], b = P.velocityHostRef()[i],
+                    c = P.angularVelocityHostRef()[i], d = P.forceHostRef()[i],
+                    e = P.torqueHostRef()[i];
+      const quaternion r = P.orientationHostRef()[i];
+      if (x) { x[3 * i] = a.x; x[3 * i + 1] = a.y; x[3 * i + 2] = a.z; }
+      if (v) { v[3 * i] = b.x; v[3 * i + 1] = b.y; v[3 * i + 2] = b.z; }
+      if (w) { w[3 * i] = c.x; w[3 * i + 1] = c.y; w[3 * i + 2] = c.z; }
+      if (f) { f[3 * i] = d.x; f[3 * i + 1] = d.y; f[3 * i + 2] = d.z; }
+      if (t) { t[3 * i] = e.x; t[3 * i + 1] = e.y; t[3 * i + 2] = e.z; }
+      if (q) { q[4 * i] = r.q0; q[4 * i + 1] = r.q1; q[4 * i + 2] = r.q2; q[4 * i + 3] = r.q3; }
+    }
+  });
+}
+long long fdmx_num_particles(void* s) {
+  return (long long)static_cast<BoundSolver*>(s)->getLSParticle().num();
+}
+// Contacts of the latest step in canonical order; returns the count (outputs
+// truncated at cap) or -1.
+long long fdmx_get_contacts(void* s, int set, int* A, int* T, int* k, double* spring, long long cap) {
+  long long n = -1;
+  guarded([&] {
+    auto c = static_cast<BoundSolver*>(s)->contacts(set);
+    n = (long long)c.size();
+    for (long long i = 0; i < n && i < cap; ++i) {
+      if (A) A[i] = c[i].A;
+      if (T) T[i] = c[i].target;
+      if (k) k[i] = c[i].node;
+      if (spring) {
+        spring[3 * i] = c[i].spring.x;
+        spring[3 * i + 1] = c[i].spring.y;
+        spring[3 * i + 2] = c[i].spring.z;
+      }
+    }
+  });
+  return n;
+}
+int fdmx_get_bonds(void* s, int* active, int* breakStep, double* sigma, double* tau) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    auto& B = S->getLSParticleBondedInteraction();
+    B.finalize(S->stream());
+    const size_t n = B.numPair();
+    if (n == 0) return;
+    auto sg = B.maxNormalStress(S->stream());
+    auto ta = B.maxShearStress(S->stream());
+    for (size_t i = 0; i < n; ++i) {
+      if (active) active[i] = B.activatedHostRef()[i];
+      if (breakStep) breakStep[i] = B.breakStepHostRef()[i];
+      if (sigma) sigma[i] = sg[i];
+      if (tau) tau[i] = ta[i];
+    }
+  });
+}
+// Hash table of the last grid build (walls != 0: wall grid): keys (hash+1)
+// and indices in sorted order, and the cell ranges.
+int fdmx_get_hash(void* s, int walls, int* hashValue, unsigned* sortedKey, int* sortedIndex,
+                  int* start, int* end) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    auto& P = walls ? S->getWall() : S->getLSParticle();
+    auto& t = P.table();
+    const size_t n = P.num();
+    const size_t ng = (size_t)P.grid().num;
+    if (hashValue) t.hashValue.download(hashValue, n, S->stream());
+    if (sortedKey) t.sortKey.download(sortedKey, n, S->stream());
+    if (sortedIndex) t.sortIndex.download(sortedIndex, n, S->stream());
+    if (start) t.cellStart.download(start, ng, S->stream());
+    if (end) t.cellEnd.download(end, ng, S->stream());
+    fundem::hipCheck(hipStreamSynchronize(S->stream()), "hash download");
+  });
+}
+int fdmx_get_grid(void* s, int walls, double* mn, double* mx, double* inv, int* size) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    const fdm_grid& g = walls ? S->getWall().grid() : S->getLSParticle().grid();
+    mn[0] = g.min.x; mn[1] = g.min.y; mn[2] = g.min.z;
+    mx[0] = g.max.x; mx[1] = g.max.y; mx[2] = g.max.z;
+    inv[0] = g.invCell.x; inv[1] = g.invCell.y; inv[2] = g.invCell.z;
+    size[0] = g.size.x; size[1] = g.size.y; size[2] = g.size.z;
+  });
+}
+int fdmx_get_geometry(void* s, int geo, double* radius, double* invMass, double* I6, double* Ii6,
+                      double* origin3) {
+  return guarded([&] {
+    const fdm_template& t = static_cast<BoundSolver*>(s)->getLSParticle().templateInfo(geo);
+    *radius = t.radius;
+    *invMass = t.invMass;
+    for (int k = 0; k < 6; ++k) {
+      I6[k] = t.inertia[k];
+      Ii6[k] = t.invInertia[k];
+    }
+    for (int k = 0; k < 3; ++k) origin3[k] = t.origin[k];
+  });
+}
+// Accumulated stage times in ms (stage timing on); names joined by '\n'.
+int fdmx_stage_times(void* s, char* names, int namesCap, double* ms, long long* calls, int cap) {
+  int count = 0;
+  guarded([&] {
+    std::string joined;
+    auto* S = static_cast<BoundSolver*>(s);
+    const auto& c = S->stageCalls();
+    for (const auto& kv : S->stageMilliseconds()) {
+      if (count < cap) {
+        ms[count] = kv.second;
+        auto it = c.find(kv.first);
+        calls[count] = it == c.end() ? 0 : it->second;
+      }
+      joined += kv.first;
+      joined += '\n';
+      ++count;
+    }
+    std::strncpy(names, joined.c_str(), (size_t)namesCap - 1);
+    names[namesCap - 1] = '\0';
+  });
+  return count;
+}
+
+}  // extern "C"

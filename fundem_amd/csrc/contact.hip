@@ -1,0 +1,649 @@
+// Contact detection and contact forces.
+//
+// The reference (LSParticleContactDetectionKernel.cu:4-607) runs one thread
+// per surface node over the 27-cell stencil, twice (count, then write), for
+// every interaction set, then one thread per contact for forces. Here:
+//
+//  broad phase  one thread per particle A lists the targets that any of A's
+//               nodes could touch under the reference's rules, sorted by
+//               target. Particle targets need |x_A - x_B| <= R_A + R_B,
+//               ghost and wall targets need |x_A - x_T| <= R_A + (SDF box
+//               reach of T); every target also needs its cell within one
+//               cell of the clamped cell range of A's bounding box. A pair is
+//               dropped when the target's interpolated SDF at A's centre
+//               exceeds Lip * R_A (Lip bounds the gradient of the trilinear
+//               interpolant), since then no node of A can reach phi <= 0.
+//  narrow phase one wavefront per candidate pair. A's nodes are swept in
+//               chunks of 64 (one node per lane). A chunk is skipped when its
+//               bounding sphere cannot satisfy the node test (the same three
+//               bounds at chunk level: sphere, stencil, SDF box, Lipschitz).
+//               Each remaining node evaluates the reference's predicate
+//               exactly in FP64: stencil membership of the target's cell,
+//               bounding sphere (particle targets, lower index only), SDF box
+//               and -phi >= 0. The wave counts its contacts, reserves that
+//               many history records, then recomputes each contact's normal
+//               and point, takes its previous spring (keyed by (A, target,
+//               node)), applies the linear spring-dashpot law and writes the
+//               updated spring. Forces and torques are summed per pair in
+//               node order, serially, so the result does not depend on the
+//               schedule.
+//
+// All culls are conservative with a relative margin of 1e-5 on reaches (a
+// unit quaternion stored in FP32 scales vectors by up to 1 +- 2.4e-7), so
+// they never remove a contact that the exact test would accept.
+#include "fdm_device.hpp"
+#include "fdm_launch.hpp"
+
+namespace fdm {
+
+constexpr double kReach = 1.0 + 1e-5;
+
+struct Target {
+  V3 x;       // detection position
+  Q4 q;
+  const fdm_template* t;
+  const float* sdf;
+  int hash;   // hash value in the target grid
+};
+
+__device__ __forceinline__ const fdm_template& tmplOf(const fdm_geometry& g, int geo) {
+  return g.info[geo];
+}
+
+// True when no point within `reach` of `local` (target frame) can have
+// phi <= 0: the point is inside the SDF box and phi - Lip*reach is positive
+// beyond a relative margin.
+__device__ __forceinline__ bool lipschitzClear(V3 local, double reach, const fdm_template& t,
+                                               const float* sdf) {
+  Sample s;
+  if (!sampleSDF(local, t, sdf, s)) return false;
+  const double phi = trilinear(s.x, s.y, s.z, s.c);
+  const double lr = t.lipschitz * reach;
+  return phi - lr > 1e-6 * (fabs(phi) + lr);
+}
+
+// True when the ball (local, reach) misses the SDF box of t (target frame).
+__device__ __forceinline__ bool outsideBox(V3 local, double reach, const fdm_template& t) {
+  const double h = 1.0 / t.invSpacing;
+  double d2 = 0.0;
+  const double lo[3] = {t.origin[0], t.origin[1], t.origin[2]};
+  const double p[3] = {local.x, local.y, local.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double hi = lo[a] + h * (double)(t.size[a] - 1);
+    const double m = 1e-6 * (hi - lo[a]);
+    double e = 0.0;
+    if (p[a] < lo[a] - m) e = lo[a] - m - p[a];
+    else if (p[a] > hi + m) e = p[a] - hi - m;
+    d2 += e * e;
+  }
+  const double r = reach * kReach;
+  return d2 > r * r;
+}
+
+// Clamped cell range of the box [c - r, c + r] (monotone in each coordinate).
+__device__ __forceinline__ void cellRange(V3 c, double r, const fdm_grid& g, I3& lo, I3& hi) {
+  lo = clampedCell(vsub(c, {r, r, r}), g);
+  hi = clampedCell(vadd(c, {r, r, r}), g);
+}
+__device__ __forceinline__ bool cellNear(I3 t, I3 lo, I3 hi) {
+  return t.x >= lo.x - 1 && t.x <= hi.x + 1 && t.y >= lo.y - 1 && t.y <= hi.y + 1 &&
+         t.z >= lo.z - 1 && t.z <= hi.z + 1;
+}
+
+// ---------------------------------------------------------------- broad phase
+struct BroadK {
+  int set;
+  fdm_bodies P;
+  fdm_geometry pgeo;
+  fdm_bodies W;
+  fdm_geometry wgeo;
+  int ghostCapacity;
+  const int* ghostCount;
+  const int* ghostReal;
+  const double* ghostPos;
+  const int* ghostBox;
+  fdm_grid grid;
+  fdm_hash_table table;
+  fdm_pairs pairs;
+  unsigned* status;
+};
+
+template <int SET>
+__device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, double RA, int T,
+                                                int G, V3 xT) {
+  double reachT;
+  Q4 qT;
+  const fdm_template* tT;
+  const float* sdf;
+  if (SET == FDM_SET_WALL) {
+    tT = &tmplOf(a.wgeo, a.W.geo[T]);
+    qT = ld4(a.W.q, a.W.n, T);
+    sdf = a.wgeo.sdf;
+    reachT = tT->boxReach;
+  } else {
+    tT = &tmplOf(a.pgeo, a.P.geo[T]);
+    qT = ld4(a.P.q, a.P.n, T);
+    sdf = a.pgeo.sdf;
+    reachT = SET == FDM_SET_PARTICLE ? tT->radius : tT->boxReach;
+  }
+  const V3 d = vsub(xA, xT);
+  const double rs = (RA + reachT) * kReach;
+  if (vlen2(d) > rs * rs) return false;
+  return !lipschitzClear(qrotateT(d, qT), RA * kReach, *tT, sdf);
+}
+
+// Visit the candidate targets of particle A; `emit(T, G)` for each.
+template <int SET, class Emit>
+__device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emit) {
+  const int n = a.P.n;
+  const V3 xA = ld3(a.P.x, n, A);
+  const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
+  I3 lo, hi;
+  cellRange(xA, RA * kReach, a.grid, lo, hi);
+  if (SET == FDM_SET_WALL) {
+    for (int w = 0; w < a.W.n; ++w) {
+      const int h = a.table.hashValue[w];
+      if (h < 0 || (long long)h >= a.grid.num) continue;
+      if (!cellNear(decodeHash(h, a.grid), lo, hi)) continue;
+      if (acceptCandidate<SET>(a, A, xA, RA, w, w, ld3(a.W.x, a.W.n, w))) emit(w, w);
+    }
+    return;
+  }
+  I3 b0 = {max(lo.x - 1, 0), max(lo.y - 1, 0), max(lo.z - 1, 0)};
+  I3 b1 = {min(hi.x + 1, a.grid.size.x - 1), min(hi.y + 1, a.grid.size.y - 1),
+           min(hi.z + 1, a.grid.size.z - 1)};
+  if (SET == FDM_SET_GHOST) {
+    if (*a.ghostCount <= 0) return;
+    b0 = {max(b0.x, a.ghostBox[0]), max(b0.y, a.ghostBox[1]), max(b0.z, a.ghostBox[2])};
+    b1 = {min(b1.x, a.ghostBox[3]), min(b1.y, a.ghostBox[4]), min(b1.z, a.ghostBox[5])};
+  }
+  const I3 sz = gsize(a.grid);
+  for (int cz = b0.z; cz <= b1.z; ++cz)
+    for (int cy = b0.y; cy <= b1.y; ++cy)
+      for (int cx = b0.x; cx <= b1.x; ++cx) {
+        const int h = lin3({cx, cy, cz}, sz);
+        const int s = a.table.cellStart[h];
+        if (s < 0) continue;
+        const int e = a.table.cellEnd[h];
+        for (int m = s; m < e; ++m) {
+          const int idx = a.table.sortIndex[m];
+          if (SET == FDM_SET_PARTICLE) {
+            if (idx <= A) continue;
+            if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ld3(a.P.x, n, idx))) emit(idx, idx);
+          } else {
+            const int j = a.ghostReal[idx];
+            if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3(a.ghostPos + 3 * idx)))
+              emit(j, idx);
+          }
+        }
+      }
+}
+
+template <int SET>
+__global__ void kBroadCount(BroadK a) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A > a.P.n) return;
+  if (A == a.P.n) {
+    a.pairs.count[A] = 0;
+    return;
+  }
+  int c = 0;
+  visitCandidates<SET>(a, A, [&](int, int) { ++c; });
+  a.pairs.count[A] = c;
+}
+
+template <int SET>
+__global__ void kBroadWrite(BroadK a) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= a.P.n) return;
+  const int begin = a.pairs.count[A];
+  const int end = a.pairs.count[A + 1];
+  if (begin == end) return;
+  if (end > a.pairs.capacity) {
+    // mark the slots that fit as empty so the narrow phase skips them
+    for (int w = begin; w < a.pairs.capacity; ++w) a.pairs.pairA[w] = -1;
+    atomicOr(a.status, FDM_STATUS_PAIR_OVERFLOW);
+    return;
+  }
+  int w = begin;
+  visitCandidates<SET>(a, A, [&](int T, int G) {
+    if (w < end) {
+      a.pairs.pairA[w] = A;
+      a.pairs.pairT[w] = T;
+      a.pairs.pairG[w] = G;
+    }
+    ++w;
+  });
+  // insertion sort of the segment by target id (lists are short)
+  for (int i = begin + 1; i < end; ++i) {
+    const int t = a.pairs.pairT[i], g = a.pairs.pairG[i];
+    int j = i - 1;
+    while (j >= begin && a.pairs.pairT[j] > t) {
+      a.pairs.pairT[j + 1] = a.pairs.pairT[j];
+      a.pairs.pairG[j + 1] = a.pairs.pairG[j];
+      --j;
+    }
+    a.pairs.pairT[j + 1] = t;
+    a.pairs.pairG[j + 1] = g;
+  }
+}
+
+// -------------------------------------------------------------- narrow phase
+struct NarrowK {
+  fdm_bodies P;
+  fdm_geometry pgeo;
+  const fdm_material* pmat;
+  fdm_bodies W;
+  fdm_geometry wgeo;
+  const fdm_material* wmat;
+  const int* ghostReal;
+  const double* ghostPos;
+  const int* ghostHash;
+  fdm_grid grid;
+  const int* targetHash;
+  fdm_grid pgrid;
+  fdm_periodic periodic;
+  fdm_pairs pairs;
+  fdm_pairs prev;
+  const int* prevStart;
+  fdm_mirror mirror;
+  double dt;
+  int updateSprings;
+  unsigned* status;
+  double* dbgPoint;
+  double* dbgNormal;
+  double* dbgOverlap;
+};
+
+// Position used for force arms: the stored position after this step's
+// periodic wrap (the reference computes forces after updateGhostSpatialGrid).
+__device__ __forceinline__ V3 wrappedArm(V3 x, const NarrowK& a) {
+  if (a.periodic.x) {
+    if (x.x < a.pgrid.min.x) x = vadd(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
+    else if (x.x >= a.pgrid.max.x) x = vsub(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
+    x = round32(x);
+  }
+  if (a.periodic.y) {
+    if (x.y < a.pgrid.min.y) x = vadd(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
+    else if (x.y >= a.pgrid.max.y) x = vsub(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
+    x = round32(x);
+  }
+  return x;
+}
+
+// The reference's node predicate for one (node, target) candidate.
+template <int SET>
+__device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const fdm_grid& g,
+                                        double RB, Sample& s) {
+  if (!inStencil(tg.hash, clampedCell(p, g), g)) return false;
+  (void)tcell;
+  const V3 rel = vsub(p, tg.x);
+  if (SET == FDM_SET_PARTICLE) {
+    if (vlen2(rel) > RB * RB) return false;
+  }
+  if (!sampleSDF(qrotateT(rel, tg.q), *tg.t, tg.sdf, s)) return false;
+  const double ov = -trilinear(s.x, s.y, s.z, s.c);
+  return ov >= 0.;
+}
+
+__device__ __forceinline__ int findPrevPair(const NarrowK& a, int A, int T, int& off) {
+  int lo = min(a.prevStart[A], a.prev.capacity), hi = min(a.prevStart[A + 1], a.prev.capacity) - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int t = a.prev.pairT[mid];
+    if (t == T) {
+      off = a.prev.recOff[mid];
+      return a.prev.recCnt[mid];
+    }
+    if (t < T) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  off = 0;
+  return 0;
+}
+
+__device__ __forceinline__ V3 findPrevSpring(const NarrowK& a, int off, int cnt, int k) {
+  int lo = off, hi = off + cnt - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const float4 r = a.prev.rec[mid];
+    const int kk = __float_as_int(r.w);
+    if (kk == k) return {(double)r.x, (double)r.y, (double)r.z};
+    if (kk < k) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return {0., 0., 0.};
+}
+
+template <int SET>
+__device__ void processPair(const NarrowK& a, int p, int lane) {
+  const int A = a.pairs.pairA[p];
+  if (A < 0) {  // slot left empty by a capacity overflow
+    if (lane == 0) {
+      a.pairs.recOff[p] = 0;
+      a.pairs.recCnt[p] = 0;
+    }
+    return;
+  }
+  const int T = a.pairs.pairT[p];
+  const int n = a.P.n;
+  const fdm_template& tA = tmplOf(a.pgeo, a.P.geo[A]);
+  const V3 xA = ld3(a.P.x, n, A);
+  const Q4 qA = ld4(a.P.q, n, A);
+  Target tg;
+  V3 armB;
+  if (SET == FDM_SET_WALL) {
+    tg.x = ld3(a.W.x, a.W.n, T);
+    tg.q = ld4(a.W.q, a.W.n, T);
+    tg.t = &tmplOf(a.wgeo, a.W.geo[T]);
+    tg.sdf = a.wgeo.sdf;
+    tg.hash = a.targetHash[T];
+    armB = tg.x;
+  } else if (SET == FDM_SET_PARTICLE) {
+    tg.x = ld3(a.P.x, n, T);
+    tg.q = ld4(a.P.q, n, T);
+    tg.t = &tmplOf(a.pgeo, a.P.geo[T]);
+    tg.sdf = a.pgeo.sdf;
+    tg.hash = a.targetHash[T];
+    armB = wrappedArm(tg.x, a);
+  } else {
+    const int G = a.pairs.pairG[p];
+    tg.x = ldd3(a.ghostPos + 3 * G);
+    tg.q = ld4(a.P.q, n, T);
+    tg.t = &tmplOf(a.pgeo, a.P.geo[T]);
+    tg.sdf = a.pgeo.sdf;
+    tg.hash = a.ghostHash[G];
+    armB = tg.x;
+  }
+  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, a);
+  const double RB = tg.t->radius;
+  const I3 tcell = decodeHash(tg.hash, a.grid);
+  const float* nodes = a.pgeo.nodes + 3 * (size_t)tA.nodeStart;
+  const float4* chunks = reinterpret_cast<const float4*>(a.pgeo.chunks) + tA.chunkStart;
+  const int nChunks = tA.chunkCount;
+
+  // pass 1: contact mask of every chunk (lane c keeps chunk c's mask)
+  unsigned long long myMask = 0ull;
+  for (int c = 0; c < nChunks; ++c) {
+    const float4 ch = chunks[c];
+    const V3 cw = vadd(qrotate(qA, {(double)ch.x, (double)ch.y, (double)ch.z}), xA);
+    const double rc = (double)ch.w * kReach;
+    if (SET == FDM_SET_PARTICLE) {
+      const double rr = (RB + rc) * (1.0 + 1e-9);
+      if (vlen2(vsub(cw, tg.x)) > rr * rr) continue;
+    }
+    I3 lo, hi;
+    cellRange(cw, rc, a.grid, lo, hi);
+    if (!cellNear(tcell, lo, hi)) continue;
+    const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
+    if (outsideBox(lc, rc, *tg.t)) continue;
+    if (lipschitzClear(lc, rc, *tg.t, tg.sdf)) continue;
+    const int k = c * FDM_CHUNK + lane;
+    bool hit = false;
+    if (k < tA.nodeCount) {
+      const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+      Sample sm;
+      hit = nodeHit<SET>(vadd(qrotate(qA, s), xA), tg, tcell, a.grid, RB, sm);
+    }
+    const unsigned long long m = ballot64(hit);
+    if (lane == c) myMask = m;
+  }
+  int total = __popcll(myMask);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
+
+  int base = 0;
+  if (total > 0) {
+    if (lane == 0) base = atomicAdd(a.pairs.recTotal, total);
+    base = readlaneI(base, 0);
+    if (base + total > a.pairs.recCapacity) {
+      if (lane == 0) atomicOr(a.status, FDM_STATUS_RECORD_OVERFLOW);
+      total = 0;
+    }
+  }
+
+  V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
+  if (total > 0) {
+    int prevOff = 0;
+    const int prevCnt = findPrevPair(a, A, T, prevOff);
+    // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
+    const int mi = a.P.mat[A];
+    const fdm_material& Mi = a.pmat[mi];
+    const double invMi = tA.invMass;
+    double kn = 0., ks = 0., effM = 0., mu, diss;
+    V3 vB, wB;
+    if (SET == FDM_SET_WALL) {
+      const fdm_material& Mw = a.wmat[a.W.mat[T]];
+      if (invMi > 0.) effM = 1. / invMi;
+      mu = fmin(Mi.mu, Mw.mu);
+      diss = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
+      kn = Mi.kn;
+      ks = Mi.ks;
+      vB = ld3(a.W.v, a.W.n, T);
+      wB = ld3(a.W.w, a.W.n, T);
+    } else {
+      const fdm_material& Mj = a.pmat[a.P.mat[T]];
+      const double invMj = tg.t->invMass;
+      if (Mi.kn > 0. && Mj.kn > 0.) kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
+      if (Mi.ks > 0. && Mj.ks > 0.) ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
+      if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
+      diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
+      mu = fmin(Mi.mu, Mj.mu);
+      vB = ld3(a.P.v, n, T);
+      wB = ld3(a.P.w, n, T);
+    }
+    const V3 vA = ld3(a.P.v, n, A);
+    const V3 wA = ld3(a.P.w, n, A);
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long chunksWith = ballot64(myMask != 0ull);
+    int running = 0;
+    for (unsigned long long cw = chunksWith; cw; cw &= cw - 1ull) {
+      const int c = __builtin_ctzll(cw);
+      const unsigned lo32 = __builtin_amdgcn_readlane((unsigned)myMask, c);
+      const unsigned hi32 = __builtin_amdgcn_readlane((unsigned)(myMask >> 32), c);
+      const unsigned long long m = ((unsigned long long)hi32 << 32) | lo32;
+      const bool hit = (m >> lane) & 1ull;
+      const int k = c * FDM_CHUNK + lane;
+      V3 F = {0., 0., 0.}, TA = {0., 0., 0.}, TB = {0., 0., 0.};
+      if (hit) {
+        const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+        const V3 pn = vadd(qrotate(qA, s), xA);
+        Sample sm;
+        nodeHit<SET>(pn, tg, tcell, a.grid, RB, sm);
+        const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+        V3 spring = findPrevSpring(a, prevOff, prevCnt, k);
+        if (a.updateSprings) {
+          const V3 vc = vsub(vadd(vA, vcross(wA, vsub(pc, armA))), vadd(vB, vcross(wB, vsub(pc, armB))));
+          linearContact(F, spring, vc, nrm, ov, a.dt, kn, ks, mu, diss, effM);
+        }
+        TA = vcross(vsub(pc, armA), F);
+        TB = vcross(vsub(pc, armB), vneg(F));
+        const int r = base + running + __popcll(m & lt);
+        a.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z,
+                                     __int_as_float(k));
+        if (a.dbgPoint) {
+          a.dbgPoint[3 * r] = pc.x;
+          a.dbgPoint[3 * r + 1] = pc.y;
+          a.dbgPoint[3 * r + 2] = pc.z;
+          a.dbgNormal[3 * r] = nrm.x;
+          a.dbgNormal[3 * r + 1] = nrm.y;
+          a.dbgNormal[3 * r + 2] = nrm.z;
+          a.dbgOverlap[r] = ov;
+        }
+      }
+      // pair sums, serially in node order
+      for (unsigned long long mm = m; mm; mm &= mm - 1ull) {
+        const int b = __builtin_ctzll(mm);
+        SF = vadd(SF, {readlaneD(F.x, b), readlaneD(F.y, b), readlaneD(F.z, b)});
+        STA = vadd(STA, {readlaneD(TA.x, b), readlaneD(TA.y, b), readlaneD(TA.z, b)});
+        STB = vadd(STB, {readlaneD(TB.x, b), readlaneD(TB.y, b), readlaneD(TB.z, b)});
+      }
+      running += __popcll(m);
+    }
+  }
+  if (lane == 0) {
+    a.pairs.recOff[p] = base;
+    a.pairs.recCnt[p] = total;
+    double* f = a.pairs.sumF + 3 * (size_t)p;
+    double* ta = a.pairs.sumTA + 3 * (size_t)p;
+    double* tb = a.pairs.sumTB + 3 * (size_t)p;
+    f[0] = SF.x; f[1] = SF.y; f[2] = SF.z;
+    ta[0] = STA.x; ta[1] = STA.y; ta[2] = STA.z;
+    tb[0] = STB.x; tb[1] = STB.y; tb[2] = STB.z;
+    if (SET != FDM_SET_WALL && total > 0 && T != A) {
+      const int pos = atomicAdd(&a.mirror.count[T], 1);
+      if (pos < a.mirror.capacity) a.mirror.list[(size_t)T * a.mirror.capacity + pos] = p;
+      else atomicOr(a.status, FDM_STATUS_BUCKET_OVERFLOW);
+    }
+  }
+}
+
+template <int SET>
+__global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
+  const int lane = laneId();
+  const int wpb = blockDim.x >> 6;
+  const int total = min(*a.pairs.total, a.pairs.capacity);
+  const int stride = gridDim.x * wpb;
+  for (int p = blockIdx.x * wpb + (threadIdx.x >> 6); p < total; p += stride)
+    processPair<SET>(a, p, lane);
+}
+
+__global__ void kResetCounter(int* c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+}  // namespace fdm
+
+using namespace fdm;
+
+extern "C" int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* workspace,
+                                    size_t workspaceBytes, hipStream_t stream) {
+  if (!args) return FDM_EINVAL;
+  const fdm_broadphase_args& g = *args;
+  const int n = g.particles.n;
+  if (n < 0) return FDM_EINVAL;
+  BroadK k;
+  k.set = g.set;
+  k.P = g.particles;
+  k.pgeo = g.pgeo;
+  k.W = g.walls;
+  k.wgeo = g.wgeo;
+  k.ghostCapacity = g.ghosts ? g.ghosts->capacity : 0;
+  k.ghostCount = g.ghosts ? g.ghosts->count : nullptr;
+  k.ghostReal = g.ghosts ? g.ghosts->realId : nullptr;
+  k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
+  k.ghostBox = g.ghosts ? g.ghosts->cellBox : nullptr;
+  k.grid = g.grid;
+  k.table = g.table;
+  k.pairs = g.pairs;
+  k.status = nullptr;
+  if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
+  const unsigned nb = blocksFor((long long)n + 1, 128);
+  switch (g.set) {
+    case FDM_SET_PARTICLE:
+      hipLaunchKernelGGL(kBroadCount<FDM_SET_PARTICLE>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    case FDM_SET_WALL:
+      hipLaunchKernelGGL(kBroadCount<FDM_SET_WALL>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    case FDM_SET_GHOST:
+      hipLaunchKernelGGL(kBroadCount<FDM_SET_GHOST>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    default:
+      return FDM_EINVAL;
+  }
+  int r = scanInts(g.pairs.count, g.pairs.count, (size_t)n + 1, false, g.pairs.total, workspace,
+                   workspaceBytes, stream);
+  if (r != FDM_OK) return r;
+  return launchStatus();
+}
+
+// The write half needs the status word; it is launched by the narrow-phase
+// entry's companion below so that the broad-phase signature stays the
+// reference-like (count, scan) pair.
+extern "C" int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsigned* status,
+                                         hipStream_t stream) {
+  if (!args) return FDM_EINVAL;
+  const fdm_broadphase_args& g = *args;
+  const int n = g.particles.n;
+  BroadK k;
+  k.set = g.set;
+  k.P = g.particles;
+  k.pgeo = g.pgeo;
+  k.W = g.walls;
+  k.wgeo = g.wgeo;
+  k.ghostCapacity = g.ghosts ? g.ghosts->capacity : 0;
+  k.ghostCount = g.ghosts ? g.ghosts->count : nullptr;
+  k.ghostReal = g.ghosts ? g.ghosts->realId : nullptr;
+  k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
+  k.ghostBox = g.ghosts ? g.ghosts->cellBox : nullptr;
+  k.grid = g.grid;
+  k.table = g.table;
+  k.pairs = g.pairs;
+  k.status = status;
+  if (n <= 0) return FDM_OK;
+  const unsigned nb = blocksFor(n, 128);
+  switch (g.set) {
+    case FDM_SET_PARTICLE:
+      hipLaunchKernelGGL(kBroadWrite<FDM_SET_PARTICLE>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    case FDM_SET_WALL:
+      hipLaunchKernelGGL(kBroadWrite<FDM_SET_WALL>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    case FDM_SET_GHOST:
+      hipLaunchKernelGGL(kBroadWrite<FDM_SET_GHOST>, dim3(nb), dim3(128), 0, stream, k);
+      break;
+    default:
+      return FDM_EINVAL;
+  }
+  return launchStatus();
+}
+
+extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream) {
+  if (!args) return FDM_EINVAL;
+  const fdm_narrowphase_args& g = *args;
+  if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
+  NarrowK k;
+  k.P = g.particles;
+  k.pgeo = g.pgeo;
+  k.pmat = g.pmat;
+  k.W = g.walls;
+  k.wgeo = g.wgeo;
+  k.wmat = g.wmat;
+  k.ghostReal = g.ghosts ? g.ghosts->realId : nullptr;
+  k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
+  k.ghostHash = g.ghosts ? g.ghosts->table.hashValue : nullptr;
+  k.grid = g.grid;
+  k.targetHash = g.targetHash;
+  k.pgrid = g.pgrid;
+  k.periodic = g.periodic;
+  k.pairs = g.pairs;
+  k.prev = g.prev;
+  k.prevStart = g.prevStart;
+  k.mirror = g.mirror;
+  k.dt = g.dt;
+  k.updateSprings = g.updateSprings;
+  k.status = g.status;
+  k.dbgPoint = g.dbgPoint;
+  k.dbgNormal = g.dbgNormal;
+  k.dbgOverlap = g.dbgOverlap;
+  hipLaunchKernelGGL(kResetCounter, dim3(1), dim3(64), 0, stream, g.pairs.recTotal);
+  const int blocks = persistentBlocks(4);
+  switch (g.set) {
+    case FDM_SET_PARTICLE:
+      hipLaunchKernelGGL(kNarrow<FDM_SET_PARTICLE>, dim3(blocks), dim3(256), 0, stream, k);
+      break;
+    case FDM_SET_WALL:
+      hipLaunchKernelGGL(kNarrow<FDM_SET_WALL>, dim3(blocks), dim3(256), 0, stream, k);
+      break;
+    case FDM_SET_GHOST:
+      hipLaunchKernelGGL(kNarrow<FDM_SET_GHOST>, dim3(blocks), dim3(256), 0, stream, k);
+      break;
+    default:
+      return FDM_EINVAL;
+  }
+  return launchStatus();
+}

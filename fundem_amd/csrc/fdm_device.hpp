@@ -1,0 +1,299 @@
+// Device-side FP64 arithmetic for the level-set DEM step.
+//
+// Every routine evaluates its expression in the reference's operation order
+// (kernel/CUDAKernelFunction/myUtility/myVec.h, myQua.h, myMat.h;
+// LSParticleContactDetectionKernel.cuh; contactKernel.cuh). The library is
+// compiled with -ffp-contract=off, so no multiply-add is fused and each FP64
+// operation rounds exactly once, as on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "fundem_amd.h"
+
+namespace fdm {
+
+struct V3 {
+  double x, y, z;
+};
+struct Q4 {
+  double q0, q1, q2, q3;
+};
+struct S6 {
+  double xx, yy, zz, xy, xz, yz;
+};
+struct I3 {
+  int x, y, z;
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ V3 vadd(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 vneg(V3 a) { return {-a.x, -a.y, -a.z}; }
+// vector times scalar, each component times s (the reference's operator* in both orders)
+__device__ __forceinline__ V3 vmul(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 vdiv(V3 a, double s) {
+  const double inv = 1.0 / s;
+  return {a.x * inv, a.y * inv, a.z * inv};
+}
+__device__ __forceinline__ double vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 vcross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double vlen2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ double vlen(V3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ bool nearZero(double x) { return fabs(x) <= 1.0e-20; }
+// normalize with a correctly rounded 1/sqrt (not a reciprocal-sqrt estimate)
+__device__ __forceinline__ V3 vnormalize(V3 v) {
+  const double l2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  if (nearZero(l2)) return {0.0, 0.0, 0.0};
+  const double inv = 1.0 / sqrt(l2);
+  return {v.x * inv, v.y * inv, v.z * inv};
+}
+__device__ __forceinline__ Q4 qnormalize(Q4 q) {
+  const double l2 = q.q0 * q.q0 + q.q1 * q.q1 + q.q2 * q.q2 + q.q3 * q.q3;
+  if (nearZero(l2)) return q;
+  const double inv = 1.0 / sqrt(l2);
+  return {q.q0 * inv, q.q1 * inv, q.q2 * inv, q.q3 * inv};
+}
+// v' = v + q0 t + u x t with t = 2 u x v (myQua.h:104-114)
+__device__ __forceinline__ V3 qrotate(Q4 q, V3 v) {
+  const V3 u = {q.q1, q.q2, q.q3};
+  const V3 t = vmul(vcross(u, v), 2.0);
+  return vadd(vadd(v, vmul(t, q.q0)), vcross(u, t));
+}
+// rotation by the conjugate (myQua.h:120-129)
+__device__ __forceinline__ V3 qrotateT(V3 v, Q4 q) {
+  const V3 u = {-q.q1, -q.q2, -q.q3};
+  const V3 t = vmul(vcross(u, v), 2.0);
+  return vadd(vadd(v, vmul(t, q.q0)), vcross(u, t));
+}
+// explicit step of dq/dt = 1/2 (0, w) (x) q, then normalise (myQua.h:77-93)
+__device__ __forceinline__ Q4 qintegrate(Q4 q, V3 w, double dt) {
+  const V3 v = vmul(w, 0.5 * dt);
+  Q4 d;
+  d.q0 = -q.q1 * v.x - q.q2 * v.y - q.q3 * v.z;
+  d.q1 = q.q0 * v.x + q.q3 * v.y - q.q2 * v.z;
+  d.q2 = -q.q3 * v.x + q.q0 * v.y + q.q1 * v.z;
+  d.q3 = q.q2 * v.x - q.q1 * v.y + q.q0 * v.z;
+  return qnormalize({q.q0 + d.q0, q.q1 + d.q1, q.q2 + d.q2, q.q3 + d.q3});
+}
+__device__ __forceinline__ V3 smul(S6 m, V3 v) {
+  return {m.xx * v.x + m.xy * v.y + m.xz * v.z, m.xy * v.x + m.yy * v.y + m.yz * v.z,
+          m.xz * v.x + m.yz * v.y + m.zz * v.z};
+}
+// R I R^T with R from q, in the explicit expansion of myMat.h:107-158
+__device__ __forceinline__ S6 srotate(Q4 q, S6 I) {
+  const double q0 = q.q0, q1 = q.q1, q2 = q.q2, q3 = q.q3;
+  const double q11 = 2 * q1 * q1, q22 = 2 * q2 * q2, q33 = 2 * q3 * q3;
+  const double a = 1.0 - q22 - q33, b = 2 * (q1 * q2 - q0 * q3), c = 2 * (q1 * q3 + q0 * q2);
+  const double d = 2 * (q1 * q2 + q0 * q3), e = 1.0 - q11 - q33, f = 2 * (q2 * q3 - q0 * q1);
+  const double g = 2 * (q1 * q3 - q0 * q2), h = 2 * (q2 * q3 + q0 * q1), i = 1.0 - q11 - q22;
+  const double r0x = I.xx * a + I.xy * b + I.xz * c, r0y = I.xy * a + I.yy * b + I.yz * c,
+               r0z = I.xz * a + I.yz * b + I.zz * c;
+  const double r1x = I.xx * d + I.xy * e + I.xz * f, r1y = I.xy * d + I.yy * e + I.yz * f,
+               r1z = I.xz * d + I.yz * e + I.zz * f;
+  const double r2x = I.xx * g + I.xy * h + I.xz * i, r2y = I.xy * g + I.yy * h + I.yz * i,
+               r2z = I.xz * g + I.yz * h + I.zz * i;
+  return {r0x * a + r0y * b + r0z * c, r1x * d + r1y * e + r1z * f, r2x * g + r2y * h + r2z * i,
+          r0x * d + r0y * e + r0z * f, r0x * g + r0y * h + r0z * i, r1x * g + r1y * h + r1z * i};
+}
+
+// ---------------------------------------------------------------- state I/O
+__device__ __forceinline__ V3 ld3(const float* p, int n, int i) {
+  return {(double)p[i], (double)p[n + i], (double)p[2 * n + i]};
+}
+__device__ __forceinline__ void st3(float* p, int n, int i, V3 v) {
+  p[i] = (float)v.x;
+  p[n + i] = (float)v.y;
+  p[2 * n + i] = (float)v.z;
+}
+__device__ __forceinline__ Q4 ld4(const float* p, int n, int i) {
+  return {(double)p[i], (double)p[n + i], (double)p[2 * n + i], (double)p[3 * n + i]};
+}
+__device__ __forceinline__ void st4(float* p, int n, int i, Q4 q) {
+  p[i] = (float)q.q0;
+  p[n + i] = (float)q.q1;
+  p[2 * n + i] = (float)q.q2;
+  p[3 * n + i] = (float)q.q3;
+}
+__device__ __forceinline__ V3 ldd3(const double* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ V3 round32(V3 v) {
+  return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z};
+}
+__device__ __forceinline__ S6 lds6(const double* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
+__host__ __device__ __forceinline__ V3 fromD3(fdm_double3 d) { return {d.x, d.y, d.z}; }
+
+// ------------------------------------------------------------------ grid
+// saturating double -> int (NaN -> 0), the device conversion made explicit
+__device__ __forceinline__ int toInt(double v) {
+  if (v != v) return 0;
+  if (v >= 2147483647.0) return INT_MAX;
+  if (v <= -2147483648.0) return INT_MIN;
+  return (int)v;
+}
+__device__ __forceinline__ int lin3(I3 c, I3 d) { return c.z * d.y * d.x + c.y * d.x + c.x; }
+__device__ __forceinline__ I3 gridPos(V3 p, const fdm_grid& g) {
+  return {toInt((p.x - g.min.x) * g.invCell.x), toInt((p.y - g.min.y) * g.invCell.y),
+          toInt((p.z - g.min.z) * g.invCell.z)};
+}
+__device__ __forceinline__ I3 gsize(const fdm_grid& g) { return {g.size.x, g.size.y, g.size.z}; }
+// calculateHashKernel (mySpatialGrid.cu:4-26)
+__device__ __forceinline__ int cellHash(V3 p, const fdm_grid& g) {
+  if (p.x < g.min.x) return -1;
+  else if (p.x >= g.max.x) return -1;
+  if (p.y < g.min.y) return -1;
+  else if (p.y >= g.max.y) return -1;
+  if (p.z < g.min.z) return -1;
+  else if (p.z >= g.max.z) return -1;
+  return lin3(gridPos(p, g), gsize(g));
+}
+// calculateGhostHashKernel (mySpatialGrid.cu:214-240); its z test compares with max.y
+__device__ __forceinline__ int ghostCellHash(V3 p, const fdm_grid& g) {
+  I3 c = gridPos(p, g);
+  bool out = false;
+  if (p.x < g.min.x) { c.x = 0; out = true; }
+  else if (p.x >= g.max.x) { c.x = g.size.x - 1; out = true; }
+  if (p.y < g.min.y) { c.y = 0; out = true; }
+  else if (p.y >= g.max.y) { c.y = g.size.y - 1; out = true; }
+  if (p.z < g.min.z) { c.z = 0; out = true; }
+  else if (p.z >= g.max.y) { c.z = g.size.z - 1; out = true; }
+  return out ? lin3(c, gsize(g)) : -1;
+}
+// node cell clamped into the grid, as the detection kernels clamp it
+// (LSParticleContactDetectionKernel.cu:38-46)
+__device__ __forceinline__ I3 clampedCell(V3 p, const fdm_grid& g) {
+  I3 c = gridPos(p, g);
+  if (c.x <= 0) c.x = 0;
+  if (c.x >= g.size.x - 1) c.x = g.size.x - 1;
+  if (c.y <= 0) c.y = 0;
+  if (c.y >= g.size.y - 1) c.y = g.size.y - 1;
+  if (c.z <= 0) c.z = 0;
+  if (c.z >= g.size.z - 1) c.z = g.size.z - 1;
+  return c;
+}
+// cell coordinates of a valid hash value
+__device__ __forceinline__ I3 decodeHash(int h, const fdm_grid& g) {
+  const int nx = g.size.x, ny = g.size.y;
+  const int z = h / (nx * ny);
+  const int r = h - z * nx * ny;
+  const int y = r / nx;
+  return {r - y * nx, y, z};
+}
+// Is a target with hash value h inside the 27-cell stencil of a node whose
+// clamped cell is c? The stencil visits exactly the in-range cells within one
+// cell of c on every axis, so the target is found iff its cell is.
+__device__ __forceinline__ bool inStencil(int h, I3 c, const fdm_grid& g) {
+  if (h < 0 || (long long)h >= g.num) return false;
+  const I3 t = decodeHash(h, g);
+  return abs(t.x - c.x) <= 1 && abs(t.y - c.y) <= 1 && abs(t.z - c.z) <= 1;
+}
+
+// --------------------------------------------------------------- level set
+// trilinear value (LSParticleContactDetectionKernel.cuh:4-32)
+__device__ __forceinline__ double trilinear(double x, double y, double z, const double* c) {
+  const double wx0 = 1.0 - x, wx1 = x, wy0 = 1.0 - y, wy1 = y, wz0 = 1.0 - z, wz1 = z;
+  return c[0] * wx0 * wy0 * wz0 + c[1] * wx1 * wy0 * wz0 + c[2] * wx0 * wy1 * wz0 +
+         c[3] * wx1 * wy1 * wz0 + c[4] * wx0 * wy0 * wz1 + c[5] * wx1 * wy0 * wz1 +
+         c[6] * wx0 * wy1 * wz1 + c[7] * wx1 * wy1 * wz1;
+}
+// gradient weights of LSParticleContactDetectionKernel.cuh:34-77, whose x
+// parameter is an int: the fractional x in [0,1) enters as 0.
+__device__ __forceinline__ V3 gradientIntX(double xf, double y, double z, const double* c) {
+  const int xi = (int)xf;
+  const double wx0 = 1.0 - xi, wx1 = xi, wy0 = 1.0 - y, wy1 = y, wz0 = 1.0 - z, wz1 = z;
+  const double dx = (c[1] - c[0]) * wy0 * wz0 + (c[3] - c[2]) * wy1 * wz0 +
+                    (c[5] - c[4]) * wy0 * wz1 + (c[7] - c[6]) * wy1 * wz1;
+  const double dy = (c[2] - c[0]) * wx0 * wz0 + (c[3] - c[1]) * wx1 * wz0 +
+                    (c[6] - c[4]) * wx0 * wz1 + (c[7] - c[5]) * wx1 * wz1;
+  const double dz = (c[4] - c[0]) * wx0 * wy0 + (c[5] - c[1]) * wx1 * wy0 +
+                    (c[6] - c[2]) * wx0 * wy1 + (c[7] - c[3]) * wx1 * wy1;
+  return {dx, dy, dz};
+}
+
+struct Sample {
+  double x, y, z;  // fractional position in the SDF cell
+  double c[8];     // corner values phi000, phi100, phi010, phi110, phi001, phi101, phi011, phi111
+};
+// Grid coordinates, SDF box test and corner fetch of the detection kernels
+// (LSParticleContactDetectionKernel.cu:72-105). Returns false outside the box.
+__device__ __forceinline__ bool sampleSDF(V3 local, const fdm_template& t, const float* sdf,
+                                          Sample& s) {
+  const double ih = t.invSpacing;
+  const double gx = (local.x - t.origin[0]) * ih;
+  const double gy = (local.y - t.origin[1]) * ih;
+  const double gz = (local.z - t.origin[2]) * ih;
+  const int i0 = toInt(floor(gx)), j0 = toInt(floor(gy)), k0 = toInt(floor(gz));
+  if (i0 < 0 || j0 < 0 || k0 < 0) return false;
+  if (i0 >= t.size[0] - 1 || j0 >= t.size[1] - 1 || k0 >= t.size[2] - 1) return false;
+  s.x = gx - (double)i0;
+  s.y = gy - (double)j0;
+  s.z = gz - (double)k0;
+  const int nx = t.size[0], nxy = t.size[0] * t.size[1];
+  const float* b = sdf + t.sdfStart + (k0 * nxy + j0 * nx + i0);
+  s.c[0] = b[0];
+  s.c[1] = b[1];
+  s.c[2] = b[nx];
+  s.c[3] = b[nx + 1];
+  s.c[4] = b[nxy];
+  s.c[5] = b[nxy + 1];
+  s.c[6] = b[nxy + nx];
+  s.c[7] = b[nxy + nx + 1];
+  return true;
+}
+
+// ------------------------------------------------------------ contact law
+// contactKernel.cuh:4-34
+__device__ __forceinline__ V3 slidingSpring(V3 prev, V3 vt, V3 n, V3 Fn, double mu, double ks,
+                                            double ct, double dt) {
+  V3 spring = {0., 0., 0.};
+  if (mu > 0. && ks > 0.) {
+    V3 p1 = vsub(prev, vmul(n, vdot(prev, n)));
+    const double a1 = vlen(p1);
+    if (!nearZero(a1)) p1 = vmul(p1, vlen(prev) / a1);
+    spring = vadd(p1, vmul(vt, dt));
+    V3 sf = vsub(vmul(spring, -ks), vmul(vt, ct));
+    const double asf = vlen(sf), afn = vlen(Fn);
+    if (!nearZero(asf) && asf > mu * afn) {
+      const double ratio = mu * afn / asf;
+      sf = vmul(sf, ratio);
+      spring = vdiv(vneg(vadd(sf, vmul(vt, ct))), ks);
+    }
+  }
+  return spring;
+}
+// contactKernel.cuh:36-77. `diss` is the material's precomputed dissipation;
+// the damping coefficients use FP32 square roots as the reference does.
+__device__ __forceinline__ void linearContact(V3& F, V3& spring, V3 v, V3 n, double delta,
+                                              double dt, double kn, double ks, double mu,
+                                              double diss, double effM) {
+  if (delta > 0.) {
+    const double cn = 2. * diss * (double)sqrtf((float)(effM * kn));
+    const V3 vn = vmul(n, vdot(v, n));
+    const V3 Fn = vsub(vmul(n, kn * delta), vmul(vn, cn));
+    const double ct = 2. * diss * (double)sqrtf((float)(effM * ks));
+    const V3 vt = vsub(v, vn);
+    spring = slidingSpring(spring, vt, n, Fn, mu, ks, ct, dt);
+    const V3 Fs = vsub(vmul(spring, -ks), vmul(vt, ct));
+    F = vadd(Fn, Fs);
+  } else {
+    spring = {0., 0., 0.};
+  }
+}
+
+// ------------------------------------------------------- wave64 helpers
+__device__ __forceinline__ int laneId() { return __lane_id(); }
+__device__ __forceinline__ double readlaneD(double v, int lane) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)bits, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(bits >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int readlaneI(int v, int lane) {
+  return (int)__builtin_amdgcn_readlane((unsigned)v, lane);
+}
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return __ballot(p); }
+
+}  // namespace fdm

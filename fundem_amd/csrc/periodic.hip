@@ -1,0 +1,260 @@
+// Cell ids, planar periodic wrap and ghost images.
+//
+// Stage launchers restate the reference's kernels one for one
+// (mySpatialGrid.cu:4-56,87-152,214-240). The fused stage used by the step
+// computes, per particle, the X wrap, the X ghost, the Y wrap, the Y ghost and
+// the XY ghost in the reference's order (PeriodicBoundary.h:221-295), keeps
+// only ghosts whose ghost hash is valid (the others are never found by any
+// node), compacts them in particle order and builds one hash table per set.
+#include <climits>
+
+#include "fdm_device.hpp"
+#include "fdm_launch.hpp"
+
+namespace fdm {
+
+__global__ void kHashOnly(const float* pos, int n, fdm_grid grid, int* hashValue, unsigned* sortKey,
+                          int* sortIndex) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int h = cellHash(ld3(pos, n, i), grid);
+  hashValue[i] = h;
+  sortKey[i] = (unsigned)(h + 1);
+  sortIndex[i] = i;
+}
+
+__global__ void kGhostHashOnly(const double* gpos, int n, fdm_grid grid, int* hashValue,
+                               unsigned* sortKey, int* sortIndex) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int h = ghostCellHash(ldd3(gpos + 3 * i), grid);
+  hashValue[i] = h;
+  sortKey[i] = (unsigned)(h + 1);
+  sortIndex[i] = i;
+}
+
+__device__ __forceinline__ V3 offsetX(const fdm_grid& g) { return {g.max.x - g.min.x, 0., 0.}; }
+__device__ __forceinline__ V3 offsetY(const fdm_grid& g) { return {0., g.max.y - g.min.y, 0.}; }
+
+// updatePositionOutOfBoundaryXD/YD (mySpatialGrid.cu:28-56); result stored FP32
+__device__ __forceinline__ V3 wrapAxis(V3 p, const fdm_grid& g, int axis) {
+  if (axis == 0) {
+    if (p.x < g.min.x) p = vadd(p, offsetX(g));
+    else if (p.x >= g.max.x) p = vsub(p, offsetX(g));
+  } else {
+    if (p.y < g.min.y) p = vadd(p, offsetY(g));
+    else if (p.y >= g.max.y) p = vsub(p, offsetY(g));
+  }
+  return round32(p);
+}
+
+// calculateGhostPosition{X,Y,XY}DKernel (mySpatialGrid.cu:87-152)
+__device__ __forceinline__ V3 ghostOf(V3 p, const fdm_grid& g, int mask) {
+  const I3 c = gridPos(p, g);
+  if (mask == 1) return c.x == 0 ? vadd(p, offsetX(g)) : p;
+  if (mask == 2) return c.y == 0 ? vadd(p, offsetY(g)) : p;
+  return (c.x == 0 && c.y == 0) ? vadd(p, vadd(offsetX(g), offsetY(g))) : p;
+}
+
+__global__ void kWrap(float* pos, int n, fdm_grid g, int axis) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st3(pos, n, i, wrapAxis(ld3(pos, n, i), g, axis));
+}
+
+__global__ void kGhostPosition(double* gpos, const float* pos, int n, fdm_grid g, int mask) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const V3 p = ghostOf(ld3(pos, n, i), g, mask);
+  gpos[3 * i] = p.x;
+  gpos[3 * i + 1] = p.y;
+  gpos[3 * i + 2] = p.z;
+}
+
+// The positions the reference holds at each point of updateGhostSpatialGrid:
+// after the X wrap (ghost X is taken here), after the Y wrap (ghost Y and XY).
+struct WrapStages {
+  V3 afterX, afterY;
+};
+__device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_periodic per) {
+  WrapStages w;
+  w.afterX = per.x ? wrapAxis(x, g, 0) : x;
+  w.afterY = per.y ? wrapAxis(w.afterX, g, 1) : w.afterX;
+  return w;
+}
+
+// Pass 1: visibility flag of each particle's ghost in each active set.
+__global__ void kGhostFlags(const float* pos, int n, fdm_grid g, fdm_periodic per, int* fx, int* fy,
+                            int* fxy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const WrapStages w = wrapStages(ld3(pos, n, i), g, per);
+  if (per.x) fx[i] = ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0 ? 1 : 0;
+  if (per.y) fy[i] = ghostCellHash(ghostOf(w.afterY, g, 2), g) >= 0 ? 1 : 0;
+  if (per.x && per.y) fxy[i] = ghostCellHash(ghostOf(w.afterY, g, 3), g) >= 0 ? 1 : 0;
+}
+
+struct GhostOut {
+  int* slot;       // exclusive scan of the flags
+  int* realId;
+  double* pos;
+  int* hashValue;
+  unsigned* sortKey;
+  int* sortIndex;
+  int* cellBox;
+  int capacity;
+  unsigned* status;
+};
+
+__device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const fdm_grid& g) {
+  const int s = o.slot[i];
+  if (s >= o.capacity) {
+    atomicOr(o.status, FDM_STATUS_GHOST_OVERFLOW);
+    return;
+  }
+  const int h = ghostCellHash(gp, g);
+  o.realId[s] = i;
+  o.pos[3 * s] = gp.x;
+  o.pos[3 * s + 1] = gp.y;
+  o.pos[3 * s + 2] = gp.z;
+  o.hashValue[s] = h;
+  o.sortKey[s] = (unsigned)(h + 1);
+  o.sortIndex[s] = s;
+  if ((long long)h < g.num) {
+    const I3 c = decodeHash(h, g);
+    atomicMin(&o.cellBox[0], c.x);
+    atomicMin(&o.cellBox[1], c.y);
+    atomicMin(&o.cellBox[2], c.z);
+    atomicMax(&o.cellBox[3], c.x);
+    atomicMax(&o.cellBox[4], c.y);
+    atomicMax(&o.cellBox[5], c.z);
+  }
+}
+
+// Pass 2 (after the flags were scanned into slots): scatter the visible
+// ghosts of each set, then store the wrapped positions.
+__global__ void kGhostScatter(float* pos, int n, fdm_grid g, fdm_periodic per, GhostOut ox,
+                              GhostOut oy, GhostOut oxy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const WrapStages w = wrapStages(ld3(pos, n, i), g, per);
+  if (per.x) {
+    const V3 gp = ghostOf(w.afterX, g, 1);
+    if (ghostCellHash(gp, g) >= 0) emitGhost(ox, i, gp, g);
+  }
+  if (per.y) {
+    const V3 gp = ghostOf(w.afterY, g, 2);
+    if (ghostCellHash(gp, g) >= 0) emitGhost(oy, i, gp, g);
+  }
+  if (per.x && per.y) {
+    const V3 gp = ghostOf(w.afterY, g, 3);
+    if (ghostCellHash(gp, g) >= 0) emitGhost(oxy, i, gp, g);
+  }
+  st3(pos, n, i, w.afterY);
+}
+
+__global__ void kResetBox(int* box0, int* box1, int* box2) {
+  const int t = threadIdx.x;
+  if (t < 6) {
+    const int v = t < 3 ? INT_MAX : INT_MIN;
+    if (box0) box0[t] = v;
+    if (box1) box1[t] = v;
+    if (box2) box2[t] = v;
+  }
+}
+
+__global__ void kClampCount(int* count, int capacity) {
+  if (threadIdx.x == 0 && *count > capacity) *count = capacity;
+}
+
+}  // namespace fdm
+
+using namespace fdm;
+
+extern "C" int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const float* position,
+                                                       int n, fdm_grid grid, void* workspace,
+                                                       size_t workspaceBytes, hipStream_t stream) {
+  if (n < 0 || grid.num <= 0) return FDM_EINVAL;
+  if (n > 0)
+    hipLaunchKernelGGL(kHashOnly, dim3(blocksFor(n, 256)), dim3(256), 0, stream, position, n, grid,
+                       table.hashValue, table.sortKey, table.sortIndex);
+  return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream);
+}
+
+extern "C" int fdm_launchUpdatePositionOutOfBoundaryXD(float* position, int n, fdm_grid grid,
+                                                       hipStream_t stream) {
+  if (n < 0) return FDM_EINVAL;
+  if (n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kWrap, dim3(blocksFor(n, 256)), dim3(256), 0, stream, position, n, grid, 0);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchUpdatePositionOutOfBoundaryYD(float* position, int n, fdm_grid grid,
+                                                       hipStream_t stream) {
+  if (n < 0) return FDM_EINVAL;
+  if (n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kWrap, dim3(blocksFor(n, 256)), dim3(256), 0, stream, position, n, grid, 1);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchCalculateGhostPosition(double* ghostPosition, const float* position, int n,
+                                                fdm_grid grid, int axisMask, hipStream_t stream) {
+  if (n < 0 || axisMask < 1 || axisMask > 3) return FDM_EINVAL;
+  if (n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kGhostPosition, dim3(blocksFor(n, 256)), dim3(256), 0, stream, ghostPosition,
+                     position, n, grid, axisMask);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchUpdateGhostSpatialGridHashStartEnd(fdm_hash_table table,
+                                                            const double* ghostPosition, int n,
+                                                            fdm_grid grid, void* workspace,
+                                                            size_t workspaceBytes,
+                                                            hipStream_t stream) {
+  if (n < 0 || grid.num <= 0) return FDM_EINVAL;
+  if (n > 0)
+    hipLaunchKernelGGL(kGhostHashOnly, dim3(blocksFor(n, 256)), dim3(256), 0, stream, ghostPosition,
+                       n, grid, table.hashValue, table.sortKey, table.sortIndex);
+  return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream);
+}
+
+extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                        fdm_ghosts* sets, unsigned* status, void* workspace,
+                                        size_t workspaceBytes, hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets || !status) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  hipLaunchKernelGGL(kResetBox, dim3(1), dim3(64), 0, stream, on[0] ? sets[0].cellBox : nullptr,
+                     on[1] ? sets[1].cellBox : nullptr, on[2] ? sets[2].cellBox : nullptr);
+  hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
+                     grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag);
+  for (int s = 0; s < 3; ++s) {
+    if (!on[s]) continue;
+    int r = scanInts(sets[s].flag, sets[s].flag, (size_t)n, false, sets[s].count, workspace,
+                     workspaceBytes, stream);
+    if (r != FDM_OK) return r;
+  }
+  GhostOut o[3];
+  for (int s = 0; s < 3; ++s) {
+    o[s].slot = sets[s].flag;
+    o[s].realId = sets[s].realId;
+    o[s].pos = sets[s].pos;
+    o[s].hashValue = sets[s].table.hashValue;
+    o[s].sortKey = sets[s].table.sortKey;
+    o[s].sortIndex = sets[s].table.sortIndex;
+    o[s].cellBox = sets[s].cellBox;
+    o[s].capacity = sets[s].capacity;
+    o[s].status = status;
+  }
+  hipLaunchKernelGGL(kGhostScatter, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
+                     grid, periodic, o[0], o[1], o[2]);
+  for (int s = 0; s < 3; ++s) {
+    if (!on[s]) continue;
+    hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity);
+    int r = buildStartEnd(sets[s].table, sets[s].capacity, sets[s].count, grid, workspace,
+                          workspaceBytes, stream);
+    if (r != FDM_OK) return r;
+  }
+  return launchStatus();
+}

@@ -1,0 +1,644 @@
+// LSSolver: the FunDEM-compatible C++ host API (kernel/LSSolver.h). Public
+// and protected methods keep the reference's names, arguments and error
+// behaviour (messages on std::cerr and a silent return on invalid input);
+// the step calls the gfx950 launchers of include/fundem_amd.h.
+//
+// Additive API (not in the reference): setCellSize, setContactCapacity,
+// initialize/advance (step without output), useGraph, contacts, status,
+// stage timing.
+#pragma once
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/fundem_amd.h"
+#include "DeviceBuffer.h"
+#include "LSParticle.h"
+#include "PeriodicBoundary.h"
+#include "SolidInteraction.h"
+#include "VBondedInteraction.h"
+#include "types.h"
+
+namespace fundem {
+
+inline void fdmCheck(int r, const char* what) {
+  if (r != FDM_OK) throw std::runtime_error(std::string(what) + " failed with code " + std::to_string(r));
+}
+
+class LSSolver {
+ public:
+  explicit LSSolver(const std::string dir = "Problem", hipStream_t stream = 0,
+                    const size_t maxGPUThread = 256, const int device = 0)
+      : dir_(dir), stream_(stream), maxGPUThread_(maxGPUThread), device_(device) {}
+  virtual ~LSSolver() {
+    for (auto& g : graphs_)
+      if (g) (void)hipGraphExecDestroy(g);
+  }
+  LSSolver(const LSSolver&) = delete;
+  LSSolver& operator=(const LSSolver&) = delete;
+
+  // ---------------------------------------------------------------- inputs
+  void addParticleMaterial(double normalStiffness, double shearStiffness,
+                           double frictionCoefficient, double restitutionCoefficient) {
+    particles_.addMaterial(normalStiffness, shearStiffness, frictionCoefficient,
+                           restitutionCoefficient);
+  }
+  void addParticleGeometry(double mass, symMatrix inertiaTensor, double3 gridNodeOrigin,
+                           double gridNodeSpacing, int3 gridNodeSize,
+                           const std::vector<double>& gridNodeSignedDistance,
+                           const std::vector<double3>& boundaryNodePosition,
+                           const std::vector<int3>& boundaryNodeConnectivity) {
+    particles_.addGeometry(mass, inertiaTensor, gridNodeOrigin, gridNodeSpacing, gridNodeSize,
+                           gridNodeSignedDistance, boundaryNodePosition, boundaryNodeConnectivity);
+  }
+  void addParticle(double3 position, double3 velocity, double3 angularVelocity,
+                   quaternion orientation, int materialID, int geometryID) {
+    particles_.addParticle(position, velocity, angularVelocity, orientation, materialID, geometryID);
+  }
+  // LSSolver::addParticleCluster (LSSolver.h:125-165)
+  void addParticleCluster(const std::vector<double3>& position, const std::vector<double3>& velocity,
+                          const std::vector<double3>& angularVelocity,
+                          const std::vector<quaternion>& orientation, int materialID,
+                          double density, double3 gridNodeOrigin, double gridNodeSpacing,
+                          int3 gridNodeSize, const std::vector<double>& gridNodeSignedDistance,
+                          const std::vector<double3>& boundaryNodePosition,
+                          const std::vector<int3>& boundaryNodeConnectivity) {
+    if (position.size() != velocity.size() || position.size() != angularVelocity.size() ||
+        position.size() != orientation.size()) {
+      std::cerr << "[LSSolver] Inconsistent particle cluster data size. Position, velocity, "
+                   "angular velocity, and orientation vectors must have the same size."
+                << std::endl;
+      return;
+    }
+    double3 localOffset;
+    addParticleGeometry(localOffset, density, gridNodeOrigin, gridNodeSpacing, gridNodeSize,
+                        gridNodeSignedDistance, boundaryNodePosition, boundaryNodeConnectivity);
+    for (size_t i = 0; i < position.size(); ++i)
+      addParticle(vec::add(position[i], rotateVectorByQuaternion(orientation[i], localOffset)),
+                  velocity[i], angularVelocity[i], orientation[i], materialID,
+                  int(particles_.numGeometry()) - 1);
+  }
+  void addWallMaterial(double frictionCoefficient, double restitutionCoefficient) {
+    walls_.addMaterial(0.0, 0.0, frictionCoefficient, restitutionCoefficient);
+  }
+  void addWallGeometry(double3 gridNodeOrigin, double gridNodeSpacing, int3 gridNodeSize,
+                       const std::vector<double>& gridNodeSignedDistance,
+                       const std::vector<double3>& boundaryNodePosition,
+                       const std::vector<int3>& boundaryNodeConnectivity) {
+    walls_.addGeometry(0.0, symMatrix(), gridNodeOrigin, gridNodeSpacing, gridNodeSize,
+                       gridNodeSignedDistance, boundaryNodePosition, boundaryNodeConnectivity);
+  }
+  void addWall(double3 position, double3 velocity, double3 angularVelocity, quaternion orientation,
+               int materialID, int geometryID) {
+    walls_.addParticle(position, velocity, angularVelocity, orientation, materialID, geometryID);
+  }
+  void moveLSParticle(size_t index, double3 offSet) { particles_.move(index, offSet); }
+  void moveWall(size_t index, double3 offset) { walls_.move(index, offset); }
+  void setFixedVelocityToWall(size_t index, double3 velocity) { walls_.setVelocity(index, velocity); }
+  void setFixedAngularVelocityToWall(size_t index, double3 angularVelocity) {
+    walls_.setAngularVelocity(index, angularVelocity);
+  }
+  // LSSolver::addBondedInteraction (LSSolver.h:306-337). The bond frames are
+  // built from the FP32 state the device holds.
+  void addBondedInteraction(int masterObjectID, int slaveObjectID, double3 bondPoint,
+                            double3 bondNormal, double radius, double initialLength,
+                            double YoungsModulus, double poissonRatio, double tensileStrength = 0.,
+                            double cohesion = 0., double frictionCoefficient = 0.) {
+    if (masterObjectID >= (int)particles_.num() || slaveObjectID >= (int)particles_.num()) return;
+    auto r3 = [](double3 v) { return make_double3((float)v.x, (float)v.y, (float)v.z); };
+    auto r4 = [](quaternion q) {
+      return make_quaternion((float)q.q0, (float)q.q1, (float)q.q2, (float)q.q3);
+    };
+    bonds_.add(masterObjectID, slaveObjectID, r3(particles_.positionHostRef()[masterObjectID]),
+               r3(particles_.positionHostRef()[slaveObjectID]),
+               r4(particles_.orientationHostRef()[masterObjectID]),
+               r4(particles_.orientationHostRef()[slaveObjectID]), bondPoint, bondNormal, radius,
+               initialLength, YoungsModulus, poissonRatio, tensileStrength, cohesion,
+               frictionCoefficient);
+  }
+  void addPeriodicBoundaryXD() { periodic_.turnXOn(); }
+  void addPeriodicBoundaryYD() { periodic_.turnYOn(); }
+  void removePeriodicBoundary() {
+    periodic_.turnXOff();
+    periodic_.turnYOff();
+  }
+
+  // ------------------------------------------------------------- additive
+  // Cell size of the particle grid (the reference derives 2 * max radius,
+  // LSParticle.h:383). Must be at least the largest template radius.
+  void setCellSize(double cellSize) { cellSize_ = cellSize; }
+  void setContactCapacity(int pairsPerParticle, int recordsPerParticle) {
+    pairsPerParticle_ = std::max(pairsPerParticle, 1);
+    recordsPerParticle_ = std::max(recordsPerParticle, 1);
+  }
+  void useGraph(bool on) { useGraph_ = on; }
+  void setStageTiming(bool on) { timing_ = on; }
+  // Accumulated device time per stage (ms) and the number of launches timed.
+  const std::map<std::string, double>& stageMilliseconds() {
+    resolveTimers();
+    return stageMs_;
+  }
+  const std::map<std::string, long>& stageCalls() {
+    resolveTimers();
+    return stageCalls_;
+  }
+  void resetStageTiming() {
+    resolveTimers();
+    stageMs_.clear();
+    stageCalls_.clear();
+  }
+
+  // LSSolver::solve (LSSolver.h:440-467).
+  void solve(const double3 minDomain, const double3 maxDomain, const double3 gravity,
+             const double timeStep, const double maximumTime, const size_t numFrame, const int argc,
+             char** argv) {
+    (void)argc;
+    (void)argv;
+    if (!checkSolverInput(minDomain, maxDomain, timeStep, maximumTime)) return;
+    if (!activateGPUDevice()) return;
+    phase_ += 1;
+    initialize(minDomain, maxDomain, gravity, timeStep);
+    const size_t numStep = size_t(maximumTime / timeStep) + 1;
+    size_t frameInterval = numStep;
+    if (numFrame > 0) frameInterval = numStep / numFrame;
+    if (frameInterval < 1) frameInterval = 1;
+    size_t iStep = 0, iFrame = 0;
+    double time = 0.;
+    output(iFrame, iStep, time);
+    while (iStep <= numStep) {
+      stepOnce(false, time);
+      iStep += 1;
+      time += timeStep;
+      if (iStep % frameInterval == 0) {
+        iFrame++;
+        output(iFrame, iStep, time);
+      }
+    }
+    iFrame++;
+    output(iFrame, iStep, time);
+    std::cout << "[Solver] Computation Completed" << std::endl;
+  }
+
+  // Upload and the pre-loop build of the compute* variants
+  // (LSSolver.h:1097-1100): grid, contact lists (springs carried over, none
+  // updated) and, with periodicity, the wrap and ghost sets.
+  void initialize(const double3 minDomain, const double3 maxDomain, const double3 gravity,
+                  const double timeStep) {
+    if (!checkSolverInput(minDomain, maxDomain, timeStep, 0.0))
+      throw std::invalid_argument("invalid domain or time step");
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    gravity_ = gravity;
+    dt_ = timeStep;
+    particles_.initialize(minDomain, maxDomain, cellSize_, stream_);
+    walls_.initialize(minDomain, maxDomain, 0.0, stream_);
+    double maxR = 0.0;
+    for (size_t g = 0; g < particles_.numGeometry(); ++g)
+      maxR = std::max(maxR, particles_.templateInfo(g).radius);
+    if (cellSize_ > 0.0 && cellSize_ < maxR)
+      throw std::invalid_argument("cell size smaller than the largest particle radius");
+    const size_t N = particles_.num(), W = walls_.num();
+    const int pairCap = (int)std::max<size_t>(size_t(pairsPerParticle_) * N, 1024);
+    const int recCap = (int)std::max<size_t>(size_t(recordsPerParticle_) * N, 4096);
+    pp_.initialize(N, pairCap, recCap, mirrorCapacity_);
+    pw_.initialize(N, (int)std::max<size_t>(N * std::max<size_t>(W, 1), 1024), recCap, 1);
+    periodic_.initialize(particles_, (int)std::max<size_t>(N, 64),
+                         (int)std::max<size_t>(4 * N, 1024), recCap, mirrorCapacity_);
+    status_.allocate(1);
+    stepCounter_.allocate(1);
+    bonds_.initialize(N, stepCounter_.get(), stream_);
+    size_t ws = fdm_radixSortWorkspaceBytes(std::max<size_t>({N, W, size_t(64)}));
+    ws = std::max(ws, fdm_scanWorkspaceBytes(N + 1));
+    workspace_.allocate(ws / sizeof(double) + 1, false);
+    for (auto& g : graphs_)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    hookMode_ = HookUnknown;
+    stepOnce(true, 0.0);
+    hipCheck(hipStreamSynchronize(stream_), "initialize");
+    checkStatus();
+  }
+
+  // Advance nSteps loop iterations without output (the loop body of
+  // compute*, LSSolver.h:1101-1110 and variants).
+  void advance(size_t nSteps, double time = 0.0) {
+    size_t done = 0;
+    if (useGraph_ && hookMode_ == HookNone && !timing_) {
+      while (nSteps - done >= 2) {
+        replayGraph();
+        done += 2;
+      }
+    }
+    for (; done < nSteps; ++done) {
+      stepOnce(false, time);
+      time += dt_;
+    }
+  }
+
+  void download() {
+    particles_.finalize(stream_);
+    walls_.finalize(stream_);
+    bonds_.finalize(stream_);
+    hipCheck(hipStreamSynchronize(stream_), "download");
+  }
+
+  // Device status word (capacity overflows); synchronises.
+  unsigned status() {
+    if (!status_.get()) return 0;
+    unsigned v = 0;
+    hipCheck(hipMemcpyAsync(&v, status_.get(), sizeof(unsigned), hipMemcpyDeviceToHost, stream_),
+             "status");
+    hipCheck(hipStreamSynchronize(stream_), "status");
+    return v;
+  }
+  void checkStatus() {
+    const unsigned s = status();
+    if (s != 0)
+      throw std::runtime_error("contact capacity exceeded (status " + std::to_string(s) +
+                               "); raise it with setContactCapacity");
+  }
+
+  // Contacts of the latest step, canonical order. set: 0 particle-particle,
+  // 1 particle-wall, 2/3/4 ghost X/Y/XY.
+  std::vector<ContactRecord> contacts(int set) {
+    if (set == 0) return pp_.contacts(stream_);
+    if (set == 1) return pw_.contacts(stream_);
+    if (set >= 2 && set <= 4 && periodic_.active(set - 2))
+      return periodic_.set(set - 2).interaction.contacts(stream_);
+    return {};
+  }
+
+  LSParticle& getLSParticle() { return particles_; }
+  LSParticle& getWall() { return walls_; }
+  SolidInteraction& getLSParticleInteraction() { return pp_; }
+  SolidInteraction& getLSParticleWallInteraction() { return pw_; }
+  VBondedInteraction& getLSParticleBondedInteraction() { return bonds_; }
+  hipStream_t stream() const { return stream_; }
+  double timeStep() const { return dt_; }
+
+ protected:
+  // LSSolver::addParticleGeometry(localOffset, density, ...) (LSSolver.h:470-574):
+  // mass, centroid and inertia from the smooth Heaviside of the SDF; nodes
+  // and SDF origin shifted by -centroid.
+  void addParticleGeometry(double3& localOffset, double density, double3 gridNodeOrigin,
+                           double gridNodeSpacing, int3 gridNodeSize,
+                           const std::vector<double>& gridNodeSignedDistance,
+                           const std::vector<double3>& boundaryNodePosition,
+                           const std::vector<int3>& boundaryNodeConnectivity) {
+    localOffset = make_double3(0., 0., 0.);
+    if (gridNodeSize.x < 2 || gridNodeSize.y < 2 || gridNodeSize.z < 2) {
+      std::cerr << "[LSSolver] Invalid grid node size: " << gridNodeSize.x << " x " << gridNodeSize.y
+                << " x " << gridNodeSize.z << ". Each dimension must be at least 2." << std::endl;
+      return;
+    }
+    if (gridNodeSpacing <= 0.0) {
+      std::cerr << "[LSSolver] Invalid grid node spacing: " << gridNodeSpacing
+                << ". Must be greater than 0." << std::endl;
+      return;
+    }
+    const size_t expected =
+        size_t(gridNodeSize.x) * size_t(gridNodeSize.y) * size_t(gridNodeSize.z);
+    if (expected != gridNodeSignedDistance.size()) {
+      std::cerr << "[LSSolver] Inconsistent grid node data size. Expected " << expected
+                << " signed distance values, but got " << gridNodeSignedDistance.size() << "."
+                << std::endl;
+      return;
+    }
+    auto heaviside = [](double phi, double eps) -> double {
+      if (eps <= 0.0) return (phi > 0.0) ? 0.0 : 1.0;
+      if (phi < -eps) return 1.0;
+      if (phi > eps) return 0.0;
+      const double x = -phi / eps;
+      return 0.5 * (1.0 + x + std::sin(pi() * x) / pi());
+    };
+    const double h = gridNodeSpacing;
+    const double mg = density * h * h * h;
+    const int3 sz = gridNodeSize;
+    double mass = 0.;
+    for (int x = 0; x < sz.x; x++)
+      for (int y = 0; y < sz.y; y++)
+        for (int z = 0; z < sz.z; z++)
+          mass += heaviside(gridNodeSignedDistance[linearIndex3D(make_int3(x, y, z), sz)] / h, 1.5);
+    mass *= mg;
+    double3 c = make_double3(0., 0., 0.);
+    symMatrix I;
+    if (mass > 0.) {
+      for (int x = 0; x < sz.x; x++)
+        for (int y = 0; y < sz.y; y++)
+          for (int z = 0; z < sz.z; z++) {
+            const double H = heaviside(gridNodeSignedDistance[linearIndex3D(make_int3(x, y, z), sz)] / h, 1.5);
+            c.x += H * (gridNodeOrigin.x + double(x) * h);
+            c.y += H * (gridNodeOrigin.y + double(y) * h);
+            c.z += H * (gridNodeOrigin.z + double(z) * h);
+          }
+      c = vec::mul(c, mg / mass);
+      for (int x = 0; x < sz.x; x++)
+        for (int y = 0; y < sz.y; y++)
+          for (int z = 0; z < sz.z; z++) {
+            const double H = heaviside(gridNodeSignedDistance[linearIndex3D(make_int3(x, y, z), sz)] / h, 1.5);
+            const double3 r = vec::sub(
+                vec::add(gridNodeOrigin, vec::mul(make_double3(double(x), double(y), double(z)), h)), c);
+            I.xx += H * (r.y * r.y + r.z * r.z) * mg;
+            I.yy += H * (r.x * r.x + r.z * r.z) * mg;
+            I.zz += H * (r.y * r.y + r.x * r.x) * mg;
+            I.xy -= H * r.x * r.y * mg;
+            I.xz -= H * r.x * r.z * mg;
+            I.yz -= H * r.y * r.z * mg;
+          }
+    }
+    std::vector<double3> shifted;
+    shifted.reserve(boundaryNodePosition.size());
+    for (const auto& p : boundaryNodePosition) shifted.push_back(vec::sub(p, c));
+    particles_.addGeometry(mass, I, vec::sub(gridNodeOrigin, c), gridNodeSpacing, gridNodeSize,
+                           gridNodeSignedDistance, shifted, boundaryNodeConnectivity);
+    localOffset = c;
+  }
+
+  // User hook between the force reduction and the second half-kick
+  // (LSSolver.h:588). Device force/torque are FP32 planar SoA.
+  virtual void addExternalForceTorque(const double time) {
+    (void)time;
+    baseHookCalled_ = true;
+  }
+
+  // Called at every output frame after the state is downloaded.
+  virtual void outputFrame(size_t iFrame, size_t iStep, double time) {
+    (void)iFrame;
+    (void)iStep;
+    (void)time;
+  }
+
+ private:
+  enum HookMode { HookUnknown, HookNone, HookUser };
+
+  bool activateGPUDevice() {
+    if (hipSetDevice(device_) != hipSuccess) {
+      std::cout << "hipSetDevice( " << device_ << " ) failed! Do you have a HIP-capable GPU installed?"
+                << std::endl;
+      std::exit(1);
+    }
+    return true;
+  }
+  bool checkSolverInput(double3 mn, double3 mx, double dt, double T) {
+    const double3 L = vec::sub(mx, mn);
+    if (L.x <= 0.0 || L.y <= 0.0 || L.z <= 0.0) {
+      std::cerr << "[Solver] Invalid simulation domain size: (" << L.x << ", " << L.y << ", " << L.z
+                << ")." << std::endl;
+      return false;
+    }
+    if (dt <= 0.0) {
+      std::cerr << "[Solver] Invalid timeStep: " << dt << "." << std::endl;
+      return false;
+    }
+    if (T < 0.0) {
+      std::cerr << "[Solver] Invalid maximumTime: " << T << "." << std::endl;
+      return false;
+    }
+    return true;
+  }
+  void output(size_t iFrame, size_t iStep, double time) {
+    std::cout << "[Solver] Phase " << phase_ << ": ------ Frame " << iFrame << " at Time " << time
+              << " ------ " << std::endl;
+    download();
+    checkStatus();
+    outputFrame(iFrame, iStep, time);
+  }
+
+  // Brackets a stage with two events on the solver stream; no host
+  // synchronisation until the times are read (resolveTimers).
+  struct Timer {
+    LSSolver* s;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(LSSolver* sv, const char* nm) : s(sv), name(nm) {
+      if (!s->timing_) return;
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, s->stream_);
+    }
+    ~Timer() {
+      if (!s->timing_) return;
+      (void)hipEventRecord(b, s->stream_);
+      s->pending_.push_back({name, a, b});
+    }
+  };
+  struct PendingTimer {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  void resolveTimers() {
+    for (auto& p : pending_) {
+      (void)hipEventSynchronize(p.b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, p.a, p.b);
+      stageMs_[p.name] += ms;
+      stageCalls_[p.name] += 1;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending_.clear();
+  }
+
+  void interactionSet(int set, SolidInteraction& si, const fdm_ghosts* ghosts, fdm_grid grid,
+                      fdm_hash_table table, const int* targetHash, bool prime) {
+    fdm_broadphase_args b{};
+    b.set = set;
+    b.particles = particles_.bodies();
+    b.pgeo = particles_.geometry();
+    b.walls = walls_.bodies();
+    b.wgeo = walls_.geometry();
+    b.ghosts = ghosts;
+    b.grid = grid;
+    b.table = table;
+    b.pairs = si.current();
+    {
+      Timer t(this, set == 0 ? "broad_pp" : set == 1 ? "broad_pw" : "broad_ghost");
+      fdmCheck(fdm_launchBroadPhase(&b, workspace_.get(), workspace_.size() * sizeof(double), stream_),
+               "broad phase");
+      fdmCheck(fdm_launchBroadPhaseWrite(&b, status_.get(), stream_), "broad phase write");
+    }
+    fdm_narrowphase_args n{};
+    n.set = set;
+    n.particles = b.particles;
+    n.pgeo = b.pgeo;
+    n.pmat = particles_.deviceMaterials();
+    n.walls = b.walls;
+    n.wgeo = b.wgeo;
+    n.wmat = walls_.deviceMaterials();
+    n.ghosts = ghosts;
+    n.grid = grid;
+    n.targetHash = targetHash;
+    n.pgrid = particles_.grid();
+    n.periodic = periodic_.flags();
+    n.pairs = si.current();
+    n.prev = si.previous();
+    n.prevStart = n.prev.count;
+    n.mirror = si.mirror();
+    n.dt = dt_;
+    n.updateSprings = prime ? 0 : 1;
+    n.status = status_.get();
+    Timer t(this, set == 0 ? "narrow_pp" : set == 1 ? "narrow_pw" : "narrow_ghost");
+    fdmCheck(fdm_launchNarrowPhase(&n, stream_), "narrow phase");
+  }
+
+  void stepOnce(bool prime, double time) {
+    fdm_bodies P = particles_.bodies();
+    fdm_geometry pgeo = particles_.geometry();
+    const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
+    const size_t N = particles_.num(), W = walls_.num();
+    void* ws = workspace_.get();
+    const size_t wsBytes = workspace_.size() * sizeof(double);
+    {
+      Timer t(this, "integrate_hash");
+      fdmCheck(fdm_launchKickDriftHash(P, pgeo, g, dt_, prime ? 0 : 1, prime ? 0 : 1,
+                                       particles_.grid(), particles_.table().view(), stream_),
+               "kick/drift");
+      if (W)
+        fdmCheck(fdm_launchKickDriftHash(walls_.bodies(), walls_.geometry(), g, dt_, 0, prime ? 0 : 1,
+                                         walls_.grid(), walls_.table().view(), stream_),
+                 "wall drift");
+    }
+    {
+      Timer t(this, "sort_cells");
+      fdmCheck(fdm_buildHashStartEnd(particles_.table().view(), (int)N, particles_.grid(), ws,
+                                     wsBytes, stream_),
+               "hash table");
+      if (W)
+        fdmCheck(fdm_buildHashStartEnd(walls_.table().view(), (int)W, walls_.grid(), ws, wsBytes,
+                                       stream_),
+                 "wall hash table");
+    }
+    interactionSet(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(), particles_.table().view(),
+                   particles_.table().hashValue.get(), prime);
+    if (W)
+      interactionSet(FDM_SET_WALL, pw_, nullptr, walls_.grid(), walls_.table().view(),
+                     walls_.table().hashValue.get(), prime);
+    fdm_ghosts gv[3];
+    const fdm_pairs* gpairs[3] = {nullptr, nullptr, nullptr};
+    const fdm_mirror* gmir[3] = {nullptr, nullptr, nullptr};
+    fdm_pairs gcur[3];
+    fdm_mirror gm[3];
+    if (periodic_.isActived()) {
+      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.set(s).view() : fdm_ghosts{};
+      {
+        Timer t(this, "periodic");
+        fdmCheck(fdm_launchPeriodicGhosts(P, particles_.grid(), periodic_.flags(), gv, status_.get(),
+                                          ws, wsBytes, stream_),
+                 "periodic ghosts");
+      }
+      for (int s = 0; s < 3; ++s) {
+        if (!periodic_.active(s)) continue;
+        GhostSet& G = periodic_.set(s);
+        interactionSet(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(), G.table.view(),
+                       G.table.hashValue.get(), prime);
+        gcur[s] = G.interaction.current();
+        gm[s] = G.interaction.mirror();
+        gpairs[s] = &gcur[s];
+        gmir[s] = &gm[s];
+      }
+    }
+    if (prime) {
+      // no forces in the pre-loop build: drop the mirror entries
+      fdm_mirror m = pp_.mirror();
+      fdmCheck(fdm_fillInt(m.count, std::max<size_t>(N, 1), 0, stream_), "clear");
+      for (int s = 0; s < 3; ++s)
+        if (gmir[s]) fdmCheck(fdm_fillInt(gmir[s]->count, std::max<size_t>(N, 1), 0, stream_), "clear");
+      swapAll();
+      return;
+    }
+    fdm_bonds bv = bonds_.view();
+    if (bv.n > 0) {
+      Timer t(this, "bonds");
+      fdmCheck(fdm_launchBondForces(P, bv, stream_), "bonds");
+    }
+    fdm_pairs ppc = pp_.current(), pwc = pw_.current();
+    fdm_mirror ppm = pp_.mirror();
+    fdm_reduce_args r{};
+    r.particles = P;
+    r.pgeo = pgeo;
+    r.pp = &ppc;
+    r.ppMirror = &ppm;
+    r.pw = W ? &pwc : nullptr;
+    for (int s = 0; s < 3; ++s) {
+      r.ghost[s] = gpairs[s];
+      r.ghostMirror[s] = gmir[s];
+    }
+    r.bonds = bv.n > 0 ? &bv : nullptr;
+    r.gravity = g;
+    r.h = 0.5 * dt_;
+    {
+      Timer t(this, "reduce_kick");
+      if (hookMode_ == HookNone) {
+        r.kick = 1;
+        fdmCheck(fdm_launchReduceForceTorque(&r, stream_), "reduce");
+      } else {
+        r.kick = 0;
+        fdmCheck(fdm_launchReduceForceTorque(&r, stream_), "reduce");
+        baseHookCalled_ = false;
+        addExternalForceTorque(time);
+        if (hookMode_ == HookUnknown) hookMode_ = baseHookCalled_ ? HookNone : HookUser;
+        fdmCheck(fdm_launchParticleVelocityAngularVelocityIntegration(P, pgeo, g, 0.5 * dt_, stream_),
+                 "kick");
+      }
+    }
+    fdmCheck(fdm_incrementCounter(stepCounter_.get(), stream_), "step counter");
+    swapAll();
+  }
+
+  void swapAll() {
+    pp_.swap();
+    pw_.swap();
+    for (int s = 0; s < 3; ++s)
+      if (periodic_.active(s)) periodic_.set(s).interaction.swap();
+  }
+
+  // Two steps (one full swap cycle, so the buffer roles after a replay equal
+  // those before it) captured once per parity and replayed.
+  void replayGraph() {
+    const int parity = pp_.currentIndex();
+    if (!graphs_[parity]) {
+      hipGraph_t graph;
+      hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal), "capture");
+      stepOnce(false, 0.0);
+      stepOnce(false, 0.0);
+      hipCheck(hipStreamEndCapture(stream_, &graph), "capture end");
+      hipCheck(hipGraphInstantiate(&graphs_[parity], graph, nullptr, nullptr, 0), "instantiate");
+      (void)hipGraphDestroy(graph);
+    }
+    hipCheck(hipGraphLaunch(graphs_[parity], stream_), "graph launch");
+  }
+
+  std::string dir_;
+  size_t phase_ = 0;
+  hipStream_t stream_;
+  size_t maxGPUThread_;
+  int device_;
+  LSParticle particles_, walls_;
+  SolidInteraction pp_, pw_;
+  VBondedInteraction bonds_;
+  PeriodicBoundaryXY2D periodic_;
+  double3 gravity_ = make_double3(0., 0., 0.);
+  double dt_ = 0.0;
+  double cellSize_ = 0.0;
+  int pairsPerParticle_ = 16, recordsPerParticle_ = 48, mirrorCapacity_ = 48;
+  DeviceBuffer<double> workspace_;
+  DeviceBuffer<unsigned> status_;
+  DeviceBuffer<int> stepCounter_;
+  bool useGraph_ = false, timing_ = false;
+  bool baseHookCalled_ = false;
+  HookMode hookMode_ = HookUnknown;
+  hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+  std::map<std::string, double> stageMs_;
+  std::map<std::string, long> stageCalls_;
+  std::vector<PendingTimer> pending_;
+};
+
+}  // namespace fundem
+
+using fundem::LSSolver;

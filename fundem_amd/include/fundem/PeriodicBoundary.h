@@ -1,0 +1,83 @@
+// PeriodicBoundaryXY2D: planar periodicity in x and/or y (kernel/
+// PeriodicBoundary.h:178-317). The reference keeps a full ghost copy of every
+// particle per set (X, Y, XY) and hashes all of them; only ghosts whose ghost
+// hash is valid can ever be found by a node, so the device keeps just those,
+// compacted in particle order, with one hash table and one interaction set
+// per active ghost set.
+#pragma once
+#include <algorithm>
+
+#include "../../../include/fundem_amd.h"
+#include "DeviceBuffer.h"
+#include "LSParticle.h"
+#include "SolidInteraction.h"
+
+namespace fundem {
+
+struct GhostSet {
+  int capacity = 0;
+  DeviceBuffer<int> count, flag, realId, cellBox;
+  DeviceBuffer<double> pos;
+  HashTableBuffers table;
+  SolidInteraction interaction;
+  void allocate(size_t n, int cap, long long numGrid) {
+    capacity = std::max(cap, 1);
+    count.allocate(1);
+    flag.allocate(std::max<size_t>(n, 1));
+    realId.allocate(capacity);
+    pos.allocate(3 * (size_t)capacity);
+    cellBox.allocate(6);
+    table.allocate(capacity, numGrid);
+  }
+  fdm_ghosts view() {
+    fdm_ghosts g;
+    g.capacity = capacity;
+    g.count = count.get();
+    g.flag = flag.get();
+    g.realId = realId.get();
+    g.pos = pos.get();
+    g.table = table.view();
+    g.cellBox = cellBox.get();
+    return g;
+  }
+};
+
+struct PeriodicBoundaryXY2D {
+ public:
+  void turnXOn() { x_ = true; }
+  void turnYOn() { y_ = true; }
+  void turnXOff() { x_ = false; }
+  void turnYOff() { y_ = false; }
+  bool isActived() const { return x_ || y_; }
+  bool active(int set) const { return set == 0 ? x_ : set == 1 ? y_ : (x_ && y_); }
+  fdm_periodic flags() const {
+    fdm_periodic p;
+    p.x = x_ ? 1 : 0;
+    p.y = y_ ? 1 : 0;
+    return p;
+  }
+
+  void initialize(LSParticle& P, int ghostCapacity, int pairCapacity, int recordCapacity,
+                  int mirrorCapacity) {
+    for (int s = 0; s < 3; ++s) {
+      if (!active(s)) continue;
+      if (sets_[s].capacity != std::max(ghostCapacity, 1) || allocatedFor_[s] != P.num())
+        sets_[s].allocate(P.num(), ghostCapacity, P.grid().num);
+      allocatedFor_[s] = P.num();
+      sets_[s].interaction.initialize(P.num(), pairCapacity, recordCapacity, mirrorCapacity);
+    }
+  }
+  GhostSet& set(int s) { return sets_[s]; }
+  double deviceMemoryGB() const {
+    double t = 0;
+    for (const auto& s : sets_) t += s.pos.memoryGB() + s.interaction.deviceMemoryGB();
+    return t;
+  }
+
+ private:
+  bool x_ = false, y_ = false;
+  GhostSet sets_[3];
+  size_t allocatedFor_[3] = {(size_t)-1, (size_t)-1, (size_t)-1};
+};
+
+}  // namespace fundem

@@ -1,0 +1,242 @@
+"""Python binding of the C++ host API (LSSolver) through libfundem_solver.so.
+
+`Solver` mirrors LSSolver's methods (kernel/LSSolver.h:14-467) with numpy
+arrays for vectors. It runs only on the HIP path: constructing it raises if
+the native libraries are missing or no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_uint)
+
+_LIB = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def solver_lib():
+    """Load libfundem_solver.so (and through it libfundem_amd.so)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not (os.path.exists(_build.SOLVER_LIB) and os.path.exists(_build.CORE_LIB)):
+        raise NativeLibraryMissing(
+            f"native libraries not built ({_build.SOLVER_LIB}); run fundem_amd.build.build_all()")
+    C.CDLL(_build.CORE_LIB, mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(_build.SOLVER_LIB)
+    I, LL, D, P = C.c_int, C.c_longlong, C.c_double, C.c_void_p
+    sig = {
+        "fdmx_last_error": (C.c_char_p, []),
+        "fdmx_create": (P, [I]),
+        "fdmx_destroy": (None, [P]),
+        "fdmx_add_material": (I, [P, D, D, D, D]),
+        "fdmx_add_wall_material": (I, [P, D, D]),
+        "fdmx_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
+        "fdmx_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
+        "fdmx_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I]),
+        "fdmx_add_particles": (I, [P, I, _dp, _dp, _dp, _dp, _ip, _ip]),
+        "fdmx_add_wall": (I, [P, _dp, _dp, _dp, _dp, I, I]),
+        "fdmx_add_bond": (I, [P, I, I, _dp, _dp, D, D, D, D, D, D, D]),
+        "fdmx_set_periodic": (I, [P, I, I]),
+        "fdmx_set_cell_size": (I, [P, D]),
+        "fdmx_set_capacity": (I, [P, I, I]),
+        "fdmx_use_graph": (I, [P, I]),
+        "fdmx_set_timing": (I, [P, I]),
+        "fdmx_initialize": (I, [P, _dp, _dp, _dp, D]),
+        "fdmx_advance": (I, [P, LL]),
+        "fdmx_sync": (I, [P]),
+        "fdmx_status": (I, [P, _up]),
+        "fdmx_get_state": (I, [P, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "fdmx_num_particles": (LL, [P]),
+        "fdmx_get_contacts": (LL, [P, I, _ip, _ip, _ip, _dp, LL]),
+        "fdmx_get_bonds": (I, [P, _ip, _ip, _dp, _dp]),
+        "fdmx_get_hash": (I, [P, I, _ip, _up, _ip, _ip, _ip]),
+        "fdmx_get_grid": (I, [P, I, _dp, _dp, _dp, _ip]),
+        "fdmx_get_geometry": (I, [P, I, _dp, _dp, _dp, _dp, _dp]),
+        "fdmx_stage_times": (I, [P, C.c_char_p, I, _dp, C.POINTER(C.c_longlong), I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _d(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+
+
+def _dp_(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _ip_(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+class Solver:
+    """LSSolver on the GPU (device `device`)."""
+
+    def __init__(self, device: int = 0):
+        lib = solver_lib()
+        self._lib = lib
+        self.h = lib.fdmx_create(int(device))
+        if not self.h:
+            raise RuntimeError(lib.fdmx_last_error().decode())
+        self._n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fdmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ok(self, r):
+        if r != 0:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+
+    # ---------------------------------------------------------------- inputs
+    def add_material(self, kn, ks, mu, e):
+        self._ok(self._lib.fdmx_add_material(self.h, kn, ks, mu, e))
+
+    def add_wall_material(self, mu, e):
+        self._ok(self._lib.fdmx_add_wall_material(self.h, mu, e))
+
+    def add_geometry(self, mass, inertia6, t):
+        self._ok(self._lib.fdmx_add_geometry(
+            self.h, mass, _dp_(_d(inertia6)), _dp_(_d(t.origin)), t.spacing,
+            _ip_(np.asarray(t.size, np.int32)), _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes)))
+
+    def add_geometry_density(self, density, t):
+        off = np.zeros(3)
+        self._ok(self._lib.fdmx_add_geometry_density(
+            self.h, density, _dp_(_d(t.origin)), t.spacing, _ip_(np.asarray(t.size, np.int32)),
+            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes), _dp_(off)))
+        return off
+
+    def add_wall_geometry(self, t):
+        self._ok(self._lib.fdmx_add_wall_geometry(
+            self.h, _dp_(_d(t.origin)), t.spacing, _ip_(np.asarray(t.size, np.int32)),
+            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes)))
+
+    def add_particles(self, x, v, w, q, mat, geo):
+        x, v, w, q = _d(x), _d(v), _d(w), _d(q)
+        n = x.size // 3
+        mat = np.ascontiguousarray(np.broadcast_to(np.asarray(mat, np.int32), (n,)))
+        geo = np.ascontiguousarray(np.broadcast_to(np.asarray(geo, np.int32), (n,)))
+        self._ok(self._lib.fdmx_add_particles(self.h, n, _dp_(x), _dp_(v), _dp_(w), _dp_(q),
+                                              _ip_(mat), _ip_(geo)))
+        self._n += n
+
+    def add_wall(self, x, v, w, q, mat, geo):
+        self._ok(self._lib.fdmx_add_wall(self.h, _dp_(_d(x)), _dp_(_d(v)), _dp_(_d(w)), _dp_(_d(q)),
+                                         mat, geo))
+
+    def add_bond(self, i, j, point, normal, radius, length, youngs, poisson, tensile=0.0,
+                 cohesion=0.0, friction=0.0):
+        self._ok(self._lib.fdmx_add_bond(self.h, i, j, _dp_(_d(point)), _dp_(_d(normal)), radius,
+                                         length, youngs, poisson, tensile, cohesion, friction))
+
+    def set_periodic(self, x, y):
+        self._ok(self._lib.fdmx_set_periodic(self.h, int(x), int(y)))
+
+    def set_cell_size(self, cs):
+        self._ok(self._lib.fdmx_set_cell_size(self.h, float(cs)))
+
+    def set_capacity(self, pairs_per_particle, records_per_particle):
+        self._ok(self._lib.fdmx_set_capacity(self.h, int(pairs_per_particle), int(records_per_particle)))
+
+    def use_graph(self, on=True):
+        self._ok(self._lib.fdmx_use_graph(self.h, int(on)))
+
+    def set_timing(self, on=True):
+        self._ok(self._lib.fdmx_set_timing(self.h, int(on)))
+
+    # ------------------------------------------------------------------ run
+    def initialize(self, mn, mx, gravity, dt):
+        self._ok(self._lib.fdmx_initialize(self.h, _dp_(_d(mn)), _dp_(_d(mx)), _dp_(_d(gravity)),
+                                           float(dt)))
+
+    def step(self, n=1):
+        self._ok(self._lib.fdmx_advance(self.h, int(n)))
+
+    def sync(self):
+        self._ok(self._lib.fdmx_sync(self.h))
+
+    def status(self) -> int:
+        v = C.c_uint(0)
+        self._ok(self._lib.fdmx_status(self.h, C.byref(v)))
+        return int(v.value)
+
+    # -------------------------------------------------------------- results
+    @property
+    def n(self) -> int:
+        return int(self._lib.fdmx_num_particles(self.h))
+
+    def state(self):
+        n = self.n
+        x, v, w, f, t = (np.zeros((n, 3)) for _ in range(5))
+        q = np.zeros((n, 4))
+        self._ok(self._lib.fdmx_get_state(self.h, _dp_(x), _dp_(v), _dp_(w), _dp_(q), _dp_(f), _dp_(t)))
+        return dict(x=x, v=v, w=w, q=q, F=f, T=t)
+
+    def contacts(self, set_id):
+        n = self._lib.fdmx_get_contacts(self.h, set_id, None, None, None, None, 0)
+        if n < 0:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+        a, b, k = (np.zeros(n, np.int32) for _ in range(3))
+        sp = np.zeros((n, 3))
+        self._lib.fdmx_get_contacts(self.h, set_id, _ip_(a), _ip_(b), _ip_(k), _dp_(sp), n)
+        return dict(A=a, B=b, k=k, spring=sp)
+
+    def bonds(self, nbonds):
+        act, brk = np.zeros(nbonds, np.int32), np.zeros(nbonds, np.int32)
+        s, t = np.zeros(nbonds), np.zeros(nbonds)
+        self._ok(self._lib.fdmx_get_bonds(self.h, _ip_(act), _ip_(brk), _dp_(s), _dp_(t)))
+        return dict(active=act, break_step=brk, sigma=s, tau=t)
+
+    def grid(self, walls=False):
+        mn, mx, inv = np.zeros(3), np.zeros(3), np.zeros(3)
+        size = np.zeros(3, np.int32)
+        self._ok(self._lib.fdmx_get_grid(self.h, int(walls), _dp_(mn), _dp_(mx), _dp_(inv), _ip_(size)))
+        return dict(min=mn, max=mx, inv=inv, size=size)
+
+    def hash_table(self, n, walls=False):
+        g = self.grid(walls)
+        ng = int(np.prod(g["size"].astype(np.int64)))
+        hv = np.zeros(n, np.int32)
+        key = np.zeros(n, np.uint32)
+        idx = np.zeros(n, np.int32)
+        st, en = np.zeros(ng, np.int32), np.zeros(ng, np.int32)
+        self._ok(self._lib.fdmx_get_hash(self.h, int(walls), _ip_(hv), key.ctypes.data_as(_up),
+                                         _ip_(idx), _ip_(st), _ip_(en)))
+        return dict(hash=hv, sorted_hash=key.astype(np.int64) - 1, index=idx, start=st, end=en)
+
+    def geometry(self, g):
+        r, im = np.zeros(1), np.zeros(1)
+        I, Ii, o = np.zeros(6), np.zeros(6), np.zeros(3)
+        self._ok(self._lib.fdmx_get_geometry(self.h, g, _dp_(r), _dp_(im), _dp_(I), _dp_(Ii), _dp_(o)))
+        return dict(radius=r[0], inv_mass=im[0], inertia=I, inv_inertia=Ii, origin=o)
+
+    def stage_times(self):
+        """{stage: (total device ms, launches timed)} since set_timing(True)."""
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(64)
+        calls = np.zeros(64, np.int64)
+        cnt = self._lib.fdmx_stage_times(self.h, names, 4096, _dp_(ms),
+                                         calls.ctypes.data_as(C.POINTER(C.c_longlong)), 64)
+        keys = names.value.decode().split("\n")[:cnt]
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(keys)}

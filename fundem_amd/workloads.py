@@ -1,0 +1,257 @@
+"""Seeded workloads: the configurations the solver is exercised and measured on.
+
+Each builder returns a `Scenario` (inputs only). `apply(scenario, target)`
+feeds it through the solver API method names, so the same scenario drives
+the device solver and any object with the same methods.
+
+Conventions recorded here:
+  * "exponents p1, p2" are superellipsoid powers (ee = 2/p1, en = 2/p2);
+  * damping ratio zeta maps to restitution e = exp(-pi zeta / sqrt(1 - zeta^2))
+    (the inverse of contactKernel.cuh:50-51);
+  * kt = 0.4 kn where a configuration does not give it;
+  * periodic boxes put max.y equal to max.z, so the ghost-hash z test, which
+    compares with max.y (mySpatialGrid.cu:236), coincides with the z bound.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import generators as gen
+
+
+def restitution_from_damping(zeta: float) -> float:
+    return math.exp(-math.pi * zeta / math.sqrt(1.0 - zeta * zeta))
+
+
+@dataclass
+class Scenario:
+    name: str
+    templates: list
+    density: object  # one value, or one per template
+    materials: list  # (kn, ks, mu, e)
+    x: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    q: np.ndarray
+    mat: np.ndarray
+    geo: np.ndarray
+    domain_min: np.ndarray
+    domain_max: np.ndarray
+    gravity: np.ndarray
+    dt: float
+    wall_templates: list = field(default_factory=list)
+    wall_materials: list = field(default_factory=list)  # (mu, e)
+    walls: list = field(default_factory=list)  # (x, v, w, q, mat, geo)
+    periodic: tuple = (False, False)
+    cell_size: float = 0.0
+    bonds: list = field(default_factory=list)  # (i, j, point, normal, R, L, E, nu, st, c, mub)
+    steps: int = 1
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+
+def apply(s: Scenario, t) -> None:
+    """Feed a scenario into a solver-like object."""
+    for m in s.materials:
+        t.add_material(*m)
+    dens = s.density if isinstance(s.density, (list, tuple)) else [s.density] * len(s.templates)
+    for d, tp in zip(dens, s.templates):
+        t.add_geometry_density(d, tp)
+    for m in s.wall_materials:
+        t.add_wall_material(*m)
+    for tp in s.wall_templates:
+        t.add_wall_geometry(tp)
+    t.add_particles(s.x, s.v, s.w, s.q, s.mat, s.geo)
+    for wl in s.walls:
+        t.add_wall(*wl)
+    for b in s.bonds:
+        t.add_bond(*b)
+    if any(s.periodic):
+        t.set_periodic(*s.periodic)
+    if s.cell_size > 0:
+        t.set_cell_size(s.cell_size)
+    t.initialize(s.domain_min, s.domain_max, s.gravity, s.dt)
+
+
+IDENT = np.array([1.0, 0.0, 0.0, 0.0])
+
+
+def floor_wall(xy_min, xy_max, z0: float = 0.0):
+    """A plane wall at height z0 covering the given xy range."""
+    return gen.plane_wall(xy_min, xy_max), (np.array([0.5 * (xy_min[0] + xy_max[0]),
+                                                     0.5 * (xy_min[1] + xy_max[1]), z0]),
+                                           np.zeros(3), np.zeros(3), IDENT, 0, 0)
+
+
+def single_particle_on_wall(n_nodes: int = 2000, seed: int = 7, steps: int = 100000) -> Scenario:
+    """One superellipsoid (semi-axes 10, 8, 6 mm, powers 4, 4, density 2500)
+    rolling on a plane under gravity: 2000 nodes, 32^3 SDF, kn = 1e5 N/m,
+    kt = 0.4 kn, damping ratio 0.3, mu = 0.3, v0 = 0.5 m/s along x, random
+    orientation, dt = 1e-5 s."""
+    rng = np.random.default_rng(seed)
+    t = gen.superellipsoid_template(0.010, 0.008, 0.006, 4.0, 4.0, n_grid=32, n_nodes=n_nodes)
+    q = gen.shoemake_quaternions(rng, 1)
+    # start with the lowest node a hair above the floor
+    R = _rotmat(q[0])
+    lowest = float(np.min((t.nodes.astype(np.float64) @ R.T)[:, 2]))
+    x = np.array([[0.0, 0.0, -lowest + 1e-5]])
+    wt, wall = floor_wall([-0.5, -0.5], [0.5, 0.5])
+    kn = 1e5
+    return Scenario(
+        name="single-particle-roll", templates=[t], density=2500.0,
+        materials=[(kn, 0.4 * kn, 0.3, restitution_from_damping(0.3))],
+        x=x, v=np.array([[0.5, 0.0, 0.0]]), w=np.zeros((1, 3)), q=q,
+        mat=np.zeros(1, np.int32), geo=np.zeros(1, np.int32),
+        domain_min=np.array([-0.5, -0.5, -0.5]), domain_max=np.array([0.5, 0.5, 0.5]),
+        gravity=np.array([0.0, 0.0, -9.81]), dt=1e-5,
+        wall_templates=[wt], wall_materials=[(0.3, restitution_from_damping(0.3))], walls=[wall],
+        steps=steps)
+
+
+def _rotmat(q):
+    q0, q1, q2, q3 = q
+    return np.array([
+        [1 - 2 * (q2 * q2 + q3 * q3), 2 * (q1 * q2 - q0 * q3), 2 * (q1 * q3 + q0 * q2)],
+        [2 * (q1 * q2 + q0 * q3), 1 - 2 * (q1 * q1 + q3 * q3), 2 * (q2 * q3 - q0 * q1)],
+        [2 * (q1 * q3 - q0 * q2), 2 * (q2 * q3 + q0 * q1), 1 - 2 * (q1 * q1 + q2 * q2)],
+    ])
+
+
+def superellipsoid_family(count: int, seed: int, n_grid: int = 32, n_nodes: int = 1000,
+                          axes=(0.8e-3, 1.2e-3), powers=(2.0, 8.0)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        a, b, c = rng.uniform(*axes, size=3)
+        p1, p2 = rng.uniform(*powers, size=2)
+        out.append(gen.superellipsoid_template(a, b, c, p1, p2, n_grid=n_grid, n_nodes=n_nodes))
+    return out
+
+
+def pour(counts=(100, 100, 100), spacing: float = 3.0e-3, n_templates: int = 16,
+         n_nodes: int = 1000, n_grid: int = 32, seed: int = 2024, cell_size: float = 2.5e-3,
+         kn: float = 1e4, mu: float = 0.5, zeta: float = 0.3, dt: float = 5e-7,
+         steps: int = 10000, templates=None) -> Scenario:
+    """Pour of superellipsoids (semi-axes U[0.8, 1.2] mm, powers U[2, 8],
+    16 templates, 32^3 SDF, 1000 nodes) from a jittered lattice with random
+    orientations, in a box periodic in x and y with a floor wall at z = 0,
+    cell size 2.5 mm, kn = 1e4 N/m, mu = 0.5, dt = 5e-7 s. Lattice centres
+    start 0.75 spacing above the floor (jitter 5% of the spacing)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = counts
+    tmpl = templates if templates is not None else superellipsoid_family(
+        n_templates, seed + 1, n_grid=n_grid, n_nodes=n_nodes)
+    n = nx * ny * nz
+    Lx, Ly = nx * spacing, ny * spacing
+    H = nz * spacing + 2.0 * spacing
+    H = max(H, Ly)
+    jitter = 0.05 * spacing
+    # y spans [H - Ly, H) so that max.y == max.z
+    x = gen.jittered_lattice(counts, spacing, (0.5 * spacing, H - Ly + 0.5 * spacing, 0.75 * spacing),
+                             jitter, rng)
+    dmin = np.array([0.0, H - Ly, 0.0])
+    dmax = np.array([Lx, H, H])
+    q = gen.shoemake_quaternions(rng, n)
+    geo = rng.integers(0, len(tmpl), size=n).astype(np.int32)
+    wt, wall = floor_wall([0.0, H - Ly], [Lx, H])
+    e = restitution_from_damping(zeta)
+    return Scenario(
+        name=f"pour-{n}", templates=tmpl, density=2500.0,
+        materials=[(kn, 0.4 * kn, mu, e)],
+        x=x, v=np.zeros((n, 3)), w=np.zeros((n, 3)), q=q,
+        mat=np.zeros(n, np.int32), geo=geo,
+        domain_min=dmin, domain_max=dmax, gravity=np.array([0.0, 0.0, -9.81]), dt=dt,
+        wall_templates=[wt], wall_materials=[(mu, e)], walls=[wall],
+        periodic=(True, True), cell_size=cell_size, steps=steps)
+
+
+def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_templates: int = 3,
+            periodic=(False, False), walls: bool = True, scale: float = 1e-3,
+            spacing: float = 3.0, speed: float = 0.5, spin: float = 20.0, dt: float = 1e-6,
+            kn: float = 1e4, mu: float = 0.4, zeta: float = 0.3, steps: int = 400) -> Scenario:
+    """A small cluster with random velocities that collides with itself, the
+    floor and (when periodic) across the periodic faces within a few hundred
+    steps. Lengths are in units of `scale` (semi-axes U[0.8, 1.2], lattice
+    spacing `spacing`); at scale 1e-3 the inertia determinant falls below the
+    reference's 1e-30 inverse threshold, so particles keep their spin, while
+    at scale 1e-2 rotation is live. Used for step-by-step parity."""
+    rng = np.random.default_rng(seed)
+    tmpl = superellipsoid_family(n_templates, seed + 100, n_grid=n_grid, n_nodes=n_nodes,
+                                 axes=(0.8 * scale, 1.2 * scale))
+    sp = spacing * scale
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    L = side * sp
+    H = L + 2 * sp
+    x = gen.jittered_lattice((side, side, side), sp, (0.5 * sp, H - L + 0.5 * sp, 0.75 * sp),
+                             0.05 * sp, rng)[:n]
+    # y spans [H - L, H) so that max.y == max.z
+    dmin = np.array([0.0, H - L, 0.0])
+    dmax = np.array([L, H, H])
+    q = gen.shoemake_quaternions(rng, n)
+    v = rng.normal(0.0, speed, size=(n, 3))
+    w = rng.normal(0.0, spin, size=(n, 3))
+    geo = rng.integers(0, n_templates, size=n).astype(np.int32)
+    e = restitution_from_damping(zeta)
+    wts, wls = [], []
+    if walls:
+        wt, wall = floor_wall([dmin[0], dmin[1]], [dmax[0], dmax[1]])
+        wts, wls = [wt], [wall]
+    return Scenario(
+        name=f"cluster-{n}-{seed}", templates=tmpl, density=2500.0,
+        materials=[(kn, 0.4 * kn, mu, e), (2 * kn, 0.8 * kn, 0.5 * mu, 0.9)],
+        x=x, v=v, w=w, q=q, mat=(np.arange(n) % 2).astype(np.int32), geo=geo,
+        domain_min=dmin, domain_max=dmax, gravity=np.array([0.0, 0.0, -9.81]), dt=dt,
+        wall_templates=wts, wall_materials=[(mu, e)] if walls else [], walls=wls,
+        periodic=tuple(periodic), cell_size=0.0, steps=steps)
+
+
+def bonded_beam(counts=(20, 4, 4), radius: float = 0.5e-3, n_nodes: int = 400, n_grid: int = 16,
+                seed: int = 3, strength_mean: float = 5e6, strength_sd: float = 0.5e6,
+                tip_speed: float = 1.0, dt: float = 1e-7, steps: int = 50000) -> Scenario:
+    """Cantilever of bonded spheres (radius 0.5 mm, 16^3 SDF, 400 nodes) on a
+    simple cubic lattice of spacing 2r: bonds to the 6 lattice neighbours with
+    normal stiffness 1e9 Pa per area (E = 1e9 * L) and shear/normal 0.5
+    (bond radius L / sqrt(6)); tensile strength and cohesion ~ N(5, 0.5) MPa.
+    The x = 0 layer is fixed (zero mass, zero velocity) and the last layer
+    is driven at tip_speed along -z (zero mass)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = counts
+    L = 2.0 * radius
+    sph = gen.sphere_template(radius, n_grid=n_grid, n_nodes=n_nodes)
+    pts = gen.jittered_lattice(counts, L, (radius, radius, radius), 0.0, rng)
+    n = len(pts)
+    ii = np.arange(n)
+    ix, iy, iz = ii % nx, (ii // nx) % ny, ii // (nx * ny)
+    geo = np.zeros(n, np.int32)
+    geo[ix == 0] = 1       # clamp
+    geo[ix == nx - 1] = 2  # driven tip
+    v = np.zeros((n, 3))
+    v[ix == nx - 1, 2] = -tip_speed
+    bonds = []
+    E = 1e9 * L
+    Rb = L / math.sqrt(6.0)
+    for i in range(n):
+        for dx, dy, dz in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            jx, jy, jz = ix[i] + dx, iy[i] + dy, iz[i] + dz
+            if jx >= nx or jy >= ny or jz >= nz:
+                continue
+            j = int(jx + nx * (jy + ny * jz))
+            c = 0.5 * (pts[i] + pts[j])
+            nrm = pts[i] - pts[j]
+            st, coh = rng.normal(strength_mean, strength_sd, size=2)
+            bonds.append((i, j, c, nrm, Rb, L, E, 0.3, float(st), float(coh), 0.0))
+    margin = 4 * L
+    dmin = pts.min(axis=0) - margin
+    dmax = pts.max(axis=0) + margin
+    s = Scenario(
+        name=f"beam-{n}", templates=[sph, sph, sph], density=[2500.0, 0.0, 0.0],
+        materials=[(1e4, 4e3, 0.3, restitution_from_damping(0.3))],
+        x=pts, v=v, w=np.zeros((n, 3)), q=np.tile(IDENT, (n, 1)),
+        mat=np.zeros(n, np.int32), geo=geo, domain_min=dmin, domain_max=dmax,
+        gravity=np.array([0.0, 0.0, -9.81]), dt=dt, bonds=bonds, steps=steps)
+    return s

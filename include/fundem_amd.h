@@ -1,0 +1,367 @@
+/*
+ * fundem_amd C-ABI: stateless gfx950 launchers for the level-set DEM timestep.
+ *
+ * This is the boundary the C++ host API (fundem_amd/include/fundem/LSSolver.h
+ * and its containers) calls, in the place of FunDEM's extern "C" launch*
+ * layer (kernel/CUDAKernelFunction/ *.cuh, myUtility/mySpatialGrid.h,
+ * myUtility/buildHashStartEnd.h). Each declaration names the reference symbol
+ * it replaces.
+ *
+ * Conventions
+ *   - Device state is FP32, planar SoA: a vector field of n bodies is 3n floats
+ *     [x0..x(n-1) | y0.. | z0..]; a quaternion field is 4n floats
+ *     [q0.. | q1.. | q2.. | q3..] (scalar first, as myQua.h:10-15).
+ *   - Per-template and per-material constants and grid parameters are FP64.
+ *   - All arithmetic that decides an integer outcome (cell ids, contact
+ *     membership, bond failure) is FP64 on the device, evaluated in the
+ *     reference's operation order without contraction.
+ *   - Every launcher is asynchronous on `stream`, allocates nothing and never
+ *     synchronises the host, so a step can be captured in a hipGraph. Counts
+ *     that the reference read back to the host (numPair) stay on the device.
+ *   - Return value: FDM_OK, or a negative FDM_E* code for invalid arguments
+ *     or a failed launch. Capacity overflow inside a step is reported through
+ *     a device-resident status word (FDM_STATUS_*), read by the host when it
+ *     chooses to synchronise.
+ */
+#ifndef FUNDEM_AMD_H
+#define FUNDEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDM_OK 0
+#define FDM_EINVAL (-1)
+#define FDM_ELAUNCH (-2)
+#define FDM_ECAPACITY (-3)
+
+/* bits of the device status word */
+#define FDM_STATUS_PAIR_OVERFLOW 1u     /* candidate pairs exceeded capacity */
+#define FDM_STATUS_RECORD_OVERFLOW 2u   /* contact records exceeded capacity */
+#define FDM_STATUS_BUCKET_OVERFLOW 4u   /* per-particle mirror list exceeded capacity */
+#define FDM_STATUS_GHOST_OVERFLOW 8u    /* ghost pair list exceeded capacity */
+
+#define FDM_CHUNK 64         /* surface nodes per culling chunk (one wavefront) */
+#define FDM_MAX_CHUNKS 64    /* chunks per template: at most 4096 nodes */
+
+typedef struct { double x, y, z; } fdm_double3;
+typedef struct { int x, y, z; } fdm_int3;
+
+/* One particle or wall template (geometryInfo, LSParticle.h:53-174), FP64. */
+typedef struct {
+  double radius;         /* max |node| after the centroid shift */
+  double invMass;
+  double inertia[6];     /* xx yy zz xy xz yz */
+  double invInertia[6];
+  double origin[3];      /* SDF grid origin in the template frame */
+  double invSpacing;
+  double lipschitz;      /* bound on |grad| of the trilinear SDF (culling only) */
+  double boxReach;       /* max distance from the template frame origin to an SDF box corner */
+  long long sdfStart;    /* offset into the concatenated SDF */
+  int size[3];           /* SDF grid nodes per axis */
+  int nodeStart;         /* offset into the node table (xyz float triples) */
+  int nodeCount;
+  int chunkStart;        /* offset into the chunk table (float4: centre xyz, radius) */
+  int chunkCount;
+  int pad_;
+} fdm_template;
+
+/* One material (materialInfo, LSParticle.h:12-51) after the clamps of
+ * LSParticle::addMaterial; dissipation = -ln e / sqrtf(ln^2 e + pi^2) as
+ * contactKernel.cuh:50-51 evaluates it. */
+typedef struct {
+  double kn, ks, mu, e;
+  double dissipation;
+  double pad_[3];
+} fdm_material;
+
+/* Device view of a set of bodies (particles or walls), planar FP32 SoA. */
+typedef struct {
+  int n;
+  float* x;
+  float* v;
+  float* w;
+  float* q;
+  float* F;
+  float* T;
+  const int* mat;
+  const int* geo;
+} fdm_bodies;
+
+/* Geometry tables of one template set. */
+typedef struct {
+  int num;
+  const fdm_template* info;
+  const float* sdf;
+  const float* nodes;    /* xyz triples */
+  const float* chunks;   /* float4 per chunk */
+} fdm_geometry;
+
+/* Uniform grid parameters (spatialGrid, myUtility/mySpatialGrid.h:162-256). */
+typedef struct {
+  fdm_double3 min, max, invCell;
+  fdm_int3 size;
+  long long num;
+} fdm_grid;
+
+/* Sorted hash table of one grid: hash value per body (by body index), the
+ * stably sorted (hash, index) lists and half-open cell ranges. */
+typedef struct {
+  int* hashValue;      /* [n] by body index */
+  unsigned* sortKey;   /* [n] hash + 1, sorted in place */
+  int* sortIndex;      /* [n] body index, sorted in place */
+  int* cellStart;      /* [grid.num], -1 if empty */
+  int* cellEnd;        /* [grid.num] */
+  int* prevKey;        /* [n] sorted keys of the previous build, to clear cells */
+  int* prevCount;      /* [1] entries in prevKey (-1: clear the whole table) */
+} fdm_hash_table;
+
+/* Candidate pairs and tangential history of one interaction set. */
+typedef struct {
+  int capacity;        /* pair capacity */
+  int* count;          /* [n+1] per-A candidate count, scanned in place to CSR starts */
+  int* total;          /* [1] number of pairs */
+  int* pairA;          /* [capacity] */
+  int* pairT;          /* [capacity] stable target id (particle, wall or real id of a ghost) */
+  int* pairG;          /* [capacity] ghost index (ghost sets only, else unused) */
+  int* recOff;         /* [capacity] first history record of the pair */
+  int* recCnt;         /* [capacity] contacts of the pair */
+  double* sumF;        /* [3*capacity] per-pair force sum (pair order, node order) */
+  double* sumTA;       /* [3*capacity] torque sum about A */
+  double* sumTB;       /* [3*capacity] torque sum about the target */
+  int recCapacity;
+  int* recTotal;       /* [1] records used */
+  float4* rec;         /* [recCapacity] (spring xyz, node id as int bits), per pair by node */
+} fdm_pairs;
+
+/* Per-particle mirror lists: pairs in which the particle is the target. */
+typedef struct {
+  int capacity;        /* entries per particle */
+  int* count;          /* [n] cleared by the reduction after use */
+  int* list;           /* [n*capacity] pair indices */
+} fdm_mirror;
+
+/* Visible ghosts of one periodic set (Ghost, PeriodicBoundary.h:7-176). */
+typedef struct {
+  int capacity;
+  int* count;          /* [1] */
+  int* flag;           /* [n] scan workspace */
+  int* realId;         /* [capacity] real particle of each ghost, ascending */
+  double* pos;         /* [3*capacity] ghost positions, AoS */
+  fdm_hash_table table;
+  int* cellBox;        /* [6] min/max occupied ghost cell per axis */
+} fdm_ghosts;
+
+/* Bonds (VBondedInteraction, VBondedInteraction.h:61-566), FP32 constants. */
+typedef struct {
+  int n;
+  const int* master;
+  const int* slave;
+  int* active;
+  int* breakStep;      /* step index at which the bond failed, -1 while intact */
+  const float* coef;   /* [n*9]: B1 B2 B3 B4 R L0 tensile cohesion friction */
+  const float* local;  /* [n*24]: n1i n2i n3i pi n1j n2j n3j pj (xyz each) */
+  double* outF;        /* [3n] bond force on master */
+  double* outTi;       /* [3n] torque on master */
+  double* outTj;       /* [3n] torque on slave */
+  int* applied;        /* [n] 1 if the bond acted this step */
+  double* maxNormalStress; /* [n] */
+  double* maxShearStress;  /* [n] */
+  const int* partStart;    /* [np+1] CSR of bonds per particle (ascending bond id) */
+  const int* partList;     /* bond id * 2 + role (0 master, 1 slave) */
+  const int* stepIndex;    /* [1] device step counter, recorded in breakStep */
+} fdm_bonds;
+
+/* Periodic configuration (PeriodicBoundaryXY2D, PeriodicBoundary.h:178-317). */
+typedef struct {
+  int x, y;
+} fdm_periodic;
+
+/* ---------------------------------------------------------------------- */
+/* Integration                                                            */
+
+/* Replaces launchParticleVelocityAngularVelocityIntegration
+ * (particleIntegrationKernel.cuh:4): half-kick of v and w with step h. */
+int fdm_launchParticleVelocityAngularVelocityIntegration(fdm_bodies bodies, fdm_geometry geo,
+                                                         fdm_double3 gravity, double h,
+                                                         hipStream_t stream);
+
+/* Replaces launchParticlePositionOrientationIntegration
+ * (particleIntegrationKernel.cuh:23): drift of x and q over dt. */
+int fdm_launchParticlePositionOrientationIntegration(fdm_bodies bodies, double dt,
+                                                     hipStream_t stream);
+
+/* Fused first half of a step (LSSolver::integration1stHalf, LSSolver.h:1033-1070,
+ * plus calculateHashKernel, mySpatialGrid.cu:4-26): optional half-kick with
+ * the stored F,T, drift, then the body's cell id and sort key/value. */
+int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_double3 gravity, double dt,
+                            int kick, int drift, fdm_grid grid, fdm_hash_table table,
+                            hipStream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Spatial hash                                                           */
+
+/* Bytes of workspace fdm_radixSortPairs needs for n keys. */
+size_t fdm_radixSortWorkspaceBytes(size_t n);
+
+/* Stable LSD radix sort of (key, value) by key, keys < 2^keyBits, in place.
+ * Replaces thrust::sort_by_key as called by buildHashStartEnd
+ * (buildHashStartEnd.cu:108-111). */
+int fdm_radixSortPairs(unsigned* keys, int* values, size_t n, int keyBits, void* workspace,
+                       size_t workspaceBytes, hipStream_t stream);
+
+/* Replaces buildHashStartEnd (myUtility/buildHashStartEnd.h:32): sorts the
+ * table's keys (hash + 1) with the body indices and writes half-open cell
+ * ranges; cells of the previous build are reset first. */
+int fdm_buildHashStartEnd(fdm_hash_table table, int n, fdm_grid grid, void* workspace,
+                          size_t workspaceBytes, hipStream_t stream);
+
+/* Replaces launchUpdateSpatialGridHashStartEnd (myUtility/mySpatialGrid.h:33):
+ * cell id of every body (calculateHashKernel) then buildHashStartEnd. */
+int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const float* position, int n,
+                                            fdm_grid grid, void* workspace, size_t workspaceBytes,
+                                            hipStream_t stream);
+
+/* Replaces buildPrefixSum (myUtility/buildHashStartEnd.h:62): inclusive scan
+ * of n ints (in may equal out). */
+size_t fdm_scanWorkspaceBytes(size_t n);
+int fdm_buildPrefixSum(int* prefixSum, const int* count, size_t n, void* workspace,
+                       size_t workspaceBytes, hipStream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Periodic boundaries                                                    */
+
+/* Replace launchUpdatePositionOutOfBoundaryXD / YD (mySpatialGrid.h:51/59). */
+int fdm_launchUpdatePositionOutOfBoundaryXD(float* position, int n, fdm_grid grid,
+                                            hipStream_t stream);
+int fdm_launchUpdatePositionOutOfBoundaryYD(float* position, int n, fdm_grid grid,
+                                            hipStream_t stream);
+/* Replace launchCalculateGhostPositionXD / YD / XYD (mySpatialGrid.h:76/90/104).
+ * axisMask: 1 = X, 2 = Y, 3 = XY. Ghost positions are FP64 AoS. */
+int fdm_launchCalculateGhostPosition(double* ghostPosition, const float* position, int n,
+                                     fdm_grid grid, int axisMask, hipStream_t stream);
+/* Replaces launchUpdateGhostSpatialGridHashStartEnd (mySpatialGrid.h:144). */
+int fdm_launchUpdateGhostSpatialGridHashStartEnd(fdm_hash_table table, const double* ghostPosition,
+                                                 int n, fdm_grid grid, void* workspace,
+                                                 size_t workspaceBytes, hipStream_t stream);
+
+/* Fused periodic stage of one step (PeriodicBoundaryXY2D::updateGhostSpatialGrid,
+ * PeriodicBoundary.h:221-295): wrap, then the visible ghosts of every active
+ * set, compacted in particle order and hashed into per-set tables. */
+int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                             fdm_ghosts* sets /* [3]: X, Y, XY */, unsigned* status,
+                             void* workspace, size_t workspaceBytes, hipStream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Contact detection and contact forces                                   */
+
+#define FDM_SET_PARTICLE 0
+#define FDM_SET_WALL 1
+#define FDM_SET_GHOST 2
+
+/* Broad phase of one interaction set: candidate pairs (A, target) sorted by
+ * (A, target), every pair that can hold a node contact under the reference's
+ * detection rules (LSParticleContactDetectionKernel.cu:35-123) included.
+ * Replaces the counting half of launchBuildLevelSetBoundaryNodeInteractions1st
+ * / FixedParticleInteractions1st (LSParticleContactDetectionKernel.cuh:79/148)
+ * together with buildPrefixSum / updateNumPair (SolidInteraction.h:92-120). */
+typedef struct {
+  int set;                 /* FDM_SET_* */
+  fdm_bodies particles;
+  fdm_geometry pgeo;
+  /* targets: particles (set 0), walls (set 1) or ghosts (set 2) */
+  fdm_bodies walls;
+  fdm_geometry wgeo;
+  const fdm_ghosts* ghosts;
+  fdm_grid grid;           /* grid of the targets */
+  fdm_hash_table table;    /* hash table of the targets */
+  fdm_pairs pairs;
+} fdm_broadphase_args;
+int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* workspace, size_t workspaceBytes,
+                         hipStream_t stream);
+
+/* Narrow phase fused with the contact law, for every candidate pair:
+ * node sweep in wave64 against the target's SDF, the reference's contact
+ * predicate, history remap keyed by (A, target, node) from the previous
+ * step's pairs, linear spring-dashpot with Coulomb-capped sliding spring,
+ * and per-pair force/torque sums in node order. Replaces
+ * launchBuildLevelSetBoundaryNode*Interactions2nd
+ * (LSParticleContactDetectionKernel.cuh:108/180) and
+ * launchAdd{,Fixed,Ghost}LevelSetParticleContactForceTorque
+ * (contactKernel.cuh:201/234/275). */
+typedef struct {
+  int set;
+  fdm_bodies particles;
+  fdm_geometry pgeo;
+  const fdm_material* pmat;
+  fdm_bodies walls;
+  fdm_geometry wgeo;
+  const fdm_material* wmat;
+  const fdm_ghosts* ghosts;
+  fdm_grid grid;               /* grid of the targets (for the stencil rule) */
+  const int* targetHash;       /* hash value per target */
+  fdm_grid pgrid;              /* particle grid (periodic wrap of force arms) */
+  fdm_periodic periodic;       /* wrap applied to positions used in forces */
+  fdm_pairs pairs;             /* this step */
+  fdm_pairs prev;              /* previous step (history) */
+  const int* prevStart;        /* [n+1] CSR of prev pairs by A */
+  fdm_mirror mirror;           /* append pairs with contacts to the target's list */
+  double dt;
+  int updateSprings;           /* 0: keep remapped springs (history filtering) */
+  unsigned* status;
+  /* optional per-record diagnostics (NULL to skip): contact point, normal, overlap */
+  double* dbgPoint;
+  double* dbgNormal;
+  double* dbgOverlap;
+} fdm_narrowphase_args;
+int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
+
+/* Replaces launchAddBondedForceTorque (contactKernel.cuh:311): VBond forces,
+ * stresses and one-way failure; results per bond. */
+int fdm_launchBondForces(fdm_bodies particles, fdm_bonds bonds, hipStream_t stream);
+
+/* Second half of the broad phase: writes the candidate pairs counted and
+ * scanned by fdm_launchBroadPhase, sorted by target within each A. */
+int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsigned* status,
+                              hipStream_t stream);
+
+/* Add one to a device counter (the step index a captured graph replays). */
+int fdm_incrementCounter(int* counter, hipStream_t stream);
+
+/* Deterministic reduction (replaces the double atomicAdds of
+ * contactKernel.cu:120-123,203-204,284-290,367-370 and the memsets of
+ * LSSolver.h:919-923), fused with the second half-kick
+ * (LSSolver::integration2ndHalf, LSSolver.h:1072-1089). Per particle, records
+ * are added in this order: particle pairs by partner (as target, then as A),
+ * wall pairs by wall, each ghost set X, Y, XY by partner real id (as A before
+ * as target), bonds by id. kick = 0 only writes F and T. */
+typedef struct {
+  fdm_bodies particles;
+  fdm_geometry pgeo;
+  const fdm_pairs* pp;
+  const fdm_mirror* ppMirror;
+  const fdm_pairs* pw;
+  const fdm_pairs* ghost[3];   /* NULL when the set is inactive */
+  const fdm_mirror* ghostMirror[3];
+  const fdm_bonds* bonds;      /* NULL when there are none */
+  fdm_double3 gravity;
+  double h;
+  int kick;
+} fdm_reduce_args;
+int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream);
+
+/* Fill n ints with a value (a kernel, not a memset node, so it replays
+ * correctly inside captured graphs). */
+int fdm_fillInt(int* ptr, size_t n, int value, hipStream_t stream);
+
+/* Device properties used by the launchers (number of CUs); 0 on success. */
+int fdm_deviceInfo(int* numCU, int* warpSize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+"""Parity of the HIP path against the oracle.
+
+The device stores state in FP32 and computes in FP64 without contraction,
+summing forces in a fixed canonical order. The oracle run with
+store_fp32=1, canonical_fold=1 performs the same operations, so states,
+contact sets (A, target, node) and tangential springs must agree bit for
+bit at every step. Against the FP64 restatement of the reference
+(store_fp32=0, canonical_fold=0) states agree within FP32 tolerances.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from fundem_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SETS = (po.SET_PP, po.SET_PW, po.SET_GX, po.SET_GY, po.SET_GXY)
+
+
+def make_pair(scn, mirror=True):
+    from fundem_amd import Solver
+    g = Solver()
+    W.apply(scn, g)
+    o = po.OracleSim(mirror, mirror)
+    W.apply(scn, o)
+    return g, o
+
+
+def assert_states_equal(gs, os_, where):
+    for key in ("x", "v", "w", "q", "F", "T"):
+        a, b = gs[key], os_[key]
+        if not np.array_equal(a, b):
+            bad = np.argwhere(a != b)[0]
+            raise AssertionError(f"{where}: {key} differs first at {tuple(bad)}: "
+                                 f"gpu={a[tuple(bad)]!r} oracle={b[tuple(bad)]!r}")
+
+
+def assert_contacts_equal(g, o, where, sets=SETS):
+    for s in sets:
+        gc, oc = g.contacts(s), o.contacts(s)
+        for key in ("A", "B", "k"):
+            assert np.array_equal(gc[key], oc[key]), \
+                f"{where}: set {s} {key} differ (gpu {gc[key].size} contacts, oracle {oc[key].size})"
+        assert np.array_equal(gc["spring"], oc["spring"]), f"{where}: set {s} springs differ"
+
+
+def run_bitexact(scn, steps, every):
+    g, o = make_pair(scn)
+    assert_states_equal(g.state(), o.state(), "after initialise")
+    total = 0
+    for i in range(1, steps + 1):
+        g.step(1)
+        o.step(1)
+        if i % every == 0 or i == steps:
+            assert_states_equal(g.state(), o.state(), f"step {i}")
+            assert_contacts_equal(g, o, f"step {i}")
+            total += sum(o.contacts(s)["A"].size for s in SETS)
+    assert g.status() == 0
+    return total
+
+
+CLUSTERS = {
+    "mm": dict(periodic=(False, False)),
+    "mm-periodic-xy": dict(periodic=(True, True)),
+    "mm-periodic-x": dict(periodic=(True, False)),
+    "mm-periodic-y": dict(periodic=(False, True), seed=5),
+    "cm-rotating": dict(periodic=(False, False), scale=1e-2, dt=1e-5, kn=1e5, spin=5.0),
+    "cm-rotating-periodic": dict(periodic=(True, True), scale=1e-2, dt=1e-5, kn=1e5, spin=5.0, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CLUSTERS))
+def test_cluster_bitexact(name):
+    scn = W.cluster(n=27, speed=1.0, **CLUSTERS[name])
+    contacts = run_bitexact(scn, steps=400, every=20)
+    assert contacts > 0, "scenario produced no contacts"
+
+
+def test_hash_table_matches_reference_sort():
+    scn = W.pour((16, 16, 8), spacing=3.0e-3, n_templates=2, n_nodes=64, n_grid=16)
+    # a few particles outside the domain on every side hash to -1
+    scn.x[:6] = scn.domain_min - 1e-3 + np.array(
+        [[0, 0, 0], [0.0, 5e-3, 5e-3], [20e-3, 0.0, 5e-3], [5e-3, 5e-3, 0.0], [0.05, 0.05, 0.05],
+         [0.06, 0.01, 0.02]])
+    from fundem_amd import Solver
+    g = Solver()
+    W.apply(scn, g)
+    x = g.state()["x"]
+    grid = g.grid()
+    h = po.hash_(x, grid["min"], grid["max"], grid["inv"], grid["size"])
+    ng = int(np.prod(grid["size"].astype(np.int64)))
+    hs, idx, st, en = po.sort_start_end(h, ng)
+    t = g.hash_table(len(x))
+    assert np.array_equal(t["hash"], h)
+    assert np.array_equal(t["sorted_hash"], hs)
+    assert np.array_equal(t["index"], idx)
+    assert np.array_equal(t["start"], st)
+    assert np.array_equal(t["end"], en)
+    assert (h == -1).sum() >= 5
+
+
+def test_single_particle_roll_bitexact_and_fp64():
+    """The rolling superellipsoid: bit-exact against the FP32-storage oracle,
+    and within FP32 tolerances of the FP64 restatement, over 3000 steps."""
+    scn = W.single_particle_on_wall(n_nodes=2000, steps=3000)
+    g, o = make_pair(scn)
+    ref = po.OracleSim(False, False)
+    W.apply(scn, ref)
+    for i in range(1, scn.steps + 1):
+        g.step(1)
+        o.step(1)
+        ref.step(1)
+        if i % 500 == 0:
+            gs = g.state()
+            assert_states_equal(gs, o.state(), f"step {i}")
+            assert_contacts_equal(g, o, f"step {i}", sets=(po.SET_PW,))
+            rs = ref.state()
+            np.testing.assert_allclose(gs["x"], rs["x"], rtol=1e-5, atol=1e-7)
+            np.testing.assert_allclose(gs["q"], rs["q"], rtol=0, atol=1e-5)
+            np.testing.assert_allclose(gs["v"], rs["v"], rtol=1e-4, atol=1e-5)
+    assert o.contacts(po.SET_PW)["A"].size > 0
+
+
+def test_deterministic_and_graph_replay():
+    """Two runs of the same input give identical bits, and replaying the
+    captured two-step graph gives the same bits as launching each stage."""
+    scn = W.cluster(n=27, periodic=(True, True), speed=1.0)
+    from fundem_amd import Solver
+    runs = []
+    for graph in (False, False, True):
+        g = Solver()
+        W.apply(scn, g)
+        g.use_graph(graph)
+        g.step(1)      # first step runs eagerly (hook detection)
+        g.step(200)
+        runs.append(g.state())
+        assert g.status() == 0
+    for r in runs[1:]:
+        assert_states_equal(r, runs[0], "repeat")
+
+
+def test_bonded_beam_bitexact_and_break_events():
+    scn = W.bonded_beam(counts=(8, 2, 2), tip_speed=5.0, strength_mean=2e5, strength_sd=5e4,
+                        n_nodes=128, dt=1e-6, steps=600)
+    g, o = make_pair(scn)
+    nb = len(scn.bonds)
+    for i in range(1, scn.steps + 1):
+        g.step(1)
+        o.step(1)
+        if i % 50 == 0:
+            assert_states_equal(g.state(), o.state(), f"step {i}")
+            ga = g.bonds(nb)["active"]
+            oa = o.bonds()[0]
+            assert np.array_equal(ga, oa), f"step {i}: bond flags differ"
+    assert o.bonds()[0].sum() < nb, "no bond broke"

@@ -1,0 +1,38 @@
+"""The native libraries exist, load, and export every C-ABI symbol that
+include/fundem_amd.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from fundem_amd import build
+
+HEADER = os.path.join(build.ROOT, "include", "fundem_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdm_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_launchers():
+    syms = declared_symbols()
+    for s in ("fdm_launchNarrowPhase", "fdm_launchBroadPhase", "fdm_radixSortPairs",
+              "fdm_launchReduceForceTorque", "fdm_launchBondForces", "fdm_launchPeriodicGhosts"):
+        assert s in syms
+
+
+@pytest.mark.skipif(not os.path.exists(build.CORE_LIB), reason="core library not built")
+def test_core_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(build.CORE_LIB)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(build.SOLVER_LIB), reason="solver library not built")
+def test_solver_binding_loads():
+    from fundem_amd import solver
+    lib = solver.solver_lib()
+    assert hasattr(lib, "fdmx_advance")
