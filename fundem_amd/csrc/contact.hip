@@ -137,7 +137,7 @@ __device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, d
 template <int SET, class Emit>
 __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emit) {
   const int n = a.P.n;
-  const V3 xA = ld3(a.P.x, n, A);
+  const V3 xA = ldp(a.P.x, n, A);
   const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
   I3 lo, hi;
   cellRange(xA, RA * kReach, a.grid, lo, hi);
@@ -146,7 +146,7 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
       const int h = a.table.hashValue[w];
       if (h < 0 || (long long)h >= a.grid.num) continue;
       if (!cellNear(decodeHash(h, a.grid), lo, hi)) continue;
-      if (acceptCandidate<SET>(a, A, xA, RA, w, w, ld3(a.W.x, a.W.n, w))) emit(w, w);
+      if (acceptCandidate<SET>(a, A, xA, RA, w, w, ldp(a.W.x, a.W.n, w))) emit(w, w);
     }
     return;
   }
@@ -170,7 +170,7 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
           const int idx = a.table.sortIndex[m];
           if (SET == FDM_SET_PARTICLE) {
             if (idx <= A) continue;
-            if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ld3(a.P.x, n, idx))) emit(idx, idx);
+            if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ldp(a.P.x, n, idx))) emit(idx, idx);
           } else {
             const int j = a.ghostReal[idx];
             if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3(a.ghostPos + 3 * idx)))
@@ -262,12 +262,10 @@ __device__ __forceinline__ V3 wrappedArm(V3 x, const NarrowK& a) {
   if (a.periodic.x) {
     if (x.x < a.pgrid.min.x) x = vadd(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
     else if (x.x >= a.pgrid.max.x) x = vsub(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
-    x = round32(x);
   }
   if (a.periodic.y) {
     if (x.y < a.pgrid.min.y) x = vadd(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
     else if (x.y >= a.pgrid.max.y) x = vsub(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
-    x = round32(x);
   }
   return x;
 }
@@ -329,19 +327,19 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
   const int T = a.pairs.pairT[p];
   const int n = a.P.n;
   const fdm_template& tA = tmplOf(a.pgeo, a.P.geo[A]);
-  const V3 xA = ld3(a.P.x, n, A);
+  const V3 xA = ldp(a.P.x, n, A);
   const Q4 qA = ld4(a.P.q, n, A);
   Target tg;
   V3 armB;
   if (SET == FDM_SET_WALL) {
-    tg.x = ld3(a.W.x, a.W.n, T);
+    tg.x = ldp(a.W.x, a.W.n, T);
     tg.q = ld4(a.W.q, a.W.n, T);
     tg.t = &tmplOf(a.wgeo, a.W.geo[T]);
     tg.sdf = a.wgeo.sdf;
     tg.hash = a.targetHash[T];
     armB = tg.x;
   } else if (SET == FDM_SET_PARTICLE) {
-    tg.x = ld3(a.P.x, n, T);
+    tg.x = ldp(a.P.x, n, T);
     tg.q = ld4(a.P.q, n, T);
     tg.t = &tmplOf(a.pgeo, a.P.geo[T]);
     tg.sdf = a.pgeo.sdf;

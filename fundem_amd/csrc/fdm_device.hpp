@@ -119,6 +119,14 @@ __device__ __forceinline__ void st4(float* p, int n, int i, Q4 q) {
   p[2 * n + i] = (float)q.q2;
   p[3 * n + i] = (float)q.q3;
 }
+// positions: planar FP64 (a per-step displacement can be far below one FP32
+// ulp of the coordinate)
+__device__ __forceinline__ V3 ldp(const double* p, int n, int i) { return {p[i], p[n + i], p[2 * n + i]}; }
+__device__ __forceinline__ void stp(double* p, int n, int i, V3 v) {
+  p[i] = v.x;
+  p[n + i] = v.y;
+  p[2 * n + i] = v.z;
+}
 __device__ __forceinline__ V3 ldd3(const double* p) { return {p[0], p[1], p[2]}; }
 __device__ __forceinline__ V3 round32(V3 v) {
   return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z};

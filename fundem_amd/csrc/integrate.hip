@@ -26,10 +26,10 @@ __device__ __forceinline__ void kickOne(const fdm_bodies& b, const fdm_template*
 // particlePositionOrientationIntegrationKernel (particleIntegrationKernel.cu:34-48)
 __device__ __forceinline__ V3 driftOne(const fdm_bodies& b, int i, double dt) {
   const int n = b.n;
-  const V3 x = vadd(ld3(b.x, n, i), vmul(ld3(b.v, n, i), dt));
-  st3(b.x, n, i, x);
+  const V3 x = vadd(ldp(b.x, n, i), vmul(ld3(b.v, n, i), dt));
+  stp(b.x, n, i, x);
   st4(b.q, n, i, qintegrate(ld4(b.q, n, i), ld3(b.w, n, i), dt));
-  return round32(x);
+  return x;
 }
 
 __global__ void kKick(fdm_bodies b, const fdm_template* tmpl, V3 g, double h) {
@@ -56,7 +56,7 @@ __global__ void kKickDriftHash(fdm_bodies b, const fdm_template* tmpl, V3 g, dou
   if (kick) kickOne(b, tmpl, i, g, 0.5 * dt);
   V3 x;
   if (drift) x = driftOne(b, i, dt);
-  else x = ld3(b.x, b.n, i);
+  else x = ldp(b.x, b.n, i);
   const int h = cellHash(x, grid);
   hashValue[i] = h;
   sortKey[i] = (unsigned)(h + 1);

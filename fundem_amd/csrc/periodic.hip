@@ -13,11 +13,11 @@
 
 namespace fdm {
 
-__global__ void kHashOnly(const float* pos, int n, fdm_grid grid, int* hashValue, unsigned* sortKey,
+__global__ void kHashOnly(const double* pos, int n, fdm_grid grid, int* hashValue, unsigned* sortKey,
                           int* sortIndex) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int h = cellHash(ld3(pos, n, i), grid);
+  const int h = cellHash(ldp(pos, n, i), grid);
   hashValue[i] = h;
   sortKey[i] = (unsigned)(h + 1);
   sortIndex[i] = i;
@@ -36,7 +36,7 @@ __global__ void kGhostHashOnly(const double* gpos, int n, fdm_grid grid, int* ha
 __device__ __forceinline__ V3 offsetX(const fdm_grid& g) { return {g.max.x - g.min.x, 0., 0.}; }
 __device__ __forceinline__ V3 offsetY(const fdm_grid& g) { return {0., g.max.y - g.min.y, 0.}; }
 
-// updatePositionOutOfBoundaryXD/YD (mySpatialGrid.cu:28-56); result stored FP32
+// updatePositionOutOfBoundaryXD/YD (mySpatialGrid.cu:28-56)
 __device__ __forceinline__ V3 wrapAxis(V3 p, const fdm_grid& g, int axis) {
   if (axis == 0) {
     if (p.x < g.min.x) p = vadd(p, offsetX(g));
@@ -45,7 +45,7 @@ __device__ __forceinline__ V3 wrapAxis(V3 p, const fdm_grid& g, int axis) {
     if (p.y < g.min.y) p = vadd(p, offsetY(g));
     else if (p.y >= g.max.y) p = vsub(p, offsetY(g));
   }
-  return round32(p);
+  return p;
 }
 
 // calculateGhostPosition{X,Y,XY}DKernel (mySpatialGrid.cu:87-152)
@@ -56,16 +56,16 @@ __device__ __forceinline__ V3 ghostOf(V3 p, const fdm_grid& g, int mask) {
   return (c.x == 0 && c.y == 0) ? vadd(p, vadd(offsetX(g), offsetY(g))) : p;
 }
 
-__global__ void kWrap(float* pos, int n, fdm_grid g, int axis) {
+__global__ void kWrap(double* pos, int n, fdm_grid g, int axis) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  st3(pos, n, i, wrapAxis(ld3(pos, n, i), g, axis));
+  stp(pos, n, i, wrapAxis(ldp(pos, n, i), g, axis));
 }
 
-__global__ void kGhostPosition(double* gpos, const float* pos, int n, fdm_grid g, int mask) {
+__global__ void kGhostPosition(double* gpos, const double* pos, int n, fdm_grid g, int mask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const V3 p = ghostOf(ld3(pos, n, i), g, mask);
+  const V3 p = ghostOf(ldp(pos, n, i), g, mask);
   gpos[3 * i] = p.x;
   gpos[3 * i + 1] = p.y;
   gpos[3 * i + 2] = p.z;
@@ -84,11 +84,11 @@ __device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_pe
 }
 
 // Pass 1: visibility flag of each particle's ghost in each active set.
-__global__ void kGhostFlags(const float* pos, int n, fdm_grid g, fdm_periodic per, int* fx, int* fy,
+__global__ void kGhostFlags(const double* pos, int n, fdm_grid g, fdm_periodic per, int* fx, int* fy,
                             int* fxy) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const WrapStages w = wrapStages(ld3(pos, n, i), g, per);
+  const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
   if (per.x) fx[i] = ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0 ? 1 : 0;
   if (per.y) fy[i] = ghostCellHash(ghostOf(w.afterY, g, 2), g) >= 0 ? 1 : 0;
   if (per.x && per.y) fxy[i] = ghostCellHash(ghostOf(w.afterY, g, 3), g) >= 0 ? 1 : 0;
@@ -133,11 +133,11 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
 // ghosts of each set, then store the wrapped positions.
-__global__ void kGhostScatter(float* pos, int n, fdm_grid g, fdm_periodic per, GhostOut ox,
+__global__ void kGhostScatter(double* pos, int n, fdm_grid g, fdm_periodic per, GhostOut ox,
                               GhostOut oy, GhostOut oxy) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const WrapStages w = wrapStages(ld3(pos, n, i), g, per);
+  const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
   if (per.x) {
     const V3 gp = ghostOf(w.afterX, g, 1);
     if (ghostCellHash(gp, g) >= 0) emitGhost(ox, i, gp, g);
@@ -150,7 +150,7 @@ __global__ void kGhostScatter(float* pos, int n, fdm_grid g, fdm_periodic per, G
     const V3 gp = ghostOf(w.afterY, g, 3);
     if (ghostCellHash(gp, g) >= 0) emitGhost(oxy, i, gp, g);
   }
-  st3(pos, n, i, w.afterY);
+  stp(pos, n, i, w.afterY);
 }
 
 __global__ void kResetBox(int* box0, int* box1, int* box2) {
@@ -171,7 +171,7 @@ __global__ void kClampCount(int* count, int capacity) {
 
 using namespace fdm;
 
-extern "C" int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const float* position,
+extern "C" int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const double* position,
                                                        int n, fdm_grid grid, void* workspace,
                                                        size_t workspaceBytes, hipStream_t stream) {
   if (n < 0 || grid.num <= 0) return FDM_EINVAL;
@@ -181,7 +181,7 @@ extern "C" int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, con
   return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream);
 }
 
-extern "C" int fdm_launchUpdatePositionOutOfBoundaryXD(float* position, int n, fdm_grid grid,
+extern "C" int fdm_launchUpdatePositionOutOfBoundaryXD(double* position, int n, fdm_grid grid,
                                                        hipStream_t stream) {
   if (n < 0) return FDM_EINVAL;
   if (n == 0) return FDM_OK;
@@ -189,7 +189,7 @@ extern "C" int fdm_launchUpdatePositionOutOfBoundaryXD(float* position, int n, f
   return launchStatus();
 }
 
-extern "C" int fdm_launchUpdatePositionOutOfBoundaryYD(float* position, int n, fdm_grid grid,
+extern "C" int fdm_launchUpdatePositionOutOfBoundaryYD(double* position, int n, fdm_grid grid,
                                                        hipStream_t stream) {
   if (n < 0) return FDM_EINVAL;
   if (n == 0) return FDM_OK;
@@ -197,7 +197,7 @@ extern "C" int fdm_launchUpdatePositionOutOfBoundaryYD(float* position, int n, f
   return launchStatus();
 }
 
-extern "C" int fdm_launchCalculateGhostPosition(double* ghostPosition, const float* position, int n,
+extern "C" int fdm_launchCalculateGhostPosition(double* ghostPosition, const double* position, int n,
                                                 fdm_grid grid, int axisMask, hipStream_t stream) {
   if (n < 0 || axisMask < 1 || axisMask > 3) return FDM_EINVAL;
   if (n == 0) return FDM_OK;

@@ -60,7 +60,7 @@ __global__ void kBonds(fdm_bodies P, fdm_bonds b) {
   auto lv = [&](int o) { return V3{(double)L[o], (double)L[o + 1], (double)L[o + 2]}; };
   const V3 n1i = qrotate(qi, lv(0)), n2i = qrotate(qi, lv(3)), n3i = qrotate(qi, lv(6));
   const V3 n1j = qrotate(qj, lv(12)), n2j = qrotate(qj, lv(15)), n3j = qrotate(qj, lv(18));
-  const V3 ri = ld3(P.x, n, i), rj = ld3(P.x, n, j);
+  const V3 ri = ldp(P.x, n, i), rj = ldp(P.x, n, j);
   const V3 rbi = vadd(qrotate(qi, lv(9)), ri), rbj = vadd(qrotate(qj, lv(21)), rj);
   const float* c = b.coef + 9 * (size_t)k;
   V3 F = {0., 0., 0.}, Mij = {0., 0., 0.}, Mji = {0., 0., 0.};

@@ -13,7 +13,10 @@
 namespace fundem {
 
 inline void hipCheck(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // do not leave the error for the next launch to report
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+  }
 }
 
 template <class T>

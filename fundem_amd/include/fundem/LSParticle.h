@@ -1,8 +1,8 @@
 // LSParticle: host container of a set of level-set bodies (particles or
 // walls): materials, templates, per-body state and the body spatial grid.
-// Public methods follow kernel/LSParticle.h:219-834. Device state is FP32
-// planar SoA (see include/fundem_amd.h); host mirrors stay double3 and are
-// refreshed by finalize().
+// Public methods follow kernel/LSParticle.h:219-834. Device state is planar
+// SoA, positions FP64 and the rest FP32 (see include/fundem_amd.h); host
+// mirrors stay double3 and are refreshed by finalize().
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -260,7 +260,17 @@ struct LSParticle {
       d.upload(buf, s);
       hipCheck(hipStreamSynchronize(s), "upload");
     };
-    put3(x_, position_);
+    {
+      std::vector<double> xb(3 * m, 0.0);
+      for (size_t i = 0; i < n; ++i) {
+        xb[i] = position_[i].x;
+        xb[n + i] = position_[i].y;
+        xb[2 * n + i] = position_[i].z;
+      }
+      x_.allocate(3 * m, false);
+      x_.upload(xb, s);
+      hipCheck(hipStreamSynchronize(s), "upload");
+    }
     put3(v_, velocity_);
     put3(w_, angularVelocity_);
     put3(F_, force_);
@@ -294,7 +304,12 @@ struct LSParticle {
       hipCheck(hipStreamSynchronize(s), "download");
       for (size_t i = 0; i < n; ++i) h[i] = make_double3(buf[i], buf[n + i], buf[2 * n + i]);
     };
-    get3(x_, position_);
+    {
+      std::vector<double> xb(3 * n);
+      x_.download(xb.data(), 3 * n, s);
+      hipCheck(hipStreamSynchronize(s), "download");
+      for (size_t i = 0; i < n; ++i) position_[i] = make_double3(xb[i], xb[n + i], xb[2 * n + i]);
+    }
     get3(v_, velocity_);
     get3(w_, angularVelocity_);
     get3(F_, force_);
@@ -343,8 +358,9 @@ struct LSParticle {
     return s;
   }
 
-  // Device state, FP32 planar SoA (n floats per component).
-  float* position() { return x_.get(); }
+  // Device state, planar SoA (n values per component): positions FP64, the
+  // rest FP32.
+  double* position() { return x_.get(); }
   float* velocity() { return v_.get(); }
   float* angularVelocity() { return w_.get(); }
   float* force() { return F_.get(); }
@@ -379,7 +395,8 @@ struct LSParticle {
   std::vector<quaternion> orientation_;
   std::vector<int> materialID_, geometryID_;
 
-  DeviceBuffer<float> x_, v_, w_, q_, F_, T_;
+  DeviceBuffer<double> x_;
+  DeviceBuffer<float> v_, w_, q_, F_, T_;
   DeviceBuffer<int> mat_, geo_;
   DeviceBuffer<fdm_material> dmat_;
   DeviceBuffer<fdm_template> dtmpl_;

@@ -38,6 +38,7 @@ class LSSolver {
   virtual ~LSSolver() {
     for (auto& g : graphs_)
       if (g) (void)hipGraphExecDestroy(g);
+    if (ownStream_) (void)hipStreamDestroy(ownStream_);
   }
   LSSolver(const LSSolver&) = delete;
   LSSolver& operator=(const LSSolver&) = delete;
@@ -104,13 +105,13 @@ class LSSolver {
     walls_.setAngularVelocity(index, angularVelocity);
   }
   // LSSolver::addBondedInteraction (LSSolver.h:306-337). The bond frames are
-  // built from the FP32 state the device holds.
+  // built from the state the device holds (FP64 positions, FP32 orientations).
   void addBondedInteraction(int masterObjectID, int slaveObjectID, double3 bondPoint,
                             double3 bondNormal, double radius, double initialLength,
                             double YoungsModulus, double poissonRatio, double tensileStrength = 0.,
                             double cohesion = 0., double frictionCoefficient = 0.) {
     if (masterObjectID >= (int)particles_.num() || slaveObjectID >= (int)particles_.num()) return;
-    auto r3 = [](double3 v) { return make_double3((float)v.x, (float)v.y, (float)v.z); };
+    auto r3 = [](double3 v) { return v; };
     auto r4 = [](quaternion q) {
       return make_quaternion((float)q.q0, (float)q.q1, (float)q.q2, (float)q.q3);
     };
@@ -192,6 +193,13 @@ class LSSolver {
     if (!checkSolverInput(minDomain, maxDomain, timeStep, 0.0))
       throw std::invalid_argument("invalid domain or time step");
     hipCheck(hipSetDevice(device_), "hipSetDevice");
+    if (!stream_) {
+      // The null stream cannot be captured into a graph: work on a stream of
+      // our own that still synchronises with the null stream, so hooks that
+      // launch on stream 0 stay ordered with the step.
+      hipCheck(hipStreamCreate(&ownStream_), "hipStreamCreate");
+      stream_ = ownStream_;
+    }
     gravity_ = gravity;
     dt_ = timeStep;
     particles_.initialize(minDomain, maxDomain, cellSize_, stream_);
@@ -617,6 +625,7 @@ class LSSolver {
   std::string dir_;
   size_t phase_ = 0;
   hipStream_t stream_;
+  hipStream_t ownStream_ = nullptr;
   size_t maxGPUThread_;
   int device_;
   LSParticle particles_, walls_;

@@ -8,9 +8,11 @@
  * it replaces.
  *
  * Conventions
- *   - Device state is FP32, planar SoA: a vector field of n bodies is 3n floats
- *     [x0..x(n-1) | y0.. | z0..]; a quaternion field is 4n floats
+ *   - Device state is planar SoA: a vector field of n bodies is 3n values
+ *     [x0..x(n-1) | y0.. | z0..]; a quaternion field is 4n values
  *     [q0.. | q1.. | q2.. | q3..] (scalar first, as myQua.h:10-15).
+ *     Positions are FP64; velocity, angular velocity, orientation, force
+ *     and torque are FP32.
  *   - Per-template and per-material constants and grid parameters are FP64.
  *   - All arithmetic that decides an integer outcome (cell ids, contact
  *     membership, bond failure) is FP64 on the device, evaluated in the
@@ -79,10 +81,12 @@ typedef struct {
   double pad_[3];
 } fdm_material;
 
-/* Device view of a set of bodies (particles or walls), planar FP32 SoA. */
+/* Device view of a set of bodies (particles or walls), planar SoA: positions
+ * FP64 (a step's displacement, v*dt, can be below one FP32 ulp of the
+ * coordinate), everything else FP32. */
 typedef struct {
   int n;
-  float* x;
+  double* x;
   float* v;
   float* w;
   float* q;
@@ -222,7 +226,7 @@ int fdm_buildHashStartEnd(fdm_hash_table table, int n, fdm_grid grid, void* work
 
 /* Replaces launchUpdateSpatialGridHashStartEnd (myUtility/mySpatialGrid.h:33):
  * cell id of every body (calculateHashKernel) then buildHashStartEnd. */
-int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const float* position, int n,
+int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const double* position, int n,
                                             fdm_grid grid, void* workspace, size_t workspaceBytes,
                                             hipStream_t stream);
 
@@ -236,13 +240,13 @@ int fdm_buildPrefixSum(int* prefixSum, const int* count, size_t n, void* workspa
 /* Periodic boundaries                                                    */
 
 /* Replace launchUpdatePositionOutOfBoundaryXD / YD (mySpatialGrid.h:51/59). */
-int fdm_launchUpdatePositionOutOfBoundaryXD(float* position, int n, fdm_grid grid,
+int fdm_launchUpdatePositionOutOfBoundaryXD(double* position, int n, fdm_grid grid,
                                             hipStream_t stream);
-int fdm_launchUpdatePositionOutOfBoundaryYD(float* position, int n, fdm_grid grid,
+int fdm_launchUpdatePositionOutOfBoundaryYD(double* position, int n, fdm_grid grid,
                                             hipStream_t stream);
 /* Replace launchCalculateGhostPositionXD / YD / XYD (mySpatialGrid.h:76/90/104).
  * axisMask: 1 = X, 2 = Y, 3 = XY. Ghost positions are FP64 AoS. */
-int fdm_launchCalculateGhostPosition(double* ghostPosition, const float* position, int n,
+int fdm_launchCalculateGhostPosition(double* ghostPosition, const double* position, int n,
                                      fdm_grid grid, int axisMask, hipStream_t stream);
 /* Replaces launchUpdateGhostSpatialGridHashStartEnd (mySpatialGrid.h:144). */
 int fdm_launchUpdateGhostSpatialGridHashStartEnd(fdm_hash_table table, const double* ghostPosition,

