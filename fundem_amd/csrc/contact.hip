@@ -361,22 +361,34 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
   const float4* chunks = reinterpret_cast<const float4*>(a.pgeo.chunks) + tA.chunkStart;
   const int nChunks = tA.chunkCount;
 
-  // pass 1: contact mask of every chunk (lane c keeps chunk c's mask)
-  unsigned long long myMask = 0ull;
-  for (int c = 0; c < nChunks; ++c) {
-    const float4 ch = chunks[c];
+  // chunk culls, one chunk per lane: a chunk survives unless its bounding
+  // sphere fails the sphere, stencil, SDF-box or Lipschitz bound
+  bool live = false;
+  if (lane < nChunks) {
+    const float4 ch = chunks[lane];
     const V3 cw = vadd(qrotate(qA, {(double)ch.x, (double)ch.y, (double)ch.z}), xA);
     const double rc = (double)ch.w * kReach;
+    live = true;
     if (SET == FDM_SET_PARTICLE) {
       const double rr = (RB + rc) * (1.0 + 1e-9);
-      if (vlen2(vsub(cw, tg.x)) > rr * rr) continue;
+      if (vlen2(vsub(cw, tg.x)) > rr * rr) live = false;
     }
-    I3 lo, hi;
-    cellRange(cw, rc, a.grid, lo, hi);
-    if (!cellNear(tcell, lo, hi)) continue;
-    const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
-    if (outsideBox(lc, rc, *tg.t)) continue;
-    if (lipschitzClear(lc, rc, *tg.t, tg.sdf)) continue;
+    if (live) {
+      I3 lo, hi;
+      cellRange(cw, rc, a.grid, lo, hi);
+      if (!cellNear(tcell, lo, hi)) live = false;
+    }
+    if (live) {
+      const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
+      if (outsideBox(lc, rc, *tg.t) || lipschitzClear(lc, rc, *tg.t, tg.sdf)) live = false;
+    }
+  }
+  const unsigned long long liveChunks = ballot64(live);
+
+  // pass 1: contact mask of every surviving chunk (lane c keeps chunk c's mask)
+  unsigned long long myMask = 0ull;
+  for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
+    const int c = __builtin_ctzll(lcm);
     const int k = c * FDM_CHUNK + lane;
     bool hit = false;
     if (k < tA.nodeCount) {

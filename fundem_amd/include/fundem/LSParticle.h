@@ -171,19 +171,31 @@ struct LSParticle {
       chunks_.push_back(fz);
       chunks_.push_back(fr);
     }
-    // gradient bound of the trilinear interpolant of the stored (FP32) SDF
-    double L[3] = {0, 0, 0};
+    // Gradient bound of the trilinear interpolant of the stored (FP32) SDF.
+    // Inside a cell, d(phi)/dx is a convex combination of the cell's four
+    // x-edge differences (likewise y, z), so |grad phi| <= sqrt(ex^2 + ey^2 +
+    // ez^2) / h with e* the cell's largest edge difference per axis; the
+    // interpolant is continuous, so the largest cell bound is a Lipschitz
+    // constant everywhere in the box.
+    double Lc = 0.0;
     const float* s = sdf_.data() + t.sdfStart;
-    for (int z = 0; z < size.z; ++z)
-      for (int y = 0; y < size.y; ++y)
-        for (int x = 0; x < size.x; ++x) {
-          const size_t i = size_t(z) * size.y * size.x + size_t(y) * size.x + x;
-          if (x + 1 < size.x) L[0] = std::max(L[0], std::fabs((double)s[i + 1] - s[i]));
-          if (y + 1 < size.y) L[1] = std::max(L[1], std::fabs((double)s[i + size.x] - s[i]));
-          if (z + 1 < size.z)
-            L[2] = std::max(L[2], std::fabs((double)s[i + size_t(size.x) * size.y] - s[i]));
+    const size_t sx = 1, sy = size_t(size.x), sz = size_t(size.x) * size.y;
+    for (int z = 0; z + 1 < size.z; ++z)
+      for (int y = 0; y + 1 < size.y; ++y)
+        for (int x = 0; x + 1 < size.x; ++x) {
+          const size_t i = size_t(z) * sz + size_t(y) * sy + x;
+          double e[3] = {0, 0, 0};
+          for (int c = 0; c < 4; ++c) {
+            const size_t ax = i + ((c & 1) ? sy : 0) + ((c & 2) ? sz : 0);  // x-edges
+            const size_t ay = i + ((c & 1) ? sx : 0) + ((c & 2) ? sz : 0);  // y-edges
+            const size_t az = i + ((c & 1) ? sx : 0) + ((c & 2) ? sy : 0);  // z-edges
+            e[0] = std::max(e[0], std::fabs((double)s[ax + sx] - s[ax]));
+            e[1] = std::max(e[1], std::fabs((double)s[ay + sy] - s[ay]));
+            e[2] = std::max(e[2], std::fabs((double)s[az + sz] - s[az]));
+          }
+          Lc = std::max(Lc, std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]));
         }
-    t.lipschitz = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]) * t.invSpacing * (1.0 + 1e-9);
+    t.lipschitz = Lc * t.invSpacing * (1.0 + 1e-9);
     double reach = 0.0;
     for (int c = 0; c < 8; ++c) {
       const double px = origin.x + ((c & 1) ? spacing * (size.x - 1) : 0.0);

@@ -122,7 +122,32 @@ typedef struct {
   int* cellEnd;        /* [grid.num] */
   int* prevKey;        /* [n] sorted keys of the previous build, to clear cells */
   int* prevCount;      /* [1] entries in prevKey (-1: clear the whole table) */
+  int* validAtBuild;   /* [n] 1 if the body's hash was valid at the last list build */
 } fdm_hash_table;
+
+/* Candidate lists are built with an extra reach `skin` and reused until the
+ * bodies may have moved (or, through rotation, moved a target-frame point) by
+ * skin/2 since the build, or until a body enters or leaves the domain, a
+ * periodic image appears or disappears, or a particle wraps. The decision is
+ * taken on the device every step, so a captured step needs no host logic.
+ * Control words (int, device): */
+#define FDM_CTL_REBUILD 0        /* lists rebuilt this step */
+#define FDM_CTL_REBUILD_GHOST 1  /* ghost lists rebuilt this step */
+#define FDM_CTL_PENDING 2        /* a particle wrapped: rebuild at the next step */
+#define FDM_CTL_VALID_CHANGED 3  /* a body entered or left the domain this step */
+#define FDM_CTL_GHOST_CHANGED 4  /* a periodic image appeared or disappeared */
+#define FDM_CTL_WRAPPED 5        /* a particle wrapped this step */
+#define FDM_CTL_MAX_RATE 6       /* largest |v| + |w| rho this step (float bits) */
+#define FDM_CTL_FORCE 7          /* host-requested rebuild (initialisation) */
+#define FDM_CTL_COUNT 8          /* number of rebuilds so far */
+#define FDM_CTL_WORDS 16
+typedef struct {
+  double skin;         /* extra reach of the candidate lists */
+  double rho;          /* lever arm turning spin into motion of target-frame points */
+  double dt;
+  int* ctl;            /* [FDM_CTL_WORDS] */
+  double* disp;        /* [1] displacement bound accumulated since the last build */
+} fdm_skin;
 
 /* Candidate pairs and tangential history of one interaction set. */
 typedef struct {
