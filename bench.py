@@ -2,15 +2,19 @@
 
 Workload (synthetic, seeded): 16 superellipsoid templates (semi-axes
 U[0.8, 1.2] mm, powers U[2, 8], 32^3 FP32 SDF, 1000 surface nodes) on a
-100^3 jittered lattice, random orientations, box periodic in x and y, floor
-wall, cell size 2.5 mm, kn = 1e4 N/m, mu = 0.5, dt = 5e-7 s.
+100^3 jittered lattice (spacing 1.05 bounding diameters, so no initial
+overlaps; the bottom layer rests on the floor), random orientations, box
+periodic in x and y, floor wall, cell size 2.5 mm, kn = 1e4 N/m,
+ks = 0.4 kn, damping ratio 0.3, mu = 0.5, dt = 5e-7 s.
 
-One step is one iteration of LSSolver's loop (kick, drift, hash, sort,
-broad/narrow phase and contact forces for particle, wall and ghost sets,
-reduction, kick). The timed region covers exactly --steps steps with the
-device synchronised on both sides; every stage is bracketed by HIP events on
-the solver's stream, which give the per-stage device times used for the
-roofline of the dominant stage. The CPU baseline is the serial FP64 oracle
+One step is one iteration of LSSolver's loop: kick, drift and cell ids,
+the device's list-rebuild decision, list builds when needed (sort, broad
+phase, ghost sets) or the kept lists, the narrow phase with contact forces
+for the particle, wall and ghost sets, the reduction and the second kick.
+The timed region covers exactly --steps steps with the device synchronised
+on both sides. Every stage is bracketed by HIP events on the solver's
+stream (the stream its kernels run on); their average per launch gives the
+roofline of the dominant kernel. The CPU baseline is the serial FP64 oracle
 on a 512-particle sample of the same workload.
 
 Prints one JSON line.
@@ -21,8 +25,6 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -32,12 +34,17 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--side", type=int, default=100, help="lattice side (side^3 particles)")
-    p.add_argument("--graph", type=int, default=0, help="replay captured two-step graphs")
+    p.add_argument("--graph", type=int, default=0,
+                   help="replay captured graphs (no per-stage timing)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--skin", type=float, default=-1.0,
+                   help="candidate-list skin in m (negative: a tenth of the largest radius)")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="first run a plain 1 GiB device copy (counter scaling under rocprofv3)")
     return p.parse_args()
 
 
@@ -45,30 +52,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic bytes per launch of each stage (FP32 state, 32-bit ids):
-#   integrate_hash  per particle: read x v w q F T (76) + template id (4),
-#                   write x v w q (52) + hash, key, index (12)          = 144
-#   sort_cells      per particle: 3 radix passes x (key read for the
-#                   histogram 4 + key/index read 8 + write 8) + cell marks
-#                   (sorted key read 4, start/end write 8, previous key
-#                   write 4, previous-cell reset read 4 + write 8)       =  88
-#   narrow_pp/pw/ghost  per contact: history read 16 + write 16; per pair:
-#                   both poses (2 x 28) + pair sums (72) + pair record 20
-#   reduce_kick     per particle: F T write (24) + v w read/write (48) +
-#                   q read (16) + template id (4) + pair-sum reads
-#                   (72 per owned or mirrored pair)
-def stage_bytes(stage, n, pairs, contacts):
-    if stage == "integrate_hash":
-        return 144.0 * n
-    if stage == "sort_cells":
-        return 88.0 * n
-    if stage.startswith("narrow"):
-        return 32.0 * contacts + 148.0 * pairs
+# Algorithmic HBM bytes per launch of the single-kernel stages (FP64
+# positions, FP32 v, w, q, F, T, 32-bit ids; template tables stay in cache):
+#   kick_drift_hash  per particle: read x 24, v 12, w 12, q 16, F 12, T 12,
+#                    template id 4, state at build 4 (= 96); write x 24,
+#                    v 12, w 12, q 16, cell id 4, sort key 4, index 4 (= 76)
+#   periodic_flags   per particle: read x 24 and visibility at build 4,
+#                    write three flags 12
+#   reduce_kick      per particle: pair counts read (particle set 8, wall 8,
+#                    three ghost sets 24, mirror counts 16), F T write 24,
+#                    template id 4, v w read and write 48, q read 16 (= 148);
+#                    per pair with contacts, per side: sums read 48
+#   narrow_*         per candidate pair: ids 12, previous ids and history
+#                    range 16, both bodies' x q template id 88, record range
+#                    write 8, pair sums write 72 (= 196); per contact:
+#                    velocities of both bodies 48 and history read 16 and
+#                    write 16 (= 80)
+def kernel_bytes(stage, n, pairs, contacts):
+    if stage == "kick_drift_hash":
+        return 172.0 * n
+    if stage == "periodic_flags":
+        return 40.0 * n
     if stage == "reduce_kick":
-        return 92.0 * n + 144.0 * pairs
-    if stage.startswith("broad"):
-        return 20.0 * n + 12.0 * pairs
-    return 0.0
+        return 148.0 * n + 96.0 * sum(contacts.values())
+    if stage.startswith("narrow_"):
+        key = {"narrow_pp": [0], "narrow_pw": [1], "narrow_ghost": [2, 3, 4]}[stage]
+        p = sum(pairs[k] for k in key)
+        c = sum(contacts[k] for k in key)
+        return (196.0 * p + 80.0 * c) / len(key)
+    return None
 
 
 def cpu_baseline(templates, budget_s):
@@ -92,12 +104,25 @@ def cpu_baseline(templates, budget_s):
                       f"serial FP64 oracle in the reference's order, {dt:.1f} s"}
 
 
+def copy_probe():
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy probe: 3 copies of 1 GiB")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards by "
-              f"slab, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the one-GPU build "
+              f"does not shard; DESIGN.md describes the slab decomposition)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
     from fundem_amd import Solver
     from fundem_amd import workloads as W
 
@@ -106,6 +131,7 @@ def main():
     log(f"workload {scn.name}: {scn.n} particles, setup {time.perf_counter() - t0:.1f} s")
     s = Solver(0)
     s.set_capacity(8, 24)
+    s.set_skin(a.skin)
     t0 = time.perf_counter()
     W.apply(scn, s)
     log(f"uploaded and initialised in {time.perf_counter() - t0:.1f} s")
@@ -117,42 +143,53 @@ def main():
         raise RuntimeError(f"capacity exceeded during warm-up (status {s.status()})")
     s.set_timing(not a.graph)
     s.sync()
+    b0, gb0, skin = s.list_builds()
     t0 = time.perf_counter()
     s.step(a.steps)
     s.sync()
     elapsed = time.perf_counter() - t0
+    b1, gb1, _ = s.list_builds()
     status = s.status()
     if status != 0:
         raise RuntimeError(f"capacity exceeded during the timed steps (status {status})")
     stages = s.stage_times() if not a.graph else {}
-    cnt = {k: s.contacts(k)["A"].size for k in range(5)}
-    contacts = sum(cnt.values())
+    contacts = {k: int(s.contacts(k)["A"].size) for k in range(5)}
+    pairs = dict(enumerate(s.pair_totals()))
     n = scn.n
     ms_step = 1e3 * elapsed / a.steps
     value = n * a.steps / elapsed
-    log(f"{ms_step:.3f} ms/step, {value:.3e} particle-steps/s, contacts at last step {cnt}")
+    log(f"{ms_step:.3f} ms/step, {value:.3e} particle-steps/s; contacts at the last step "
+        f"{contacts}, candidate pairs {pairs}; skin {skin:.3e} m, {b1 - b0} list builds and "
+        f"{gb1 - gb0} ghost-list builds in {a.steps} steps")
     for k, (ms, calls) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
-        log(f"  {k:16s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches")
+        log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches  "
+            f"{ms / a.steps * 1e3:8.1f} us/step")
 
     roof = None
-    if stages:
-        dom, (ms, calls) = max(stages.items(), key=lambda kv: kv[1][0])
-        pairs_est = contacts  # pairs with contacts are at most the contact count
-        bytes_launch = stage_bytes(dom, n, pairs_est, contacts)
+    modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}
+    if modeled:
+        dom, (ms, calls) = max(modeled.items(), key=lambda kv: kv[1][0])
+        nb = kernel_bytes(dom, n, pairs, contacts)
         avg_s = ms / max(calls, 1) * 1e-3
-        achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+        achieved = nb / avg_s / 1e9 if avg_s > 0 else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": bytes_launch}
-    step_bytes = 224.0 * n + 2 * 20.0 * contacts + 16 * (32 ** 3 * 4 + 1000 * 12)
+                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb}
+    # algorithmic bytes of a whole step: state 224 B per particle and 20 B per
+    # contact record read and written, plus the template tables
+    c_total = sum(contacts.values())
+    step_bytes = 224.0 * n + 2 * 20.0 * c_total + 16 * (32 ** 3 * 4 + 1000 * 12)
     out = {
         "metric": "particle-steps/s (1M level-set particle pour, one GPU)",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"pour {n} superellipsoids (16 templates, 32^3 FP32 SDF, 1000 nodes), "
-                               "periodic x/y + floor, cell 2.5 mm, dt 5e-7 s; FP32 state, FP64 arithmetic",
-                   "particles": n, "contacts_last_step": cnt, "graph": bool(a.graph)},
+        "config": {"workload": f"pour of {n} superellipsoids (16 templates, 32^3 FP32 SDF, 1000 "
+                               "nodes), periodic x/y + floor, cell 2.5 mm, dt 5e-7 s; FP64 "
+                               "positions, FP32 v/w/q/F/T, FP64 arithmetic",
+                   "particles": n, "contacts_last_step": contacts, "candidate_pairs": pairs,
+                   "graph": bool(a.graph), "skin_m": skin, "list_builds": b1 - b0,
+                   "ghost_list_builds": gb1 - gb0},
         "step_hbm_fraction": step_bytes * a.steps / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
     }

@@ -133,6 +133,24 @@ int fdmx_set_cell_size(void* s, double cs) {
 int fdmx_set_capacity(void* s, int pairs, int records) {
   return guarded([&] { static_cast<BoundSolver*>(s)->setContactCapacity(pairs, records); });
 }
+int fdmx_set_skin(void* s, double skin) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setNeighborSkin(skin); });
+}
+int fdmx_list_builds(void* s, long long* builds, long long* ghostBuilds, double* skin) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    const auto b = S->listBuilds();
+    if (builds) *builds = b.first;
+    if (ghostBuilds) *ghostBuilds = b.second;
+    if (skin) *skin = S->neighborSkin();
+  });
+}
+int fdmx_pair_totals(void* s, int* out5) {
+  return guarded([&] {
+    const auto t = static_cast<BoundSolver*>(s)->pairTotals();
+    for (int k = 0; k < 5; ++k) out5[k] = t[k];
+  });
+}
 int fdmx_use_graph(void* s, int on) {
   return guarded([&] { static_cast<BoundSolver*>(s)->useGraph(on != 0); });
 }

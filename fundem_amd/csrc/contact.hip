@@ -6,13 +6,15 @@
 //
 //  broad phase  one thread per particle A lists the targets that any of A's
 //               nodes could touch under the reference's rules, sorted by
-//               target. Particle targets need |x_A - x_B| <= R_A + R_B,
+//               target. Particle targets need |x_A - x_B| <= R_A + R_B + s,
 //               ghost and wall targets need |x_A - x_T| <= R_A + (SDF box
-//               reach of T); every target also needs its cell within one
-//               cell of the clamped cell range of A's bounding box. A pair is
-//               dropped when the target's interpolated SDF at A's centre
-//               exceeds Lip * R_A (Lip bounds the gradient of the trilinear
-//               interpolant), since then no node of A can reach phi <= 0.
+//               reach of T) + s. A pair is dropped when the target's
+//               interpolated SDF at A's centre exceeds Lip * (R_A + s) (Lip
+//               bounds the gradient of the trilinear interpolant), since
+//               then no node of A can reach phi <= 0. The skin s lets a list
+//               serve later steps while bodies move less than s/2 (the step
+//               decides on the device when to rebuild); s = 0 gives lists
+//               for the current positions only.
 //  narrow phase one wavefront per candidate pair. A's nodes are swept in
 //               chunks of 64 (one node per lane). A chunk is skipped when its
 //               bounding sphere cannot satisfy the node test (the same three
@@ -107,6 +109,9 @@ struct BroadK {
   fdm_hash_table table;
   fdm_pairs pairs;
   unsigned* status;
+  double skin;
+  double maxReach;
+  const int* gate;
 };
 
 template <int SET>
@@ -128,32 +133,51 @@ __device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, d
     reachT = SET == FDM_SET_PARTICLE ? tT->radius : tT->boxReach;
   }
   const V3 d = vsub(xA, xT);
-  const double rs = (RA + reachT) * kReach;
+  const double rs = (RA + reachT) * kReach + a.skin;
   if (vlen2(d) > rs * rs) return false;
-  return !lipschitzClear(qrotateT(d, qT), RA * kReach, *tT, sdf);
+  const V3 local = qrotateT(d, qT);
+  const double reach = RA * kReach + a.skin;
+  // no node can enter the SDF box, or none can reach phi <= 0 inside it
+  return !outsideBox(local, reach, *tT) && !lipschitzClear(local, reach, *tT, sdf);
 }
 
 // Visit the candidate targets of particle A; `emit(T, G)` for each.
+//
+// Every accepted pair satisfies a necessary condition of a node contact with
+// the reach widened by the skin: a node lies within R_A of x_A, and a contact
+// needs it within R_T of x_T (particle targets, sphere test) or inside T's
+// SDF box, hence within the box reach of x_T (walls, ghosts). The cells
+// visited cover all targets within that distance, so lists built at one step
+// stay complete while bodies move less than skin/2:
+//  - particles are stored under their cell id; the visited range is the
+//    geometric range of A's reach plus the largest target radius, joined with
+//    the reference's own stencil (one cell around A's node cells), which also
+//    covers a target whose id aliases another cell in the step it is built;
+//  - walls are all visited;
+//  - ghosts are stored under their clamped geometric cell, which is monotone
+//    in the position, so the geometric range alone suffices.
 template <int SET, class Emit>
 __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emit) {
   const int n = a.P.n;
   const V3 xA = ldp(a.P.x, n, A);
   const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
-  I3 lo, hi;
-  cellRange(xA, RA * kReach, a.grid, lo, hi);
   if (SET == FDM_SET_WALL) {
     for (int w = 0; w < a.W.n; ++w) {
       const int h = a.table.hashValue[w];
       if (h < 0 || (long long)h >= a.grid.num) continue;
-      if (!cellNear(decodeHash(h, a.grid), lo, hi)) continue;
       if (acceptCandidate<SET>(a, A, xA, RA, w, w, ldp(a.W.x, a.W.n, w))) emit(w, w);
     }
     return;
   }
-  I3 b0 = {max(lo.x - 1, 0), max(lo.y - 1, 0), max(lo.z - 1, 0)};
-  I3 b1 = {min(hi.x + 1, a.grid.size.x - 1), min(hi.y + 1, a.grid.size.y - 1),
-           min(hi.z + 1, a.grid.size.z - 1)};
-  if (SET == FDM_SET_GHOST) {
+  I3 b0, b1;
+  cellRange(xA, (RA + a.maxReach) * kReach + a.skin, a.grid, b0, b1);
+  if (SET == FDM_SET_PARTICLE) {
+    I3 lo, hi;
+    cellRange(xA, RA * kReach + a.skin, a.grid, lo, hi);
+    b0 = {min(b0.x, max(lo.x - 1, 0)), min(b0.y, max(lo.y - 1, 0)), min(b0.z, max(lo.z - 1, 0))};
+    b1 = {max(b1.x, min(hi.x + 1, a.grid.size.x - 1)), max(b1.y, min(hi.y + 1, a.grid.size.y - 1)),
+          max(b1.z, min(hi.z + 1, a.grid.size.z - 1))};
+  } else {
     if (*a.ghostCount <= 0) return;
     b0 = {max(b0.x, a.ghostBox[0]), max(b0.y, a.ghostBox[1]), max(b0.z, a.ghostBox[2])};
     b1 = {min(b1.x, a.ghostBox[3]), min(b1.y, a.ghostBox[4]), min(b1.z, a.ghostBox[5])};
@@ -182,6 +206,7 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
 
 template <int SET>
 __global__ void kBroadCount(BroadK a) {
+  if (gateOff(a.gate)) return;
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
   if (A > a.P.n) return;
   if (A == a.P.n) {
@@ -195,6 +220,7 @@ __global__ void kBroadCount(BroadK a) {
 
 template <int SET>
 __global__ void kBroadWrite(BroadK a) {
+  if (gateOff(a.gate)) return;
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= a.P.n) return;
   const int begin = a.pairs.count[A];
@@ -285,7 +311,12 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
   return ov >= 0.;
 }
 
-__device__ __forceinline__ int findPrevPair(const NarrowK& a, int A, int T, int& off) {
+__device__ __forceinline__ int findPrevPair(const NarrowK& a, int p, int A, int T, int& off) {
+  // lists kept from the previous step hold the pair at the same index
+  if (p < min(*a.prev.total, a.prev.capacity) && a.prev.pairA[p] == A && a.prev.pairT[p] == T) {
+    off = a.prev.recOff[p];
+    return a.prev.recCnt[p];
+  }
   int lo = min(a.prevStart[A], a.prev.capacity), hi = min(a.prevStart[A + 1], a.prev.capacity) - 1;
   while (lo <= hi) {
     const int mid = (lo + hi) >> 1;
@@ -416,7 +447,7 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   if (total > 0) {
     int prevOff = 0;
-    const int prevCnt = findPrevPair(a, A, T, prevOff);
+    const int prevCnt = findPrevPair(a, p, A, T, prevOff);
     // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
     const int mi = a.P.mat[A];
     const fdm_material& Mi = a.pmat[mi];
@@ -550,7 +581,11 @@ extern "C" int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* works
   k.table = g.table;
   k.pairs = g.pairs;
   k.status = nullptr;
+  k.skin = g.skin;
+  k.maxReach = g.maxReach;
+  k.gate = g.gate;
   if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
+  if (!(g.skin >= 0.0) || !(g.maxReach >= 0.0)) return FDM_EINVAL;
   const unsigned nb = blocksFor((long long)n + 1, 128);
   switch (g.set) {
     case FDM_SET_PARTICLE:
@@ -566,7 +601,7 @@ extern "C" int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* works
       return FDM_EINVAL;
   }
   int r = scanInts(g.pairs.count, g.pairs.count, (size_t)n + 1, false, g.pairs.total, workspace,
-                   workspaceBytes, stream);
+                   workspaceBytes, stream, g.gate);
   if (r != FDM_OK) return r;
   return launchStatus();
 }
@@ -594,6 +629,9 @@ extern "C" int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsign
   k.table = g.table;
   k.pairs = g.pairs;
   k.status = status;
+  k.skin = g.skin;
+  k.maxReach = g.maxReach;
+  k.gate = g.gate;
   if (n <= 0) return FDM_OK;
   const unsigned nb = blocksFor(n, 128);
   switch (g.set) {

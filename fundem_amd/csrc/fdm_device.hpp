@@ -170,6 +170,14 @@ __device__ __forceinline__ int ghostCellHash(V3 p, const fdm_grid& g) {
   else if (p.z >= g.max.y) { c.z = g.size.z - 1; out = true; }
   return out ? lin3(c, gsize(g)) : -1;
 }
+// Grid state of a body: 0 outside the grid (hash -1), 1 inside, 2 inside but
+// with a cell index rounded up to the grid size on some axis, so that its
+// cell id aliases another cell.
+__device__ __forceinline__ int bodyState(V3 p, const fdm_grid& g) {
+  if (cellHash(p, g) < 0) return 0;
+  const I3 c = gridPos(p, g);
+  return (c.x >= g.size.x || c.y >= g.size.y || c.z >= g.size.z) ? 2 : 1;
+}
 // node cell clamped into the grid, as the detection kernels clamp it
 // (LSParticleContactDetectionKernel.cu:38-46)
 __device__ __forceinline__ I3 clampedCell(V3 p, const fdm_grid& g) {

@@ -6,6 +6,7 @@
 // the XY ghost in the reference's order (PeriodicBoundary.h:221-295), keeps
 // only ghosts whose ghost hash is valid (the others are never found by any
 // node), compacts them in particle order and builds one hash table per set.
+#include <algorithm>
 #include <climits>
 
 #include "fdm_device.hpp"
@@ -83,19 +84,47 @@ __device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_pe
   return w;
 }
 
-// Pass 1: visibility flag of each particle's ghost in each active set.
+__device__ __forceinline__ bool sameV3(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// Pass 1: visibility flag of each particle's ghost in each active set. With
+// list reuse (ctl != null) it flags a wrap and any visibility change since
+// the last ghost list build.
 __global__ void kGhostFlags(const double* pos, int n, fdm_grid g, fdm_periodic per, int* fx, int* fy,
-                            int* fxy) {
+                            int* fxy, const int* atBuild, int* ctl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
-  if (per.x) fx[i] = ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0 ? 1 : 0;
-  if (per.y) fy[i] = ghostCellHash(ghostOf(w.afterY, g, 2), g) >= 0 ? 1 : 0;
-  if (per.x && per.y) fxy[i] = ghostCellHash(ghostOf(w.afterY, g, 3), g) >= 0 ? 1 : 0;
+  bool changed = false, wrapped = false;
+  if (i < n) {
+    const V3 x = ldp(pos, n, i);
+    const WrapStages w = wrapStages(x, g, per);
+    int bits = 0;
+    if (per.x) {
+      const int f = ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0 ? 1 : 0;
+      fx[i] = f;
+      bits |= f;
+    }
+    if (per.y) {
+      const int f = ghostCellHash(ghostOf(w.afterY, g, 2), g) >= 0 ? 1 : 0;
+      fy[i] = f;
+      bits |= f << 1;
+    }
+    if (per.x && per.y) {
+      const int f = ghostCellHash(ghostOf(w.afterY, g, 3), g) >= 0 ? 1 : 0;
+      fxy[i] = f;
+      bits |= f << 2;
+    }
+    if (ctl) {
+      wrapped = !sameV3(w.afterY, x);
+      changed = atBuild && bits != atBuild[i];
+    }
+  }
+  if (!ctl) return;
+  if (ballot64(changed) != 0ull && laneId() == 0) ctl[FDM_CTL_GHOST_CHANGED] = 1;
+  if (ballot64(wrapped) != 0ull && laneId() == 0) ctl[FDM_CTL_WRAPPED] = 1;
 }
 
 struct GhostOut {
   int* slot;       // exclusive scan of the flags
+  const int* gate;
   int* realId;
   double* pos;
   int* hashValue;
@@ -118,42 +147,74 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
   o.pos[3 * s + 1] = gp.y;
   o.pos[3 * s + 2] = gp.z;
   o.hashValue[s] = h;
-  o.sortKey[s] = (unsigned)(h + 1);
+  // lists are built by the clamped geometric cell (monotone in the position);
+  // the reference's ghost cell id above decides the stencil rule
+  const I3 c = clampedCell(gp, g);
+  o.sortKey[s] = (unsigned)(lin3(c, gsize(g)) + 1);
   o.sortIndex[s] = s;
-  if ((long long)h < g.num) {
-    const I3 c = decodeHash(h, g);
-    atomicMin(&o.cellBox[0], c.x);
-    atomicMin(&o.cellBox[1], c.y);
-    atomicMin(&o.cellBox[2], c.z);
-    atomicMax(&o.cellBox[3], c.x);
-    atomicMax(&o.cellBox[4], c.y);
-    atomicMax(&o.cellBox[5], c.z);
-  }
+  atomicMin(&o.cellBox[0], c.x);
+  atomicMin(&o.cellBox[1], c.y);
+  atomicMin(&o.cellBox[2], c.z);
+  atomicMax(&o.cellBox[3], c.x);
+  atomicMax(&o.cellBox[4], c.y);
+  atomicMax(&o.cellBox[5], c.z);
 }
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
 // ghosts of each set, then store the wrapped positions.
 __global__ void kGhostScatter(double* pos, int n, fdm_grid g, fdm_periodic per, GhostOut ox,
-                              GhostOut oy, GhostOut oxy) {
+                              GhostOut oy, GhostOut oxy, int* atBuild) {
+  if (gateOff(ox.gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
+  int bits = 0;
   if (per.x) {
     const V3 gp = ghostOf(w.afterX, g, 1);
-    if (ghostCellHash(gp, g) >= 0) emitGhost(ox, i, gp, g);
+    if (ghostCellHash(gp, g) >= 0) {
+      emitGhost(ox, i, gp, g);
+      bits |= 1;
+    }
   }
   if (per.y) {
     const V3 gp = ghostOf(w.afterY, g, 2);
-    if (ghostCellHash(gp, g) >= 0) emitGhost(oy, i, gp, g);
+    if (ghostCellHash(gp, g) >= 0) {
+      emitGhost(oy, i, gp, g);
+      bits |= 2;
+    }
   }
   if (per.x && per.y) {
     const V3 gp = ghostOf(w.afterY, g, 3);
-    if (ghostCellHash(gp, g) >= 0) emitGhost(oxy, i, gp, g);
+    if (ghostCellHash(gp, g) >= 0) {
+      emitGhost(oxy, i, gp, g);
+      bits |= 4;
+    }
   }
+  if (atBuild) atBuild[i] = bits;
   stp(pos, n, i, w.afterY);
 }
 
-__global__ void kResetBox(int* box0, int* box1, int* box2) {
+// Steps without a ghost rebuild: no particle wrapped and every visibility bit
+// is unchanged, so the kept ghosts are exactly this step's; refresh their
+// positions and the reference's cell ids.
+__global__ void kGhostRefresh(const double* pos, int n, fdm_grid g, fdm_periodic per, int mask,
+                              const int* count, int capacity, const int* realId, double* gpos,
+                              int* hashValue, const int* gate) {
+  if (gate && *gate != 0) return;
+  const int c = min(*count, capacity);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < c; k += gridDim.x * blockDim.x) {
+    const int i = realId[k];
+    const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
+    const V3 gp = ghostOf(mask == 1 ? w.afterX : w.afterY, g, mask);
+    gpos[3 * k] = gp.x;
+    gpos[3 * k + 1] = gp.y;
+    gpos[3 * k + 2] = gp.z;
+    hashValue[k] = ghostCellHash(gp, g);
+  }
+}
+
+__global__ void kResetBox(int* box0, int* box1, int* box2, const int* gate) {
+  if (gateOff(gate)) return;
   const int t = threadIdx.x;
   if (t < 6) {
     const int v = t < 3 ? INT_MAX : INT_MIN;
@@ -163,7 +224,8 @@ __global__ void kResetBox(int* box0, int* box1, int* box2) {
   }
 }
 
-__global__ void kClampCount(int* count, int capacity) {
+__global__ void kClampCount(int* count, int capacity, const int* gate) {
+  if (gateOff(gate)) return;
   if (threadIdx.x == 0 && *count > capacity) *count = capacity;
 }
 
@@ -218,26 +280,32 @@ extern "C" int fdm_launchUpdateGhostSpatialGridHashStartEnd(fdm_hash_table table
   return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream);
 }
 
-extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
-                                        fdm_ghosts* sets, unsigned* status, void* workspace,
-                                        size_t workspaceBytes, hipStream_t stream) {
+namespace {
+
+const int* firstAtBuild(const fdm_ghosts* sets, const bool* on) {
+  for (int s = 0; s < 3; ++s)
+    if (on[s]) return sets[s].atBuild;
+  return nullptr;
+}
+
+// scans of the flags, scatter of the visible ghosts (storing the wrapped
+// positions) and one table per set; every kernel gated by *gate.
+int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_ghosts* sets,
+                const bool* on, int* atBuild, const int* gate, unsigned* status, void* workspace,
+                size_t workspaceBytes, hipStream_t stream) {
   const int n = particles.n;
-  if (n < 0 || !sets || !status) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
   hipLaunchKernelGGL(kResetBox, dim3(1), dim3(64), 0, stream, on[0] ? sets[0].cellBox : nullptr,
-                     on[1] ? sets[1].cellBox : nullptr, on[2] ? sets[2].cellBox : nullptr);
-  hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
-                     grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag);
+                     on[1] ? sets[1].cellBox : nullptr, on[2] ? sets[2].cellBox : nullptr, gate);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
     int r = scanInts(sets[s].flag, sets[s].flag, (size_t)n, false, sets[s].count, workspace,
-                     workspaceBytes, stream);
+                     workspaceBytes, stream, gate);
     if (r != FDM_OK) return r;
   }
   GhostOut o[3];
   for (int s = 0; s < 3; ++s) {
     o[s].slot = sets[s].flag;
+    o[s].gate = gate;
     o[s].realId = sets[s].realId;
     o[s].pos = sets[s].pos;
     o[s].hashValue = sets[s].table.hashValue;
@@ -248,13 +316,71 @@ extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm
     o[s].status = status;
   }
   hipLaunchKernelGGL(kGhostScatter, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
-                     grid, periodic, o[0], o[1], o[2]);
+                     grid, periodic, o[0], o[1], o[2], atBuild);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
-    hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity);
+    hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity,
+                       gate);
     int r = buildStartEnd(sets[s].table, sets[s].capacity, sets[s].count, grid, workspace,
-                          workspaceBytes, stream);
+                          workspaceBytes, stream, gate);
     if (r != FDM_OK) return r;
+  }
+  return launchStatus();
+}
+
+}  // namespace
+
+extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                        fdm_ghosts* sets, unsigned* status, void* workspace,
+                                        size_t workspaceBytes, hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets || !status) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
+                     grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag, nullptr, nullptr);
+  return buildGhosts(particles, grid, periodic, sets, on, nullptr, nullptr, status, workspace,
+                     workspaceBytes, stream);
+}
+
+extern "C" int fdm_launchPeriodicPrepare(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                         fdm_ghosts* sets, const fdm_skin* skin,
+                                         hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets || !skin || !skin->ctl) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  const int* atBuild = firstAtBuild(sets, on);
+  if (!atBuild) return FDM_EINVAL;
+  hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
+                     grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag, atBuild, skin->ctl);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                       fdm_ghosts* sets, const int* gate, unsigned* status,
+                                       void* workspace, size_t workspaceBytes,
+                                       hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets || !status) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), gate, status,
+                     workspace, workspaceBytes, stream);
+}
+
+extern "C" int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                         fdm_ghosts* sets, const int* gate, hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  for (int s = 0; s < 3; ++s) {
+    if (!on[s]) continue;
+    const unsigned nb = std::min<unsigned>(blocksFor(sets[s].capacity, 256), 256);
+    hipLaunchKernelGGL(kGhostRefresh, dim3(nb), dim3(256), 0, stream, particles.x, n, grid,
+                       periodic, s == 0 ? 1 : s == 1 ? 2 : 3, sets[s].count, sets[s].capacity,
+                       sets[s].realId, sets[s].pos, sets[s].table.hashValue, gate);
   }
   return launchStatus();
 }

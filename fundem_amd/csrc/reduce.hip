@@ -134,7 +134,7 @@ __global__ void kReduce(ReduceK a) {
       last = p;
       addRec(F, T, a.pp.sumF + 3 * (size_t)p, a.pp.sumTB + 3 * (size_t)p, true);
     }
-    a.ppm.count[i] = 0;
+    if (m > 0) a.ppm.count[i] = 0;
     for (int p = segBegin(a.pp, i); p < segEnd(a.pp, i); ++p)
       if (a.pp.recCnt[p] > 0) addRec(F, T, a.pp.sumF + 3 * (size_t)p, a.pp.sumTA + 3 * (size_t)p, false);
   }
@@ -167,7 +167,7 @@ __global__ void kReduce(ReduceK a) {
         mp = used < m ? nextMirror(list, m, mp) : INT_MAX;
       }
     }
-    M.count[i] = 0;
+    if (m > 0) M.count[i] = 0;
   }
   if (a.haveBonds) {
     const fdm_bonds& b = a.bonds;

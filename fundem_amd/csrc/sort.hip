@@ -18,7 +18,8 @@
 namespace fdm {
 
 // ------------------------------------------------------------------- fill
-__global__ void kFillInt(int* p, size_t n, int v) {
+__global__ void kFillInt(int* p, size_t n, int v, const int* gate) {
+  if (gateOff(gate)) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -63,7 +64,9 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* lds, int& total) {
   return r;
 }
 
-__global__ void __launch_bounds__(kScanThreads) kScanReduce(const int* in, size_t n, int* sums) {
+__global__ void __launch_bounds__(kScanThreads) kScanReduce(const int* in, size_t n, int* sums,
+                                                            const int* gate) {
+  if (gateOff(gate)) return;
   __shared__ int lds[kScanThreads / 64 + 1];
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   int s = 0;
@@ -76,7 +79,8 @@ __global__ void __launch_bounds__(kScanThreads) kScanReduce(const int* in, size_
 }
 
 // Single workgroup: exclusive scan of the tile sums in place, total to *total.
-__global__ void __launch_bounds__(1024) kScanSums(int* sums, int nb, int* total) {
+__global__ void __launch_bounds__(1024) kScanSums(int* sums, int nb, int* total, const int* gate) {
+  if (gateOff(gate)) return;
   __shared__ int lds[1024 / 64 + 1];
   int carry = 0;
   for (int base = 0; base < nb; base += 1024) {
@@ -91,7 +95,9 @@ __global__ void __launch_bounds__(1024) kScanSums(int* sums, int nb, int* total)
 }
 
 __global__ void __launch_bounds__(kScanThreads) kScanApply(const int* in, int* out, size_t n,
-                                                           const int* sums, int inclusive) {
+                                                           const int* sums, int inclusive,
+                                                           const int* gate) {
+  if (gateOff(gate)) return;
   __shared__ int lds[kScanThreads / 64 + 1];
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -114,18 +120,18 @@ __global__ void __launch_bounds__(kScanThreads) kScanApply(const int* in, int* o
 size_t scanWorkspace(size_t n) { return alignUp((blocksFor((long long)n, kScanTile) + 1) * sizeof(int)); }
 
 int scanInts(const int* in, int* out, size_t n, bool inclusive, int* total, void* ws, size_t wsBytes,
-             hipStream_t stream) {
+             hipStream_t stream, const int* gate) {
   if (n == 0) {
-    if (total) hipLaunchKernelGGL(kFillInt, dim3(1), dim3(64), 0, stream, total, (size_t)1, 0);
+    if (total) hipLaunchKernelGGL(kFillInt, dim3(1), dim3(64), 0, stream, total, (size_t)1, 0, gate);
     return launchStatus();
   }
   if (wsBytes < scanWorkspace(n)) return FDM_EINVAL;
   const unsigned nb = blocksFor((long long)n, kScanTile);
   int* sums = (int*)ws;
-  hipLaunchKernelGGL(kScanReduce, dim3(nb), dim3(kScanThreads), 0, stream, in, n, sums);
-  hipLaunchKernelGGL(kScanSums, dim3(1), dim3(1024), 0, stream, sums, (int)nb, total);
+  hipLaunchKernelGGL(kScanReduce, dim3(nb), dim3(kScanThreads), 0, stream, in, n, sums, gate);
+  hipLaunchKernelGGL(kScanSums, dim3(1), dim3(1024), 0, stream, sums, (int)nb, total, gate);
   hipLaunchKernelGGL(kScanApply, dim3(nb), dim3(kScanThreads), 0, stream, in, out, n, sums,
-                     inclusive ? 1 : 0);
+                     inclusive ? 1 : 0, gate);
   return launchStatus();
 }
 
@@ -147,7 +153,8 @@ __device__ __forceinline__ size_t effectiveCount(size_t n, const int* nDev) {
 
 __global__ void __launch_bounds__(kSortThreads) kRadixUpsweep(const unsigned* keys, size_t nCap,
                                                               const int* nDev, int shift, int* hist,
-                                                              int nblocks) {
+                                                              int nblocks, const int* gate) {
+  if (gateOff(gate)) return;
   __shared__ int bins[kRadixBins];
   const size_t n = effectiveCount(nCap, nDev);
   for (int d = threadIdx.x; d < kRadixBins; d += kSortThreads) bins[d] = 0;
@@ -178,7 +185,9 @@ __global__ void __launch_bounds__(kSortThreads) kRadixDownsweep(const unsigned* 
                                                                 const int* valsIn, unsigned* keysOut,
                                                                 int* valsOut, size_t nCap,
                                                                 const int* nDev, int shift,
-                                                                const int* offsets, int nblocks) {
+                                                                const int* offsets, int nblocks,
+                                                                const int* gate) {
+  if (gateOff(gate)) return;
   __shared__ int cnt[kSortWaves][kRadixBins];
   const size_t n = effectiveCount(nCap, nDev);
   const int lane = laneId();
@@ -237,7 +246,8 @@ __global__ void __launch_bounds__(kSortThreads) kRadixDownsweep(const unsigned* 
 }
 
 __global__ void kCopyPairs(const unsigned* ks, const int* vs, unsigned* kd, int* vd, size_t nCap,
-                           const int* nDev) {
+                           const int* nDev, const int* gate) {
+  if (gateOff(gate)) return;
   const size_t n = effectiveCount(nCap, nDev);
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -254,7 +264,7 @@ size_t radixWorkspace(size_t n) {
 
 // Sorts n pairs, or the first *nDev of them when nDev is given (n bounds it).
 int radixSort(unsigned* keys, int* vals, size_t n, const int* nDev, int keyBits, void* ws,
-              size_t wsBytes, hipStream_t stream) {
+              size_t wsBytes, hipStream_t stream, const int* gate) {
   if (n <= 1) return FDM_OK;
   if (keyBits < 1) keyBits = 1;
   if (keyBits > 32) keyBits = 32;
@@ -274,11 +284,12 @@ int radixSort(unsigned* keys, int* vals, size_t n, const int* nDev, int keyBits,
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kRadixBits;
     hipLaunchKernelGGL(kRadixUpsweep, dim3(nb), dim3(kSortThreads), 0, stream, ki, n, nDev, shift,
-                       hist, (int)nb);
-    int r = scanInts(hist, hist, (size_t)nb * kRadixBins, false, nullptr, sws, swsBytes, stream);
+                       hist, (int)nb, gate);
+    int r = scanInts(hist, hist, (size_t)nb * kRadixBins, false, nullptr, sws, swsBytes, stream,
+                     gate);
     if (r != FDM_OK) return r;
     hipLaunchKernelGGL(kRadixDownsweep, dim3(nb), dim3(kSortThreads), 0, stream, ki, vi, ko, vo, n,
-                       nDev, shift, hist, (int)nb);
+                       nDev, shift, hist, (int)nb, gate);
     unsigned* tk = ki;
     ki = ko;
     ko = tk;
@@ -288,14 +299,15 @@ int radixSort(unsigned* keys, int* vals, size_t n, const int* nDev, int keyBits,
   }
   if (ki != keys)
     hipLaunchKernelGGL(kCopyPairs, dim3(blocksFor((long long)n, 256)), dim3(256), 0, stream, ki, vi,
-                       keys, vals, n, nDev);
+                       keys, vals, n, nDev, gate);
   return launchStatus();
 }
 
 // ------------------------------------------------------------ cell ranges
 // Reset the cells marked by the previous build (count < 0: all cells).
 __global__ void kClearCells(const int* prevKey, const int* prevCount, int* start, int* end,
-                            long long num) {
+                            long long num, const int* gate) {
+  if (gateOff(gate)) return;
   const long long stride = (long long)gridDim.x * blockDim.x;
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int pc = *prevCount;
@@ -318,7 +330,8 @@ __global__ void kClearCells(const int* prevKey, const int* prevCount, int* start
 // findStartAndEnd (buildHashStartEnd.cu:47-67) on keys hash+1, then remember
 // the keys for the next reset.
 __global__ void kMarkCells(const unsigned* key, int nCap, const int* nDev, int* start, int* end,
-                           long long num, int* prevKey, int* prevCount) {
+                           long long num, int* prevKey, int* prevCount, const int* gate) {
+  if (gateOff(gate)) return;
   const int n = (int)effectiveCount((size_t)nCap, nDev);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *prevCount = n;
@@ -343,17 +356,24 @@ int keyBitsFor(long long numGrid) {
 // Sort the table's (key, index) lists and mark cell ranges; with nDev the
 // list length is read on the device (n is its bound).
 int buildStartEnd(fdm_hash_table t, int n, const int* nDev, fdm_grid grid, void* ws,
-                  size_t wsBytes, hipStream_t stream) {
+                  size_t wsBytes, hipStream_t stream, const int* gate) {
   int r = radixSort(t.sortKey, t.sortIndex, (size_t)n, nDev, keyBitsFor(grid.num), ws, wsBytes,
-                    stream);
+                    stream, gate);
   if (r != FDM_OK) return r;
   hipLaunchKernelGGL(kClearCells, dim3(2048), dim3(256), 0, stream, t.prevKey, t.prevCount,
-                     t.cellStart, t.cellEnd, grid.num);
+                     t.cellStart, t.cellEnd, grid.num, gate);
   if (n > 0)
     hipLaunchKernelGGL(kMarkCells, dim3(blocksFor(n, 256)), dim3(256), 0, stream, t.sortKey, n,
-                       nDev, t.cellStart, t.cellEnd, grid.num, t.prevKey, t.prevCount);
+                       nDev, t.cellStart, t.cellEnd, grid.num, t.prevKey, t.prevCount, gate);
   else
-    hipLaunchKernelGGL(kFillInt, dim3(1), dim3(64), 0, stream, t.prevCount, (size_t)1, 0);
+    hipLaunchKernelGGL(kFillInt, dim3(1), dim3(64), 0, stream, t.prevCount, (size_t)1, 0, gate);
+  return launchStatus();
+}
+
+int fillInts(int* p, size_t n, int v, hipStream_t stream, const int* gate) {
+  if (n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kFillInt, dim3(blocksFor((long long)n, 256)), dim3(256), 0, stream, p, n, v,
+                     gate);
   return launchStatus();
 }
 
@@ -377,9 +397,7 @@ using namespace fdm;
 extern "C" int fdm_fillInt(int* ptr, size_t n, int value, hipStream_t stream) {
   if (n == 0) return FDM_OK;
   if (!ptr) return FDM_EINVAL;
-  hipLaunchKernelGGL(kFillInt, dim3(blocksFor((long long)n, 256)), dim3(256), 0, stream, ptr, n,
-                     value);
-  return launchStatus();
+  return fillInts(ptr, n, value, stream);
 }
 
 extern "C" size_t fdm_radixSortWorkspaceBytes(size_t n) { return radixWorkspace(n); }
@@ -401,6 +419,13 @@ extern "C" int fdm_buildHashStartEnd(fdm_hash_table table, int n, fdm_grid grid,
                                      size_t workspaceBytes, hipStream_t stream) {
   if (n < 0 || grid.num <= 0) return FDM_EINVAL;
   return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream);
+}
+
+extern "C" int fdm_buildHashStartEndGated(fdm_hash_table table, int n, fdm_grid grid,
+                                          const int* gate, void* workspace, size_t workspaceBytes,
+                                          hipStream_t stream) {
+  if (n < 0 || grid.num <= 0) return FDM_EINVAL;
+  return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream, gate);
 }
 
 extern "C" int fdm_deviceInfo(int* numCU, int* warpSize) {

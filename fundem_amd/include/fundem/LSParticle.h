@@ -42,11 +42,12 @@ inline fdm_grid makeGrid(double3 mn, double3 mx, double cs) {
 
 // Device hash table of one grid.
 struct HashTableBuffers {
-  DeviceBuffer<int> hashValue, sortIndex, cellStart, cellEnd, prevKey, prevCount;
+  DeviceBuffer<int> hashValue, sortIndex, cellStart, cellEnd, prevKey, prevCount, validAtBuild;
   DeviceBuffer<unsigned> sortKey;
   void allocate(size_t n, long long numGrid) {
     const size_t m = std::max<size_t>(n, 1);
     hashValue.allocate(m);
+    validAtBuild.allocate(m);
     sortIndex.allocate(m);
     sortKey.allocate(m);
     prevKey.allocate(m);
@@ -65,6 +66,7 @@ struct HashTableBuffers {
     t.cellEnd = cellEnd.get();
     t.prevKey = prevKey.get();
     t.prevCount = prevCount.get();
+    t.validAtBuild = validAtBuild.get();
     return t;
   }
 };

@@ -8,6 +8,7 @@
 // stage timing.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <iostream>
@@ -36,8 +37,8 @@ class LSSolver {
                     const size_t maxGPUThread = 256, const int device = 0)
       : dir_(dir), stream_(stream), maxGPUThread_(maxGPUThread), device_(device) {}
   virtual ~LSSolver() {
-    for (auto& g : graphs_)
-      if (g) (void)hipGraphExecDestroy(g);
+    destroyGraphs();
+    if (hostDecision_) (void)hipHostFree(hostDecision_);
     if (ownStream_) (void)hipStreamDestroy(ownStream_);
   }
   LSSolver(const LSSolver&) = delete;
@@ -139,6 +140,18 @@ class LSSolver {
   }
   void useGraph(bool on) { useGraph_ = on; }
   void setStageTiming(bool on) { timing_ = on; }
+  // Extra reach of the candidate lists, which are then rebuilt only when a
+  // body may have moved by half of it (decided on the device each step).
+  // Results do not depend on it. Negative: a tenth of the largest particle
+  // radius. Takes effect at the next initialize.
+  void setNeighborSkin(double skin) { skinRequest_ = skin; }
+  double neighborSkin() const { return skin_; }
+  // Candidate-list builds and ghost-list builds since initialize; synchronises.
+  std::pair<long, long> listBuilds() {
+    if (!ctl_.get()) return {0, 0};
+    std::vector<int> c = ctl_.toHost(stream_, FDM_CTL_WORDS);
+    return {c[FDM_CTL_COUNT], c[FDM_CTL_GHOST_COUNT]};
+  }
   // Accumulated device time per stage (ms) and the number of launches timed.
   const std::map<std::string, double>& stageMilliseconds() {
     resolveTimers();
@@ -210,6 +223,26 @@ class LSSolver {
     if (cellSize_ > 0.0 && cellSize_ < maxR)
       throw std::invalid_argument("cell size smaller than the largest particle radius");
     const size_t N = particles_.num(), W = walls_.num();
+    maxRadius_ = maxR;
+    maxBoxReach_ = 0.0;
+    rhoMax_ = 0.0;
+    for (size_t g = 0; g < particles_.numGeometry(); ++g) {
+      const fdm_template& t = particles_.templateInfo(g);
+      maxBoxReach_ = std::max(maxBoxReach_, t.boxReach);
+      rhoMax_ = std::max({rhoMax_, t.radius, t.boxReach});
+    }
+    for (size_t g = 0; g < walls_.numGeometry(); ++g) {
+      const fdm_template& t = walls_.templateInfo(g);
+      rhoMax_ = std::max({rhoMax_, t.radius, t.boxReach});
+    }
+    skin_ = skinRequest_ >= 0.0 ? skinRequest_ : 0.1 * maxR;
+    ctl_.allocate(FDM_CTL_WORDS);
+    disp_.allocate(1);
+    {
+      const int one = 1;  // the first step builds every list
+      hipCheck(hipMemcpy(ctl_.get() + FDM_CTL_FORCE, &one, sizeof(int), hipMemcpyHostToDevice),
+               "ctl");
+    }
     const int pairCap = (int)std::max<size_t>(size_t(pairsPerParticle_) * N, 1024);
     const int recCap = (int)std::max<size_t>(size_t(recordsPerParticle_) * N, 4096);
     pp_.initialize(N, pairCap, recCap, mirrorCapacity_);
@@ -222,11 +255,17 @@ class LSSolver {
     size_t ws = fdm_radixSortWorkspaceBytes(std::max<size_t>({N, W, size_t(64)}));
     ws = std::max(ws, fdm_scanWorkspaceBytes(N + 1));
     workspace_.allocate(ws / sizeof(double) + 1, false);
-    for (auto& g : graphs_)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    destroyGraphs();
+    if (!hostDecision_) {
+      hipCheck(hipHostMalloc((void**)&hostDecision_, 4 * sizeof(int),
+                             hipHostMallocMapped | hipHostMallocCoherent),
+               "hipHostMalloc");
+      hipCheck(hipHostGetDevicePointer((void**)&hostDecisionDev_, hostDecision_, 0),
+               "hipHostGetDevicePointer");
+    }
+    hipCheck(hipStreamSynchronize(stream_), "initialize");
+    for (int k = 0; k < 4; ++k) hostDecision_[k] = 0;
+    decisionSeq_ = 0;
     hookMode_ = HookUnknown;
     stepOnce(true, 0.0);
     hipCheck(hipStreamSynchronize(stream_), "initialize");
@@ -238,10 +277,7 @@ class LSSolver {
   void advance(size_t nSteps, double time = 0.0) {
     size_t done = 0;
     if (useGraph_ && hookMode_ == HookNone && !timing_) {
-      while (nSteps - done >= 2) {
-        replayGraph();
-        done += 2;
-      }
+      for (; done < nSteps; ++done) graphStep();
     }
     for (; done < nSteps; ++done) {
       stepOnce(false, time);
@@ -270,6 +306,17 @@ class LSSolver {
     if (s != 0)
       throw std::runtime_error("contact capacity exceeded (status " + std::to_string(s) +
                                "); raise it with setContactCapacity");
+  }
+
+  // Candidate pairs of each set (particle, wall, ghost X, Y, XY) at the
+  // latest step; synchronises.
+  std::vector<int> pairTotals() {
+    std::vector<int> out(5, 0);
+    out[0] = pp_.lastTotal(stream_);
+    out[1] = pw_.lastTotal(stream_);
+    for (int s = 0; s < 3; ++s)
+      if (periodic_.active(s)) out[2 + s] = periodic_.set(s).interaction.lastTotal(stream_);
+    return out;
   }
 
   // Contacts of the latest step, canonical order. set: 0 particle-particle,
@@ -453,9 +500,23 @@ class LSSolver {
     pending_.clear();
   }
 
+  fdm_skin skinView() {
+    fdm_skin k;
+    k.skin = skin_;
+    k.rhoMax = rhoMax_;
+    k.dt = dt_;
+    k.ctl = ctl_.get();
+    k.disp = disp_.get();
+    k.hostDecision = hostDecisionDev_;
+    return k;
+  }
+
   void interactionSet(int set, SolidInteraction& si, const fdm_ghosts* ghosts, fdm_grid grid,
-                      fdm_hash_table table, const int* targetHash, bool prime) {
+                      fdm_hash_table table, const int* targetHash, bool prime, bool rebuild) {
     fdm_broadphase_args b{};
+    b.skin = skin_;
+    b.maxReach = set == FDM_SET_PARTICLE ? maxRadius_ : set == FDM_SET_GHOST ? maxBoxReach_ : 0.0;
+    b.gate = nullptr;
     b.set = set;
     b.particles = particles_.bodies();
     b.pgeo = particles_.geometry();
@@ -466,10 +527,19 @@ class LSSolver {
     b.table = table;
     b.pairs = si.current();
     {
-      Timer t(this, set == 0 ? "broad_pp" : set == 1 ? "broad_pw" : "broad_ghost");
-      fdmCheck(fdm_launchBroadPhase(&b, workspace_.get(), workspace_.size() * sizeof(double), stream_),
-               "broad phase");
-      fdmCheck(fdm_launchBroadPhaseWrite(&b, status_.get(), stream_), "broad phase write");
+      Timer t(this, set == 0 ? (rebuild ? "broad_pp" : "carry_pp")
+                             : set == 1 ? (rebuild ? "broad_pw" : "carry_pw")
+                                        : (rebuild ? "broad_ghost" : "carry_ghost"));
+      if (rebuild) {
+        fdmCheck(fdm_launchBroadPhase(&b, workspace_.get(), workspace_.size() * sizeof(double),
+                                      stream_),
+                 "broad phase");
+        fdmCheck(fdm_launchBroadPhaseWrite(&b, status_.get(), stream_), "broad phase write");
+      } else {
+        const fdm_pairs from = si.previous();
+        fdmCheck(fdm_launchCarryPairs(&from, &b.pairs, (int)particles_.num(), nullptr, stream_),
+                 "carry pairs");
+      }
     }
     fdm_narrowphase_args n{};
     n.set = set;
@@ -495,56 +565,116 @@ class LSSolver {
     fdmCheck(fdm_launchNarrowPhase(&n, stream_), "narrow phase");
   }
 
-  void stepOnce(bool prime, double time) {
+  // A step runs in two phases. Phase A (kick, drift, cell ids, periodic
+  // visibility flags) ends with the device's decision whether the candidate
+  // lists are rebuilt (fdm_launchSkinDecide), written to host-visible memory.
+  // The host waits for that word and launches phase B with only the work the
+  // step needs: list builds, or carrying the previous lists forward, then
+  // the narrow phase, bonds and the reduction. Waiting here replaces dozens
+  // of launches that would otherwise test a device flag and exit.
+  void phaseA(bool prime) {
+    fdm_bodies P = particles_.bodies();
+    const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
+    const fdm_skin sk = skinView();
+    {
+      Timer t(this, "kick_drift_hash");
+      fdmCheck(fdm_launchKickDriftHash(P, particles_.geometry(), g, dt_, prime ? 0 : 1,
+                                       prime ? 0 : 1, particles_.grid(), particles_.table().view(),
+                                       &sk, stream_),
+               "kick/drift");
+    }
+    if (walls_.num()) {
+      Timer t(this, "wall_drift");
+      fdmCheck(fdm_launchKickDriftHash(walls_.bodies(), walls_.geometry(), g, dt_, 0, prime ? 0 : 1,
+                                       walls_.grid(), walls_.table().view(), &sk, stream_),
+               "wall drift");
+    }
+    if (periodic_.isActived()) {
+      Timer t(this, "periodic_flags");
+      fdm_ghosts gv[3];
+      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
+      fdmCheck(fdm_launchPeriodicPrepare(P, particles_.grid(), periodic_.flags(), gv, &sk, stream_),
+               "periodic flags");
+    }
+    Timer t(this, "decide");
+    fdmCheck(fdm_launchSkinDecide(&sk, stream_), "list decision");
+  }
+
+  struct Decision {
+    bool rebuild, ghost;
+  };
+  // Spin on the host-visible decision of the phase A just launched.
+  Decision waitDecision() {
+    const int seq = ++decisionSeq_;
+    volatile int* h = hostDecision_;
+    for (long spin = 1;; ++spin) {
+      if (h[0] == seq) break;
+      if ((spin & 1023) == 0) {
+        const hipError_t e = hipStreamQuery(stream_);
+        if (e == hipSuccess) {
+          if (h[0] == seq) break;
+          throw std::runtime_error("list decision missing after the stream drained");
+        }
+        if (e != hipErrorNotReady) hipCheck(e, "step");
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return {h[1] != 0, h[2] != 0};
+  }
+
+  void phaseB(bool prime, double time, Decision d) {
     fdm_bodies P = particles_.bodies();
     fdm_geometry pgeo = particles_.geometry();
     const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
     const size_t N = particles_.num(), W = walls_.num();
     void* ws = workspace_.get();
     const size_t wsBytes = workspace_.size() * sizeof(double);
-    {
-      Timer t(this, "integrate_hash");
-      fdmCheck(fdm_launchKickDriftHash(P, pgeo, g, dt_, prime ? 0 : 1, prime ? 0 : 1,
-                                       particles_.grid(), particles_.table().view(), stream_),
-               "kick/drift");
-      if (W)
-        fdmCheck(fdm_launchKickDriftHash(walls_.bodies(), walls_.geometry(), g, dt_, 0, prime ? 0 : 1,
-                                         walls_.grid(), walls_.table().view(), stream_),
-                 "wall drift");
-    }
-    {
+    const fdm_skin sk = skinView();
+    if (d.rebuild) {
       Timer t(this, "sort_cells");
+      fdmCheck(fdm_launchCommitBodyState(P, particles_.grid(), particles_.table().view(), &sk,
+                                         stream_),
+               "body state");
       fdmCheck(fdm_buildHashStartEnd(particles_.table().view(), (int)N, particles_.grid(), ws,
                                      wsBytes, stream_),
                "hash table");
-      if (W)
+      if (W) {
+        fdmCheck(fdm_launchCommitBodyState(walls_.bodies(), walls_.grid(), walls_.table().view(),
+                                           &sk, stream_),
+                 "wall state");
         fdmCheck(fdm_buildHashStartEnd(walls_.table().view(), (int)W, walls_.grid(), ws, wsBytes,
                                        stream_),
                  "wall hash table");
+      }
     }
     interactionSet(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(), particles_.table().view(),
-                   particles_.table().hashValue.get(), prime);
+                   particles_.table().hashValue.get(), prime, d.rebuild);
     if (W)
       interactionSet(FDM_SET_WALL, pw_, nullptr, walls_.grid(), walls_.table().view(),
-                     walls_.table().hashValue.get(), prime);
+                     walls_.table().hashValue.get(), prime, d.rebuild);
     fdm_ghosts gv[3];
     const fdm_pairs* gpairs[3] = {nullptr, nullptr, nullptr};
     const fdm_mirror* gmir[3] = {nullptr, nullptr, nullptr};
     fdm_pairs gcur[3];
     fdm_mirror gm[3];
     if (periodic_.isActived()) {
-      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.set(s).view() : fdm_ghosts{};
+      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
       {
-        Timer t(this, "periodic");
-        fdmCheck(fdm_launchPeriodicGhosts(P, particles_.grid(), periodic_.flags(), gv, status_.get(),
-                                          ws, wsBytes, stream_),
-                 "periodic ghosts");
+        Timer t(this, d.ghost ? "periodic_build" : "periodic_refresh");
+        if (d.ghost)
+          fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
+                                           status_.get(), ws, wsBytes, stream_),
+                   "periodic ghosts");
+        else
+          fdmCheck(fdm_launchPeriodicRefresh(P, particles_.grid(), periodic_.flags(), gv, nullptr,
+                                             stream_),
+                   "periodic refresh");
       }
       for (int s = 0; s < 3; ++s) {
         if (!periodic_.active(s)) continue;
         GhostSet& G = periodic_.set(s);
         interactionSet(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(), G.table.view(),
-                       G.table.hashValue.get(), prime);
+                       G.table.hashValue.get(), prime, d.ghost);
         gcur[s] = G.interaction.current();
         gm[s] = G.interaction.mirror();
         gpairs[s] = &gcur[s];
@@ -599,6 +729,11 @@ class LSSolver {
     swapAll();
   }
 
+  void stepOnce(bool prime, double time) {
+    phaseA(prime);
+    phaseB(prime, time, waitDecision());
+  }
+
   void swapAll() {
     pp_.swap();
     pw_.swap();
@@ -606,20 +741,43 @@ class LSSolver {
       if (periodic_.active(s)) periodic_.set(s).interaction.swap();
   }
 
-  // Two steps (one full swap cycle, so the buffer roles after a replay equal
-  // those before it) captured once per parity and replayed.
-  void replayGraph() {
+  // One step from captured graphs: phase A (the same every step) and the
+  // phase B variant for this step's decision and buffer parity, each
+  // captured at first use.
+  void graphStep() {
+    if (!graphA_) graphA_ = capture([&] { phaseA(false); });
+    hipCheck(hipGraphLaunch(graphA_, stream_), "graph launch");
+    const Decision d = waitDecision();
     const int parity = pp_.currentIndex();
-    if (!graphs_[parity]) {
-      hipGraph_t graph;
-      hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal), "capture");
-      stepOnce(false, 0.0);
-      stepOnce(false, 0.0);
-      hipCheck(hipStreamEndCapture(stream_, &graph), "capture end");
-      hipCheck(hipGraphInstantiate(&graphs_[parity], graph, nullptr, nullptr, 0), "instantiate");
-      (void)hipGraphDestroy(graph);
+    hipGraphExec_t& gb = graphsB_[parity][d.rebuild ? 1 : 0][d.ghost ? 1 : 0];
+    if (!gb) {
+      gb = capture([&] { phaseB(false, 0.0, d); });
+      // capturing ran phaseB's host side (buffer swaps) without executing it
+      swapAll();
     }
-    hipCheck(hipGraphLaunch(graphs_[parity], stream_), "graph launch");
+    hipCheck(hipGraphLaunch(gb, stream_), "graph launch");
+    swapAll();
+  }
+  template <class F>
+  hipGraphExec_t capture(F&& body) {
+    hipGraph_t graph;
+    hipGraphExec_t exec = nullptr;
+    hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal), "capture");
+    body();
+    hipCheck(hipStreamEndCapture(stream_, &graph), "capture end");
+    hipCheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+    (void)hipGraphDestroy(graph);
+    return exec;
+  }
+  void destroyGraphs() {
+    if (graphA_) (void)hipGraphExecDestroy(graphA_);
+    graphA_ = nullptr;
+    for (auto& a : graphsB_)
+      for (auto& b : a)
+        for (auto& g : b) {
+          if (g) (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
   }
 
   std::string dir_;
@@ -639,10 +797,18 @@ class LSSolver {
   DeviceBuffer<double> workspace_;
   DeviceBuffer<unsigned> status_;
   DeviceBuffer<int> stepCounter_;
+  DeviceBuffer<int> ctl_;
+  DeviceBuffer<double> disp_;
+  double skinRequest_ = -1.0, skin_ = 0.0;
+  double maxRadius_ = 0.0, maxBoxReach_ = 0.0, rhoMax_ = 0.0;
   bool useGraph_ = false, timing_ = false;
   bool baseHookCalled_ = false;
   HookMode hookMode_ = HookUnknown;
-  hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+  hipGraphExec_t graphA_ = nullptr;
+  hipGraphExec_t graphsB_[2][2][2] = {};
+  int* hostDecision_ = nullptr;     // pinned, host view
+  int* hostDecisionDev_ = nullptr;  // the same memory, device view
+  int decisionSeq_ = 0;
   std::map<std::string, double> stageMs_;
   std::map<std::string, long> stageCalls_;
   std::vector<PendingTimer> pending_;

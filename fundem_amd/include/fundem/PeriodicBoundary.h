@@ -29,8 +29,9 @@ struct GhostSet {
     cellBox.allocate(6);
     table.allocate(capacity, numGrid);
   }
-  fdm_ghosts view() {
+  fdm_ghosts view(int* atBuild) {
     fdm_ghosts g;
+    g.atBuild = atBuild;
     g.capacity = capacity;
     g.count = count.get();
     g.flag = flag.get();
@@ -66,8 +67,11 @@ struct PeriodicBoundaryXY2D {
       allocatedFor_[s] = P.num();
       sets_[s].interaction.initialize(P.num(), pairCapacity, recordCapacity, mirrorCapacity);
     }
+    if (isActived()) atBuild_.allocate(std::max<size_t>(P.num(), 1));
   }
   GhostSet& set(int s) { return sets_[s]; }
+  // view of set s; every set shares the per-particle visibility bits
+  fdm_ghosts view(int s) { return sets_[s].view(atBuild_.get()); }
   double deviceMemoryGB() const {
     double t = 0;
     for (const auto& s : sets_) t += s.pos.memoryGB() + s.interaction.deviceMemoryGB();
@@ -77,6 +81,7 @@ struct PeriodicBoundaryXY2D {
  private:
   bool x_ = false, y_ = false;
   GhostSet sets_[3];
+  DeviceBuffer<int> atBuild_;
   size_t allocatedFor_[3] = {(size_t)-1, (size_t)-1, (size_t)-1};
 };
 

@@ -120,6 +120,12 @@ struct SolidInteraction {
       }
     return out;
   }
+  // Candidate pairs of the generation written last; synchronises.
+  int lastTotal(hipStream_t s) {
+    PairBuffers& g = gen_[cur_ ^ 1];
+    if (!g.total.get()) return 0;
+    return std::min(g.total.toHost(s, 1)[0], g.capacity);
+  }
   double deviceMemoryGB() const { return gen_[0].memoryGB() + gen_[1].memoryGB(); }
 
  private:

@@ -50,6 +50,10 @@ def solver_lib():
         "fdmx_set_periodic": (I, [P, I, I]),
         "fdmx_set_cell_size": (I, [P, D]),
         "fdmx_set_capacity": (I, [P, I, I]),
+        "fdmx_set_skin": (I, [P, D]),
+        "fdmx_pair_totals": (I, [P, C.POINTER(I)]),
+        "fdmx_list_builds": (I, [P, C.POINTER(LL), C.POINTER(LL),
+                                 C.POINTER(D)]),
         "fdmx_use_graph": (I, [P, I]),
         "fdmx_set_timing": (I, [P, I]),
         "fdmx_initialize": (I, [P, _dp, _dp, _dp, D]),
@@ -158,6 +162,23 @@ class Solver:
 
     def set_capacity(self, pairs_per_particle, records_per_particle):
         self._ok(self._lib.fdmx_set_capacity(self.h, int(pairs_per_particle), int(records_per_particle)))
+
+    def set_skin(self, skin):
+        """Extra reach of the candidate lists (negative: automatic); takes
+        effect at initialize."""
+        self._ok(self._lib.fdmx_set_skin(self.h, float(skin)))
+
+    def list_builds(self):
+        """(candidate-list builds, ghost-list builds, skin in use)."""
+        b, gb, sk = C.c_longlong(), C.c_longlong(), C.c_double()
+        self._ok(self._lib.fdmx_list_builds(self.h, C.byref(b), C.byref(gb), C.byref(sk)))
+        return b.value, gb.value, sk.value
+
+    def pair_totals(self):
+        """Candidate pairs per set (particle, wall, ghost X, Y, XY) at the latest step."""
+        out = (C.c_int * 5)()
+        self._ok(self._lib.fdmx_pair_totals(self.h, out))
+        return list(out)
 
     def use_graph(self, on=True):
         self._ok(self._lib.fdmx_use_graph(self.h, int(on)))

@@ -133,31 +133,51 @@ def superellipsoid_family(count: int, seed: int, n_grid: int = 32, n_nodes: int 
     return out
 
 
-def pour(counts=(100, 100, 100), spacing: float = 3.0e-3, n_templates: int = 16,
+def pour(counts=(100, 100, 100), spacing: float = None, n_templates: int = 16,
          n_nodes: int = 1000, n_grid: int = 32, seed: int = 2024, cell_size: float = 2.5e-3,
          kn: float = 1e4, mu: float = 0.5, zeta: float = 0.3, dt: float = 5e-7,
-         steps: int = 10000, templates=None) -> Scenario:
+         steps: int = 10000, templates=None, clearance: float = 0.05,
+         rest_on_floor: bool = True) -> Scenario:
     """Pour of superellipsoids (semi-axes U[0.8, 1.2] mm, powers U[2, 8],
     16 templates, 32^3 SDF, 1000 nodes) from a jittered lattice with random
     orientations, in a box periodic in x and y with a floor wall at z = 0,
-    cell size 2.5 mm, kn = 1e4 N/m, mu = 0.5, dt = 5e-7 s. Lattice centres
-    start 0.75 spacing above the floor (jitter 5% of the spacing)."""
+    cell size 2.5 mm, kn = 1e4 N/m, ks = 0.4 kn, damping ratio 0.3, mu = 0.5,
+    dt = 5e-7 s.
+
+    The lattice spacing defaults to (1 + clearance) times the largest
+    bounding diameter and the jitter to 1% of the spacing, so no two bounding
+    spheres (hence no two level sets) overlap initially. With rest_on_floor
+    every particle of the bottom layer is lowered until its lowest surface
+    node touches the floor, so wall contacts are live from the first step;
+    the layers above fall onto it."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = counts
     tmpl = templates if templates is not None else superellipsoid_family(
         n_templates, seed + 1, n_grid=n_grid, n_nodes=n_nodes)
+    rmax = max(gen.template_radius(t) for t in tmpl)
+    if spacing is None:
+        spacing = 2.0 * rmax * (1.0 + clearance)
+    jitter = 0.01 * spacing
+    if spacing - 2.0 * np.sqrt(3.0) * jitter < 2.0 * rmax:
+        raise ValueError("lattice spacing too small for non-overlapping particles")
     n = nx * ny * nz
     Lx, Ly = nx * spacing, ny * spacing
     H = nz * spacing + 2.0 * spacing
     H = max(H, Ly)
-    jitter = 0.05 * spacing
+    z0 = rmax + jitter
     # y spans [H - Ly, H) so that max.y == max.z
-    x = gen.jittered_lattice(counts, spacing, (0.5 * spacing, H - Ly + 0.5 * spacing, 0.75 * spacing),
+    x = gen.jittered_lattice(counts, spacing, (0.5 * spacing, H - Ly + 0.5 * spacing, z0),
                              jitter, rng)
     dmin = np.array([0.0, H - Ly, 0.0])
     dmax = np.array([Lx, H, H])
     q = gen.shoemake_quaternions(rng, n)
     geo = rng.integers(0, len(tmpl), size=n).astype(np.int32)
+    if rest_on_floor:
+        bottom = np.arange(nx * ny)
+        for i in bottom:
+            R = _rotmat(q[i])
+            low = float(np.min((tmpl[geo[i]].nodes.astype(np.float64) @ R.T)[:, 2]))
+            x[i, 2] = -low
     wt, wall = floor_wall([0.0, H - Ly], [Lx, H])
     e = restitution_from_damping(zeta)
     return Scenario(

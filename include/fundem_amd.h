@@ -122,31 +122,40 @@ typedef struct {
   int* cellEnd;        /* [grid.num] */
   int* prevKey;        /* [n] sorted keys of the previous build, to clear cells */
   int* prevCount;      /* [1] entries in prevKey (-1: clear the whole table) */
-  int* validAtBuild;   /* [n] 1 if the body's hash was valid at the last list build */
+  int* validAtBuild;   /* [n] body state at the last list build (0 outside the
+                          grid, 1 inside, 2 index rounded up to the grid size);
+                          NULL when candidate lists are rebuilt every step */
 } fdm_hash_table;
 
-/* Candidate lists are built with an extra reach `skin` and reused until the
- * bodies may have moved (or, through rotation, moved a target-frame point) by
- * skin/2 since the build, or until a body enters or leaves the domain, a
- * periodic image appears or disappears, or a particle wraps. The decision is
- * taken on the device every step, so a captured step needs no host logic.
- * Control words (int, device): */
+/* Candidate-list reuse. Lists are built with an extra reach `skin` and kept
+ * while every body has moved (or, through rotation, moved a point within
+ * `rho` of its centre) by less than skin/2 since the build, no body changed
+ * grid state, no periodic image appeared or disappeared and no particle
+ * wrapped. Each step re-evaluates the exact contact predicate on the kept
+ * candidates, so results do not depend on `skin`. The decision is taken on
+ * the device every step; a captured step needs no host logic. Control words
+ * (int, device): */
 #define FDM_CTL_REBUILD 0        /* lists rebuilt this step */
 #define FDM_CTL_REBUILD_GHOST 1  /* ghost lists rebuilt this step */
 #define FDM_CTL_PENDING 2        /* a particle wrapped: rebuild at the next step */
-#define FDM_CTL_VALID_CHANGED 3  /* a body entered or left the domain this step */
+#define FDM_CTL_VALID_CHANGED 3  /* a body changed grid state this step */
 #define FDM_CTL_GHOST_CHANGED 4  /* a periodic image appeared or disappeared */
 #define FDM_CTL_WRAPPED 5        /* a particle wrapped this step */
 #define FDM_CTL_MAX_RATE 6       /* largest |v| + |w| rho this step (float bits) */
-#define FDM_CTL_FORCE 7          /* host-requested rebuild (initialisation) */
-#define FDM_CTL_COUNT 8          /* number of rebuilds so far */
+#define FDM_CTL_FORCE 7          /* host-requested rebuild */
+#define FDM_CTL_COUNT 8          /* list builds so far */
+#define FDM_CTL_GHOST_COUNT 9    /* ghost list builds so far */
+#define FDM_CTL_SEQ 10           /* decisions taken so far */
 #define FDM_CTL_WORDS 16
 typedef struct {
-  double skin;         /* extra reach of the candidate lists */
-  double rho;          /* lever arm turning spin into motion of target-frame points */
+  double skin;         /* extra reach of the candidate lists (0: rebuild every step) */
+  double rhoMax;       /* largest lever arm of any body (FP32 orientation rounding) */
   double dt;
   int* ctl;            /* [FDM_CTL_WORDS] */
   double* disp;        /* [1] displacement bound accumulated since the last build */
+  int* hostDecision;   /* optional host-visible [4]: decision number, rebuild, ghost
+                          rebuild (written with system scope, for a host that launches
+                          only the work a step needs); NULL to skip */
 } fdm_skin;
 
 /* Candidate pairs and tangential history of one interaction set. */
@@ -181,8 +190,11 @@ typedef struct {
   int* flag;           /* [n] scan workspace */
   int* realId;         /* [capacity] real particle of each ghost, ascending */
   double* pos;         /* [3*capacity] ghost positions, AoS */
-  fdm_hash_table table;
-  int* cellBox;        /* [6] min/max occupied ghost cell per axis */
+  fdm_hash_table table; /* hashValue: the reference's ghost cell id (stencil rule);
+                           sortKey: 1 + id of the clamped geometric cell (list builds) */
+  int* cellBox;        /* [6] min/max occupied sort cell per axis */
+  int* atBuild;        /* [n] per particle: visibility bits (X 1, Y 2, XY 4) at the
+                          last ghost list build; NULL when rebuilt every step */
 } fdm_ghosts;
 
 /* Bonds (VBondedInteraction, VBondedInteraction.h:61-566), FP32 constants. */
@@ -226,10 +238,28 @@ int fdm_launchParticlePositionOrientationIntegration(fdm_bodies bodies, double d
 
 /* Fused first half of a step (LSSolver::integration1stHalf, LSSolver.h:1033-1070,
  * plus calculateHashKernel, mySpatialGrid.cu:4-26): optional half-kick with
- * the stored F,T, drift, then the body's cell id and sort key/value. */
+ * the stored F,T, drift, then the body's cell id and sort key/value. With
+ * `skin` (may be NULL) it also folds the body's motion rate into
+ * ctl[FDM_CTL_MAX_RATE] and flags ctl[FDM_CTL_VALID_CHANGED] when the body's
+ * grid state differs from table.validAtBuild. */
 int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_double3 gravity, double dt,
                             int kick, int drift, fdm_grid grid, fdm_hash_table table,
-                            hipStream_t stream);
+                            const fdm_skin* skin, hipStream_t stream);
+/* No counterpart in the reference (which rebuilds every list every step).
+ * Decides, on the device, whether this step rebuilds the candidate lists:
+ * accumulates the displacement bound and sets ctl[FDM_CTL_REBUILD] and
+ * ctl[FDM_CTL_REBUILD_GHOST] (after fdm_launchPeriodicPrepare, when
+ * periodic); a wrap seen by the prepare stage also rebuilds the next step. */
+int fdm_launchSkinDecide(const fdm_skin* skin, hipStream_t stream);
+/* No counterpart in the reference. When ctl[FDM_CTL_REBUILD] is set, records
+ * every body's grid state in table.validAtBuild. */
+int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
+                              const fdm_skin* skin, hipStream_t stream);
+/* No counterpart in the reference. When *gate is 0, copies the candidate
+ * lists (count, total, pairA/T/G) of `from` into `to`, so a step without a
+ * rebuild sees the lists of the last build. */
+int fdm_launchCarryPairs(const fdm_pairs* from, const fdm_pairs* to, int n, const int* gate,
+                         hipStream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Spatial hash                                                           */
@@ -248,6 +278,9 @@ int fdm_radixSortPairs(unsigned* keys, int* values, size_t n, int keyBits, void*
  * ranges; cells of the previous build are reset first. */
 int fdm_buildHashStartEnd(fdm_hash_table table, int n, fdm_grid grid, void* workspace,
                           size_t workspaceBytes, hipStream_t stream);
+/* The same, as a no-op unless *gate is nonzero (gate may be NULL: always). */
+int fdm_buildHashStartEndGated(fdm_hash_table table, int n, fdm_grid grid, const int* gate,
+                               void* workspace, size_t workspaceBytes, hipStream_t stream);
 
 /* Replaces launchUpdateSpatialGridHashStartEnd (myUtility/mySpatialGrid.h:33):
  * cell id of every body (calculateHashKernel) then buildHashStartEnd. */
@@ -284,6 +317,20 @@ int fdm_launchUpdateGhostSpatialGridHashStartEnd(fdm_hash_table table, const dou
 int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
                              fdm_ghosts* sets /* [3]: X, Y, XY */, unsigned* status,
                              void* workspace, size_t workspaceBytes, hipStream_t stream);
+/* The same stage split for candidate-list reuse (no counterpart in the
+ * reference). Prepare: visibility flags, flagging ctl[FDM_CTL_WRAPPED] and
+ * ctl[FDM_CTL_GHOST_CHANGED] against sets[].atBuild; it runs before
+ * fdm_launchSkinDecide. Build: the ghost sets from those flags (as
+ * fdm_launchPeriodicGhosts), recording the visibility bits; it runs when
+ * *gate != 0 (gate NULL: always). Refresh: positions and cell ids of the
+ * kept ghosts; it runs when *gate == 0 (gate NULL: always). */
+int fdm_launchPeriodicPrepare(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                              fdm_ghosts* sets, const fdm_skin* skin, hipStream_t stream);
+int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                            fdm_ghosts* sets, const int* gate, unsigned* status, void* workspace,
+                            size_t workspaceBytes, hipStream_t stream);
+int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                              fdm_ghosts* sets, const int* gate, hipStream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Contact detection and contact forces                                   */
@@ -309,6 +356,10 @@ typedef struct {
   fdm_grid grid;           /* grid of the targets */
   fdm_hash_table table;    /* hash table of the targets */
   fdm_pairs pairs;
+  double skin;             /* extra reach (candidate-list reuse), 0 for exact lists */
+  double maxReach;         /* largest target reach: particle radius (set 0) or
+                              SDF box reach (set 2) over all templates */
+  const int* gate;         /* run only when *gate != 0 (NULL: always) */
 } fdm_broadphase_args;
 int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* workspace, size_t workspaceBytes,
                          hipStream_t stream);

@@ -18,9 +18,11 @@ pytestmark = pytest.mark.gpu
 SETS = (po.SET_PP, po.SET_PW, po.SET_GX, po.SET_GY, po.SET_GXY)
 
 
-def make_pair(scn, mirror=True):
+def make_pair(scn, mirror=True, skin=None):
     from fundem_amd import Solver
     g = Solver()
+    if skin is not None:
+        g.set_skin(skin)
     W.apply(scn, g)
     o = po.OracleSim(mirror, mirror)
     W.apply(scn, o)
@@ -45,8 +47,8 @@ def assert_contacts_equal(g, o, where, sets=SETS):
         assert np.array_equal(gc["spring"], oc["spring"]), f"{where}: set {s} springs differ"
 
 
-def run_bitexact(scn, steps, every):
-    g, o = make_pair(scn)
+def run_bitexact(scn, steps, every, skin=None):
+    g, o = make_pair(scn, skin=skin)
     assert_states_equal(g.state(), o.state(), "after initialise")
     total = 0
     for i in range(1, steps + 1):
@@ -57,6 +59,10 @@ def run_bitexact(scn, steps, every):
             assert_contacts_equal(g, o, f"step {i}")
             total += sum(o.contacts(s)["A"].size for s in SETS)
     assert g.status() == 0
+    builds, ghost_builds, used = g.list_builds()
+    print(f"skin {used:.3e}: {builds} list builds, {ghost_builds} ghost-list builds in {steps} steps")
+    if used == 0.0:
+        assert builds == steps + 1
     return total
 
 
@@ -77,8 +83,37 @@ def test_cluster_bitexact(name):
     assert contacts > 0, "scenario produced no contacts"
 
 
+@pytest.mark.parametrize("name,skin", [("mm-periodic-xy", 0.0), ("mm-periodic-xy", 4e-4),
+                                       ("cm-rotating-periodic", 0.0), ("cm-rotating-periodic", 3e-3),
+                                       ("mm", 1e-3)])
+def test_cluster_bitexact_any_skin(name, skin):
+    """Candidate lists rebuilt every step (skin 0) or kept across steps (a
+    skin of 0.4 to 1 particle radius) give the oracle's bits."""
+    scn = W.cluster(n=27, speed=1.0, **CLUSTERS[name])
+    contacts = run_bitexact(scn, steps=400, every=25, skin=skin)
+    assert contacts > 0, "scenario produced no contacts"
+
+
+def test_list_reuse_slow_pour_bitexact():
+    """A slow, settling pour (periodic, wall, many steps): lists are reused
+    for many steps, particles wrap and ghosts come and go, and every checked
+    step still equals the oracle."""
+    scn = W.cluster(n=64, speed=0.3, spin=2.0, periodic=(True, True), seed=11, dt=2e-6)
+    g, o = make_pair(scn, skin=3e-4)
+    for i in range(1, 1201):
+        g.step(1)
+        o.step(1)
+        if i % 100 == 0:
+            assert_states_equal(g.state(), o.state(), f"step {i}")
+            assert_contacts_equal(g, o, f"step {i}")
+    builds, ghost_builds, _ = g.list_builds()
+    print(f"{builds} list builds, {ghost_builds} ghost-list builds in 1200 steps")
+    assert builds < 600, "lists were not reused"
+    assert g.status() == 0
+
+
 def test_hash_table_matches_reference_sort():
-    scn = W.pour((16, 16, 8), spacing=3.0e-3, n_templates=2, n_nodes=64, n_grid=16)
+    scn = W.pour((16, 16, 8), n_templates=2, n_nodes=64, n_grid=16)
     scn.periodic = (False, False)  # no wrap: stored positions are the hashed ones
     # a few particles outside the domain on every side hash to -1
     scn.x[:6] = scn.domain_min - 1e-3 + np.array(
