@@ -33,6 +33,8 @@
 // All culls are conservative with a relative margin of 1e-5 on reaches (a
 // unit quaternion stored in FP32 scales vectors by up to 1 +- 2.4e-7), so
 // they never remove a contact that the exact test would accept.
+#include <algorithm>
+
 #include "fdm_device.hpp"
 #include "fdm_launch.hpp"
 
@@ -256,24 +258,16 @@ __global__ void kBroadWrite(BroadK a) {
 }
 
 // -------------------------------------------------------------- narrow phase
-struct NarrowK {
+// Fields shared by every interaction set of a step.
+struct NarrowCommon {
   fdm_bodies P;
   fdm_geometry pgeo;
   const fdm_material* pmat;
   fdm_bodies W;
   fdm_geometry wgeo;
   const fdm_material* wmat;
-  const int* ghostReal;
-  const double* ghostPos;
-  const int* ghostHash;
-  fdm_grid grid;
-  const int* targetHash;
   fdm_grid pgrid;
   fdm_periodic periodic;
-  fdm_pairs pairs;
-  fdm_pairs prev;
-  const int* prevStart;
-  fdm_mirror mirror;
   double dt;
   int updateSprings;
   unsigned* status;
@@ -282,9 +276,35 @@ struct NarrowK {
   double* dbgOverlap;
 };
 
+// One interaction set.
+struct NarrowSet {
+  int kind;           // FDM_SET_*
+  int ghostMask;      // ghost sets: 1 X, 2 Y, 3 XY
+  int ghostFromReal;  // ghost sets: derive positions from the real particles
+  const double* ghostPos;
+  const int* ghostHash;
+  fdm_grid grid;
+  const int* targetHash;
+  fdm_pairs pairs;
+  fdm_pairs prev;
+  fdm_mirror mirror;
+};
+
+struct NarrowK {
+  NarrowCommon c;
+  NarrowSet s;
+};
+
+constexpr int kMaxSets = 5;
+struct NarrowMulti {
+  NarrowCommon c;
+  int nsets;
+  NarrowSet s[kMaxSets];
+};
+
 // Position used for force arms: the stored position after this step's
 // periodic wrap (the reference computes forces after updateGhostSpatialGrid).
-__device__ __forceinline__ V3 wrappedArm(V3 x, const NarrowK& a) {
+__device__ __forceinline__ V3 wrappedArm(V3 x, const NarrowCommon& a) {
   if (a.periodic.x) {
     if (x.x < a.pgrid.min.x) x = vadd(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
     else if (x.x >= a.pgrid.max.x) x = vsub(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
@@ -311,13 +331,13 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
   return ov >= 0.;
 }
 
-__device__ __forceinline__ int findPrevPair(const NarrowK& a, int p, int A, int T, int& off) {
+__device__ __forceinline__ int findPrevPair(const NarrowSet& a, int p, int A, int T, int& off) {
   // lists kept from the previous step hold the pair at the same index
   if (p < min(*a.prev.total, a.prev.capacity) && a.prev.pairA[p] == A && a.prev.pairT[p] == T) {
     off = a.prev.recOff[p];
     return a.prev.recCnt[p];
   }
-  int lo = min(a.prevStart[A], a.prev.capacity), hi = min(a.prevStart[A + 1], a.prev.capacity) - 1;
+  int lo = min(a.prev.count[A], a.prev.capacity), hi = min(a.prev.count[A + 1], a.prev.capacity) - 1;
   while (lo <= hi) {
     const int mid = (lo + hi) >> 1;
     const int t = a.prev.pairT[mid];
@@ -332,7 +352,7 @@ __device__ __forceinline__ int findPrevPair(const NarrowK& a, int p, int A, int 
   return 0;
 }
 
-__device__ __forceinline__ V3 findPrevSpring(const NarrowK& a, int off, int cnt, int k) {
+__device__ __forceinline__ V3 findPrevSpring(const NarrowSet& a, int off, int cnt, int k) {
   int lo = off, hi = off + cnt - 1;
   while (lo <= hi) {
     const int mid = (lo + hi) >> 1;
@@ -345,52 +365,86 @@ __device__ __forceinline__ V3 findPrevSpring(const NarrowK& a, int off, int cnt,
   return {0., 0., 0.};
 }
 
+// The target of pair p as this step sees it: position, orientation, template,
+// SDF, cell id, and the position its force arm uses.
 template <int SET>
-__device__ void processPair(const NarrowK& a, int p, int lane) {
-  const int A = a.pairs.pairA[p];
+__device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const NarrowSet& S, int p, int T,
+                                              Target& tg, V3& armB) {
+  const int n = C.P.n;
+  if (SET == FDM_SET_WALL) {
+    tg.x = ldp(C.W.x, C.W.n, T);
+    tg.q = ld4(C.W.q, C.W.n, T);
+    tg.t = &tmplOf(C.wgeo, C.W.geo[T]);
+    tg.sdf = C.wgeo.sdf;
+    tg.hash = S.targetHash[T];
+    armB = tg.x;
+  } else if (SET == FDM_SET_PARTICLE) {
+    tg.x = ldp(C.P.x, n, T);
+    tg.q = ld4(C.P.q, n, T);
+    tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
+    tg.sdf = C.pgeo.sdf;
+    tg.hash = S.targetHash[T];
+    armB = wrappedArm(tg.x, C);
+  } else {
+    if (S.ghostFromReal) {
+      // no wrap happened this step: the ghost is the real particle shifted
+      // by the periods, as the ghost stage computes it
+      const WrapStages w = wrapStages(ldp(C.P.x, n, T), C.pgrid, C.periodic);
+      tg.x = ghostOf(S.ghostMask == 1 ? w.afterX : w.afterY, C.pgrid, S.ghostMask);
+      tg.hash = ghostCellHash(tg.x, C.pgrid);
+    } else {
+      const int G = S.pairs.pairG[p];
+      tg.x = ldd3(S.ghostPos + 3 * G);
+      tg.hash = S.ghostHash[G];
+    }
+    tg.q = ld4(C.P.q, n, T);
+    tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
+    tg.sdf = C.pgeo.sdf;
+    armB = tg.x;
+  }
+}
+
+// Contacts staged per wave in LDS between the node sweep and the contact law.
+constexpr int kMaxHits = 64;
+struct HitSlots {
+  double pc[3][kMaxHits];
+  double nrm[3][kMaxHits];
+  double ov[kMaxHits];
+  int node[kMaxHits];
+};
+
+// One wave per candidate pair: chunk culls (one chunk per lane), the node
+// sweep over surviving chunks (one node per lane) evaluating the reference's
+// predicate, the contact geometry of every hit staged in LDS, then the
+// contact law with one hit per lane and the pair sums folded in node order.
+// A pair with more than kMaxHits contacts recomputes its hits chunk by chunk
+// instead of staging them.
+template <int SET>
+__device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, int lane,
+                           HitSlots& hs) {
+  const int A = S.pairs.pairA[p];
   if (A < 0) {  // slot left empty by a capacity overflow
     if (lane == 0) {
-      a.pairs.recOff[p] = 0;
-      a.pairs.recCnt[p] = 0;
+      S.pairs.recOff[p] = 0;
+      S.pairs.recCnt[p] = 0;
     }
     return;
   }
-  const int T = a.pairs.pairT[p];
-  const int n = a.P.n;
-  const fdm_template& tA = tmplOf(a.pgeo, a.P.geo[A]);
-  const V3 xA = ldp(a.P.x, n, A);
-  const Q4 qA = ld4(a.P.q, n, A);
+  const int T = S.pairs.pairT[p];
+  const int n = C.P.n;
+  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const V3 xA = ldp(C.P.x, n, A);
+  const Q4 qA = ld4(C.P.q, n, A);
   Target tg;
   V3 armB;
-  if (SET == FDM_SET_WALL) {
-    tg.x = ldp(a.W.x, a.W.n, T);
-    tg.q = ld4(a.W.q, a.W.n, T);
-    tg.t = &tmplOf(a.wgeo, a.W.geo[T]);
-    tg.sdf = a.wgeo.sdf;
-    tg.hash = a.targetHash[T];
-    armB = tg.x;
-  } else if (SET == FDM_SET_PARTICLE) {
-    tg.x = ldp(a.P.x, n, T);
-    tg.q = ld4(a.P.q, n, T);
-    tg.t = &tmplOf(a.pgeo, a.P.geo[T]);
-    tg.sdf = a.pgeo.sdf;
-    tg.hash = a.targetHash[T];
-    armB = wrappedArm(tg.x, a);
-  } else {
-    const int G = a.pairs.pairG[p];
-    tg.x = ldd3(a.ghostPos + 3 * G);
-    tg.q = ld4(a.P.q, n, T);
-    tg.t = &tmplOf(a.pgeo, a.P.geo[T]);
-    tg.sdf = a.pgeo.sdf;
-    tg.hash = a.ghostHash[G];
-    armB = tg.x;
-  }
-  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, a);
+  resolveTarget<SET>(C, S, p, T, tg, armB);
+  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, C);
   const double RB = tg.t->radius;
-  const I3 tcell = decodeHash(tg.hash, a.grid);
-  const float* nodes = a.pgeo.nodes + 3 * (size_t)tA.nodeStart;
-  const float4* chunks = reinterpret_cast<const float4*>(a.pgeo.chunks) + tA.chunkStart;
+  const I3 tcell = decodeHash(tg.hash, S.grid);
+  const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
+  const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
   const int nChunks = tA.chunkCount;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
   // chunk culls, one chunk per lane: a chunk survives unless its bounding
   // sphere fails the sphere, stencil, SDF-box or Lipschitz bound
@@ -406,7 +460,7 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
     }
     if (live) {
       I3 lo, hi;
-      cellRange(cw, rc, a.grid, lo, hi);
+      cellRange(cw, rc, S.grid, lo, hi);
       if (!cellNear(tcell, lo, hi)) live = false;
     }
     if (live) {
@@ -416,30 +470,48 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
   }
   const unsigned long long liveChunks = ballot64(live);
 
-  // pass 1: contact mask of every surviving chunk (lane c keeps chunk c's mask)
+  // node sweep: lane c keeps chunk c's hit mask; hits in node order get LDS
+  // slots holding their contact point, normal, overlap and node id
   unsigned long long myMask = 0ull;
+  int total = 0;
   for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
     const int c = __builtin_ctzll(lcm);
     const int k = c * FDM_CHUNK + lane;
     bool hit = false;
+    Sample sm;
+    V3 pn;
     if (k < tA.nodeCount) {
       const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
-      Sample sm;
-      hit = nodeHit<SET>(vadd(qrotate(qA, s), xA), tg, tcell, a.grid, RB, sm);
+      pn = vadd(qrotate(qA, s), xA);
+      hit = nodeHit<SET>(pn, tg, tcell, S.grid, RB, sm);
     }
     const unsigned long long m = ballot64(hit);
     if (lane == c) myMask = m;
+    if (hit) {
+      const int slot = total + __popcll(m & lt);
+      if (slot < kMaxHits) {
+        const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+        hs.pc[0][slot] = pc.x;
+        hs.pc[1][slot] = pc.y;
+        hs.pc[2][slot] = pc.z;
+        hs.nrm[0][slot] = nrm.x;
+        hs.nrm[1][slot] = nrm.y;
+        hs.nrm[2][slot] = nrm.z;
+        hs.ov[slot] = ov;
+        hs.node[slot] = k;
+      }
+    }
+    total += __popcll(m);
   }
-  int total = __popcll(myMask);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
 
   int base = 0;
   if (total > 0) {
-    if (lane == 0) base = atomicAdd(a.pairs.recTotal, total);
+    if (lane == 0) base = atomicAdd(S.pairs.recTotal, total);
     base = readlaneI(base, 0);
-    if (base + total > a.pairs.recCapacity) {
-      if (lane == 0) atomicOr(a.status, FDM_STATUS_RECORD_OVERFLOW);
+    if (base + total > S.pairs.recCapacity) {
+      if (lane == 0) atomicOr(C.status, FDM_STATUS_RECORD_OVERFLOW);
       total = 0;
     }
   }
@@ -447,109 +519,214 @@ __device__ void processPair(const NarrowK& a, int p, int lane) {
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   if (total > 0) {
     int prevOff = 0;
-    const int prevCnt = findPrevPair(a, p, A, T, prevOff);
+    const int prevCnt = findPrevPair(S, p, A, T, prevOff);
     // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
-    const int mi = a.P.mat[A];
-    const fdm_material& Mi = a.pmat[mi];
+    const int mi = C.P.mat[A];
+    const fdm_material& Mi = C.pmat[mi];
     const double invMi = tA.invMass;
     double kn = 0., ks = 0., effM = 0., mu, diss;
     V3 vB, wB;
     if (SET == FDM_SET_WALL) {
-      const fdm_material& Mw = a.wmat[a.W.mat[T]];
+      const fdm_material& Mw = C.wmat[C.W.mat[T]];
       if (invMi > 0.) effM = 1. / invMi;
       mu = fmin(Mi.mu, Mw.mu);
       diss = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
       kn = Mi.kn;
       ks = Mi.ks;
-      vB = ld3(a.W.v, a.W.n, T);
-      wB = ld3(a.W.w, a.W.n, T);
+      vB = ld3(C.W.v, C.W.n, T);
+      wB = ld3(C.W.w, C.W.n, T);
     } else {
-      const fdm_material& Mj = a.pmat[a.P.mat[T]];
+      const fdm_material& Mj = C.pmat[C.P.mat[T]];
       const double invMj = tg.t->invMass;
       if (Mi.kn > 0. && Mj.kn > 0.) kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
       if (Mi.ks > 0. && Mj.ks > 0.) ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
       if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
       diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
       mu = fmin(Mi.mu, Mj.mu);
-      vB = ld3(a.P.v, n, T);
-      wB = ld3(a.P.w, n, T);
+      vB = ld3(C.P.v, n, T);
+      wB = ld3(C.P.w, n, T);
     }
-    const V3 vA = ld3(a.P.v, n, A);
-    const V3 wA = ld3(a.P.w, n, A);
-    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const unsigned long long chunksWith = ballot64(myMask != 0ull);
-    int running = 0;
-    for (unsigned long long cw = chunksWith; cw; cw &= cw - 1ull) {
-      const int c = __builtin_ctzll(cw);
-      const unsigned lo32 = __builtin_amdgcn_readlane((unsigned)myMask, c);
-      const unsigned hi32 = __builtin_amdgcn_readlane((unsigned)(myMask >> 32), c);
-      const unsigned long long m = ((unsigned long long)hi32 << 32) | lo32;
-      const bool hit = (m >> lane) & 1ull;
-      const int k = c * FDM_CHUNK + lane;
+    const V3 vA = ld3(C.P.v, n, A);
+    const V3 wA = ld3(C.P.w, n, A);
+    // contact law of one contact: its spring is updated in place and stored
+    // with its node id as record r
+    auto law = [&](V3 pc, V3 nrm, double ov, int k, int r, V3& F, V3& TA, V3& TB) {
+      V3 spring = findPrevSpring(S, prevOff, prevCnt, k);
+      if (C.updateSprings) {
+        const V3 vc = vsub(vadd(vA, vcross(wA, vsub(pc, armA))), vadd(vB, vcross(wB, vsub(pc, armB))));
+        linearContact(F, spring, vc, nrm, ov, C.dt, kn, ks, mu, diss, effM);
+      }
+      TA = vcross(vsub(pc, armA), F);
+      TB = vcross(vsub(pc, armB), vneg(F));
+      S.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z,
+                                   __int_as_float(k));
+      if (C.dbgPoint) {
+        C.dbgPoint[3 * r] = pc.x;
+        C.dbgPoint[3 * r + 1] = pc.y;
+        C.dbgPoint[3 * r + 2] = pc.z;
+        C.dbgNormal[3 * r] = nrm.x;
+        C.dbgNormal[3 * r + 1] = nrm.y;
+        C.dbgNormal[3 * r + 2] = nrm.z;
+        C.dbgOverlap[r] = ov;
+      }
+    };
+    if (total <= kMaxHits) {
+      // one staged contact per lane (the slots were written by other lanes)
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       V3 F = {0., 0., 0.}, TA = {0., 0., 0.}, TB = {0., 0., 0.};
-      if (hit) {
-        const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
-        const V3 pn = vadd(qrotate(qA, s), xA);
-        Sample sm;
-        nodeHit<SET>(pn, tg, tcell, a.grid, RB, sm);
-        const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
-        const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
-        V3 spring = findPrevSpring(a, prevOff, prevCnt, k);
-        if (a.updateSprings) {
-          const V3 vc = vsub(vadd(vA, vcross(wA, vsub(pc, armA))), vadd(vB, vcross(wB, vsub(pc, armB))));
-          linearContact(F, spring, vc, nrm, ov, a.dt, kn, ks, mu, diss, effM);
-        }
-        TA = vcross(vsub(pc, armA), F);
-        TB = vcross(vsub(pc, armB), vneg(F));
-        const int r = base + running + __popcll(m & lt);
-        a.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z,
-                                     __int_as_float(k));
-        if (a.dbgPoint) {
-          a.dbgPoint[3 * r] = pc.x;
-          a.dbgPoint[3 * r + 1] = pc.y;
-          a.dbgPoint[3 * r + 2] = pc.z;
-          a.dbgNormal[3 * r] = nrm.x;
-          a.dbgNormal[3 * r + 1] = nrm.y;
-          a.dbgNormal[3 * r + 2] = nrm.z;
-          a.dbgOverlap[r] = ov;
-        }
+      if (lane < total) {
+        const V3 pc = {hs.pc[0][lane], hs.pc[1][lane], hs.pc[2][lane]};
+        const V3 nrm = {hs.nrm[0][lane], hs.nrm[1][lane], hs.nrm[2][lane]};
+        law(pc, nrm, hs.ov[lane], hs.node[lane], base + lane, F, TA, TB);
       }
       // pair sums, serially in node order
-      for (unsigned long long mm = m; mm; mm &= mm - 1ull) {
-        const int b = __builtin_ctzll(mm);
+      for (int b = 0; b < total; ++b) {
         SF = vadd(SF, {readlaneD(F.x, b), readlaneD(F.y, b), readlaneD(F.z, b)});
         STA = vadd(STA, {readlaneD(TA.x, b), readlaneD(TA.y, b), readlaneD(TA.z, b)});
         STB = vadd(STB, {readlaneD(TB.x, b), readlaneD(TB.y, b), readlaneD(TB.z, b)});
       }
-      running += __popcll(m);
+    } else {
+      // more contacts than slots: recompute each chunk's hits
+      const unsigned long long chunksWith = ballot64(myMask != 0ull);
+      int running = 0;
+      for (unsigned long long cw = chunksWith; cw; cw &= cw - 1ull) {
+        const int c = __builtin_ctzll(cw);
+        const unsigned lo32 = __builtin_amdgcn_readlane((unsigned)myMask, c);
+        const unsigned hi32 = __builtin_amdgcn_readlane((unsigned)(myMask >> 32), c);
+        const unsigned long long m = ((unsigned long long)hi32 << 32) | lo32;
+        const bool hit = (m >> lane) & 1ull;
+        const int k = c * FDM_CHUNK + lane;
+        V3 F = {0., 0., 0.}, TA = {0., 0., 0.}, TB = {0., 0., 0.};
+        if (hit) {
+          const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+          const V3 pn = vadd(qrotate(qA, s), xA);
+          Sample sm;
+          nodeHit<SET>(pn, tg, tcell, S.grid, RB, sm);
+          const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+          const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+          const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+          law(pc, nrm, ov, k, base + running + __popcll(m & lt), F, TA, TB);
+        }
+        for (unsigned long long mm = m; mm; mm &= mm - 1ull) {
+          const int b = __builtin_ctzll(mm);
+          SF = vadd(SF, {readlaneD(F.x, b), readlaneD(F.y, b), readlaneD(F.z, b)});
+          STA = vadd(STA, {readlaneD(TA.x, b), readlaneD(TA.y, b), readlaneD(TA.z, b)});
+          STB = vadd(STB, {readlaneD(TB.x, b), readlaneD(TB.y, b), readlaneD(TB.z, b)});
+        }
+        running += __popcll(m);
+      }
     }
   }
   if (lane == 0) {
-    a.pairs.recOff[p] = base;
-    a.pairs.recCnt[p] = total;
-    double* f = a.pairs.sumF + 3 * (size_t)p;
-    double* ta = a.pairs.sumTA + 3 * (size_t)p;
-    double* tb = a.pairs.sumTB + 3 * (size_t)p;
+    S.pairs.recOff[p] = base;
+    S.pairs.recCnt[p] = total;
+    double* f = S.pairs.sumF + 3 * (size_t)p;
+    double* ta = S.pairs.sumTA + 3 * (size_t)p;
+    double* tb = S.pairs.sumTB + 3 * (size_t)p;
     f[0] = SF.x; f[1] = SF.y; f[2] = SF.z;
     ta[0] = STA.x; ta[1] = STA.y; ta[2] = STA.z;
     tb[0] = STB.x; tb[1] = STB.y; tb[2] = STB.z;
     if (SET != FDM_SET_WALL && total > 0 && T != A) {
-      const int pos = atomicAdd(&a.mirror.count[T], 1);
-      if (pos < a.mirror.capacity) a.mirror.list[(size_t)T * a.mirror.capacity + pos] = p;
-      else atomicOr(a.status, FDM_STATUS_BUCKET_OVERFLOW);
+      const int pos = atomicAdd(&S.mirror.count[T], 1);
+      if (pos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + pos] = p;
+      else atomicOr(C.status, FDM_STATUS_BUCKET_OVERFLOW);
     }
   }
 }
 
 template <int SET>
 __global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
+  __shared__ HitSlots slots[4];
   const int lane = laneId();
   const int wpb = blockDim.x >> 6;
-  const int total = min(*a.pairs.total, a.pairs.capacity);
+  const int total = min(*a.s.pairs.total, a.s.pairs.capacity);
   const int stride = gridDim.x * wpb;
+  HitSlots& hs = slots[threadIdx.x >> 6];
   for (int p = blockIdx.x * wpb + (threadIdx.x >> 6); p < total; p += stride)
-    processPair<SET>(a, p, lane);
+    processPair<SET>(a.c, a.s, p, lane, hs);
+}
+
+// Pair-level cull of every set's candidates at this step's positions (one
+// thread per pair): the pair is kept only if a node of A can still reach
+// the target (the broad-phase tests without the skin). Culled pairs get
+// empty record ranges; kept pairs go to the work list as (p << 3 | set).
+__global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* workCount) {
+  int start[kMaxSets + 1];
+  start[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxSets; ++k)
+    start[k + 1] = start[k] + (k < a.nsets ? min(*a.s[k].pairs.total, a.s[k].pairs.capacity) : 0);
+  const int lane = laneId();
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int stride = gridDim.x * blockDim.x;
+  // whole waves iterate together so the append below stays wave-uniform
+  for (int g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < start[kMaxSets]; g0 += stride) {
+    const int g = g0 + lane;
+    bool keep = false;
+    int code = 0;
+    if (g < start[kMaxSets]) {
+      int k = 0;
+      while (g >= start[k + 1]) ++k;
+      const NarrowSet& S = a.s[k];
+      const int p = g - start[k];
+      code = (p << 3) | k;
+      const int A = S.pairs.pairA[p];
+      if (A >= 0) {
+        const int T = S.pairs.pairT[p];
+        const V3 xA = ldp(a.c.P.x, a.c.P.n, A);
+        const double RA = tmplOf(a.c.pgeo, a.c.P.geo[A]).radius * kReach;
+        Target tg;
+        V3 armB;
+        double reachT;
+        if (S.kind == FDM_SET_WALL) {
+          resolveTarget<FDM_SET_WALL>(a.c, S, p, T, tg, armB);
+          reachT = tg.t->boxReach;
+        } else if (S.kind == FDM_SET_PARTICLE) {
+          resolveTarget<FDM_SET_PARTICLE>(a.c, S, p, T, tg, armB);
+          reachT = tg.t->radius;
+        } else {
+          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, tg, armB);
+          reachT = tg.t->boxReach;
+        }
+        const V3 d = vsub(xA, tg.x);
+        const double rs = RA + reachT * kReach;
+        if (vlen2(d) <= rs * rs) {
+          const V3 local = qrotateT(d, tg.q);
+          keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
+        }
+      }
+      if (!keep) {
+        S.pairs.recOff[p] = 0;
+        S.pairs.recCnt[p] = 0;
+      }
+    }
+    const unsigned long long m = ballot64(keep);
+    if (m) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(workCount, __popcll(m));
+      base = readlaneI(base, 0);
+      if (keep) work[base + __popcll(m & lt)] = code;
+    }
+  }
+}
+
+// One wave per kept pair of the work list.
+__global__ void __launch_bounds__(256) kNarrowWork(NarrowMulti a, const int* work,
+                                                   const int* workCount) {
+  __shared__ HitSlots slots[4];
+  const int lane = laneId();
+  const int wpb = blockDim.x >> 6;
+  HitSlots& hs = slots[threadIdx.x >> 6];
+  const int total = *workCount;
+  const int stride = gridDim.x * wpb;
+  for (int w = blockIdx.x * wpb + (threadIdx.x >> 6); w < total; w += stride) {
+    const int code = work[w];
+    const NarrowSet& s = a.s[code & 7];
+    const int p = code >> 3;
+    if (s.kind == FDM_SET_PARTICLE) processPair<FDM_SET_PARTICLE>(a.c, s, p, lane, hs);
+    else if (s.kind == FDM_SET_WALL) processPair<FDM_SET_WALL>(a.c, s, p, lane, hs);
+    else processPair<FDM_SET_GHOST>(a.c, s, p, lane, hs);
+  }
 }
 
 __global__ void kResetCounter(int* c) {
@@ -650,34 +827,55 @@ extern "C" int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsign
   return launchStatus();
 }
 
-extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream) {
-  if (!args) return FDM_EINVAL;
-  const fdm_narrowphase_args& g = *args;
-  if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
-  NarrowK k;
+namespace {
+
+void fillCommon(NarrowCommon& k, const fdm_narrowphase_args& g) {
   k.P = g.particles;
   k.pgeo = g.pgeo;
   k.pmat = g.pmat;
   k.W = g.walls;
   k.wgeo = g.wgeo;
   k.wmat = g.wmat;
-  k.ghostReal = g.ghosts ? g.ghosts->realId : nullptr;
-  k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
-  k.ghostHash = g.ghosts ? g.ghosts->table.hashValue : nullptr;
-  k.grid = g.grid;
-  k.targetHash = g.targetHash;
   k.pgrid = g.pgrid;
   k.periodic = g.periodic;
-  k.pairs = g.pairs;
-  k.prev = g.prev;
-  k.prevStart = g.prevStart;
-  k.mirror = g.mirror;
   k.dt = g.dt;
   k.updateSprings = g.updateSprings;
   k.status = g.status;
   k.dbgPoint = g.dbgPoint;
   k.dbgNormal = g.dbgNormal;
   k.dbgOverlap = g.dbgOverlap;
+}
+
+void fillSet(NarrowSet& k, const fdm_narrowphase_args& g) {
+  k.kind = g.set;
+  k.ghostMask = g.ghostMask;
+  k.ghostFromReal = g.ghostFromReal;
+  k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
+  k.ghostHash = g.ghosts ? g.ghosts->table.hashValue : nullptr;
+  k.grid = g.grid;
+  k.targetHash = g.targetHash;
+  k.pairs = g.pairs;
+  k.prev = g.prev;
+  k.mirror = g.mirror;
+}
+
+bool validSet(const fdm_narrowphase_args& g) {
+  if (g.set < FDM_SET_PARTICLE || g.set > FDM_SET_GHOST) return false;
+  if (g.set == FDM_SET_GHOST) {
+    if (g.ghostMask < 1 || g.ghostMask > 3) return false;
+    if (!g.ghostFromReal && !g.ghosts) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream) {
+  if (!args || !validSet(*args)) return FDM_EINVAL;
+  const fdm_narrowphase_args& g = *args;
+  NarrowK k;
+  fillCommon(k.c, g);
+  fillSet(k.s, g);
   hipLaunchKernelGGL(kResetCounter, dim3(1), dim3(64), 0, stream, g.pairs.recTotal);
   const int blocks = persistentBlocks(4);
   switch (g.set) {
@@ -687,11 +885,31 @@ extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream
     case FDM_SET_WALL:
       hipLaunchKernelGGL(kNarrow<FDM_SET_WALL>, dim3(blocks), dim3(256), 0, stream, k);
       break;
-    case FDM_SET_GHOST:
+    default:
       hipLaunchKernelGGL(kNarrow<FDM_SET_GHOST>, dim3(blocks), dim3(256), 0, stream, k);
       break;
-    default:
-      return FDM_EINVAL;
   }
+  return launchStatus();
+}
+
+extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets,
+                                         int* workList, int workCapacity, int* workCount,
+                                         hipStream_t stream) {
+  if (!sets || nsets < 1 || nsets > kMaxSets || !workList || !workCount) return FDM_EINVAL;
+  NarrowMulti k;
+  fillCommon(k.c, sets[0]);
+  k.nsets = nsets;
+  long long cap = 0;
+  for (int i = 0; i < nsets; ++i) {
+    if (!validSet(sets[i])) return FDM_EINVAL;
+    fillSet(k.s[i], sets[i]);
+    cap += sets[i].pairs.capacity;
+  }
+  if (cap > workCapacity || cap >= (1ll << 28)) return FDM_EINVAL;
+  for (int i = nsets; i < kMaxSets; ++i) k.s[i] = k.s[0];
+  const unsigned cullBlocks = std::max<unsigned>(1, std::min<unsigned>(blocksFor(cap, 256), 2048));
+  hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, workList, workCount);
+  hipLaunchKernelGGL(kNarrowWork, dim3(persistentBlocks(4)), dim3(256), 0, stream, k,
+                     (const int*)workList, (const int*)workCount);
   return launchStatus();
 }

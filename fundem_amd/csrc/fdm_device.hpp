@@ -170,6 +170,41 @@ __device__ __forceinline__ int ghostCellHash(V3 p, const fdm_grid& g) {
   else if (p.z >= g.max.y) { c.z = g.size.z - 1; out = true; }
   return out ? lin3(c, gsize(g)) : -1;
 }
+__device__ __forceinline__ V3 offsetX(const fdm_grid& g) { return {g.max.x - g.min.x, 0., 0.}; }
+__device__ __forceinline__ V3 offsetY(const fdm_grid& g) { return {0., g.max.y - g.min.y, 0.}; }
+
+// updatePositionOutOfBoundaryXD/YD (mySpatialGrid.cu:28-56)
+__device__ __forceinline__ V3 wrapAxis(V3 p, const fdm_grid& g, int axis) {
+  if (axis == 0) {
+    if (p.x < g.min.x) p = vadd(p, offsetX(g));
+    else if (p.x >= g.max.x) p = vsub(p, offsetX(g));
+  } else {
+    if (p.y < g.min.y) p = vadd(p, offsetY(g));
+    else if (p.y >= g.max.y) p = vsub(p, offsetY(g));
+  }
+  return p;
+}
+
+// calculateGhostPosition{X,Y,XY}DKernel (mySpatialGrid.cu:87-152)
+__device__ __forceinline__ V3 ghostOf(V3 p, const fdm_grid& g, int mask) {
+  const I3 c = gridPos(p, g);
+  if (mask == 1) return c.x == 0 ? vadd(p, offsetX(g)) : p;
+  if (mask == 2) return c.y == 0 ? vadd(p, offsetY(g)) : p;
+  return (c.x == 0 && c.y == 0) ? vadd(p, vadd(offsetX(g), offsetY(g))) : p;
+}
+
+// The positions the reference holds at each point of updateGhostSpatialGrid:
+// after the X wrap (ghost X is taken here), after the Y wrap (ghost Y and XY).
+struct WrapStages {
+  V3 afterX, afterY;
+};
+__device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_periodic per) {
+  WrapStages w;
+  w.afterX = per.x ? wrapAxis(x, g, 0) : x;
+  w.afterY = per.y ? wrapAxis(w.afterX, g, 1) : w.afterX;
+  return w;
+}
+
 // Grid state of a body: 0 outside the grid (hash -1), 1 inside, 2 inside but
 // with a cell index rounded up to the grid size on some axis, so that its
 // cell id aliases another cell.

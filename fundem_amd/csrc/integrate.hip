@@ -109,21 +109,23 @@ __global__ void kCommitState(const double* x, int n, fdm_grid grid, int* atBuild
 // of orientations to FP32 (a relative 2^-24 per component moves a point at
 // distance rho by well under 1e-6 rho). Ghost lists are also rebuilt after
 // a visibility change or a wrap; a wrap (positions jump by a period)
-// rebuilds the next step's lists as well.
-__global__ void kSkinDecide(fdm_skin s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// rebuilds the next step's lists as well. Control words written by other
+// workgroups of the same launch are read through atomics (at the device
+// cache, not a workgroup's own).
+__device__ void decideStep(const fdm_skin& s) {
   int* c = s.ctl;
-  const float rate = __int_as_float(c[FDM_CTL_MAX_RATE]);
+  auto rd = [&](int w) { return atomicAdd(c + w, 0); };
+  const float rate = __int_as_float(rd(FDM_CTL_MAX_RATE));
   double d = *s.disp + s.dt * (double)rate * (1.0 + 1e-6) + 1e-6 * s.rhoMax;
-  const bool rebuild = c[FDM_CTL_FORCE] != 0 || c[FDM_CTL_PENDING] != 0 ||
-                       c[FDM_CTL_VALID_CHANGED] != 0 || !(2.0 * d <= s.skin);
-  const bool wrapped = c[FDM_CTL_WRAPPED] != 0;
-  const bool ghost = rebuild || wrapped || c[FDM_CTL_GHOST_CHANGED] != 0;
+  const bool rebuild = rd(FDM_CTL_FORCE) != 0 || rd(FDM_CTL_PENDING) != 0 ||
+                       rd(FDM_CTL_VALID_CHANGED) != 0 || !(2.0 * d <= s.skin);
+  const bool wrapped = rd(FDM_CTL_WRAPPED) != 0;
+  const bool ghost = rebuild || wrapped || rd(FDM_CTL_GHOST_CHANGED) != 0;
   if (rebuild) {
     d = 0.0;
-    c[FDM_CTL_COUNT] += 1;
+    c[FDM_CTL_COUNT] = rd(FDM_CTL_COUNT) + 1;
   }
-  if (ghost) c[FDM_CTL_GHOST_COUNT] += 1;
+  if (ghost) c[FDM_CTL_GHOST_COUNT] = rd(FDM_CTL_GHOST_COUNT) + 1;
   *s.disp = d;
   c[FDM_CTL_REBUILD] = rebuild ? 1 : 0;
   c[FDM_CTL_REBUILD_GHOST] = ghost ? 1 : 0;
@@ -133,7 +135,7 @@ __global__ void kSkinDecide(fdm_skin s) {
   c[FDM_CTL_VALID_CHANGED] = 0;
   c[FDM_CTL_WRAPPED] = 0;
   c[FDM_CTL_GHOST_CHANGED] = 0;
-  const int seq = c[FDM_CTL_SEQ] + 1;
+  const int seq = rd(FDM_CTL_SEQ) + 1;
   c[FDM_CTL_SEQ] = seq;
   if (s.hostDecision) {
     volatile int* h = s.hostDecision;
@@ -143,6 +145,127 @@ __global__ void kSkinDecide(fdm_skin s) {
     h[0] = seq;
     __threadfence_system();
   }
+}
+
+__global__ void kSkinDecide(fdm_skin s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  decideStep(s);
+}
+
+struct StepBeginK {
+  fdm_step_begin_args a;
+  fdm_skin s;
+  float* blockRate;  // [gridDim.x] largest motion rate of each workgroup
+  int* blockFlags;   // [gridDim.x] bit 0 grid state changed, 1 visibility changed, 2 wrapped
+};
+
+// Phase A of a step (one thread per particle, then per wall). Each workgroup
+// leaves its largest motion rate and its change flags in a slot of its own
+// (no atomics on shared words); kStepDecide folds the slots.
+__global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
+  const fdm_step_begin_args& a = k.a;
+  __shared__ float waveMax[4];
+  const int nP = a.particles.n, nW = a.walls.n;
+  const V3 g = fromD3(a.gravity);
+  const bool per = a.periodic.x || a.periodic.y;
+  float rate = 0.f;
+  int flags = 0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nP + nW) {
+    const bool wall = i >= nP;
+    const fdm_bodies& b = wall ? a.walls : a.particles;
+    const fdm_template* tmpl = wall ? a.wgeo.info : a.pgeo.info;
+    const fdm_grid& grid = wall ? a.wgrid : a.pgrid;
+    const fdm_hash_table& t = wall ? a.wtable : a.ptable;
+    const int j = wall ? i - nP : i;
+    if (a.kick && !wall) kickOne(b, tmpl, j, g, 0.5 * a.dt);
+    V3 x;
+    if (a.drift) x = driftOne(b, j, a.dt);
+    else x = ldp(b.x, b.n, j);
+    const int h = cellHash(x, grid);
+    t.hashValue[j] = h;
+    t.sortKey[j] = (unsigned)(h + 1);
+    t.sortIndex[j] = j;
+    if (a.drift) {
+      const fdm_template& tp = tmpl[b.geo[j]];
+      const double rho = fmax(tp.radius, tp.boxReach);
+      const double r = vlen(ld3(b.v, b.n, j)) + vlen(ld3(b.w, b.n, j)) * rho;
+      rate = (float)r * (1.0f + 1e-6f);
+    }
+    const int st = bodyState(x, grid);
+    if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
+    if (!wall && per) {
+      const WrapStages w = wrapStages(x, a.pgrid, a.periodic);
+      int bits = 0;
+      if (a.periodic.x) {
+        const int f = ghostCellHash(ghostOf(w.afterX, a.pgrid, 1), a.pgrid) >= 0 ? 1 : 0;
+        a.ghostFlag[0][j] = f;
+        bits |= f;
+      }
+      if (a.periodic.y) {
+        const int f = ghostCellHash(ghostOf(w.afterY, a.pgrid, 2), a.pgrid) >= 0 ? 1 : 0;
+        a.ghostFlag[1][j] = f;
+        bits |= f << 1;
+      }
+      if (a.periodic.x && a.periodic.y) {
+        const int f = ghostCellHash(ghostOf(w.afterY, a.pgrid, 3), a.pgrid) >= 0 ? 1 : 0;
+        a.ghostFlag[2][j] = f;
+        bits |= f << 2;
+      }
+      if (bits != a.ghostAtBuild[j]) flags |= 2;
+      if (!(w.afterY.x == x.x && w.afterY.y == x.y && w.afterY.z == x.z)) flags |= 4;
+    }
+  }
+  const int f1 = __syncthreads_or(flags & 1), f2 = __syncthreads_or(flags & 2),
+            f4 = __syncthreads_or(flags & 4);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) rate = fmaxf(rate, __shfl_xor(rate, off, 64));
+  if (laneId() == 0) waveMax[threadIdx.x >> 6] = rate;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = waveMax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, waveMax[w]);
+    k.blockRate[blockIdx.x] = m;
+    k.blockFlags[blockIdx.x] = (f1 ? 1 : 0) | (f2 ? 2 : 0) | (f4 ? 4 : 0);
+  }
+}
+
+// Folds the workgroup slots of kStepBegin into the control words, takes the
+// list decision and clears the given counters (one workgroup).
+__global__ void __launch_bounds__(1024) kStepDecide(StepBeginK k, int nblocks) {
+  __shared__ float rmax[16];
+  __shared__ int fl[16];
+  float r = 0.f;
+  int f = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    r = fmaxf(r, k.blockRate[b]);
+    f |= k.blockFlags[b];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    r = fmaxf(r, __shfl_xor(r, off, 64));
+    f |= __shfl_xor(f, off, 64);
+  }
+  if (laneId() == 0) {
+    rmax[threadIdx.x >> 6] = r;
+    fl[threadIdx.x >> 6] = f;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+    r = fmaxf(r, rmax[w]);
+    f |= fl[w];
+  }
+  r = fmaxf(r, rmax[0]);
+  f |= fl[0];
+  int* c = k.s.ctl;
+  if (r > 0.f) atomicMax(c + FDM_CTL_MAX_RATE, __float_as_int(r));
+  if (f & 1) c[FDM_CTL_VALID_CHANGED] = 1;
+  if (f & 2) c[FDM_CTL_GHOST_CHANGED] = 1;
+  if (f & 4) c[FDM_CTL_WRAPPED] = 1;
+  decideStep(k.s);
+  for (int q = 0; q < 8; ++q)
+    if (k.a.clearCounters[q]) *k.a.clearCounters[q] = 0;
 }
 
 // Copies the candidate lists of the previous step when no rebuild happens.
@@ -195,6 +318,39 @@ extern "C" int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_
                      fromD3(gravity), dt, kick, drift, grid, table.hashValue, table.sortKey,
                      table.sortIndex, skin ? (const int*)table.validAtBuild : nullptr,
                      skin ? skin->ctl : nullptr);
+  return launchStatus();
+}
+
+extern "C" size_t fdm_stepBeginWorkspaceBytes(size_t bodies) {
+  const size_t nb = std::max<size_t>(1, (bodies + 255) / 256);
+  return alignUp(nb * sizeof(float)) + alignUp(nb * sizeof(int));
+}
+
+extern "C" int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin,
+                                   void* workspace, size_t workspaceBytes, hipStream_t stream) {
+  if (!args || !skin || !skin->ctl || !skin->disp) return FDM_EINVAL;
+  const fdm_step_begin_args& a = *args;
+  if (a.particles.n < 0 || a.walls.n < 0) return FDM_EINVAL;
+  if (a.particles.n > 0 && !a.ptable.validAtBuild) return FDM_EINVAL;
+  if (a.walls.n > 0 && !a.wtable.validAtBuild) return FDM_EINVAL;
+  if (a.periodic.x || a.periodic.y) {
+    if (!a.ghostAtBuild) return FDM_EINVAL;
+    if (a.periodic.x && !a.ghostFlag[0]) return FDM_EINVAL;
+    if (a.periodic.y && !a.ghostFlag[1]) return FDM_EINVAL;
+    if (a.periodic.x && a.periodic.y && !a.ghostFlag[2]) return FDM_EINVAL;
+  }
+  const long long total = (long long)a.particles.n + a.walls.n;
+  if (workspaceBytes < fdm_stepBeginWorkspaceBytes((size_t)total)) return FDM_EINVAL;
+  const unsigned nb = std::max<unsigned>(1, blocksFor(total, 256));
+  Carve cv(workspace, workspaceBytes);
+  StepBeginK k;
+  k.a = a;
+  k.s = *skin;
+  k.blockRate = cv.take<float>(nb);
+  k.blockFlags = cv.take<int>(nb);
+  if (!cv.ok) return FDM_EINVAL;
+  hipLaunchKernelGGL(kStepBegin, dim3(nb), dim3(256), 0, stream, k);
+  hipLaunchKernelGGL(kStepDecide, dim3(1), dim3(1024), 0, stream, k, (int)nb);
   return launchStatus();
 }
 

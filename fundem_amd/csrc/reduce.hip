@@ -93,6 +93,7 @@ struct ReduceK {
   V3 gravity;
   double h;
   int kick;
+  int* stepCounter;
 };
 
 __device__ __forceinline__ void addRec(V3& F, V3& T, const double* f, const double* t, bool negF) {
@@ -122,6 +123,7 @@ __device__ __forceinline__ int segEnd(const fdm_pairs& s, int i) {
 __global__ void kReduce(ReduceK a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = a.P.n;
+  if (i == 0 && a.stepCounter) *a.stepCounter += 1;  // read by the bond stage before this launch
   if (i >= n) return;
   V3 F = {0., 0., 0.}, T = {0., 0., 0.};
   // particle pairs: as target (partners below i), then owned (partners above)
@@ -239,7 +241,11 @@ extern "C" int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStrea
   k.gravity = fromD3(g.gravity);
   k.h = g.h;
   k.kick = g.kick;
-  if (g.particles.n <= 0) return FDM_OK;
+  k.stepCounter = g.stepCounter;
+  if (g.particles.n <= 0) {
+    if (g.stepCounter) hipLaunchKernelGGL(kIncrement, dim3(1), dim3(64), 0, stream, g.stepCounter);
+    return launchStatus();
+  }
   hipLaunchKernelGGL(kReduce, dim3(blocksFor(g.particles.n, 256)), dim3(256), 0, stream, k);
   return launchStatus();
 }

@@ -249,11 +249,25 @@ class LSSolver {
     pw_.initialize(N, (int)std::max<size_t>(N * std::max<size_t>(W, 1), 1024), recCap, 1);
     periodic_.initialize(particles_, (int)std::max<size_t>(N, 64),
                          (int)std::max<size_t>(4 * N, 1024), recCap, mirrorCapacity_);
+    pp_.resetGenerations();
+    pw_.resetGenerations();
+    for (int s = 0; s < 3; ++s)
+      if (periodic_.active(s)) periodic_.set(s).interaction.resetGenerations();
+    {
+      // work list of the narrow phase: every candidate pair of every set
+      long long cap = (long long)pairCap + std::max<long long>((long long)N * std::max<size_t>(W, 1), 1024);
+      for (int s = 0; s < 3; ++s)
+        if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
+      if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
+      workList_.allocate((size_t)cap, false);
+      workCount_.allocate(1);
+    }
     status_.allocate(1);
     stepCounter_.allocate(1);
     bonds_.initialize(N, stepCounter_.get(), stream_);
     size_t ws = fdm_radixSortWorkspaceBytes(std::max<size_t>({N, W, size_t(64)}));
     ws = std::max(ws, fdm_scanWorkspaceBytes(N + 1));
+    ws = std::max(ws, fdm_stepBeginWorkspaceBytes(N + W));
     workspace_.allocate(ws / sizeof(double) + 1, false);
     destroyGraphs();
     if (!hostDecision_) {
@@ -511,8 +525,9 @@ class LSSolver {
     return k;
   }
 
-  void interactionSet(int set, SolidInteraction& si, const fdm_ghosts* ghosts, fdm_grid grid,
-                      fdm_hash_table table, const int* targetHash, bool prime, bool rebuild) {
+  // Broad phase of one set into its (new) current list generation.
+  void broadPhase(int set, SolidInteraction& si, const fdm_ghosts* ghosts, fdm_grid grid,
+                  fdm_hash_table table) {
     fdm_broadphase_args b{};
     b.skin = skin_;
     b.maxReach = set == FDM_SET_PARTICLE ? maxRadius_ : set == FDM_SET_GHOST ? maxBoxReach_ : 0.0;
@@ -526,28 +541,22 @@ class LSSolver {
     b.grid = grid;
     b.table = table;
     b.pairs = si.current();
-    {
-      Timer t(this, set == 0 ? (rebuild ? "broad_pp" : "carry_pp")
-                             : set == 1 ? (rebuild ? "broad_pw" : "carry_pw")
-                                        : (rebuild ? "broad_ghost" : "carry_ghost"));
-      if (rebuild) {
-        fdmCheck(fdm_launchBroadPhase(&b, workspace_.get(), workspace_.size() * sizeof(double),
-                                      stream_),
-                 "broad phase");
-        fdmCheck(fdm_launchBroadPhaseWrite(&b, status_.get(), stream_), "broad phase write");
-      } else {
-        const fdm_pairs from = si.previous();
-        fdmCheck(fdm_launchCarryPairs(&from, &b.pairs, (int)particles_.num(), nullptr, stream_),
-                 "carry pairs");
-      }
-    }
+    Timer t(this, set == 0 ? "broad_pp" : set == 1 ? "broad_pw" : "broad_ghost");
+    fdmCheck(fdm_launchBroadPhase(&b, workspace_.get(), workspace_.size() * sizeof(double), stream_),
+             "broad phase");
+    fdmCheck(fdm_launchBroadPhaseWrite(&b, status_.get(), stream_), "broad phase write");
+  }
+
+  fdm_narrowphase_args narrowArgs(int set, SolidInteraction& si, const fdm_ghosts* ghosts,
+                                  fdm_grid grid, const int* targetHash, int ghostMask,
+                                  bool ghostFromReal, bool prime) {
     fdm_narrowphase_args n{};
     n.set = set;
-    n.particles = b.particles;
-    n.pgeo = b.pgeo;
+    n.particles = particles_.bodies();
+    n.pgeo = particles_.geometry();
     n.pmat = particles_.deviceMaterials();
-    n.walls = b.walls;
-    n.wgeo = b.wgeo;
+    n.walls = walls_.bodies();
+    n.wgeo = walls_.geometry();
     n.wmat = walls_.deviceMaterials();
     n.ghosts = ghosts;
     n.grid = grid;
@@ -556,48 +565,53 @@ class LSSolver {
     n.periodic = periodic_.flags();
     n.pairs = si.current();
     n.prev = si.previous();
-    n.prevStart = n.prev.count;
     n.mirror = si.mirror();
+    n.ghostMask = ghostMask;
+    n.ghostFromReal = ghostFromReal ? 1 : 0;
     n.dt = dt_;
     n.updateSprings = prime ? 0 : 1;
     n.status = status_.get();
-    Timer t(this, set == 0 ? "narrow_pp" : set == 1 ? "narrow_pw" : "narrow_ghost");
-    fdmCheck(fdm_launchNarrowPhase(&n, stream_), "narrow phase");
+    return n;
   }
 
-  // A step runs in two phases. Phase A (kick, drift, cell ids, periodic
-  // visibility flags) ends with the device's decision whether the candidate
-  // lists are rebuilt (fdm_launchSkinDecide), written to host-visible memory.
-  // The host waits for that word and launches phase B with only the work the
-  // step needs: list builds, or carrying the previous lists forward, then
-  // the narrow phase, bonds and the reduction. Waiting here replaces dozens
-  // of launches that would otherwise test a device flag and exit.
+  // A step runs in two phases. Phase A (one launch: kick, drift, cell ids,
+  // periodic visibility flags) ends with the device's decision whether the
+  // candidate lists are rebuilt, written to host-visible memory. The host
+  // waits for that word and launches phase B with only the work the step
+  // needs: list builds when decided, then the narrow phase of every set in
+  // one launch, bonds and the reduction with the second kick.
   void phaseA(bool prime) {
-    fdm_bodies P = particles_.bodies();
-    const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
-    const fdm_skin sk = skinView();
-    {
-      Timer t(this, "kick_drift_hash");
-      fdmCheck(fdm_launchKickDriftHash(P, particles_.geometry(), g, dt_, prime ? 0 : 1,
-                                       prime ? 0 : 1, particles_.grid(), particles_.table().view(),
-                                       &sk, stream_),
-               "kick/drift");
-    }
-    if (walls_.num()) {
-      Timer t(this, "wall_drift");
-      fdmCheck(fdm_launchKickDriftHash(walls_.bodies(), walls_.geometry(), g, dt_, 0, prime ? 0 : 1,
-                                       walls_.grid(), walls_.table().view(), &sk, stream_),
-               "wall drift");
-    }
+    fdm_step_begin_args a{};
+    a.particles = particles_.bodies();
+    a.pgeo = particles_.geometry();
+    a.pgrid = particles_.grid();
+    a.ptable = particles_.table().view();
+    a.walls = walls_.bodies();
+    a.walls.n = (int)walls_.num();
+    a.wgeo = walls_.geometry();
+    a.wgrid = walls_.grid();
+    a.wtable = walls_.table().view();
+    a.gravity = {gravity_.x, gravity_.y, gravity_.z};
+    a.dt = dt_;
+    a.kick = prime ? 0 : 1;
+    a.drift = prime ? 0 : 1;
+    a.periodic = periodic_.flags();
     if (periodic_.isActived()) {
-      Timer t(this, "periodic_flags");
-      fdm_ghosts gv[3];
-      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
-      fdmCheck(fdm_launchPeriodicPrepare(P, particles_.grid(), periodic_.flags(), gv, &sk, stream_),
-               "periodic flags");
+      for (int s = 0; s < 3; ++s)
+        a.ghostFlag[s] = periodic_.active(s) ? periodic_.set(s).flag.get() : nullptr;
+      a.ghostAtBuild = periodic_.view(periodic_.active(0) ? 0 : 1).atBuild;
     }
-    Timer t(this, "decide");
-    fdmCheck(fdm_launchSkinDecide(&sk, stream_), "list decision");
+    int q = 0;
+    a.clearCounters[q++] = pp_.nextRecordCounter();
+    if (walls_.num()) a.clearCounters[q++] = pw_.nextRecordCounter();
+    for (int s = 0; s < 3; ++s)
+      if (periodic_.active(s)) a.clearCounters[q++] = periodic_.set(s).interaction.nextRecordCounter();
+    a.clearCounters[q++] = workCount_.get();
+    const fdm_skin sk = skinView();
+    Timer t(this, "step_begin");
+    fdmCheck(fdm_launchStepBegin(&a, &sk, workspace_.get(), workspace_.size() * sizeof(double),
+                                 stream_),
+             "step begin");
   }
 
   struct Decision {
@@ -622,6 +636,14 @@ class LSSolver {
     return {h[1] != 0, h[2] != 0};
   }
 
+  // New result generations for every set, new list generations where rebuilt.
+  void beginSets(Decision d) {
+    pp_.beginStep(d.rebuild);
+    pw_.beginStep(d.rebuild);
+    for (int s = 0; s < 3; ++s)
+      if (periodic_.active(s)) periodic_.set(s).interaction.beginStep(d.ghost);
+  }
+
   void phaseB(bool prime, double time, Decision d) {
     fdm_bodies P = particles_.bodies();
     fdm_geometry pgeo = particles_.geometry();
@@ -630,28 +652,35 @@ class LSSolver {
     void* ws = workspace_.get();
     const size_t wsBytes = workspace_.size() * sizeof(double);
     const fdm_skin sk = skinView();
+    beginSets(d);
     if (d.rebuild) {
-      Timer t(this, "sort_cells");
-      fdmCheck(fdm_launchCommitBodyState(P, particles_.grid(), particles_.table().view(), &sk,
+      {
+        Timer t(this, "sort_cells");
+        fdmCheck(fdm_launchCommitBodyState(P, particles_.grid(), particles_.table().view(), &sk,
+                                           stream_),
+                 "body state");
+        fdmCheck(fdm_buildHashStartEnd(particles_.table().view(), (int)N, particles_.grid(), ws,
+                                       wsBytes, stream_),
+                 "hash table");
+        if (W) {
+          fdmCheck(fdm_launchCommitBodyState(walls_.bodies(), walls_.grid(), walls_.table().view(),
+                                             &sk, stream_),
+                   "wall state");
+          fdmCheck(fdm_buildHashStartEnd(walls_.table().view(), (int)W, walls_.grid(), ws, wsBytes,
                                          stream_),
-               "body state");
-      fdmCheck(fdm_buildHashStartEnd(particles_.table().view(), (int)N, particles_.grid(), ws,
-                                     wsBytes, stream_),
-               "hash table");
-      if (W) {
-        fdmCheck(fdm_launchCommitBodyState(walls_.bodies(), walls_.grid(), walls_.table().view(),
-                                           &sk, stream_),
-                 "wall state");
-        fdmCheck(fdm_buildHashStartEnd(walls_.table().view(), (int)W, walls_.grid(), ws, wsBytes,
-                                       stream_),
-                 "wall hash table");
+                   "wall hash table");
+        }
       }
+      broadPhase(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(), particles_.table().view());
+      if (W) broadPhase(FDM_SET_WALL, pw_, nullptr, walls_.grid(), walls_.table().view());
     }
-    interactionSet(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(), particles_.table().view(),
-                   particles_.table().hashValue.get(), prime, d.rebuild);
+    fdm_narrowphase_args na[5];
+    int ns = 0;
+    na[ns++] = narrowArgs(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(),
+                          particles_.table().hashValue.get(), 0, false, prime);
     if (W)
-      interactionSet(FDM_SET_WALL, pw_, nullptr, walls_.grid(), walls_.table().view(),
-                     walls_.table().hashValue.get(), prime, d.rebuild);
+      na[ns++] = narrowArgs(FDM_SET_WALL, pw_, nullptr, walls_.grid(),
+                            walls_.table().hashValue.get(), 0, false, prime);
     fdm_ghosts gv[3];
     const fdm_pairs* gpairs[3] = {nullptr, nullptr, nullptr};
     const fdm_mirror* gmir[3] = {nullptr, nullptr, nullptr};
@@ -659,27 +688,34 @@ class LSSolver {
     fdm_mirror gm[3];
     if (periodic_.isActived()) {
       for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
-      {
-        Timer t(this, d.ghost ? "periodic_build" : "periodic_refresh");
-        if (d.ghost)
+      if (d.ghost) {
+        {
+          Timer t(this, "periodic_build");
           fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
                                            status_.get(), ws, wsBytes, stream_),
                    "periodic ghosts");
-        else
-          fdmCheck(fdm_launchPeriodicRefresh(P, particles_.grid(), periodic_.flags(), gv, nullptr,
-                                             stream_),
-                   "periodic refresh");
+        }
+        for (int s = 0; s < 3; ++s)
+          if (periodic_.active(s))
+            broadPhase(FDM_SET_GHOST, periodic_.set(s).interaction, &gv[s], particles_.grid(),
+                       periodic_.set(s).table.view());
       }
       for (int s = 0; s < 3; ++s) {
         if (!periodic_.active(s)) continue;
         GhostSet& G = periodic_.set(s);
-        interactionSet(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(), G.table.view(),
-                       G.table.hashValue.get(), prime, d.ghost);
+        na[ns++] = narrowArgs(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(),
+                              G.table.hashValue.get(), s + 1, !d.ghost, prime);
         gcur[s] = G.interaction.current();
         gm[s] = G.interaction.mirror();
         gpairs[s] = &gcur[s];
         gmir[s] = &gm[s];
       }
+    }
+    {
+      Timer t(this, "narrow");
+      fdmCheck(fdm_launchNarrowPhaseSets(na, ns, workList_.get(), (int)workList_.size(),
+                                         workCount_.get(), stream_),
+               "narrow phase");
     }
     if (prime) {
       // no forces in the pre-loop build: drop the mirror entries
@@ -687,7 +723,6 @@ class LSSolver {
       fdmCheck(fdm_fillInt(m.count, std::max<size_t>(N, 1), 0, stream_), "clear");
       for (int s = 0; s < 3; ++s)
         if (gmir[s]) fdmCheck(fdm_fillInt(gmir[s]->count, std::max<size_t>(N, 1), 0, stream_), "clear");
-      swapAll();
       return;
     }
     fdm_bonds bv = bonds_.view();
@@ -710,6 +745,7 @@ class LSSolver {
     r.bonds = bv.n > 0 ? &bv : nullptr;
     r.gravity = g;
     r.h = 0.5 * dt_;
+    r.stepCounter = stepCounter_.get();
     {
       Timer t(this, "reduce_kick");
       if (hookMode_ == HookNone) {
@@ -725,8 +761,6 @@ class LSSolver {
                  "kick");
       }
     }
-    fdmCheck(fdm_incrementCounter(stepCounter_.get(), stream_), "step counter");
-    swapAll();
   }
 
   void stepOnce(bool prime, double time) {
@@ -734,29 +768,27 @@ class LSSolver {
     phaseB(prime, time, waitDecision());
   }
 
-  void swapAll() {
-    pp_.swap();
-    pw_.swap();
+  int ghostListIndex() {
     for (int s = 0; s < 3; ++s)
-      if (periodic_.active(s)) periodic_.set(s).interaction.swap();
+      if (periodic_.active(s)) return periodic_.set(s).interaction.listIndex();
+    return 0;
   }
 
-  // One step from captured graphs: phase A (the same every step) and the
-  // phase B variant for this step's decision and buffer parity, each
-  // captured at first use.
+  // One step from captured graphs: phase A (keyed by the result generation
+  // it clears) and the phase B variant for this step's decision and buffer
+  // generations, each captured at first use.
   void graphStep() {
-    if (!graphA_) graphA_ = capture([&] { phaseA(false); });
-    hipCheck(hipGraphLaunch(graphA_, stream_), "graph launch");
+    const int stepNext = pp_.nextStepIndex();
+    hipGraphExec_t& ga = graphsA_[stepNext];
+    if (!ga) ga = capture([&] { phaseA(false); });
+    hipCheck(hipGraphLaunch(ga, stream_), "graph launch");
     const Decision d = waitDecision();
-    const int parity = pp_.currentIndex();
-    hipGraphExec_t& gb = graphsB_[parity][d.rebuild ? 1 : 0][d.ghost ? 1 : 0];
-    if (!gb) {
-      gb = capture([&] { phaseB(false, 0.0, d); });
-      // capturing ran phaseB's host side (buffer swaps) without executing it
-      swapAll();
-    }
+    const int mainNext = pp_.listIndex() ^ (d.rebuild ? 1 : 0);
+    const int ghostNext = ghostListIndex() ^ (d.ghost ? 1 : 0);
+    hipGraphExec_t& gb = graphsB_[stepNext][mainNext][ghostNext][d.rebuild ? 1 : 0][d.ghost ? 1 : 0];
+    if (!gb) gb = capture([&] { phaseB(false, 0.0, d); });  // advances the generations
+    else beginSets(d);
     hipCheck(hipGraphLaunch(gb, stream_), "graph launch");
-    swapAll();
   }
   template <class F>
   hipGraphExec_t capture(F&& body) {
@@ -770,14 +802,15 @@ class LSSolver {
     return exec;
   }
   void destroyGraphs() {
-    if (graphA_) (void)hipGraphExecDestroy(graphA_);
-    graphA_ = nullptr;
-    for (auto& a : graphsB_)
-      for (auto& b : a)
-        for (auto& g : b) {
-          if (g) (void)hipGraphExecDestroy(g);
-          g = nullptr;
-        }
+    for (auto& g : graphsA_) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+    hipGraphExec_t* b = &graphsB_[0][0][0][0][0];
+    for (size_t k = 0; k < sizeof(graphsB_) / sizeof(graphsB_[0][0][0][0][0]); ++k) {
+      if (b[k]) (void)hipGraphExecDestroy(b[k]);
+      b[k] = nullptr;
+    }
   }
 
   std::string dir_;
@@ -798,14 +831,15 @@ class LSSolver {
   DeviceBuffer<unsigned> status_;
   DeviceBuffer<int> stepCounter_;
   DeviceBuffer<int> ctl_;
+  DeviceBuffer<int> workList_, workCount_;
   DeviceBuffer<double> disp_;
   double skinRequest_ = -1.0, skin_ = 0.0;
   double maxRadius_ = 0.0, maxBoxReach_ = 0.0, rhoMax_ = 0.0;
   bool useGraph_ = false, timing_ = false;
   bool baseHookCalled_ = false;
   HookMode hookMode_ = HookUnknown;
-  hipGraphExec_t graphA_ = nullptr;
-  hipGraphExec_t graphsB_[2][2][2] = {};
+  hipGraphExec_t graphsA_[2] = {};
+  hipGraphExec_t graphsB_[2][2][2][2][2] = {};
   int* hostDecision_ = nullptr;     // pinned, host view
   int* hostDecisionDev_ = nullptr;  // the same memory, device view
   int decisionSeq_ = 0;

@@ -1,9 +1,12 @@
 // SolidInteraction: one interaction set (particle-particle, particle-wall or
 // one ghost set). The reference stores per-node counts and prefix sums plus
 // contact arrays, copied to "previous" arrays every step
-// (kernel/SolidInteraction.h:92-151). Here the set holds two generations of
-// candidate pairs with their tangential-history records; a step writes one
-// generation while reading the other, and the roles swap afterwards.
+// (kernel/SolidInteraction.h:92-151). Here the set holds
+//  - two generations of candidate lists (CSR counts and pairs), of which one
+//    is current; a step that rebuilds the lists writes the other one and
+//    makes it current, while the kept one still resolves the history;
+//  - two generations of per-step results (history records, per-pair record
+//    ranges and force/torque sums); each step writes one and reads the other.
 #pragma once
 #include <algorithm>
 #include <cstring>
@@ -15,49 +18,57 @@
 
 namespace fundem {
 
-struct PairBuffers {
-  int capacity = 0, recCapacity = 0;
-  DeviceBuffer<int> count, total, pairA, pairT, pairG, recOff, recCnt, recTotal;
-  DeviceBuffer<double> sumF, sumTA, sumTB;
-  DeviceBuffer<float4> rec;
-  void allocate(size_t nBodies, int cap, int recCap) {
+struct ListBuffers {
+  int capacity = 0;
+  DeviceBuffer<int> count, total, pairA, pairT, pairG;
+  void allocate(size_t nBodies, int cap) {
     capacity = std::max(cap, 1);
-    recCapacity = std::max(recCap, 1);
     count.allocate(nBodies + 1);
     total.allocate(1);
-    recTotal.allocate(1);
     pairA.allocate(capacity);
     pairT.allocate(capacity);
     pairG.allocate(capacity);
-    recOff.allocate(capacity);
-    recCnt.allocate(capacity);
-    sumF.allocate(3 * (size_t)capacity);
-    sumTA.allocate(3 * (size_t)capacity);
-    sumTB.allocate(3 * (size_t)capacity);
+  }
+  double memoryGB() const { return count.memoryGB() + pairA.memoryGB() * 3; }
+};
+
+struct StepBuffers {
+  int recCapacity = 0;
+  DeviceBuffer<int> recOff, recCnt, recTotal;
+  DeviceBuffer<double> sumF, sumTA, sumTB;
+  DeviceBuffer<float4> rec;
+  void allocate(int pairCap, int recCap) {
+    const int cap = std::max(pairCap, 1);
+    recCapacity = std::max(recCap, 1);
+    recOff.allocate(cap);
+    recCnt.allocate(cap);
+    recTotal.allocate(1);
+    sumF.allocate(3 * (size_t)cap);
+    sumTA.allocate(3 * (size_t)cap);
+    sumTB.allocate(3 * (size_t)cap);
     rec.allocate(recCapacity);
   }
-  fdm_pairs view() {
-    fdm_pairs p;
-    p.capacity = capacity;
-    p.count = count.get();
-    p.total = total.get();
-    p.pairA = pairA.get();
-    p.pairT = pairT.get();
-    p.pairG = pairG.get();
-    p.recOff = recOff.get();
-    p.recCnt = recCnt.get();
-    p.sumF = sumF.get();
-    p.sumTA = sumTA.get();
-    p.sumTB = sumTB.get();
-    p.recCapacity = recCapacity;
-    p.recTotal = recTotal.get();
-    p.rec = rec.get();
-    return p;
-  }
-  double memoryGB() const {
-    return count.memoryGB() + pairA.memoryGB() * 5 + sumF.memoryGB() * 3 + rec.memoryGB();
-  }
+  double memoryGB() const { return recOff.memoryGB() * 2 + sumF.memoryGB() * 3 + rec.memoryGB(); }
 };
+
+inline fdm_pairs pairsView(ListBuffers& l, StepBuffers& s) {
+  fdm_pairs p;
+  p.capacity = l.capacity;
+  p.count = l.count.get();
+  p.total = l.total.get();
+  p.pairA = l.pairA.get();
+  p.pairT = l.pairT.get();
+  p.pairG = l.pairG.get();
+  p.recOff = s.recOff.get();
+  p.recCnt = s.recCnt.get();
+  p.sumF = s.sumF.get();
+  p.sumTA = s.sumTA.get();
+  p.sumTB = s.sumTB.get();
+  p.recCapacity = s.recCapacity;
+  p.recTotal = s.recTotal.get();
+  p.rec = s.rec.get();
+  return p;
+}
 
 // One contact of a set, in canonical order (A, target, node).
 struct ContactRecord {
@@ -75,18 +86,38 @@ struct SolidInteraction {
   // survives a re-initialisation with the same sizes (as the reference's
   // SolidInteraction::initialize keeps its arrays, SolidInteraction.h:23-49).
   void initialize(size_t nBodies, int pairCapacity, int recordCapacity, int mirrorCapacity) {
-    if (nBodies == nBodies_ && pairCapacity == gen_[0].capacity &&
-        recordCapacity == gen_[0].recCapacity && mirrorCapacity == mirrorCap_)
+    if (nBodies == nBodies_ && pairCapacity == list_[0].capacity &&
+        recordCapacity == step_[0].recCapacity && mirrorCapacity == mirrorCap_)
       return;
     nBodies_ = nBodies;
-    for (auto& g : gen_) g.allocate(nBodies, pairCapacity, recordCapacity);
+    for (auto& l : list_) l.allocate(nBodies, pairCapacity);
+    for (auto& s : step_) s.allocate(pairCapacity, recordCapacity);
     mirrorCap_ = std::max(mirrorCapacity, 1);
     mirrorCount_.allocate(std::max<size_t>(nBodies, 1));
     mirrorList_.allocate(std::max<size_t>(nBodies, 1) * mirrorCap_);
-    cur_ = 0;
+    list_cur_ = list_prev_ = step_cur_ = 0;
   }
-  fdm_pairs current() { return gen_[cur_].view(); }
-  fdm_pairs previous() { return gen_[cur_ ^ 1].view(); }
+
+  // Generation roles back to their initial state (every set of a solver
+  // then flips in step).
+  void resetGenerations() { list_cur_ = list_prev_ = step_cur_ = 0; }
+
+  // Start of a step: a new results generation, and a new list generation
+  // when the lists are rebuilt.
+  void beginStep(bool rebuild) {
+    list_prev_ = list_cur_;
+    if (rebuild) list_cur_ ^= 1;
+    step_cur_ ^= 1;
+  }
+  // Generations the step after beginStep(rebuild) would use, without
+  // changing anything (keys of captured graphs).
+  int nextStepIndex() const { return step_cur_ ^ 1; }
+  int listIndex() const { return list_cur_; }
+
+  fdm_pairs current() { return pairsView(list_[list_cur_], step_[step_cur_]); }
+  fdm_pairs previous() { return pairsView(list_[list_prev_], step_[step_cur_ ^ 1]); }
+  // record counter of the results generation the next step writes
+  int* nextRecordCounter() { return step_[step_cur_ ^ 1].recTotal.get(); }
   fdm_mirror mirror() {
     fdm_mirror m;
     m.capacity = mirrorCap_;
@@ -94,19 +125,25 @@ struct SolidInteraction {
     m.list = mirrorList_.get();
     return m;
   }
-  void swap() { cur_ ^= 1; }
-  int currentIndex() const { return cur_; }
-  void setCurrentIndex(int c) { cur_ = c & 1; }
 
-  // Contacts of the most recent step (the generation written last), sorted
-  // by (A, target, node). Synchronises the stream.
+  // Candidate pairs of the latest step; synchronises.
+  int lastTotal(hipStream_t s) {
+    ListBuffers& l = list_[list_cur_];
+    if (!l.total.get()) return 0;
+    return std::min(l.total.toHost(s, 1)[0], l.capacity);
+  }
+
+  // Contacts of the most recent step, sorted by (A, target, node).
+  // Synchronises the stream.
   std::vector<ContactRecord> contacts(hipStream_t s) {
-    PairBuffers& g = gen_[cur_ ^ 1];
-    std::vector<int> total = g.total.toHost(s, 1);
-    const int np = std::min(total[0], g.capacity);
+    ListBuffers& l = list_[list_cur_];
+    StepBuffers& g = step_[step_cur_];
     std::vector<ContactRecord> out;
+    if (!l.total.get()) return out;
+    std::vector<int> total = l.total.toHost(s, 1);
+    const int np = std::min(total[0], l.capacity);
     if (np <= 0) return out;
-    std::vector<int> A = g.pairA.toHost(s, np), T = g.pairT.toHost(s, np);
+    std::vector<int> A = l.pairA.toHost(s, np), T = l.pairT.toHost(s, np);
     std::vector<int> off = g.recOff.toHost(s, np), cnt = g.recCnt.toHost(s, np);
     std::vector<int> rt = g.recTotal.toHost(s, 1);
     const int nr = std::min(rt[0], g.recCapacity);
@@ -120,18 +157,15 @@ struct SolidInteraction {
       }
     return out;
   }
-  // Candidate pairs of the generation written last; synchronises.
-  int lastTotal(hipStream_t s) {
-    PairBuffers& g = gen_[cur_ ^ 1];
-    if (!g.total.get()) return 0;
-    return std::min(g.total.toHost(s, 1)[0], g.capacity);
+  double deviceMemoryGB() const {
+    return list_[0].memoryGB() + list_[1].memoryGB() + step_[0].memoryGB() + step_[1].memoryGB();
   }
-  double deviceMemoryGB() const { return gen_[0].memoryGB() + gen_[1].memoryGB(); }
 
  private:
   size_t nBodies_ = (size_t)-1;
-  PairBuffers gen_[2];
-  int cur_ = 0;
+  ListBuffers list_[2];
+  StepBuffers step_[2];
+  int list_cur_ = 0, list_prev_ = 0, step_cur_ = 0;
   int mirrorCap_ = 0;
   DeviceBuffer<int> mirrorCount_, mirrorList_;
 };

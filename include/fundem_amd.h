@@ -251,6 +251,33 @@ int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_double3 gra
  * ctl[FDM_CTL_REBUILD_GHOST] (after fdm_launchPeriodicPrepare, when
  * periodic); a wrap seen by the prepare stage also rebuilds the next step. */
 int fdm_launchSkinDecide(const fdm_skin* skin, hipStream_t stream);
+/* No counterpart in the reference: the first phase of a step in one launch.
+ * Kick (when `kick`), drift (when `drift`) and cell id of every particle,
+ * drift and cell id of every wall (as fdm_launchKickDriftHash), the periodic
+ * visibility flags of every particle (as fdm_launchPeriodicPrepare), then the
+ * list decision of fdm_launchSkinDecide from per-workgroup partials held in
+ * `workspace` (fdm_stepBeginWorkspaceBytes(particles + walls) bytes). It also
+ * clears the given counters for the narrow phase. */
+typedef struct {
+  fdm_bodies particles;
+  fdm_geometry pgeo;
+  fdm_grid pgrid;
+  fdm_hash_table ptable;
+  fdm_bodies walls;            /* n may be 0 */
+  fdm_geometry wgeo;
+  fdm_grid wgrid;
+  fdm_hash_table wtable;
+  fdm_double3 gravity;
+  double dt;
+  int kick, drift;
+  fdm_periodic periodic;
+  int* ghostFlag[3];           /* per particle visibility flag of sets X, Y, XY */
+  int* ghostAtBuild;           /* [n] visibility bits at the last ghost build */
+  int* clearCounters[8];       /* counters set to 0 (NULL entries skipped) */
+} fdm_step_begin_args;
+size_t fdm_stepBeginWorkspaceBytes(size_t bodies);
+int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
+                        size_t workspaceBytes, hipStream_t stream);
 /* No counterpart in the reference. When ctl[FDM_CTL_REBUILD] is set, records
  * every body's grid state in table.validAtBuild. */
 int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
@@ -387,9 +414,12 @@ typedef struct {
   fdm_grid pgrid;              /* particle grid (periodic wrap of force arms) */
   fdm_periodic periodic;       /* wrap applied to positions used in forces */
   fdm_pairs pairs;             /* this step */
-  fdm_pairs prev;              /* previous step (history) */
-  const int* prevStart;        /* [n+1] CSR of prev pairs by A */
+  fdm_pairs prev;              /* previous step (history); prev.count is its CSR by A */
   fdm_mirror mirror;           /* append pairs with contacts to the target's list */
+  int ghostMask;               /* ghost sets: 1 X, 2 Y, 3 XY */
+  int ghostFromReal;           /* ghost sets: take each ghost's position and cell id from
+                                  its real particle (valid when no particle wrapped this
+                                  step) instead of ghosts->pos / ghosts->table.hashValue */
   double dt;
   int updateSprings;           /* 0: keep remapped springs (history filtering) */
   unsigned* status;
@@ -399,6 +429,17 @@ typedef struct {
   double* dbgOverlap;
 } fdm_narrowphase_args;
 int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
+/* No counterpart in the reference: the narrow phase of up to five sets at
+ * once. A first kernel (one thread per pair) keeps the pairs whose bodies can
+ * still touch at this step's positions (the broad-phase tests without the
+ * skin) in workList and gives the others empty record ranges; a second runs
+ * the wave-per-pair narrow phase over workList. Shared fields (bodies,
+ * geometry, materials, pgrid, periodic, dt, updateSprings, status, dbg*) come
+ * from sets[0]. workCapacity must cover the sum of the sets' pair capacities.
+ * Unlike fdm_launchNarrowPhase it clears no counter: the sets' record
+ * counters (pairs.recTotal) and *workCount must be zero at launch. */
+int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets, int* workList,
+                              int workCapacity, int* workCount, hipStream_t stream);
 
 /* Replaces launchAddBondedForceTorque (contactKernel.cuh:311): VBond forces,
  * stresses and one-way failure; results per bond. */
@@ -431,6 +472,7 @@ typedef struct {
   fdm_double3 gravity;
   double h;
   int kick;
+  int* stepCounter;    /* incremented by one after the reduction (NULL: skip) */
 } fdm_reduce_args;
 int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream);
 
