@@ -52,35 +52,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic HBM bytes per launch of the single-kernel stages (FP64
-# positions, FP32 v, w, q, F, T, 32-bit ids; template tables stay in cache):
-#   kick_drift_hash  per particle: read x 24, v 12, w 12, q 16, F 12, T 12,
-#                    template id 4, state at build 4 (= 96); write x 24,
-#                    v 12, w 12, q 16, cell id 4, sort key 4, index 4 (= 76)
-#   periodic_flags   per particle: read x 24 and visibility at build 4,
-#                    write three flags 12
-#   reduce_kick      per particle: pair counts read (particle set 8, wall 8,
-#                    three ghost sets 24, mirror counts 16), F T write 24,
-#                    template id 4, v w read and write 48, q read 16 (= 148);
-#                    per pair with contacts, per side: sums read 48
-#   narrow_*         per candidate pair: ids 12, previous ids and history
-#                    range 16, both bodies' x q template id 88, record range
-#                    write 8, pair sums write 72 (= 196); per contact:
-#                    velocities of both bodies 48 and history read 16 and
-#                    write 16 (= 80)
+# Algorithmic HBM bytes per launch of the timed stages (FP64 positions, FP32
+# v, w, q, F, T, 32-bit ids; template tables stay in cache):
+#   step_begin   (kStepBegin alone; the decision kernel is timed as "decide")
+#                per particle: read x 24, v 12, w 12, q 16, F 12, T 12,
+#                template id 4, grid state at build 4, visibility at build 4
+#                (= 100); write x 24, v 12, w 12, q 16, cell id 4, sort key
+#                4, index 4, three visibility flags 12 (= 88)
+#   reduce_kick  per particle: pair counts read (particle set 8, wall 8,
+#                three ghost sets 24, mirror counts 16), F T write 24,
+#                template id 4, v w read and write 48, q read 16 (= 148);
+#                per contact record pair side: sums read 48 (bounded by 96
+#                per contact)
+#   narrow       per candidate pair: ids 8, both bodies' x q template id 88,
+#                empty record range or the kept pair's record range and pair
+#                sums 80, history ids 16 (= 192); per contact: velocities of
+#                both bodies 48, history read 16 and write 16 (= 80)
 def kernel_bytes(stage, n, pairs, contacts):
-    if stage == "kick_drift_hash":
-        return 172.0 * n
-    if stage == "periodic_flags":
-        return 40.0 * n
+    if stage == "step_begin":
+        return 188.0 * n
     if stage == "reduce_kick":
         return 148.0 * n + 96.0 * sum(contacts.values())
-    if stage.startswith("narrow_"):
-        key = {"narrow_pp": [0], "narrow_pw": [1], "narrow_ghost": [2, 3, 4]}[stage]
-        p = sum(pairs[k] for k in key)
-        c = sum(contacts[k] for k in key)
-        return (196.0 * p + 80.0 * c) / len(key)
+    if stage == "narrow":
+        return 192.0 * sum(pairs.values()) + 80.0 * sum(contacts.values())
     return None
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage from the committed counter summary
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("stages", {}).get(stage)
+    return None if e is None else float(e["bytes_per_launch"])
 
 
 def cpu_baseline(templates, budget_s):
@@ -173,7 +182,7 @@ def main():
         avg_s = ms / max(calls, 1) * 1e-3
         achieved = nb / avg_s / 1e9 if avg_s > 0 else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb}
     # algorithmic bytes of a whole step: state 224 B per particle and 20 B per
     # contact record read and written, plus the template tables

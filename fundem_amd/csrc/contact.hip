@@ -298,6 +298,7 @@ struct NarrowK {
 constexpr int kMaxSets = 5;
 struct NarrowMulti {
   NarrowCommon c;
+  const int* gate;
   int nsets;
   NarrowSet s[kMaxSets];
 };
@@ -421,7 +422,7 @@ struct HitSlots {
 // instead of staging them.
 template <int SET>
 __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, int lane,
-                           HitSlots& hs) {
+                           HitSlots& hs, int pool) {
   const int A = S.pairs.pairA[p];
   if (A < 0) {  // slot left empty by a capacity overflow
     if (lane == 0) {
@@ -474,14 +475,37 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   // slots holding their contact point, normal, overlap and node id
   unsigned long long myMask = 0ull;
   int total = 0;
+  // the history range of the pair does not depend on the sweep: start it now
+  int prevOff = 0;
+  const int prevCnt = liveChunks ? findPrevPair(S, p, A, T, prevOff) : 0;
+  // node coordinates of the next surviving chunk are loaded one iteration
+  // ahead, so their latency overlaps the current chunk's SDF fetches
+  float nx = 0.f, ny = 0.f, nz = 0.f;
+  if (liveChunks) {
+    const int k0 = __builtin_ctzll(liveChunks) * FDM_CHUNK + lane;
+    if (k0 < tA.nodeCount) {
+      nx = nodes[3 * k0];
+      ny = nodes[3 * k0 + 1];
+      nz = nodes[3 * k0 + 2];
+    }
+  }
   for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
     const int c = __builtin_ctzll(lcm);
     const int k = c * FDM_CHUNK + lane;
+    const V3 s = {(double)nx, (double)ny, (double)nz};
+    const unsigned long long rest = lcm & (lcm - 1ull);
+    if (rest) {
+      const int k2 = __builtin_ctzll(rest) * FDM_CHUNK + lane;
+      if (k2 < tA.nodeCount) {
+        nx = nodes[3 * k2];
+        ny = nodes[3 * k2 + 1];
+        nz = nodes[3 * k2 + 2];
+      }
+    }
     bool hit = false;
     Sample sm;
     V3 pn;
     if (k < tA.nodeCount) {
-      const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
       pn = vadd(qrotate(qA, s), xA);
       hit = nodeHit<SET>(pn, tg, tcell, S.grid, RB, sm);
     }
@@ -508,18 +532,19 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
 
   int base = 0;
   if (total > 0) {
-    if (lane == 0) base = atomicAdd(S.pairs.recTotal, total);
+    const int poolCap = S.pairs.recCapacity / FDM_REC_POOLS;
+    if (lane == 0) base = atomicAdd(S.pairs.recTotal + pool * FDM_REC_STRIDE, total);
     base = readlaneI(base, 0);
-    if (base + total > S.pairs.recCapacity) {
+    if (base + total > poolCap) {
       if (lane == 0) atomicOr(C.status, FDM_STATUS_RECORD_OVERFLOW);
       total = 0;
+      base = 0;
     }
+    base += pool * poolCap;
   }
 
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   if (total > 0) {
-    int prevOff = 0;
-    const int prevCnt = findPrevPair(S, p, A, T, prevOff);
     // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
     const int mi = C.P.mat[A];
     const fdm_material& Mi = C.pmat[mi];
@@ -643,7 +668,7 @@ __global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
   const int stride = gridDim.x * wpb;
   HitSlots& hs = slots[threadIdx.x >> 6];
   for (int p = blockIdx.x * wpb + (threadIdx.x >> 6); p < total; p += stride)
-    processPair<SET>(a.c, a.s, p, lane, hs);
+    processPair<SET>(a.c, a.s, p, lane, hs, p % FDM_REC_POOLS);
 }
 
 // Pair-level cull of every set's candidates at this step's positions (one
@@ -651,6 +676,7 @@ __global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
 // the target (the broad-phase tests without the skin). Culled pairs get
 // empty record ranges; kept pairs go to the work list as (p << 3 | set).
 __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* workCount) {
+  if (gateOff(a.gate)) return;
   int start[kMaxSets + 1];
   start[0] = 0;
 #pragma unroll
@@ -711,9 +737,11 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
 }
 
 // One wave per kept pair of the work list.
-__global__ void __launch_bounds__(256) kNarrowWork(NarrowMulti a, const int* work,
-                                                   const int* workCount) {
-  __shared__ HitSlots slots[4];
+// One wave per workgroup, so a finished wave frees its slot at once.
+__global__ void __launch_bounds__(64) kNarrowWork(NarrowMulti a, const int* work,
+                                                  const int* workCount) {
+  if (gateOff(a.gate)) return;
+  __shared__ HitSlots slots[1];
   const int lane = laneId();
   const int wpb = blockDim.x >> 6;
   HitSlots& hs = slots[threadIdx.x >> 6];
@@ -723,14 +751,15 @@ __global__ void __launch_bounds__(256) kNarrowWork(NarrowMulti a, const int* wor
     const int code = work[w];
     const NarrowSet& s = a.s[code & 7];
     const int p = code >> 3;
-    if (s.kind == FDM_SET_PARTICLE) processPair<FDM_SET_PARTICLE>(a.c, s, p, lane, hs);
-    else if (s.kind == FDM_SET_WALL) processPair<FDM_SET_WALL>(a.c, s, p, lane, hs);
-    else processPair<FDM_SET_GHOST>(a.c, s, p, lane, hs);
+    const int pool = w % FDM_REC_POOLS;
+    if (s.kind == FDM_SET_PARTICLE) processPair<FDM_SET_PARTICLE>(a.c, s, p, lane, hs, pool);
+    else if (s.kind == FDM_SET_WALL) processPair<FDM_SET_WALL>(a.c, s, p, lane, hs, pool);
+    else processPair<FDM_SET_GHOST>(a.c, s, p, lane, hs, pool);
   }
 }
 
 __global__ void kResetCounter(int* c) {
-  if (threadIdx.x == 0) *c = 0;
+  for (int i = threadIdx.x; i < FDM_REC_COUNTERS; i += blockDim.x) c[i] = 0;
 }
 
 }  // namespace fdm
@@ -898,6 +927,7 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   if (!sets || nsets < 1 || nsets > kMaxSets || !workList || !workCount) return FDM_EINVAL;
   NarrowMulti k;
   fillCommon(k.c, sets[0]);
+  k.gate = sets[0].gate;
   k.nsets = nsets;
   long long cap = 0;
   for (int i = 0; i < nsets; ++i) {
@@ -909,7 +939,7 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   for (int i = nsets; i < kMaxSets; ++i) k.s[i] = k.s[0];
   const unsigned cullBlocks = std::max<unsigned>(1, std::min<unsigned>(blocksFor(cap, 256), 2048));
   hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, workList, workCount);
-  hipLaunchKernelGGL(kNarrowWork, dim3(persistentBlocks(4)), dim3(256), 0, stream, k,
+  hipLaunchKernelGGL(kNarrowWork, dim3(persistentBlocks(1)), dim3(64), 0, stream, k,
                      (const int*)workList, (const int*)workCount);
   return launchStatus();
 }

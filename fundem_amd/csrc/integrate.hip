@@ -109,12 +109,11 @@ __global__ void kCommitState(const double* x, int n, fdm_grid grid, int* atBuild
 // of orientations to FP32 (a relative 2^-24 per component moves a point at
 // distance rho by well under 1e-6 rho). Ghost lists are also rebuilt after
 // a visibility change or a wrap; a wrap (positions jump by a period)
-// rebuilds the next step's lists as well. Control words written by other
-// workgroups of the same launch are read through atomics (at the device
-// cache, not a workgroup's own).
+// rebuilds the next step's lists as well. One thread of one workgroup runs
+// it, after every writer of the control words has finished.
 __device__ void decideStep(const fdm_skin& s) {
   int* c = s.ctl;
-  auto rd = [&](int w) { return atomicAdd(c + w, 0); };
+  auto rd = [&](int w) { return *(volatile int*)(c + w); };
   const float rate = __int_as_float(rd(FDM_CTL_MAX_RATE));
   double d = *s.disp + s.dt * (double)rate * (1.0 + 1e-6) + 1e-6 * s.rhoMax;
   const bool rebuild = rd(FDM_CTL_FORCE) != 0 || rd(FDM_CTL_PENDING) != 0 ||
@@ -129,6 +128,7 @@ __device__ void decideStep(const fdm_skin& s) {
   *s.disp = d;
   c[FDM_CTL_REBUILD] = rebuild ? 1 : 0;
   c[FDM_CTL_REBUILD_GHOST] = ghost ? 1 : 0;
+  c[FDM_CTL_KEEP] = (rebuild || ghost) ? 0 : 1;
   c[FDM_CTL_PENDING] = wrapped ? 1 : 0;
   c[FDM_CTL_MAX_RATE] = 0;
   c[FDM_CTL_FORCE] = 0;
@@ -230,11 +230,14 @@ __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
   }
 }
 
-// Folds the workgroup slots of kStepBegin into the control words, takes the
-// list decision and clears the given counters (one workgroup).
+// Clears the given counter arrays, folds the workgroup slots of kStepBegin
+// into the control words and takes the list decision (one workgroup).
 __global__ void __launch_bounds__(1024) kStepDecide(StepBeginK k, int nblocks) {
   __shared__ float rmax[16];
   __shared__ int fl[16];
+  for (int q = 0; q < 8; ++q)
+    if (k.a.clearCounters[q])
+      for (int i = threadIdx.x; i < FDM_REC_COUNTERS; i += blockDim.x) k.a.clearCounters[q][i] = 0;
   float r = 0.f;
   int f = 0;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
@@ -264,8 +267,6 @@ __global__ void __launch_bounds__(1024) kStepDecide(StepBeginK k, int nblocks) {
   if (f & 2) c[FDM_CTL_GHOST_CHANGED] = 1;
   if (f & 4) c[FDM_CTL_WRAPPED] = 1;
   decideStep(k.s);
-  for (int q = 0; q < 8; ++q)
-    if (k.a.clearCounters[q]) *k.a.clearCounters[q] = 0;
 }
 
 // Copies the candidate lists of the previous step when no rebuild happens.
@@ -326,8 +327,10 @@ extern "C" size_t fdm_stepBeginWorkspaceBytes(size_t bodies) {
   return alignUp(nb * sizeof(float)) + alignUp(nb * sizeof(int));
 }
 
-extern "C" int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin,
-                                   void* workspace, size_t workspaceBytes, hipStream_t stream) {
+namespace {
+
+int stepBeginSetup(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
+                   size_t workspaceBytes, StepBeginK& k, unsigned& nb) {
   if (!args || !skin || !skin->ctl || !skin->disp) return FDM_EINVAL;
   const fdm_step_begin_args& a = *args;
   if (a.particles.n < 0 || a.walls.n < 0) return FDM_EINVAL;
@@ -341,15 +344,33 @@ extern "C" int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_sk
   }
   const long long total = (long long)a.particles.n + a.walls.n;
   if (workspaceBytes < fdm_stepBeginWorkspaceBytes((size_t)total)) return FDM_EINVAL;
-  const unsigned nb = std::max<unsigned>(1, blocksFor(total, 256));
+  nb = std::max<unsigned>(1, blocksFor(total, 256));
   Carve cv(workspace, workspaceBytes);
-  StepBeginK k;
   k.a = a;
   k.s = *skin;
   k.blockRate = cv.take<float>(nb);
   k.blockFlags = cv.take<int>(nb);
-  if (!cv.ok) return FDM_EINVAL;
+  return cv.ok ? FDM_OK : FDM_EINVAL;
+}
+
+}  // namespace
+
+extern "C" int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin,
+                                   void* workspace, size_t workspaceBytes, hipStream_t stream) {
+  StepBeginK k;
+  unsigned nb = 0;
+  const int r = stepBeginSetup(args, skin, workspace, workspaceBytes, k, nb);
+  if (r != FDM_OK) return r;
   hipLaunchKernelGGL(kStepBegin, dim3(nb), dim3(256), 0, stream, k);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchStepDecide(const fdm_step_begin_args* args, const fdm_skin* skin,
+                                    void* workspace, size_t workspaceBytes, hipStream_t stream) {
+  StepBeginK k;
+  unsigned nb = 0;
+  const int r = stepBeginSetup(args, skin, workspace, workspaceBytes, k, nb);
+  if (r != FDM_OK) return r;
   hipLaunchKernelGGL(kStepDecide, dim3(1), dim3(1024), 0, stream, k, (int)nb);
   return launchStatus();
 }

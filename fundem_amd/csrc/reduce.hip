@@ -45,6 +45,7 @@ __device__ __forceinline__ int vbond(V3& F, V3& Mij, V3& Mji, double& sig, doubl
 
 // addBondedForceTorqueKernel (contactKernel.cu:293-371), results per bond.
 __global__ void kBonds(fdm_bodies P, fdm_bonds b) {
+  if (gateOff(b.gate)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= b.n) return;
   if (b.active[k] == 0) {
@@ -94,6 +95,7 @@ struct ReduceK {
   double h;
   int kick;
   int* stepCounter;
+  const int* gate;
 };
 
 __device__ __forceinline__ void addRec(V3& F, V3& T, const double* f, const double* t, bool negF) {
@@ -121,6 +123,7 @@ __device__ __forceinline__ int segEnd(const fdm_pairs& s, int i) {
 }
 
 __global__ void kReduce(ReduceK a) {
+  if (gateOff(a.gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = a.P.n;
   if (i == 0 && a.stepCounter) *a.stepCounter += 1;  // read by the bond stage before this launch
@@ -242,6 +245,7 @@ extern "C" int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStrea
   k.h = g.h;
   k.kick = g.kick;
   k.stepCounter = g.stepCounter;
+  k.gate = g.gate;
   if (g.particles.n <= 0) {
     if (g.stepCounter) hipLaunchKernelGGL(kIncrement, dim3(1), dim3(64), 0, stream, g.stepCounter);
     return launchStatus();

@@ -244,7 +244,9 @@ class LSSolver {
                "ctl");
     }
     const int pairCap = (int)std::max<size_t>(size_t(pairsPerParticle_) * N, 1024);
-    const int recCap = (int)std::max<size_t>(size_t(recordsPerParticle_) * N, 4096);
+    // records come from FDM_REC_POOLS pools: each gets a share plus 1024
+    const int recCap = (int)(std::max<size_t>(size_t(recordsPerParticle_) * N, 4096) +
+                             size_t(FDM_REC_POOLS) * 1024);
     pp_.initialize(N, pairCap, recCap, mirrorCapacity_);
     pw_.initialize(N, (int)std::max<size_t>(N * std::max<size_t>(W, 1), 1024), recCap, 1);
     periodic_.initialize(particles_, (int)std::max<size_t>(N, 64),
@@ -260,7 +262,7 @@ class LSSolver {
         if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
       workList_.allocate((size_t)cap, false);
-      workCount_.allocate(1);
+      workCount_.allocate(FDM_REC_COUNTERS);
     }
     status_.allocate(1);
     stepCounter_.allocate(1);
@@ -608,10 +610,13 @@ class LSSolver {
       if (periodic_.active(s)) a.clearCounters[q++] = periodic_.set(s).interaction.nextRecordCounter();
     a.clearCounters[q++] = workCount_.get();
     const fdm_skin sk = skinView();
-    Timer t(this, "step_begin");
-    fdmCheck(fdm_launchStepBegin(&a, &sk, workspace_.get(), workspace_.size() * sizeof(double),
-                                 stream_),
-             "step begin");
+    const size_t wsBytes = workspace_.size() * sizeof(double);
+    {
+      Timer t(this, "step_begin");
+      fdmCheck(fdm_launchStepBegin(&a, &sk, workspace_.get(), wsBytes, stream_), "step begin");
+    }
+    Timer t(this, "decide");
+    fdmCheck(fdm_launchStepDecide(&a, &sk, workspace_.get(), wsBytes, stream_), "step decision");
   }
 
   struct Decision {
@@ -644,15 +649,14 @@ class LSSolver {
       if (periodic_.active(s)) periodic_.set(s).interaction.beginStep(d.ghost);
   }
 
-  void phaseB(bool prime, double time, Decision d) {
+  // List builds of a step (the part of phase B that runs only when the
+  // decision asks for it).
+  void phaseBRebuild(Decision d) {
     fdm_bodies P = particles_.bodies();
-    fdm_geometry pgeo = particles_.geometry();
-    const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
     const size_t N = particles_.num(), W = walls_.num();
     void* ws = workspace_.get();
     const size_t wsBytes = workspace_.size() * sizeof(double);
     const fdm_skin sk = skinView();
-    beginSets(d);
     if (d.rebuild) {
       {
         Timer t(this, "sort_cells");
@@ -674,6 +678,30 @@ class LSSolver {
       broadPhase(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(), particles_.table().view());
       if (W) broadPhase(FDM_SET_WALL, pw_, nullptr, walls_.grid(), walls_.table().view());
     }
+    if (periodic_.isActived() && d.ghost) {
+      fdm_ghosts gv[3];
+      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
+      {
+        Timer t(this, "periodic_build");
+        fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
+                                         status_.get(), ws, wsBytes, stream_),
+                 "periodic ghosts");
+      }
+      for (int s = 0; s < 3; ++s)
+        if (periodic_.active(s))
+          broadPhase(FDM_SET_GHOST, periodic_.set(s).interaction, &gv[s], particles_.grid(),
+                     periodic_.set(s).table.view());
+    }
+  }
+
+  // The part of phase B every step runs: the narrow phase of all sets, bonds
+  // and the reduction with the second kick. With `gate` each kernel runs
+  // only if *gate != 0 (queued before the decision is known).
+  void phaseBCommon(bool prime, double time, Decision d, const int* gate) {
+    fdm_bodies P = particles_.bodies();
+    fdm_geometry pgeo = particles_.geometry();
+    const fdm_double3 g = {gravity_.x, gravity_.y, gravity_.z};
+    const size_t N = particles_.num(), W = walls_.num();
     fdm_narrowphase_args na[5];
     int ns = 0;
     na[ns++] = narrowArgs(FDM_SET_PARTICLE, pp_, nullptr, particles_.grid(),
@@ -686,31 +714,18 @@ class LSSolver {
     const fdm_mirror* gmir[3] = {nullptr, nullptr, nullptr};
     fdm_pairs gcur[3];
     fdm_mirror gm[3];
-    if (periodic_.isActived()) {
-      for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
-      if (d.ghost) {
-        {
-          Timer t(this, "periodic_build");
-          fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
-                                           status_.get(), ws, wsBytes, stream_),
-                   "periodic ghosts");
-        }
-        for (int s = 0; s < 3; ++s)
-          if (periodic_.active(s))
-            broadPhase(FDM_SET_GHOST, periodic_.set(s).interaction, &gv[s], particles_.grid(),
-                       periodic_.set(s).table.view());
-      }
-      for (int s = 0; s < 3; ++s) {
-        if (!periodic_.active(s)) continue;
-        GhostSet& G = periodic_.set(s);
-        na[ns++] = narrowArgs(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(),
-                              G.table.hashValue.get(), s + 1, !d.ghost, prime);
-        gcur[s] = G.interaction.current();
-        gm[s] = G.interaction.mirror();
-        gpairs[s] = &gcur[s];
-        gmir[s] = &gm[s];
-      }
+    for (int s = 0; s < 3; ++s) {
+      if (!periodic_.active(s)) continue;
+      gv[s] = periodic_.view(s);
+      GhostSet& G = periodic_.set(s);
+      na[ns++] = narrowArgs(FDM_SET_GHOST, G.interaction, &gv[s], particles_.grid(),
+                            G.table.hashValue.get(), s + 1, !d.ghost, prime);
+      gcur[s] = G.interaction.current();
+      gm[s] = G.interaction.mirror();
+      gpairs[s] = &gcur[s];
+      gmir[s] = &gm[s];
     }
+    na[0].gate = gate;
     {
       Timer t(this, "narrow");
       fdmCheck(fdm_launchNarrowPhaseSets(na, ns, workList_.get(), (int)workList_.size(),
@@ -726,6 +741,7 @@ class LSSolver {
       return;
     }
     fdm_bonds bv = bonds_.view();
+    bv.gate = gate;
     if (bv.n > 0) {
       Timer t(this, "bonds");
       fdmCheck(fdm_launchBondForces(P, bv, stream_), "bonds");
@@ -746,6 +762,7 @@ class LSSolver {
     r.gravity = g;
     r.h = 0.5 * dt_;
     r.stepCounter = stepCounter_.get();
+    r.gate = gate;
     {
       Timer t(this, "reduce_kick");
       if (hookMode_ == HookNone) {
@@ -763,9 +780,46 @@ class LSSolver {
     }
   }
 
+  void phaseB(bool prime, double time, Decision d) {
+    beginSets(d);
+    if (d.rebuild || d.ghost) phaseBRebuild(d);
+    phaseBCommon(prime, time, d, nullptr);
+  }
+
+  struct GenState {
+    SolidInteraction::Generations g[5];
+  };
+  GenState saveGenerations() {
+    GenState s;
+    s.g[0] = pp_.generations();
+    s.g[1] = pw_.generations();
+    for (int k = 0; k < 3; ++k) s.g[2 + k] = periodic_.set(k).interaction.generations();
+    return s;
+  }
+  void restoreGenerations(const GenState& s) {
+    pp_.setGenerations(s.g[0]);
+    pw_.setGenerations(s.g[1]);
+    for (int k = 0; k < 3; ++k) periodic_.set(k).interaction.setGenerations(s.g[2 + k]);
+  }
+
   void stepOnce(bool prime, double time) {
     phaseA(prime);
-    phaseB(prime, time, waitDecision());
+    if (prime || hookMode_ != HookNone) {
+      phaseB(prime, time, waitDecision());
+      return;
+    }
+    // Queue the step that keeps every list before the decision is known
+    // (its kernels test the keep word), so the device does not wait for the
+    // host; when the decision asks for a rebuild, queue the full step after.
+    const GenState before = saveGenerations();
+    const Decision keep = {false, false};
+    beginSets(keep);
+    phaseBCommon(false, time, keep, ctl_.get() + FDM_CTL_KEEP);
+    const Decision d = waitDecision();
+    if (d.rebuild || d.ghost) {
+      restoreGenerations(before);
+      phaseB(false, time, d);
+    }
   }
 
   int ghostListIndex() {
@@ -782,7 +836,21 @@ class LSSolver {
     hipGraphExec_t& ga = graphsA_[stepNext];
     if (!ga) ga = capture([&] { phaseA(false); });
     hipCheck(hipGraphLaunch(ga, stream_), "graph launch");
+    const GenState before = saveGenerations();
+    const Decision keep = {false, false};
+    hipGraphExec_t& gs = graphsKeep_[stepNext][pp_.listIndex()][ghostListIndex()];
+    if (!gs) {
+      gs = capture([&] {
+        beginSets(keep);
+        phaseBCommon(false, 0.0, keep, ctl_.get() + FDM_CTL_KEEP);
+      });
+    } else {
+      beginSets(keep);
+    }
+    hipCheck(hipGraphLaunch(gs, stream_), "graph launch");
     const Decision d = waitDecision();
+    if (!(d.rebuild || d.ghost)) return;
+    restoreGenerations(before);
     const int mainNext = pp_.listIndex() ^ (d.rebuild ? 1 : 0);
     const int ghostNext = ghostListIndex() ^ (d.ghost ? 1 : 0);
     hipGraphExec_t& gb = graphsB_[stepNext][mainNext][ghostNext][d.rebuild ? 1 : 0][d.ghost ? 1 : 0];
@@ -810,6 +878,11 @@ class LSSolver {
     for (size_t k = 0; k < sizeof(graphsB_) / sizeof(graphsB_[0][0][0][0][0]); ++k) {
       if (b[k]) (void)hipGraphExecDestroy(b[k]);
       b[k] = nullptr;
+    }
+    hipGraphExec_t* c = &graphsKeep_[0][0][0];
+    for (size_t k = 0; k < sizeof(graphsKeep_) / sizeof(graphsKeep_[0][0][0]); ++k) {
+      if (c[k]) (void)hipGraphExecDestroy(c[k]);
+      c[k] = nullptr;
     }
   }
 
@@ -840,6 +913,7 @@ class LSSolver {
   HookMode hookMode_ = HookUnknown;
   hipGraphExec_t graphsA_[2] = {};
   hipGraphExec_t graphsB_[2][2][2][2][2] = {};
+  hipGraphExec_t graphsKeep_[2][2][2] = {};
   int* hostDecision_ = nullptr;     // pinned, host view
   int* hostDecisionDev_ = nullptr;  // the same memory, device view
   int decisionSeq_ = 0;

@@ -42,7 +42,7 @@ struct StepBuffers {
     recCapacity = std::max(recCap, 1);
     recOff.allocate(cap);
     recCnt.allocate(cap);
-    recTotal.allocate(1);
+    recTotal.allocate(FDM_REC_COUNTERS);
     sumF.allocate(3 * (size_t)cap);
     sumTA.allocate(3 * (size_t)cap);
     sumTB.allocate(3 * (size_t)cap);
@@ -101,6 +101,15 @@ struct SolidInteraction {
   // Generation roles back to their initial state (every set of a solver
   // then flips in step).
   void resetGenerations() { list_cur_ = list_prev_ = step_cur_ = 0; }
+  struct Generations {
+    int listCur, listPrev, stepCur;
+  };
+  Generations generations() const { return {list_cur_, list_prev_, step_cur_}; }
+  void setGenerations(Generations g) {
+    list_cur_ = g.listCur;
+    list_prev_ = g.listPrev;
+    step_cur_ = g.stepCur;
+  }
 
   // Start of a step: a new results generation, and a new list generation
   // when the lists are rebuilt.
@@ -145,8 +154,10 @@ struct SolidInteraction {
     if (np <= 0) return out;
     std::vector<int> A = l.pairA.toHost(s, np), T = l.pairT.toHost(s, np);
     std::vector<int> off = g.recOff.toHost(s, np), cnt = g.recCnt.toHost(s, np);
-    std::vector<int> rt = g.recTotal.toHost(s, 1);
-    const int nr = std::min(rt[0], g.recCapacity);
+    int nr = 0;  // records are spread over pools: read up to the last one used
+    for (int p = 0; p < np; ++p)
+      if (cnt[p] > 0) nr = std::max(nr, off[p] + cnt[p]);
+    nr = std::min(nr, g.recCapacity);
     std::vector<float4> rec = g.rec.toHost(s, std::max(nr, 0));
     for (int p = 0; p < np; ++p)
       for (int r = off[p]; r < off[p] + cnt[p] && r < nr; ++r) {

@@ -48,6 +48,12 @@ extern "C" {
 #define FDM_STATUS_GHOST_OVERFLOW 8u    /* ghost pair list exceeded capacity */
 
 #define FDM_CHUNK 64         /* surface nodes per culling chunk (one wavefront) */
+/* History records are reserved from FDM_REC_POOLS pools, each with its own
+ * counter on its own cache line, so concurrent reservations do not all
+ * serialise on one address. */
+#define FDM_REC_POOLS 64
+#define FDM_REC_STRIDE 32
+#define FDM_REC_COUNTERS (FDM_REC_POOLS * FDM_REC_STRIDE)
 #define FDM_MAX_CHUNKS 64    /* chunks per template: at most 4096 nodes */
 
 typedef struct { double x, y, z; } fdm_double3;
@@ -146,6 +152,7 @@ typedef struct {
 #define FDM_CTL_COUNT 8          /* list builds so far */
 #define FDM_CTL_GHOST_COUNT 9    /* ghost list builds so far */
 #define FDM_CTL_SEQ 10           /* decisions taken so far */
+#define FDM_CTL_KEEP 11          /* 1 when this step keeps every list (no rebuild at all) */
 #define FDM_CTL_WORDS 16
 typedef struct {
   double skin;         /* extra reach of the candidate lists (0: rebuild every step) */
@@ -171,8 +178,9 @@ typedef struct {
   double* sumF;        /* [3*capacity] per-pair force sum (pair order, node order) */
   double* sumTA;       /* [3*capacity] torque sum about A */
   double* sumTB;       /* [3*capacity] torque sum about the target */
-  int recCapacity;
-  int* recTotal;       /* [1] records used */
+  int recCapacity;     /* records, split into FDM_REC_POOLS equal pools */
+  int* recTotal;       /* [FDM_REC_COUNTERS] records used per pool, pool j at
+                          recTotal[j * FDM_REC_STRIDE] */
   float4* rec;         /* [recCapacity] (spring xyz, node id as int bits), per pair by node */
 } fdm_pairs;
 
@@ -215,6 +223,7 @@ typedef struct {
   const int* partStart;    /* [np+1] CSR of bonds per particle (ascending bond id) */
   const int* partList;     /* bond id * 2 + role (0 master, 1 slave) */
   const int* stepIndex;    /* [1] device step counter, recorded in breakStep */
+  const int* gate;         /* run only when *gate != 0 (NULL: always) */
 } fdm_bonds;
 
 /* Periodic configuration (PeriodicBoundaryXY2D, PeriodicBoundary.h:178-317). */
@@ -251,13 +260,15 @@ int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_double3 gra
  * ctl[FDM_CTL_REBUILD_GHOST] (after fdm_launchPeriodicPrepare, when
  * periodic); a wrap seen by the prepare stage also rebuilds the next step. */
 int fdm_launchSkinDecide(const fdm_skin* skin, hipStream_t stream);
-/* No counterpart in the reference: the first phase of a step in one launch.
- * Kick (when `kick`), drift (when `drift`) and cell id of every particle,
- * drift and cell id of every wall (as fdm_launchKickDriftHash), the periodic
- * visibility flags of every particle (as fdm_launchPeriodicPrepare), then the
- * list decision of fdm_launchSkinDecide from per-workgroup partials held in
- * `workspace` (fdm_stepBeginWorkspaceBytes(particles + walls) bytes). It also
- * clears the given counters for the narrow phase. */
+/* No counterpart in the reference: the first phase of a step in two
+ * launches. fdm_launchStepBegin: kick (when `kick`), drift (when `drift`)
+ * and cell id of every particle, drift and cell id of every wall (as
+ * fdm_launchKickDriftHash), the periodic visibility flags of every particle
+ * (as fdm_launchPeriodicPrepare), leaving per-workgroup partials in
+ * `workspace` (fdm_stepBeginWorkspaceBytes(particles + walls) bytes).
+ * fdm_launchStepDecide (same arguments, after it): clears the given counter
+ * arrays and takes the list decision of fdm_launchSkinDecide from the
+ * partials. */
 typedef struct {
   fdm_bodies particles;
   fdm_geometry pgeo;
@@ -273,11 +284,13 @@ typedef struct {
   fdm_periodic periodic;
   int* ghostFlag[3];           /* per particle visibility flag of sets X, Y, XY */
   int* ghostAtBuild;           /* [n] visibility bits at the last ghost build */
-  int* clearCounters[8];       /* counters set to 0 (NULL entries skipped) */
+  int* clearCounters[8];       /* arrays of FDM_REC_COUNTERS ints set to 0 (NULL: skipped) */
 } fdm_step_begin_args;
 size_t fdm_stepBeginWorkspaceBytes(size_t bodies);
 int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
                         size_t workspaceBytes, hipStream_t stream);
+int fdm_launchStepDecide(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
+                         size_t workspaceBytes, hipStream_t stream);
 /* No counterpart in the reference. When ctl[FDM_CTL_REBUILD] is set, records
  * every body's grid state in table.validAtBuild. */
 int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
@@ -420,6 +433,8 @@ typedef struct {
   int ghostFromReal;           /* ghost sets: take each ghost's position and cell id from
                                   its real particle (valid when no particle wrapped this
                                   step) instead of ghosts->pos / ghosts->table.hashValue */
+  const int* gate;             /* fdm_launchNarrowPhaseSets: run only when *gate != 0
+                                  (NULL: always; taken from sets[0]) */
   double dt;
   int updateSprings;           /* 0: keep remapped springs (history filtering) */
   unsigned* status;
@@ -437,7 +452,8 @@ int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
  * geometry, materials, pgrid, periodic, dt, updateSprings, status, dbg*) come
  * from sets[0]. workCapacity must cover the sum of the sets' pair capacities.
  * Unlike fdm_launchNarrowPhase it clears no counter: the sets' record
- * counters (pairs.recTotal) and *workCount must be zero at launch. */
+ * counters (pairs.recTotal) and *workCount must be zero at launch. Pair
+ * work item w reserves its records from pool w % FDM_REC_POOLS. */
 int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets, int* workList,
                               int workCapacity, int* workCount, hipStream_t stream);
 
@@ -473,6 +489,7 @@ typedef struct {
   double h;
   int kick;
   int* stepCounter;    /* incremented by one after the reduction (NULL: skip) */
+  const int* gate;     /* run only when *gate != 0 (NULL: always) */
 } fdm_reduce_args;
 int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream);
 
