@@ -127,6 +127,13 @@ int fdmx_set_periodic(void* s, int x, int y) {
     if (y) S->addPeriodicBoundaryYD();
   });
 }
+int fdmx_add_system(void* s, const double* mn, const double* mx, int px, int py) {
+  int k = -1;
+  const int r = guarded([&] {
+    k = static_cast<BoundSolver*>(s)->addSystem(d3(mn), d3(mx), px != 0, py != 0);
+  });
+  return r != 0 ? r : k;
+}
 int fdmx_set_cell_size(void* s, double cs) {
   return guarded([&] { static_cast<BoundSolver*>(s)->setCellSize(cs); });
 }

@@ -114,6 +114,7 @@ struct BroadK {
   double skin;
   double maxReach;
   const int* gate;
+  fdm_instances inst;
 };
 
 template <int SET>
@@ -163,32 +164,40 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
   const int n = a.P.n;
   const V3 xA = ldp(a.P.x, n, A);
   const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
+  // with several instances: A's own grid, and its instance's cells and walls
+  const bool many = a.inst.count > 0;
+  const int in = many ? a.inst.particleInstance[A] : 0;
   if (SET == FDM_SET_WALL) {
-    for (int w = 0; w < a.W.n; ++w) {
+    const fdm_grid& wg = many ? a.inst.wgrid[in] : a.grid;
+    const int w0 = many ? a.inst.wallStart[in] : 0;
+    const int w1 = many ? a.inst.wallStart[in + 1] : a.W.n;
+    for (int w = w0; w < w1; ++w) {
       const int h = a.table.hashValue[w];
-      if (h < 0 || (long long)h >= a.grid.num) continue;
+      if (h < 0 || (long long)h >= wg.num) continue;
       if (acceptCandidate<SET>(a, A, xA, RA, w, w, ldp(a.W.x, a.W.n, w))) emit(w, w);
     }
     return;
   }
+  const fdm_grid& grid = many ? a.inst.pgrid[in] : a.grid;
+  const long long cellOffset = many ? a.inst.pcellOffset[in] : 0;
   I3 b0, b1;
-  cellRange(xA, (RA + a.maxReach) * kReach + a.skin, a.grid, b0, b1);
+  cellRange(xA, (RA + a.maxReach) * kReach + a.skin, grid, b0, b1);
   if (SET == FDM_SET_PARTICLE) {
     I3 lo, hi;
-    cellRange(xA, RA * kReach + a.skin, a.grid, lo, hi);
+    cellRange(xA, RA * kReach + a.skin, grid, lo, hi);
     b0 = {min(b0.x, max(lo.x - 1, 0)), min(b0.y, max(lo.y - 1, 0)), min(b0.z, max(lo.z - 1, 0))};
-    b1 = {max(b1.x, min(hi.x + 1, a.grid.size.x - 1)), max(b1.y, min(hi.y + 1, a.grid.size.y - 1)),
-          max(b1.z, min(hi.z + 1, a.grid.size.z - 1))};
+    b1 = {max(b1.x, min(hi.x + 1, grid.size.x - 1)), max(b1.y, min(hi.y + 1, grid.size.y - 1)),
+          max(b1.z, min(hi.z + 1, grid.size.z - 1))};
   } else {
     if (*a.ghostCount <= 0) return;
     b0 = {max(b0.x, a.ghostBox[0]), max(b0.y, a.ghostBox[1]), max(b0.z, a.ghostBox[2])};
     b1 = {min(b1.x, a.ghostBox[3]), min(b1.y, a.ghostBox[4]), min(b1.z, a.ghostBox[5])};
   }
-  const I3 sz = gsize(a.grid);
+  const I3 sz = gsize(grid);
   for (int cz = b0.z; cz <= b1.z; ++cz)
     for (int cy = b0.y; cy <= b1.y; ++cy)
       for (int cx = b0.x; cx <= b1.x; ++cx) {
-        const int h = lin3({cx, cy, cz}, sz);
+        const long long h = cellOffset + lin3({cx, cy, cz}, sz);
         const int s = a.table.cellStart[h];
         if (s < 0) continue;
         const int e = a.table.cellEnd[h];
@@ -268,6 +277,7 @@ struct NarrowCommon {
   const fdm_material* wmat;
   fdm_grid pgrid;
   fdm_periodic periodic;
+  fdm_instances inst;
   double dt;
   int updateSprings;
   unsigned* status;
@@ -305,16 +315,39 @@ struct NarrowMulti {
 
 // Position used for force arms: the stored position after this step's
 // periodic wrap (the reference computes forces after updateGhostSpatialGrid).
-__device__ __forceinline__ V3 wrappedArm(V3 x, const NarrowCommon& a) {
-  if (a.periodic.x) {
-    if (x.x < a.pgrid.min.x) x = vadd(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
-    else if (x.x >= a.pgrid.max.x) x = vsub(x, {a.pgrid.max.x - a.pgrid.min.x, 0., 0.});
+__device__ __forceinline__ V3 wrappedArm(V3 x, const fdm_grid& g, fdm_periodic per) {
+  if (per.x) {
+    if (x.x < g.min.x) x = vadd(x, {g.max.x - g.min.x, 0., 0.});
+    else if (x.x >= g.max.x) x = vsub(x, {g.max.x - g.min.x, 0., 0.});
   }
-  if (a.periodic.y) {
-    if (x.y < a.pgrid.min.y) x = vadd(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
-    else if (x.y >= a.pgrid.max.y) x = vsub(x, {0., a.pgrid.max.y - a.pgrid.min.y, 0.});
+  if (per.y) {
+    if (x.y < g.min.y) x = vadd(x, {0., g.max.y - g.min.y, 0.});
+    else if (x.y >= g.max.y) x = vsub(x, {0., g.max.y - g.min.y, 0.});
   }
   return x;
+}
+
+// The grids and periodic flags that apply to a pair whose A is `A`: its
+// instance's, or the single system's.
+struct PairFrame {
+  const fdm_grid* pgrid;   // particle grid (wrap, ghosts, particle cell ids)
+  const fdm_grid* tgrid;   // grid of the target's cell ids (the wall grid for walls)
+  fdm_periodic periodic;
+};
+template <int SET>
+__device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const NarrowSet& S, int A) {
+  PairFrame f;
+  if (C.inst.count > 0) {
+    const int in = C.inst.particleInstance[A];
+    f.pgrid = &C.inst.pgrid[in];
+    f.tgrid = SET == FDM_SET_WALL ? &C.inst.wgrid[in] : f.pgrid;
+    f.periodic = C.inst.periodic[in];
+  } else {
+    f.pgrid = &C.pgrid;
+    f.tgrid = &S.grid;
+    f.periodic = C.periodic;
+  }
+  return f;
 }
 
 // The reference's node predicate for one (node, target) candidate.
@@ -370,7 +403,7 @@ __device__ __forceinline__ V3 findPrevSpring(const NarrowSet& a, int off, int cn
 // SDF, cell id, and the position its force arm uses.
 template <int SET>
 __device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const NarrowSet& S, int p, int T,
-                                              Target& tg, V3& armB) {
+                                              const PairFrame& f, Target& tg, V3& armB) {
   const int n = C.P.n;
   if (SET == FDM_SET_WALL) {
     tg.x = ldp(C.W.x, C.W.n, T);
@@ -385,14 +418,14 @@ __device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const Narro
     tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
     tg.sdf = C.pgeo.sdf;
     tg.hash = S.targetHash[T];
-    armB = wrappedArm(tg.x, C);
+    armB = wrappedArm(tg.x, *f.pgrid, f.periodic);
   } else {
     if (S.ghostFromReal) {
       // no wrap happened this step: the ghost is the real particle shifted
       // by the periods, as the ghost stage computes it
-      const WrapStages w = wrapStages(ldp(C.P.x, n, T), C.pgrid, C.periodic);
-      tg.x = ghostOf(S.ghostMask == 1 ? w.afterX : w.afterY, C.pgrid, S.ghostMask);
-      tg.hash = ghostCellHash(tg.x, C.pgrid);
+      const WrapStages w = wrapStages(ldp(C.P.x, n, T), *f.pgrid, f.periodic);
+      tg.x = ghostOf(S.ghostMask == 1 ? w.afterX : w.afterY, *f.pgrid, S.ghostMask);
+      tg.hash = ghostCellHash(tg.x, *f.pgrid);
     } else {
       const int G = S.pairs.pairG[p];
       tg.x = ldd3(S.ghostPos + 3 * G);
@@ -436,12 +469,14 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
   const V3 xA = ldp(C.P.x, n, A);
   const Q4 qA = ld4(C.P.q, n, A);
+  const PairFrame fr = pairFrame<SET>(C, S, A);
+  const fdm_grid& tgrid = *fr.tgrid;
   Target tg;
   V3 armB;
-  resolveTarget<SET>(C, S, p, T, tg, armB);
-  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, C);
+  resolveTarget<SET>(C, S, p, T, fr, tg, armB);
+  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
   const double RB = tg.t->radius;
-  const I3 tcell = decodeHash(tg.hash, S.grid);
+  const I3 tcell = decodeHash(tg.hash, tgrid);
   const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
   const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
   const int nChunks = tA.chunkCount;
@@ -461,7 +496,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     }
     if (live) {
       I3 lo, hi;
-      cellRange(cw, rc, S.grid, lo, hi);
+      cellRange(cw, rc, tgrid, lo, hi);
       if (!cellNear(tcell, lo, hi)) live = false;
     }
     if (live) {
@@ -507,7 +542,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     V3 pn;
     if (k < tA.nodeCount) {
       pn = vadd(qrotate(qA, s), xA);
-      hit = nodeHit<SET>(pn, tg, tcell, S.grid, RB, sm);
+      hit = nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
     }
     const unsigned long long m = ballot64(hit);
     if (lane == c) myMask = m;
@@ -626,7 +661,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
           const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
           const V3 pn = vadd(qrotate(qA, s), xA);
           Sample sm;
-          nodeHit<SET>(pn, tg, tcell, S.grid, RB, sm);
+          nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
           const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
           const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
           const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
@@ -705,13 +740,13 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
         V3 armB;
         double reachT;
         if (S.kind == FDM_SET_WALL) {
-          resolveTarget<FDM_SET_WALL>(a.c, S, p, T, tg, armB);
+          resolveTarget<FDM_SET_WALL>(a.c, S, p, T, pairFrame<FDM_SET_WALL>(a.c, S, A), tg, armB);
           reachT = tg.t->boxReach;
         } else if (S.kind == FDM_SET_PARTICLE) {
-          resolveTarget<FDM_SET_PARTICLE>(a.c, S, p, T, tg, armB);
+          resolveTarget<FDM_SET_PARTICLE>(a.c, S, p, T, pairFrame<FDM_SET_PARTICLE>(a.c, S, A), tg, armB);
           reachT = tg.t->radius;
         } else {
-          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, tg, armB);
+          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, pairFrame<FDM_SET_GHOST>(a.c, S, A), tg, armB);
           reachT = tg.t->boxReach;
         }
         const V3 d = vsub(xA, tg.x);
@@ -790,6 +825,7 @@ extern "C" int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* works
   k.skin = g.skin;
   k.maxReach = g.maxReach;
   k.gate = g.gate;
+  k.inst = g.inst;
   if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
   if (!(g.skin >= 0.0) || !(g.maxReach >= 0.0)) return FDM_EINVAL;
   const unsigned nb = blocksFor((long long)n + 1, 128);
@@ -838,6 +874,7 @@ extern "C" int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsign
   k.skin = g.skin;
   k.maxReach = g.maxReach;
   k.gate = g.gate;
+  k.inst = g.inst;
   if (n <= 0) return FDM_OK;
   const unsigned nb = blocksFor(n, 128);
   switch (g.set) {
@@ -867,6 +904,7 @@ void fillCommon(NarrowCommon& k, const fdm_narrowphase_args& g) {
   k.wmat = g.wmat;
   k.pgrid = g.pgrid;
   k.periodic = g.periodic;
+  k.inst = g.inst;
   k.dt = g.dt;
   k.updateSprings = g.updateSprings;
   k.status = g.status;

@@ -205,6 +205,13 @@ __device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_pe
   return w;
 }
 
+// Sort key of a body of one instance among several: 1 + global cell, or 0
+// when the body is outside its instance's grid (ids rounded up past the last
+// cell are kept out too, as the table never holds them).
+__device__ __forceinline__ unsigned instanceKey(int h, const fdm_grid& g, long long cellOffset) {
+  return (h < 0 || (long long)h >= g.num) ? 0u : (unsigned)(cellOffset + h + 1);
+}
+
 // Grid state of a body: 0 outside the grid (hash -1), 1 inside, 2 inside but
 // with a cell index rounded up to the grid size on some axis, so that its
 // cell id aliases another cell.

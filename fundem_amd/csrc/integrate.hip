@@ -104,6 +104,15 @@ __global__ void kCommitState(const double* x, int n, fdm_grid grid, int* atBuild
     atBuild[i] = bodyState(ldp(x, n, i), grid);
 }
 
+__global__ void kCommitStateInstances(const double* x, int n, fdm_instances I, int isWall,
+                                      int* atBuild, const int* gate) {
+  if (gateOff(gate)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const fdm_grid& g = isWall ? I.wgrid[I.wallInstance[i]] : I.pgrid[I.particleInstance[i]];
+    atBuild[i] = bodyState(ldp(x, n, i), g);
+  }
+}
+
 // Rebuild decision of one step (one thread). The displacement bound grows by
 // dt times the largest motion rate, plus 1e-6 rho per step for the rounding
 // of orientations to FP32 (a relative 2^-24 per component moves a point at
@@ -175,16 +184,21 @@ __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
     const bool wall = i >= nP;
     const fdm_bodies& b = wall ? a.walls : a.particles;
     const fdm_template* tmpl = wall ? a.wgeo.info : a.pgeo.info;
-    const fdm_grid& grid = wall ? a.wgrid : a.pgrid;
     const fdm_hash_table& t = wall ? a.wtable : a.ptable;
     const int j = wall ? i - nP : i;
+    const bool many = a.inst.count > 0;
+    const int in = !many ? 0 : (wall ? a.inst.wallInstance[j] : a.inst.particleInstance[j]);
+    const fdm_grid& grid =
+        !many ? (wall ? a.wgrid : a.pgrid) : (wall ? a.inst.wgrid[in] : a.inst.pgrid[in]);
+    const fdm_periodic perI = many ? a.inst.periodic[in] : a.periodic;
     if (a.kick && !wall) kickOne(b, tmpl, j, g, 0.5 * a.dt);
     V3 x;
     if (a.drift) x = driftOne(b, j, a.dt);
     else x = ldp(b.x, b.n, j);
     const int h = cellHash(x, grid);
     t.hashValue[j] = h;
-    t.sortKey[j] = (unsigned)(h + 1);
+    t.sortKey[j] = !many ? (unsigned)(h + 1)
+                         : instanceKey(h, grid, wall ? a.inst.wcellOffset[in] : a.inst.pcellOffset[in]);
     t.sortIndex[j] = j;
     if (a.drift) {
       const fdm_template& tp = tmpl[b.geo[j]];
@@ -195,20 +209,26 @@ __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
     const int st = bodyState(x, grid);
     if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
     if (!wall && per) {
-      const WrapStages w = wrapStages(x, a.pgrid, a.periodic);
+      // sets exist where any instance is periodic; an instance contributes
+      // only to the sets of its own periodic axes
+      const WrapStages w = wrapStages(x, grid, perI);
       int bits = 0;
       if (a.periodic.x) {
-        const int f = ghostCellHash(ghostOf(w.afterX, a.pgrid, 1), a.pgrid) >= 0 ? 1 : 0;
+        const int f =
+            perI.x && ghostCellHash(ghostOf(w.afterX, grid, 1), grid) >= 0 ? 1 : 0;
         a.ghostFlag[0][j] = f;
         bits |= f;
       }
       if (a.periodic.y) {
-        const int f = ghostCellHash(ghostOf(w.afterY, a.pgrid, 2), a.pgrid) >= 0 ? 1 : 0;
+        const int f =
+            perI.y && ghostCellHash(ghostOf(w.afterY, grid, 2), grid) >= 0 ? 1 : 0;
         a.ghostFlag[1][j] = f;
         bits |= f << 1;
       }
       if (a.periodic.x && a.periodic.y) {
-        const int f = ghostCellHash(ghostOf(w.afterY, a.pgrid, 3), a.pgrid) >= 0 ? 1 : 0;
+        const int f = perI.x && perI.y && ghostCellHash(ghostOf(w.afterY, grid, 3), grid) >= 0
+                          ? 1
+                          : 0;
         a.ghostFlag[2][j] = f;
         bits |= f << 2;
       }
@@ -388,6 +408,17 @@ extern "C" int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_h
   const unsigned nb = std::min<unsigned>(blocksFor(bodies.n, 256), kIntegrateBlocks);
   hipLaunchKernelGGL(kCommitState, dim3(nb), dim3(256), 0, stream, bodies.x, bodies.n, grid,
                      table.validAtBuild, skin->ctl + FDM_CTL_REBUILD);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchCommitBodyStateInstances(fdm_bodies bodies, const fdm_instances* inst,
+                                                  int isWall, fdm_hash_table table,
+                                                  const fdm_skin* skin, hipStream_t stream) {
+  if (bodies.n < 0 || !inst || inst->count <= 0 || !skin || !table.validAtBuild) return FDM_EINVAL;
+  if (bodies.n == 0) return FDM_OK;
+  const unsigned nb = std::min<unsigned>(blocksFor(bodies.n, 256), kIntegrateBlocks);
+  hipLaunchKernelGGL(kCommitStateInstances, dim3(nb), dim3(256), 0, stream, bodies.x, bodies.n,
+                     *inst, isWall, table.validAtBuild, skin->ctl + FDM_CTL_REBUILD);
   return launchStatus();
 }
 

@@ -100,7 +100,8 @@ struct GhostOut {
   unsigned* status;
 };
 
-__device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const fdm_grid& g) {
+__device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const fdm_grid& g,
+                                          long long cellOffset) {
   const int s = o.slot[i];
   if (s >= o.capacity) {
     atomicOr(o.status, FDM_STATUS_GHOST_OVERFLOW);
@@ -115,7 +116,7 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
   // lists are built by the clamped geometric cell (monotone in the position);
   // the reference's ghost cell id above decides the stencil rule
   const I3 c = clampedCell(gp, g);
-  o.sortKey[s] = (unsigned)(lin3(c, gsize(g)) + 1);
+  o.sortKey[s] = (unsigned)(cellOffset + lin3(c, gsize(g)) + 1);
   o.sortIndex[s] = s;
   atomicMin(&o.cellBox[0], c.x);
   atomicMin(&o.cellBox[1], c.y);
@@ -127,31 +128,37 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
 // ghosts of each set, then store the wrapped positions.
-__global__ void kGhostScatter(double* pos, int n, fdm_grid g, fdm_periodic per, GhostOut ox,
-                              GhostOut oy, GhostOut oxy, int* atBuild) {
+// With several instances (I.count > 0) each particle uses its instance's
+// grid and periodic flags; `per` then only says which sets exist.
+__global__ void kGhostScatter(double* pos, int n, fdm_grid g0, fdm_periodic per, GhostOut ox,
+                              GhostOut oy, GhostOut oxy, int* atBuild, fdm_instances I) {
   if (gateOff(ox.gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const WrapStages w = wrapStages(ldp(pos, n, i), g, per);
+  const int in = I.count > 0 ? I.particleInstance[i] : 0;
+  const fdm_grid& g = I.count > 0 ? I.pgrid[in] : g0;
+  const fdm_periodic pi = I.count > 0 ? I.periodic[in] : per;
+  const long long off = I.count > 0 ? I.pcellOffset[in] : 0;
+  const WrapStages w = wrapStages(ldp(pos, n, i), g, pi);
   int bits = 0;
-  if (per.x) {
+  if (per.x && pi.x) {
     const V3 gp = ghostOf(w.afterX, g, 1);
     if (ghostCellHash(gp, g) >= 0) {
-      emitGhost(ox, i, gp, g);
+      emitGhost(ox, i, gp, g, off);
       bits |= 1;
     }
   }
-  if (per.y) {
+  if (per.y && pi.y) {
     const V3 gp = ghostOf(w.afterY, g, 2);
     if (ghostCellHash(gp, g) >= 0) {
-      emitGhost(oy, i, gp, g);
+      emitGhost(oy, i, gp, g, off);
       bits |= 2;
     }
   }
-  if (per.x && per.y) {
+  if (per.x && per.y && pi.x && pi.y) {
     const V3 gp = ghostOf(w.afterY, g, 3);
     if (ghostCellHash(gp, g) >= 0) {
-      emitGhost(oxy, i, gp, g);
+      emitGhost(oxy, i, gp, g, off);
       bits |= 4;
     }
   }
@@ -257,7 +264,9 @@ const int* firstAtBuild(const fdm_ghosts* sets, const bool* on) {
 // positions) and one table per set; every kernel gated by *gate.
 int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_ghosts* sets,
                 const bool* on, int* atBuild, const int* gate, unsigned* status, void* workspace,
-                size_t workspaceBytes, hipStream_t stream) {
+                size_t workspaceBytes, hipStream_t stream, const fdm_instances* inst = nullptr) {
+  fdm_instances none{};
+  const fdm_instances I = inst ? *inst : none;
   const int n = particles.n;
   hipLaunchKernelGGL(kResetBox, dim3(1), dim3(64), 0, stream, on[0] ? sets[0].cellBox : nullptr,
                      on[1] ? sets[1].cellBox : nullptr, on[2] ? sets[2].cellBox : nullptr, gate);
@@ -281,7 +290,7 @@ int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_
     o[s].status = status;
   }
   hipLaunchKernelGGL(kGhostScatter, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
-                     grid, periodic, o[0], o[1], o[2], atBuild);
+                     grid, periodic, o[0], o[1], o[2], atBuild, I);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
     hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity,
@@ -332,6 +341,19 @@ extern "C" int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_
   const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
   return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), gate, status,
                      workspace, workspaceBytes, stream);
+}
+
+extern "C" int fdm_launchPeriodicBuildInstances(fdm_bodies particles, fdm_grid grid,
+                                                fdm_periodic periodic, const fdm_instances* inst,
+                                                fdm_ghosts* sets, unsigned* status,
+                                                void* workspace, size_t workspaceBytes,
+                                                hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0 || !sets || !status || !inst || inst->count <= 0) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), nullptr,
+                     status, workspace, workspaceBytes, stream, inst);
 }
 
 extern "C" int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,

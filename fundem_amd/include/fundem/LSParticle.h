@@ -230,6 +230,43 @@ struct LSParticle {
     orientation_.push_back(normalize(orientation));
     materialID_.push_back(materialID);
     geometryID_.push_back(geometryID);
+    system_.push_back(currentSystem_);
+  }
+  // Bodies added from now on belong to system k (several independent systems
+  // in one solver).
+  void setCurrentSystem(int k) { currentSystem_ = k; }
+  const std::vector<int>& systemOfBody() const { return system_; }
+
+  // Cell size of the grid rule for the bodies of system k (k < 0: all
+  // bodies): 2 * the largest radius over every template, as
+  // LSParticle::initialize computes it (LSParticle.h:383), or `cellSize` when
+  // positive.
+  double cellSizeRule(double cellSize) const {
+    double cs = 0.0;
+    if (num() > 0 && !templates_.empty()) {
+      for (const auto& t : templates_) cs = std::max(cs, t.radius);
+      cs *= 2.0;
+    }
+    if (cellSize > 0.0) cs = cellSize;
+    return cs;
+  }
+  // 2 * the largest radius over the templates the bodies of system k use.
+  double cellSizeRuleOfSystem(int k, double cellSize) const {
+    if (cellSize > 0.0) return cellSize;
+    double cs = 0.0;
+    for (size_t i = 0; i < num(); ++i)
+      if (system_[i] == k) cs = std::max(cs, templates_[geometryID_[i]].radius);
+    return 2.0 * cs;
+  }
+
+  // Several systems: the table spans every system's cells end to end
+  // (globalCells), the grid returned by grid() only carries that count.
+  void initializeSystems(long long globalCells, hipStream_t s) {
+    grid_ = fdm_grid{};
+    grid_.num = std::max<long long>(globalCells, 1);
+    grid_.size = {1, 1, 1};
+    upload(s);
+    table_.allocate(num(), grid_.num);
   }
 
   void move(size_t index, double3 offset) {
@@ -407,7 +444,8 @@ struct LSParticle {
   std::vector<std::vector<int3>> connectivity_;
   std::vector<double3> position_, velocity_, angularVelocity_, force_, torque_;
   std::vector<quaternion> orientation_;
-  std::vector<int> materialID_, geometryID_;
+  std::vector<int> materialID_, geometryID_, system_;
+  int currentSystem_ = 0;
 
   DeviceBuffer<double> x_;
   DeviceBuffer<float> v_, w_, q_, F_, T_;

@@ -131,6 +131,23 @@ class LSSolver {
   }
 
   // ------------------------------------------------------------- additive
+  // Several independent systems in one solver (no counterpart in the
+  // reference): bodies, walls and materials added after addSystem(k) belong
+  // to system k, which has its own domain, grids and periodic axes; walls
+  // must be added system by system. initialize() then ignores its domain
+  // arguments. Each system evolves exactly as it would in a solver of its
+  // own with the same templates.
+  int addSystem(double3 minDomain, double3 maxDomain, bool periodicX, bool periodicY) {
+    if (!(maxDomain.x > minDomain.x && maxDomain.y > minDomain.y && maxDomain.z > minDomain.z))
+      throw std::invalid_argument("invalid system domain");
+    systems_.push_back({minDomain, maxDomain, periodicX, periodicY});
+    const int k = (int)systems_.size() - 1;
+    particles_.setCurrentSystem(k);
+    walls_.setCurrentSystem(k);
+    return k;
+  }
+  size_t numSystems() const { return systems_.size(); }
+
   // Cell size of the particle grid (the reference derives 2 * max radius,
   // LSParticle.h:383). Must be at least the largest template radius.
   void setCellSize(double cellSize) { cellSize_ = cellSize; }
@@ -215,8 +232,13 @@ class LSSolver {
     }
     gravity_ = gravity;
     dt_ = timeStep;
-    particles_.initialize(minDomain, maxDomain, cellSize_, stream_);
-    walls_.initialize(minDomain, maxDomain, 0.0, stream_);
+    if (systems_.empty()) {
+      particles_.initialize(minDomain, maxDomain, cellSize_, stream_);
+      walls_.initialize(minDomain, maxDomain, 0.0, stream_);
+      inst_ = fdm_instances{};
+    } else {
+      initializeSystems();
+    }
     double maxR = 0.0;
     for (size_t g = 0; g < particles_.numGeometry(); ++g)
       maxR = std::max(maxR, particles_.templateInfo(g).radius);
@@ -516,6 +538,69 @@ class LSSolver {
     pending_.clear();
   }
 
+  struct SystemDesc {
+    double3 min, max;
+    bool px, py;
+  };
+
+  // Grids, cell offsets and instance tables of the systems; the bodies'
+  // tables span every system's cells end to end.
+  void initializeSystems() {
+    const int K = (int)systems_.size();
+    const double pcs = particles_.cellSizeRule(cellSize_);
+    std::vector<fdm_grid> pg(K), wg(K);
+    std::vector<long long> po(K), wo(K);
+    std::vector<fdm_periodic> per(K);
+    long long pc = 0, wc = 0;
+    bool anyX = false, anyY = false;
+    for (int k = 0; k < K; ++k) {
+      const SystemDesc& d = systems_[k];
+      pg[k] = makeGrid(d.min, d.max, pcs);
+      wg[k] = makeGrid(d.min, d.max, walls_.cellSizeRuleOfSystem(k, 0.0));
+      po[k] = pc;
+      wo[k] = wc;
+      pc += pg[k].num;
+      wc += wg[k].num;
+      per[k].x = d.px ? 1 : 0;
+      per[k].y = d.py ? 1 : 0;
+      anyX |= d.px;
+      anyY |= d.py;
+    }
+    if (pc >= (1ll << 31) || wc >= (1ll << 31)) throw std::invalid_argument("too many cells");
+    const std::vector<int>& ws = walls_.systemOfBody();
+    std::vector<int> wallStart(K + 1, 0);
+    for (size_t i = 0; i < ws.size(); ++i) {
+      if (i > 0 && ws[i] < ws[i - 1]) throw std::invalid_argument("walls must be added system by system");
+      wallStart[ws[i] + 1] += 1;
+    }
+    for (int k = 0; k < K; ++k) wallStart[k + 1] += wallStart[k];
+    particles_.initializeSystems(pc, stream_);
+    walls_.initializeSystems(wc, stream_);
+    removePeriodicBoundary();
+    if (anyX) addPeriodicBoundaryXD();
+    if (anyY) addPeriodicBoundaryYD();
+    instParticle_.upload(particles_.systemOfBody(), stream_);
+    if (particles_.systemOfBody().empty()) instParticle_.allocate(1);
+    instWall_.upload(ws, stream_);
+    if (ws.empty()) instWall_.allocate(1);
+    instWallStart_.upload(wallStart, stream_);
+    instPGrid_.upload(pg, stream_);
+    instWGrid_.upload(wg, stream_);
+    instPOff_.upload(po, stream_);
+    instWOff_.upload(wo, stream_);
+    instPeriodic_.upload(per, stream_);
+    hipCheck(hipStreamSynchronize(stream_), "systems");
+    inst_.count = K;
+    inst_.particleInstance = instParticle_.get();
+    inst_.wallInstance = instWall_.get();
+    inst_.wallStart = instWallStart_.get();
+    inst_.pgrid = instPGrid_.get();
+    inst_.wgrid = instWGrid_.get();
+    inst_.pcellOffset = instPOff_.get();
+    inst_.wcellOffset = instWOff_.get();
+    inst_.periodic = instPeriodic_.get();
+  }
+
   fdm_skin skinView() {
     fdm_skin k;
     k.skin = skin_;
@@ -534,6 +619,7 @@ class LSSolver {
     b.skin = skin_;
     b.maxReach = set == FDM_SET_PARTICLE ? maxRadius_ : set == FDM_SET_GHOST ? maxBoxReach_ : 0.0;
     b.gate = nullptr;
+    b.inst = inst_;
     b.set = set;
     b.particles = particles_.bodies();
     b.pgeo = particles_.geometry();
@@ -570,6 +656,7 @@ class LSSolver {
     n.mirror = si.mirror();
     n.ghostMask = ghostMask;
     n.ghostFromReal = ghostFromReal ? 1 : 0;
+    n.inst = inst_;
     n.dt = dt_;
     n.updateSprings = prime ? 0 : 1;
     n.status = status_.get();
@@ -598,6 +685,7 @@ class LSSolver {
     a.kick = prime ? 0 : 1;
     a.drift = prime ? 0 : 1;
     a.periodic = periodic_.flags();
+    a.inst = inst_;
     if (periodic_.isActived()) {
       for (int s = 0; s < 3; ++s)
         a.ghostFlag[s] = periodic_.active(s) ? periodic_.set(s).flag.get() : nullptr;
@@ -660,16 +748,26 @@ class LSSolver {
     if (d.rebuild) {
       {
         Timer t(this, "sort_cells");
-        fdmCheck(fdm_launchCommitBodyState(P, particles_.grid(), particles_.table().view(), &sk,
-                                           stream_),
-                 "body state");
+        if (inst_.count > 0)
+          fdmCheck(fdm_launchCommitBodyStateInstances(P, &inst_, 0, particles_.table().view(), &sk,
+                                                      stream_),
+                   "body state");
+        else
+          fdmCheck(fdm_launchCommitBodyState(P, particles_.grid(), particles_.table().view(), &sk,
+                                             stream_),
+                   "body state");
         fdmCheck(fdm_buildHashStartEnd(particles_.table().view(), (int)N, particles_.grid(), ws,
                                        wsBytes, stream_),
                  "hash table");
         if (W) {
-          fdmCheck(fdm_launchCommitBodyState(walls_.bodies(), walls_.grid(), walls_.table().view(),
-                                             &sk, stream_),
-                   "wall state");
+          if (inst_.count > 0)
+            fdmCheck(fdm_launchCommitBodyStateInstances(walls_.bodies(), &inst_, 1,
+                                                        walls_.table().view(), &sk, stream_),
+                     "wall state");
+          else
+            fdmCheck(fdm_launchCommitBodyState(walls_.bodies(), walls_.grid(),
+                                               walls_.table().view(), &sk, stream_),
+                     "wall state");
           fdmCheck(fdm_buildHashStartEnd(walls_.table().view(), (int)W, walls_.grid(), ws, wsBytes,
                                          stream_),
                    "wall hash table");
@@ -683,9 +781,14 @@ class LSSolver {
       for (int s = 0; s < 3; ++s) gv[s] = periodic_.active(s) ? periodic_.view(s) : fdm_ghosts{};
       {
         Timer t(this, "periodic_build");
-        fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
-                                         status_.get(), ws, wsBytes, stream_),
-                 "periodic ghosts");
+        if (inst_.count > 0)
+          fdmCheck(fdm_launchPeriodicBuildInstances(P, particles_.grid(), periodic_.flags(), &inst_,
+                                                    gv, status_.get(), ws, wsBytes, stream_),
+                   "periodic ghosts");
+        else
+          fdmCheck(fdm_launchPeriodicBuild(P, particles_.grid(), periodic_.flags(), gv, nullptr,
+                                           status_.get(), ws, wsBytes, stream_),
+                   "periodic ghosts");
       }
       for (int s = 0; s < 3; ++s)
         if (periodic_.active(s))
@@ -904,6 +1007,12 @@ class LSSolver {
   DeviceBuffer<unsigned> status_;
   DeviceBuffer<int> stepCounter_;
   DeviceBuffer<int> ctl_;
+  std::vector<SystemDesc> systems_;
+  fdm_instances inst_{};
+  DeviceBuffer<int> instParticle_, instWall_, instWallStart_;
+  DeviceBuffer<fdm_grid> instPGrid_, instWGrid_;
+  DeviceBuffer<long long> instPOff_, instWOff_;
+  DeviceBuffer<fdm_periodic> instPeriodic_;
   DeviceBuffer<int> workList_, workCount_;
   DeviceBuffer<double> disp_;
   double skinRequest_ = -1.0, skin_ = 0.0;

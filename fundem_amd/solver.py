@@ -51,6 +51,7 @@ def solver_lib():
         "fdmx_set_cell_size": (I, [P, D]),
         "fdmx_set_capacity": (I, [P, I, I]),
         "fdmx_set_skin": (I, [P, D]),
+        "fdmx_add_system": (I, [P, C.POINTER(D), C.POINTER(D), I, I]),
         "fdmx_pair_totals": (I, [P, C.POINTER(I)]),
         "fdmx_list_builds": (I, [P, C.POINTER(LL), C.POINTER(LL),
                                  C.POINTER(D)]),
@@ -156,6 +157,17 @@ class Solver:
 
     def set_periodic(self, x, y):
         self._ok(self._lib.fdmx_set_periodic(self.h, int(x), int(y)))
+
+    def add_system(self, dmin, dmax, periodic_x=False, periodic_y=False):
+        """Start an independent system with its own domain and periodic axes;
+        bodies, walls and materials added afterwards belong to it."""
+        mn = np.ascontiguousarray(dmin, dtype=np.float64)
+        mx = np.ascontiguousarray(dmax, dtype=np.float64)
+        k = self._lib.fdmx_add_system(self.h, _dp_(mn), _dp_(mx), int(bool(periodic_x)),
+                                      int(bool(periodic_y)))
+        if k < 0:
+            self._ok(k)
+        return k
 
     def set_cell_size(self, cs):
         self._ok(self._lib.fdmx_set_cell_size(self.h, float(cs)))

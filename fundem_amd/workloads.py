@@ -78,6 +78,40 @@ def apply(s: Scenario, t) -> None:
     t.initialize(s.domain_min, s.domain_max, s.gravity, s.dt)
 
 
+def apply_batch(scenarios, t) -> None:
+    """Feed several scenarios into one solver as independent systems. They
+    share the particle templates (added once, from the first scenario), the
+    time step and gravity; materials, walls and wall templates are per system."""
+    base = scenarios[0]
+    dens = base.density if isinstance(base.density, (list, tuple)) else [base.density] * len(base.templates)
+    for d, tp in zip(dens, base.templates):
+        t.add_geometry_density(d, tp)
+    mat_off = wmat_off = wt_off = 0
+    for s in scenarios:
+        if s.templates is not base.templates and len(s.templates) != len(base.templates):
+            raise ValueError("systems of one batch share their particle templates")
+        if s.bonds:
+            raise ValueError("bonded systems are not batched")
+        if s.dt != base.dt or not np.array_equal(s.gravity, base.gravity):
+            raise ValueError("systems of one batch share dt and gravity")
+        t.add_system(s.domain_min, s.domain_max, s.periodic[0], s.periodic[1])
+        for m in s.materials:
+            t.add_material(*m)
+        for m in s.wall_materials:
+            t.add_wall_material(*m)
+        for tp in s.wall_templates:
+            t.add_wall_geometry(tp)
+        t.add_particles(s.x, s.v, s.w, s.q, np.asarray(s.mat, np.int32) + mat_off, s.geo)
+        for pos, v, w, q, mat, geo in s.walls:
+            t.add_wall(pos, v, w, q, mat + wmat_off, geo + wt_off)
+        mat_off += len(s.materials)
+        wmat_off += len(s.wall_materials)
+        wt_off += len(s.wall_templates)
+    if base.cell_size > 0:
+        t.set_cell_size(base.cell_size)
+    t.initialize(base.domain_min, base.domain_max, base.gravity, base.dt)
+
+
 IDENT = np.array([1.0, 0.0, 0.0, 0.0])
 
 
@@ -193,7 +227,8 @@ def pour(counts=(100, 100, 100), spacing: float = None, n_templates: int = 16,
 def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_templates: int = 3,
             periodic=(False, False), walls: bool = True, scale: float = 1e-3,
             spacing: float = 3.0, speed: float = 0.5, spin: float = 20.0, dt: float = 1e-6,
-            kn: float = 1e4, mu: float = 0.4, zeta: float = 0.3, steps: int = 400) -> Scenario:
+            kn: float = 1e4, mu: float = 0.4, zeta: float = 0.3, steps: int = 400,
+            templates=None) -> Scenario:
     """A small cluster with random velocities that collides with itself, the
     floor and (when periodic) across the periodic faces within a few hundred
     steps. Lengths are in units of `scale` (semi-axes U[0.8, 1.2], lattice
@@ -201,8 +236,9 @@ def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_
     reference's 1e-30 inverse threshold, so particles keep their spin, while
     at scale 1e-2 rotation is live. Used for step-by-step parity."""
     rng = np.random.default_rng(seed)
-    tmpl = superellipsoid_family(n_templates, seed + 100, n_grid=n_grid, n_nodes=n_nodes,
-                                 axes=(0.8 * scale, 1.2 * scale))
+    tmpl = templates if templates is not None else superellipsoid_family(
+        n_templates, seed + 100, n_grid=n_grid, n_nodes=n_nodes, axes=(0.8 * scale, 1.2 * scale))
+    n_templates = len(tmpl)
     sp = spacing * scale
     side = int(math.ceil(n ** (1.0 / 3.0)))
     L = side * sp

@@ -133,6 +133,7 @@ typedef struct {
                           NULL when candidate lists are rebuilt every step */
 } fdm_hash_table;
 
+
 /* Candidate-list reuse. Lists are built with an extra reach `skin` and kept
  * while every body has moved (or, through rotation, moved a point within
  * `rho` of its centre) by less than skin/2 since the build, no body changed
@@ -231,6 +232,26 @@ typedef struct {
   int x, y;
 } fdm_periodic;
 
+/* Independent systems advanced together (no counterpart in the reference,
+ * which runs one system per solver). Every body carries an instance id; each
+ * instance keeps its own coordinates, particle grid, wall grid, periodic
+ * flags and materials, so it reproduces its own plain run bit for bit. The
+ * instances' cells are laid end to end in one global cell space: a body's
+ * sort key is 1 + (first global cell of its instance) + (its cell id in its
+ * instance's grid), so one sort and one table serve every instance. count = 0
+ * means a single system described by the launchers' own grid arguments. */
+typedef struct {
+  int count;
+  const int* particleInstance;  /* [particles] */
+  const int* wallInstance;      /* [walls]; walls are stored by instance */
+  const int* wallStart;         /* [count+1] walls of instance k: wallStart[k] .. wallStart[k+1] */
+  const fdm_grid* pgrid;        /* [count] particle grid of each instance */
+  const fdm_grid* wgrid;        /* [count] wall grid of each instance */
+  const long long* pcellOffset; /* [count] first global particle cell of each instance */
+  const long long* wcellOffset; /* [count] first global wall cell of each instance */
+  const fdm_periodic* periodic; /* [count] */
+} fdm_instances;
+
 /* ---------------------------------------------------------------------- */
 /* Integration                                                            */
 
@@ -284,6 +305,7 @@ typedef struct {
   fdm_periodic periodic;
   int* ghostFlag[3];           /* per particle visibility flag of sets X, Y, XY */
   int* ghostAtBuild;           /* [n] visibility bits at the last ghost build */
+  fdm_instances inst;          /* count 0: one system (pgrid, wgrid, periodic above) */
   int* clearCounters[8];       /* arrays of FDM_REC_COUNTERS ints set to 0 (NULL: skipped) */
 } fdm_step_begin_args;
 size_t fdm_stepBeginWorkspaceBytes(size_t bodies);
@@ -295,6 +317,10 @@ int fdm_launchStepDecide(const fdm_step_begin_args* args, const fdm_skin* skin, 
  * every body's grid state in table.validAtBuild. */
 int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
                               const fdm_skin* skin, hipStream_t stream);
+/* The same for bodies of several instances (isWall selects the wall grids). */
+int fdm_launchCommitBodyStateInstances(fdm_bodies bodies, const fdm_instances* inst, int isWall,
+                                       fdm_hash_table table, const fdm_skin* skin,
+                                       hipStream_t stream);
 /* No counterpart in the reference. When *gate is 0, copies the candidate
  * lists (count, total, pairA/T/G) of `from` into `to`, so a step without a
  * rebuild sees the lists of the last build. */
@@ -369,6 +395,11 @@ int fdm_launchPeriodicPrepare(fdm_bodies particles, fdm_grid grid, fdm_periodic 
 int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
                             fdm_ghosts* sets, const int* gate, unsigned* status, void* workspace,
                             size_t workspaceBytes, hipStream_t stream);
+/* fdm_launchPeriodicBuild for several instances: `grid` only gives the
+ * global cell count (grid.num); `periodic` says which sets exist. */
+int fdm_launchPeriodicBuildInstances(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                     const fdm_instances* inst, fdm_ghosts* sets, unsigned* status,
+                                     void* workspace, size_t workspaceBytes, hipStream_t stream);
 int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
                               fdm_ghosts* sets, const int* gate, hipStream_t stream);
 
@@ -396,6 +427,7 @@ typedef struct {
   fdm_grid grid;           /* grid of the targets */
   fdm_hash_table table;    /* hash table of the targets */
   fdm_pairs pairs;
+  fdm_instances inst;      /* count 0: one system on `grid` */
   double skin;             /* extra reach (candidate-list reuse), 0 for exact lists */
   double maxReach;         /* largest target reach: particle radius (set 0) or
                               SDF box reach (set 2) over all templates */
@@ -435,6 +467,7 @@ typedef struct {
                                   step) instead of ghosts->pos / ghosts->table.hashValue */
   const int* gate;             /* fdm_launchNarrowPhaseSets: run only when *gate != 0
                                   (NULL: always; taken from sets[0]) */
+  fdm_instances inst;          /* count 0: one system on grid / pgrid / periodic */
   double dt;
   int updateSprings;           /* 0: keep remapped springs (history filtering) */
   unsigned* status;

@@ -1,0 +1,71 @@
+"""Several independent systems advanced by one solver: each system must
+evolve exactly as the oracle evolves it alone (same templates, its own
+domain, periodic axes, materials and walls)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from fundem_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SETS = (po.SET_PP, po.SET_PW, po.SET_GX, po.SET_GY, po.SET_GXY)
+
+
+def systems():
+    base = W.cluster(n=27, seed=1, periodic=(True, True), speed=1.0)
+    tm = base.templates
+    return [
+        base,
+        W.cluster(n=20, seed=2, periodic=(False, False), speed=1.0, kn=2e4, mu=0.3, templates=tm),
+        W.cluster(n=33, seed=3, periodic=(True, False), speed=1.5, templates=tm),
+        W.cluster(n=8, seed=4, periodic=(False, True), speed=1.0, zeta=0.5, templates=tm),
+        W.cluster(n=27, seed=5, periodic=(True, True), speed=0.5, walls=False, templates=tm),
+    ]
+
+
+def split_contacts(c, a_off, a_n, b_off):
+    sel = (c["A"] >= a_off) & (c["A"] < a_off + a_n)
+    return {"A": c["A"][sel] - a_off, "B": c["B"][sel] - b_off, "k": c["k"][sel],
+            "spring": c["spring"][sel]}
+
+
+@pytest.mark.parametrize("skin", [None, 0.0])
+def test_batch_systems_equal_their_plain_runs(skin):
+    from fundem_amd import Solver
+    scns = systems()
+    g = Solver()
+    if skin is not None:
+        g.set_skin(skin)
+    W.apply_batch(scns, g)
+    oracles = []
+    for s in scns:
+        o = po.OracleSim(True, True)
+        W.apply(s, o)
+        oracles.append(o)
+    p_off = np.cumsum([0] + [s.n for s in scns])
+    w_off = np.cumsum([0] + [len(s.walls) for s in scns])
+    total = 0
+    for step in range(1, 301):
+        g.step(1)
+        for o in oracles:
+            o.step(1)
+        if step % 50 and step != 300:
+            continue
+        gs = g.state()
+        gc = {k: g.contacts(k) for k in SETS}
+        for i, (s, o) in enumerate(zip(scns, oracles)):
+            os_ = o.state()
+            for key in ("x", "v", "w", "q", "F", "T"):
+                a = gs[key][p_off[i]:p_off[i + 1]]
+                assert np.array_equal(a, os_[key]), f"step {step} system {i}: {key} differs"
+            for k in SETS:
+                b_off = w_off[i] if k == po.SET_PW else p_off[i]
+                c = split_contacts(gc[k], p_off[i], s.n, b_off)
+                oc = o.contacts(k)
+                for key in ("A", "B", "k", "spring"):
+                    assert np.array_equal(c[key], oc[key]), \
+                        f"step {step} system {i} set {k}: {key} differs ({c['A'].size} vs {oc['A'].size})"
+                total += oc["A"].size
+    assert total > 0
+    assert g.status() == 0
