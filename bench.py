@@ -45,6 +45,11 @@ def parse():
                    help="candidate-list skin in m (negative: a tenth of the largest radius)")
     p.add_argument("--copy-probe", action="store_true",
                    help="first run a plain 1 GiB device copy (counter scaling under rocprofv3)")
+    p.add_argument("--workload", choices=("pour", "batch"), default="pour",
+                   help="pour: the 1M-particle pour; batch: 512 packings in one solver against "
+                        "plain runs of the same packings one after another")
+    p.add_argument("--batch-sample", type=int, default=32,
+                   help="packings timed one by one for the sequential reference")
     return p.parse_args()
 
 
@@ -124,8 +129,89 @@ def copy_probe():
     log("copy probe: 3 copies of 1 GiB")
 
 
+def run_steps(s, steps, warmup, graph):
+    s.use_graph(bool(graph))
+    s.step(1)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    return time.perf_counter() - t0
+
+
+def batch_main(a):
+    """512 packings (log-uniform 500..20000 particles, 16 shared templates,
+    per-packing kn, damping ratio and mu) advanced by one solver, against the
+    same packings run one after another in solvers of their own."""
+    import numpy as np
+    from fundem_amd import Solver
+    from fundem_amd import workloads as W
+    t0 = time.perf_counter()
+    scns = W.batch_packings()
+    n_total = sum(sc.n for sc in scns)
+    log(f"batch of {len(scns)} packings, {n_total} particles, set-up {time.perf_counter() - t0:.1f} s")
+    s = Solver(0)
+    s.set_capacity(8, 24)
+    s.set_skin(a.skin)
+    t0 = time.perf_counter()
+    W.apply_batch(scns, s)
+    log(f"batch uploaded and initialised in {time.perf_counter() - t0:.1f} s")
+    s.set_timing(not a.graph)
+    elapsed = run_steps(s, a.steps, a.warmup, a.graph)
+    if s.status() != 0:
+        raise RuntimeError(f"capacity exceeded in the batch (status {s.status()})")
+    batched = n_total * a.steps / elapsed
+    b, gb, skin = s.list_builds()
+    log(f"batched: {1e3 * elapsed / a.steps:.3f} ms/step, {batched:.3e} particle-steps/s; "
+        f"candidate pairs {s.pair_totals()}, {b} list builds")
+    if not a.graph:
+        for k, (ms, calls) in sorted(s.stage_times().items(), key=lambda kv: -kv[1][0]):
+            log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches")
+    del s
+    rng = np.random.default_rng(17)
+    pick = sorted(rng.choice(len(scns), size=min(a.batch_sample, len(scns)), replace=False))
+    seq_time, seq_work = 0.0, 0
+    for k in pick:
+        sc = scns[k]
+        p = Solver(0)
+        p.set_capacity(8, 24)
+        p.set_skin(a.skin)
+        W.apply(sc, p)
+        t = run_steps(p, a.steps, a.warmup, a.graph)
+        if p.status() != 0:
+            raise RuntimeError(f"capacity exceeded in packing {k}")
+        seq_time += t
+        seq_work += sc.n * a.steps
+        del p
+    sequential = seq_work / seq_time
+    log(f"sequential plain runs ({len(pick)} packings): {sequential:.3e} particle-steps/s")
+    out = {
+        "metric": "batched vs sequential plain particle-steps/s (512 packings, one GPU)",
+        "value": batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(scns)} periodic packings of 500..20000 superellipsoids "
+                               f"(log-uniform, {n_total} in total), 16 shared templates (32^3 "
+                               "SDF, 1000 nodes), per-packing kn in [5e3, 2e4], damping ratio "
+                               "in [0.1, 0.5], mu in [0.2, 0.8], dt 5e-7 s",
+                   "graph": bool(a.graph), "skin_m": skin, "list_builds": b},
+        "sequential": {"value": sequential, "unit": "particle-steps/s",
+                       "sample": f"{len(pick)} of the {len(scns)} packings (seeded choice), each "
+                                 f"in a solver of its own, {a.steps} steps each, one after another"},
+        "speedup": batched / sequential,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.workload == "batch":
+        if a.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        batch_main(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the one-GPU build "
               f"does not shard; DESIGN.md describes the slab decomposition)", file=sys.stderr)

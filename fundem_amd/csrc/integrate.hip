@@ -171,71 +171,72 @@ struct StepBeginK {
 // Phase A of a step (one thread per particle, then per wall). Each workgroup
 // leaves its largest motion rate and its change flags in a slot of its own
 // (no atomics on shared words); kStepDecide folds the slots.
+// One body of phase A. Everything it reads from the launch arguments is
+// copied into registers first (selecting between argument structs by
+// reference would make the compiler copy them to per-lane scratch).
+template <bool WALL>
+__device__ __forceinline__ void stepBeginBody(const fdm_step_begin_args& a, int j, float& rate,
+                                              int& flags) {
+  const fdm_bodies b = WALL ? a.walls : a.particles;
+  const fdm_template* tmpl = WALL ? a.wgeo.info : a.pgeo.info;
+  const fdm_hash_table t = WALL ? a.wtable : a.ptable;
+  const bool many = a.inst.count > 0;
+  const int in = !many ? 0 : (WALL ? a.inst.wallInstance[j] : a.inst.particleInstance[j]);
+  const fdm_grid grid =
+      !many ? (WALL ? a.wgrid : a.pgrid) : (WALL ? a.inst.wgrid[in] : a.inst.pgrid[in]);
+  const fdm_periodic perI = many ? a.inst.periodic[in] : a.periodic;
+  if (!WALL && a.kick) kickOne(b, tmpl, j, fromD3(a.gravity), 0.5 * a.dt);
+  V3 x;
+  if (a.drift) x = driftOne(b, j, a.dt);
+  else x = ldp(b.x, b.n, j);
+  const int h = cellHash(x, grid);
+  t.hashValue[j] = h;
+  t.sortKey[j] = !many ? (unsigned)(h + 1)
+                       : instanceKey(h, grid, WALL ? a.inst.wcellOffset[in] : a.inst.pcellOffset[in]);
+  t.sortIndex[j] = j;
+  if (a.drift) {
+    const fdm_template& tp = tmpl[b.geo[j]];
+    const double rho = fmax(tp.radius, tp.boxReach);
+    const double r = vlen(ld3(b.v, b.n, j)) + vlen(ld3(b.w, b.n, j)) * rho;
+    rate = (float)r * (1.0f + 1e-6f);
+  }
+  const int st = bodyState(x, grid);
+  if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
+  if (!WALL && (a.periodic.x || a.periodic.y)) {
+    // sets exist where any instance is periodic; an instance contributes
+    // only to the sets of its own periodic axes
+    const WrapStages w = wrapStages(x, grid, perI);
+    int bits = 0;
+    if (a.periodic.x) {
+      const int f = perI.x && ghostCellHash(ghostOf(w.afterX, grid, 1), grid) >= 0 ? 1 : 0;
+      a.ghostFlag[0][j] = f;
+      bits |= f;
+    }
+    if (a.periodic.y) {
+      const int f = perI.y && ghostCellHash(ghostOf(w.afterY, grid, 2), grid) >= 0 ? 1 : 0;
+      a.ghostFlag[1][j] = f;
+      bits |= f << 1;
+    }
+    if (a.periodic.x && a.periodic.y) {
+      const int f =
+          perI.x && perI.y && ghostCellHash(ghostOf(w.afterY, grid, 3), grid) >= 0 ? 1 : 0;
+      a.ghostFlag[2][j] = f;
+      bits |= f << 2;
+    }
+    if (bits != a.ghostAtBuild[j]) flags |= 2;
+    if (!(w.afterY.x == x.x && w.afterY.y == x.y && w.afterY.z == x.z)) flags |= 4;
+  }
+}
+
 __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
   const fdm_step_begin_args& a = k.a;
   __shared__ float waveMax[4];
   const int nP = a.particles.n, nW = a.walls.n;
-  const V3 g = fromD3(a.gravity);
-  const bool per = a.periodic.x || a.periodic.y;
   float rate = 0.f;
   int flags = 0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nP + nW) {
-    const bool wall = i >= nP;
-    const fdm_bodies& b = wall ? a.walls : a.particles;
-    const fdm_template* tmpl = wall ? a.wgeo.info : a.pgeo.info;
-    const fdm_hash_table& t = wall ? a.wtable : a.ptable;
-    const int j = wall ? i - nP : i;
-    const bool many = a.inst.count > 0;
-    const int in = !many ? 0 : (wall ? a.inst.wallInstance[j] : a.inst.particleInstance[j]);
-    const fdm_grid& grid =
-        !many ? (wall ? a.wgrid : a.pgrid) : (wall ? a.inst.wgrid[in] : a.inst.pgrid[in]);
-    const fdm_periodic perI = many ? a.inst.periodic[in] : a.periodic;
-    if (a.kick && !wall) kickOne(b, tmpl, j, g, 0.5 * a.dt);
-    V3 x;
-    if (a.drift) x = driftOne(b, j, a.dt);
-    else x = ldp(b.x, b.n, j);
-    const int h = cellHash(x, grid);
-    t.hashValue[j] = h;
-    t.sortKey[j] = !many ? (unsigned)(h + 1)
-                         : instanceKey(h, grid, wall ? a.inst.wcellOffset[in] : a.inst.pcellOffset[in]);
-    t.sortIndex[j] = j;
-    if (a.drift) {
-      const fdm_template& tp = tmpl[b.geo[j]];
-      const double rho = fmax(tp.radius, tp.boxReach);
-      const double r = vlen(ld3(b.v, b.n, j)) + vlen(ld3(b.w, b.n, j)) * rho;
-      rate = (float)r * (1.0f + 1e-6f);
-    }
-    const int st = bodyState(x, grid);
-    if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
-    if (!wall && per) {
-      // sets exist where any instance is periodic; an instance contributes
-      // only to the sets of its own periodic axes
-      const WrapStages w = wrapStages(x, grid, perI);
-      int bits = 0;
-      if (a.periodic.x) {
-        const int f =
-            perI.x && ghostCellHash(ghostOf(w.afterX, grid, 1), grid) >= 0 ? 1 : 0;
-        a.ghostFlag[0][j] = f;
-        bits |= f;
-      }
-      if (a.periodic.y) {
-        const int f =
-            perI.y && ghostCellHash(ghostOf(w.afterY, grid, 2), grid) >= 0 ? 1 : 0;
-        a.ghostFlag[1][j] = f;
-        bits |= f << 1;
-      }
-      if (a.periodic.x && a.periodic.y) {
-        const int f = perI.x && perI.y && ghostCellHash(ghostOf(w.afterY, grid, 3), grid) >= 0
-                          ? 1
-                          : 0;
-        a.ghostFlag[2][j] = f;
-        bits |= f << 2;
-      }
-      if (bits != a.ghostAtBuild[j]) flags |= 2;
-      if (!(w.afterY.x == x.x && w.afterY.y == x.y && w.afterY.z == x.z)) flags |= 4;
-    }
-  }
+  if (i < nP) stepBeginBody<false>(a, i, rate, flags);
+  else if (i < nP + nW) stepBeginBody<true>(a, i - nP, rate, flags);
   const int f1 = __syncthreads_or(flags & 1), f2 = __syncthreads_or(flags & 2),
             f4 = __syncthreads_or(flags & 4);
 #pragma unroll

@@ -270,7 +270,17 @@ class LSSolver {
     const int recCap = (int)(std::max<size_t>(size_t(recordsPerParticle_) * N, 4096) +
                              size_t(FDM_REC_POOLS) * 1024);
     pp_.initialize(N, pairCap, recCap, mirrorCapacity_);
-    pw_.initialize(N, (int)std::max<size_t>(N * std::max<size_t>(W, 1), 1024), recCap, 1);
+    // a particle can pair with every wall of its own system
+    size_t wallsPerSystem = W;
+    if (!systems_.empty()) {
+      std::vector<size_t> perSystem(systems_.size(), 0);
+      for (int k : walls_.systemOfBody()) perSystem[k] += 1;
+      wallsPerSystem = *std::max_element(perSystem.begin(), perSystem.end());
+    }
+    const long long wallPairCap =
+        std::max<long long>((long long)N * (long long)std::max<size_t>(wallsPerSystem, 1), 1024);
+    if (wallPairCap >= (1ll << 31)) throw std::invalid_argument("too many wall pairs");
+    pw_.initialize(N, (int)wallPairCap, recCap, 1);
     periodic_.initialize(particles_, (int)std::max<size_t>(N, 64),
                          (int)std::max<size_t>(4 * N, 1024), recCap, mirrorCapacity_);
     pp_.resetGenerations();
@@ -279,7 +289,7 @@ class LSSolver {
       if (periodic_.active(s)) periodic_.set(s).interaction.resetGenerations();
     {
       // work list of the narrow phase: every candidate pair of every set
-      long long cap = (long long)pairCap + std::max<long long>((long long)N * std::max<size_t>(W, 1), 1024);
+      long long cap = (long long)pairCap + wallPairCap;
       for (int s = 0; s < 3; ++s)
         if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");

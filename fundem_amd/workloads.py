@@ -116,10 +116,16 @@ IDENT = np.array([1.0, 0.0, 0.0, 0.0])
 
 
 def floor_wall(xy_min, xy_max, z0: float = 0.0):
-    """A plane wall at height z0 covering the given xy range."""
-    return gen.plane_wall(xy_min, xy_max), (np.array([0.5 * (xy_min[0] + xy_max[0]),
-                                                     0.5 * (xy_min[1] + xy_max[1]), z0]),
-                                           np.zeros(3), np.zeros(3), IDENT, 0, 0)
+    """A plane wall at height z0 covering the given xy range: the reference's
+    plane construction (apps/test/LSSphereBouncing.cpp:86-127) built around
+    the plane's own centre, and the wall placed at that centre. The wall's
+    position must lie inside the domain, or its cell id is -1 and no node
+    ever finds it."""
+    c = np.array([0.5 * (xy_min[0] + xy_max[0]), 0.5 * (xy_min[1] + xy_max[1])])
+    lo = np.asarray(xy_min, dtype=np.float64) - c
+    hi = np.asarray(xy_max, dtype=np.float64) - c
+    return gen.plane_wall(lo, hi), (np.array([c[0], c[1], z0]), np.zeros(3), np.zeros(3),
+                                   IDENT, 0, 0)
 
 
 def single_particle_on_wall(n_nodes: int = 2000, seed: int = 7, steps: int = 100000) -> Scenario:
@@ -222,6 +228,58 @@ def pour(counts=(100, 100, 100), spacing: float = None, n_templates: int = 16,
         domain_min=dmin, domain_max=dmax, gravity=np.array([0.0, 0.0, -9.81]), dt=dt,
         wall_templates=[wt], wall_materials=[(mu, e)], walls=[wall],
         periodic=(True, True), cell_size=cell_size, steps=steps)
+
+
+def packing(n: int, seed: int, templates, kn: float = 1e4, mu: float = 0.5, zeta: float = 0.3,
+            dt: float = 5e-7, cell_size: float = 2.5e-3, clearance: float = 0.05) -> Scenario:
+    """One packing of a batch: n particles of the shared templates on a
+    jittered lattice (spacing 1.05 bounding diameters, no initial overlap)
+    filling a box periodic in x and y, about twice as tall as wide, with a
+    floor; the bottom layer rests on the floor."""
+    rng = np.random.default_rng(seed)
+    rmax = max(gen.template_radius(t) for t in templates)
+    spacing = 2.0 * rmax * (1.0 + clearance)
+    side = max(2, int(math.ceil((n / 2.0) ** (1.0 / 3.0))))
+    layers = int(math.ceil(n / (side * side)))
+    jitter = 0.01 * spacing
+    L = side * spacing
+    H = layers * spacing + 2.0 * spacing
+    H = max(H, L)
+    x = gen.jittered_lattice((side, side, layers), spacing,
+                             (0.5 * spacing, H - L + 0.5 * spacing, rmax + jitter), jitter, rng)[:n]
+    q = gen.shoemake_quaternions(rng, n)
+    geo = rng.integers(0, len(templates), size=n).astype(np.int32)
+    for i in range(min(n, side * side)):
+        R = _rotmat(q[i])
+        x[i, 2] = -float(np.min((templates[geo[i]].nodes.astype(np.float64) @ R.T)[:, 2]))
+    dmin = np.array([0.0, H - L, 0.0])
+    dmax = np.array([L, H, H])
+    wt, wall = floor_wall([0.0, H - L], [L, H])
+    e = restitution_from_damping(zeta)
+    return Scenario(
+        name=f"packing-{n}-{seed}", templates=templates, density=2500.0,
+        materials=[(kn, 0.4 * kn, mu, e)],
+        x=x, v=np.zeros((n, 3)), w=np.zeros((n, 3)), q=q,
+        mat=np.zeros(n, np.int32), geo=geo,
+        domain_min=dmin, domain_max=dmax, gravity=np.array([0.0, 0.0, -9.81]), dt=dt,
+        wall_templates=[wt], wall_materials=[(mu, e)], walls=[wall],
+        periodic=(True, True), cell_size=cell_size, steps=5000)
+
+
+def batch_packings(count: int = 512, seed: int = 5, n_range=(500, 20000), templates=None,
+                   kn_range=(5e3, 2e4), zeta_range=(0.1, 0.5), mu_range=(0.2, 0.8)):
+    """The batched calibration sweep: `count` packings whose particle counts
+    are log-uniform in n_range, sharing 16 superellipsoid templates (32^3 SDF,
+    1000 nodes), each with its own kn, damping ratio and mu drawn uniformly
+    from the given ranges."""
+    rng = np.random.default_rng(seed)
+    tmpl = templates if templates is not None else superellipsoid_family(16, seed + 1)
+    ns = np.round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), size=count))).astype(int)
+    out = []
+    for k in range(count):
+        out.append(packing(int(ns[k]), seed * 100003 + k, tmpl, kn=rng.uniform(*kn_range),
+                           mu=rng.uniform(*mu_range), zeta=rng.uniform(*zeta_range)))
+    return out
 
 
 def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_templates: int = 3,

@@ -69,3 +69,40 @@ def test_batch_systems_equal_their_plain_runs(skin):
                 total += oc["A"].size
     assert total > 0
     assert g.status() == 0
+
+
+def test_batch_of_packings_matches_plain_solvers():
+    """Packings of the batched sweep (shared templates, per-packing
+    materials, periodic boxes with floors): the batch equals plain solver runs
+    of the same packings, bit for bit, including the wall contacts."""
+    from fundem_amd import Solver
+    tm = W.superellipsoid_family(4, 9, n_grid=16, n_nodes=128)
+    scns = W.batch_packings(count=5, seed=3, n_range=(60, 300), templates=tm)
+    g = Solver()
+    W.apply_batch(scns, g)
+    plain = []
+    for s in scns:
+        p = Solver()
+        W.apply(s, p)
+        plain.append(p)
+    p_off = np.cumsum([0] + [s.n for s in scns])
+    w_off = np.cumsum([0] + [len(s.walls) for s in scns])
+    g.step(400)
+    for p in plain:
+        p.step(400)
+    gs = g.state()
+    wall_contacts = 0
+    for i, (s, p) in enumerate(zip(scns, plain)):
+        ps = p.state()
+        for key in ("x", "v", "w", "q", "F", "T"):
+            assert np.array_equal(gs[key][p_off[i]:p_off[i + 1]], ps[key]), f"system {i}: {key}"
+        for k in SETS:
+            b_off = w_off[i] if k == po.SET_PW else p_off[i]
+            c = split_contacts(g.contacts(k), p_off[i], s.n, b_off)
+            pc = p.contacts(k)
+            for key in ("A", "B", "k", "spring"):
+                assert np.array_equal(c[key], pc[key]), f"system {i} set {k}: {key}"
+            if k == po.SET_PW:
+                wall_contacts += pc["A"].size
+    assert wall_contacts > 0
+    assert g.status() == 0
