@@ -706,6 +706,278 @@ __global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
     processPair<SET>(a.c, a.s, p, lane, hs, p % FDM_REC_POOLS);
 }
 
+// ---------------------------------------------------------------------------
+// Narrow phase of several sets in three kernels: a pair-level cull (one thread
+// per candidate pair), the node sweep (one wave per kept pair) that stages the
+// geometry of every contact, and the contact law (one thread per kept pair)
+// that walks its contacts in node order.
+
+struct NarrowScratch {
+  int* workHit;      // [work item] first staged contact
+  double* hitGeom;   // [8 * hitCapacity]: contact point, normal, overlap, node id
+  int hitCapacity;
+  int* hitCount;     // [FDM_REC_COUNTERS] pool counters of the staged contacts
+};
+
+__device__ __forceinline__ void stageHit(double* g, V3 pc, V3 nrm, double ov, int k) {
+  g[0] = pc.x;
+  g[1] = pc.y;
+  g[2] = pc.z;
+  g[3] = nrm.x;
+  g[4] = nrm.y;
+  g[5] = nrm.z;
+  g[6] = ov;
+  g[7] = (double)k;
+}
+
+// Reserve n entries from pool `pool` of a pooled buffer of `capacity`
+// entries; returns the first index, or -1 when the pool is full (the whole
+// wave gets the same answer).
+__device__ __forceinline__ int reservePooled(int* counters, int capacity, int pool, int n, int lane) {
+  const int poolCap = capacity / FDM_REC_POOLS;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(counters + pool * FDM_REC_STRIDE, n);
+  base = readlaneI(base, 0);
+  if (base + n > poolCap) return -1;
+  return pool * poolCap + base;
+}
+
+// Node sweep of one kept pair (one wave): the culls and exact predicate of
+// processPair; the contacts' geometry is staged in node order (through LDS,
+// or recomputed chunk by chunk past kMaxHits) for the contact-law kernel,
+// and the pair's record range is reserved.
+template <int SET>
+__device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
+                          int p, int lane, HitSlots& hs, int pool) {
+  const int A = S.pairs.pairA[p];
+  const int T = S.pairs.pairT[p];
+  const int n = C.P.n;
+  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const V3 xA = ldp(C.P.x, n, A);
+  const Q4 qA = ld4(C.P.q, n, A);
+  const PairFrame fr = pairFrame<SET>(C, S, A);
+  const fdm_grid& tgrid = *fr.tgrid;
+  Target tg;
+  V3 armB;
+  resolveTarget<SET>(C, S, p, T, fr, tg, armB);
+  const double RB = tg.t->radius;
+  const I3 tcell = decodeHash(tg.hash, tgrid);
+  const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
+  const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
+  const int nChunks = tA.chunkCount;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+  bool live = false;
+  if (lane < nChunks) {
+    const float4 ch = chunks[lane];
+    const V3 cw = vadd(qrotate(qA, {(double)ch.x, (double)ch.y, (double)ch.z}), xA);
+    const double rc = (double)ch.w * kReach;
+    live = true;
+    if (SET == FDM_SET_PARTICLE) {
+      const double rr = (RB + rc) * (1.0 + 1e-9);
+      if (vlen2(vsub(cw, tg.x)) > rr * rr) live = false;
+    }
+    if (live) {
+      I3 lo, hi;
+      cellRange(cw, rc, tgrid, lo, hi);
+      if (!cellNear(tcell, lo, hi)) live = false;
+    }
+    if (live) {
+      const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
+      if (outsideBox(lc, rc, *tg.t) || lipschitzClear(lc, rc, *tg.t, tg.sdf)) live = false;
+    }
+  }
+  const unsigned long long liveChunks = ballot64(live);
+
+  unsigned long long myMask = 0ull;
+  int total = 0;
+  float nx = 0.f, ny = 0.f, nz = 0.f;
+  if (liveChunks) {
+    const int k0 = __builtin_ctzll(liveChunks) * FDM_CHUNK + lane;
+    if (k0 < tA.nodeCount) {
+      nx = nodes[3 * k0];
+      ny = nodes[3 * k0 + 1];
+      nz = nodes[3 * k0 + 2];
+    }
+  }
+  for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
+    const int c = __builtin_ctzll(lcm);
+    const int k = c * FDM_CHUNK + lane;
+    const V3 s = {(double)nx, (double)ny, (double)nz};
+    const unsigned long long rest = lcm & (lcm - 1ull);
+    if (rest) {
+      const int k2 = __builtin_ctzll(rest) * FDM_CHUNK + lane;
+      if (k2 < tA.nodeCount) {
+        nx = nodes[3 * k2];
+        ny = nodes[3 * k2 + 1];
+        nz = nodes[3 * k2 + 2];
+      }
+    }
+    bool hit = false;
+    Sample sm;
+    V3 pn;
+    if (k < tA.nodeCount) {
+      pn = vadd(qrotate(qA, s), xA);
+      hit = nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
+    }
+    const unsigned long long m = ballot64(hit);
+    if (lane == c) myMask = m;
+    if (hit) {
+      const int slot = total + __popcll(m & lt);
+      if (slot < kMaxHits) {
+        const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+        hs.pc[0][slot] = pc.x;
+        hs.pc[1][slot] = pc.y;
+        hs.pc[2][slot] = pc.z;
+        hs.nrm[0][slot] = nrm.x;
+        hs.nrm[1][slot] = nrm.y;
+        hs.nrm[2][slot] = nrm.z;
+        hs.ov[slot] = ov;
+        hs.node[slot] = k;
+      }
+    }
+    total += __popcll(m);
+  }
+
+  int base = 0, gbase = 0;
+  if (total > 0) {
+    base = reservePooled(S.pairs.recTotal, S.pairs.recCapacity, pool, total, lane);
+    gbase = reservePooled(X.hitCount, X.hitCapacity, pool, total, lane);
+    if (base < 0 || gbase < 0) {
+      if (lane == 0) atomicOr(C.status, FDM_STATUS_RECORD_OVERFLOW);
+      total = 0;
+      base = gbase = 0;
+    }
+  }
+  if (total > 0 && total <= kMaxHits) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < total)
+      stageHit(X.hitGeom + 8 * (size_t)(gbase + lane), {hs.pc[0][lane], hs.pc[1][lane], hs.pc[2][lane]},
+               {hs.nrm[0][lane], hs.nrm[1][lane], hs.nrm[2][lane]}, hs.ov[lane], hs.node[lane]);
+  } else if (total > 0) {
+    // more contacts than slots: recompute each chunk's hits
+    const unsigned long long chunksWith = ballot64(myMask != 0ull);
+    int running = 0;
+    for (unsigned long long cw = chunksWith; cw; cw &= cw - 1ull) {
+      const int c = __builtin_ctzll(cw);
+      const unsigned lo32 = __builtin_amdgcn_readlane((unsigned)myMask, c);
+      const unsigned hi32 = __builtin_amdgcn_readlane((unsigned)(myMask >> 32), c);
+      const unsigned long long m = ((unsigned long long)hi32 << 32) | lo32;
+      if ((m >> lane) & 1ull) {
+        const int k = c * FDM_CHUNK + lane;
+        const V3 s = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+        const V3 pn = vadd(qrotate(qA, s), xA);
+        Sample sm;
+        nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
+        const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+        stageHit(X.hitGeom + 8 * (size_t)(gbase + running + __popcll(m & lt)), pc, nrm, ov, k);
+      }
+      running += __popcll(m);
+    }
+  }
+  if (lane == 0) {
+    S.pairs.recOff[p] = base;
+    S.pairs.recCnt[p] = total;
+    X.workHit[w] = gbase;
+    if (SET != FDM_SET_WALL && total > 0 && T != A) {
+      const int pos = atomicAdd(&S.mirror.count[T], 1);
+      if (pos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + pos] = p;
+      else atomicOr(C.status, FDM_STATUS_BUCKET_OVERFLOW);
+    }
+  }
+}
+
+// Contact law of one kept pair with contacts (one thread): the previous
+// springs by (A, target, node), the linear spring-dashpot law per contact in
+// node order, and the pair's force and torque sums in that order.
+template <int SET>
+__device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
+                        int p) {
+  const int cnt = S.pairs.recCnt[p];
+  if (cnt <= 0) return;
+  const int base = S.pairs.recOff[p];
+  const int gbase = X.workHit[w];
+  const int A = S.pairs.pairA[p];
+  const int T = S.pairs.pairT[p];
+  const int n = C.P.n;
+  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const V3 xA = ldp(C.P.x, n, A);
+  const PairFrame fr = pairFrame<SET>(C, S, A);
+  Target tg;
+  V3 armB;
+  resolveTarget<SET>(C, S, p, T, fr, tg, armB);
+  const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
+  int prevOff = 0;
+  const int prevCnt = findPrevPair(S, p, A, T, prevOff);
+  // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
+  const fdm_material& Mi = C.pmat[C.P.mat[A]];
+  const double invMi = tA.invMass;
+  double kn = 0., ks = 0., effM = 0., mu, diss;
+  V3 vB, wB;
+  if (SET == FDM_SET_WALL) {
+    const fdm_material& Mw = C.wmat[C.W.mat[T]];
+    if (invMi > 0.) effM = 1. / invMi;
+    mu = fmin(Mi.mu, Mw.mu);
+    diss = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
+    kn = Mi.kn;
+    ks = Mi.ks;
+    vB = ld3(C.W.v, C.W.n, T);
+    wB = ld3(C.W.w, C.W.n, T);
+  } else {
+    const fdm_material& Mj = C.pmat[C.P.mat[T]];
+    const double invMj = tg.t->invMass;
+    if (Mi.kn > 0. && Mj.kn > 0.) kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
+    if (Mi.ks > 0. && Mj.ks > 0.) ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
+    if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
+    diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
+    mu = fmin(Mi.mu, Mj.mu);
+    vB = ld3(C.P.v, n, T);
+    wB = ld3(C.P.w, n, T);
+  }
+  const V3 vA = ld3(C.P.v, n, A);
+  const V3 wA = ld3(C.P.w, n, A);
+  V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
+  for (int h = 0; h < cnt; ++h) {
+    const double* g = X.hitGeom + 8 * (size_t)(gbase + h);
+    const V3 pc = {g[0], g[1], g[2]};
+    const V3 nrm = {g[3], g[4], g[5]};
+    const double ov = g[6];
+    const int k = (int)g[7];
+    V3 F = {0., 0., 0.};
+    V3 spring = findPrevSpring(S, prevOff, prevCnt, k);
+    if (C.updateSprings) {
+      const V3 vc = vsub(vadd(vA, vcross(wA, vsub(pc, armA))), vadd(vB, vcross(wB, vsub(pc, armB))));
+      linearContact(F, spring, vc, nrm, ov, C.dt, kn, ks, mu, diss, effM);
+    }
+    const V3 TA = vcross(vsub(pc, armA), F);
+    const V3 TB = vcross(vsub(pc, armB), vneg(F));
+    SF = vadd(SF, F);
+    STA = vadd(STA, TA);
+    STB = vadd(STB, TB);
+    const int r = base + h;
+    S.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z, __int_as_float(k));
+    if (C.dbgPoint) {
+      C.dbgPoint[3 * r] = pc.x;
+      C.dbgPoint[3 * r + 1] = pc.y;
+      C.dbgPoint[3 * r + 2] = pc.z;
+      C.dbgNormal[3 * r] = nrm.x;
+      C.dbgNormal[3 * r + 1] = nrm.y;
+      C.dbgNormal[3 * r + 2] = nrm.z;
+      C.dbgOverlap[r] = ov;
+    }
+  }
+  double* f = S.pairs.sumF + 3 * (size_t)p;
+  double* ta = S.pairs.sumTA + 3 * (size_t)p;
+  double* tb = S.pairs.sumTB + 3 * (size_t)p;
+  f[0] = SF.x; f[1] = SF.y; f[2] = SF.z;
+  ta[0] = STA.x; ta[1] = STA.y; ta[2] = STA.z;
+  tb[0] = STB.x; tb[1] = STB.y; tb[2] = STB.z;
+}
+
 // Pair-level cull of every set's candidates at this step's positions (one
 // thread per pair): the pair is kept only if a node of A can still reach
 // the target (the broad-phase tests without the skin). Culled pairs get
@@ -772,24 +1044,37 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
 }
 
 // One wave per kept pair of the work list.
-// One wave per workgroup, so a finished wave frees its slot at once.
-__global__ void __launch_bounds__(64) kNarrowWork(NarrowMulti a, const int* work,
-                                                  const int* workCount) {
+// One wave per kept pair; one-wave workgroups, so a finished wave frees its
+// slot at once.
+__global__ void __launch_bounds__(64) kNarrowSweep(NarrowMulti a, NarrowScratch x, const int* work,
+                                                   const int* workCount) {
   if (gateOff(a.gate)) return;
   __shared__ HitSlots slots[1];
   const int lane = laneId();
-  const int wpb = blockDim.x >> 6;
-  HitSlots& hs = slots[threadIdx.x >> 6];
   const int total = *workCount;
-  const int stride = gridDim.x * wpb;
-  for (int w = blockIdx.x * wpb + (threadIdx.x >> 6); w < total; w += stride) {
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
     const int code = work[w];
     const NarrowSet& s = a.s[code & 7];
     const int p = code >> 3;
     const int pool = w % FDM_REC_POOLS;
-    if (s.kind == FDM_SET_PARTICLE) processPair<FDM_SET_PARTICLE>(a.c, s, p, lane, hs, pool);
-    else if (s.kind == FDM_SET_WALL) processPair<FDM_SET_WALL>(a.c, s, p, lane, hs, pool);
-    else processPair<FDM_SET_GHOST>(a.c, s, p, lane, hs, pool);
+    if (s.kind == FDM_SET_PARTICLE) sweepPair<FDM_SET_PARTICLE>(a.c, s, x, w, p, lane, slots[0], pool);
+    else if (s.kind == FDM_SET_WALL) sweepPair<FDM_SET_WALL>(a.c, s, x, w, p, lane, slots[0], pool);
+    else sweepPair<FDM_SET_GHOST>(a.c, s, x, w, p, lane, slots[0], pool);
+  }
+}
+
+// One thread per kept pair.
+__global__ void __launch_bounds__(256) kContactLaw(NarrowMulti a, NarrowScratch x, const int* work,
+                                                   const int* workCount) {
+  if (gateOff(a.gate)) return;
+  const int total = *workCount;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < total; w += gridDim.x * blockDim.x) {
+    const int code = work[w];
+    const NarrowSet& s = a.s[code & 7];
+    const int p = code >> 3;
+    if (s.kind == FDM_SET_PARTICLE) lawPair<FDM_SET_PARTICLE>(a.c, s, x, w, p);
+    else if (s.kind == FDM_SET_WALL) lawPair<FDM_SET_WALL>(a.c, s, x, w, p);
+    else lawPair<FDM_SET_GHOST>(a.c, s, x, w, p);
   }
 }
 
@@ -960,9 +1245,10 @@ extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream
 }
 
 extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets,
-                                         int* workList, int workCapacity, int* workCount,
-                                         hipStream_t stream) {
-  if (!sets || nsets < 1 || nsets > kMaxSets || !workList || !workCount) return FDM_EINVAL;
+                                         const fdm_narrow_workspace* ws, hipStream_t stream) {
+  if (!sets || nsets < 1 || nsets > kMaxSets || !ws || !ws->workList || !ws->workCount ||
+      !ws->workHit || !ws->hitGeom || !ws->hitCount || ws->hitCapacity < FDM_REC_POOLS)
+    return FDM_EINVAL;
   NarrowMulti k;
   fillCommon(k.c, sets[0]);
   k.gate = sets[0].gate;
@@ -973,11 +1259,19 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
     fillSet(k.s[i], sets[i]);
     cap += sets[i].pairs.capacity;
   }
-  if (cap > workCapacity || cap >= (1ll << 28)) return FDM_EINVAL;
+  if (cap > ws->workCapacity || cap >= (1ll << 28)) return FDM_EINVAL;
   for (int i = nsets; i < kMaxSets; ++i) k.s[i] = k.s[0];
+  NarrowScratch x;
+  x.workHit = ws->workHit;
+  x.hitGeom = ws->hitGeom;
+  x.hitCapacity = ws->hitCapacity;
+  x.hitCount = ws->hitCount;
   const unsigned cullBlocks = std::max<unsigned>(1, std::min<unsigned>(blocksFor(cap, 256), 2048));
-  hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, workList, workCount);
-  hipLaunchKernelGGL(kNarrowWork, dim3(persistentBlocks(1)), dim3(64), 0, stream, k,
-                     (const int*)workList, (const int*)workCount);
+  hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, ws->workList,
+                     ws->workCount);
+  hipLaunchKernelGGL(kNarrowSweep, dim3(persistentBlocks(1)), dim3(64), 0, stream, k, x,
+                     (const int*)ws->workList, (const int*)ws->workCount);
+  hipLaunchKernelGGL(kContactLaw, dim3(persistentBlocks(4)), dim3(256), 0, stream, k, x,
+                     (const int*)ws->workList, (const int*)ws->workCount);
   return launchStatus();
 }

@@ -295,6 +295,11 @@ class LSSolver {
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
       workList_.allocate((size_t)cap, false);
       workCount_.allocate(FDM_REC_COUNTERS);
+      workHit_.allocate((size_t)cap, false);
+      // staged contacts of one step, all sets together
+      hitCount_.allocate(FDM_REC_COUNTERS);
+      hitGeom_.allocate(8 * (size_t)recCap, false);
+      hitCapacity_ = recCap;
     }
     status_.allocate(1);
     stepCounter_.allocate(1);
@@ -707,6 +712,7 @@ class LSSolver {
     for (int s = 0; s < 3; ++s)
       if (periodic_.active(s)) a.clearCounters[q++] = periodic_.set(s).interaction.nextRecordCounter();
     a.clearCounters[q++] = workCount_.get();
+    a.clearCounters[q++] = hitCount_.get();
     const fdm_skin sk = skinView();
     const size_t wsBytes = workspace_.size() * sizeof(double);
     {
@@ -841,9 +847,15 @@ class LSSolver {
     na[0].gate = gate;
     {
       Timer t(this, "narrow");
-      fdmCheck(fdm_launchNarrowPhaseSets(na, ns, workList_.get(), (int)workList_.size(),
-                                         workCount_.get(), stream_),
-               "narrow phase");
+      fdm_narrow_workspace w;
+      w.workList = workList_.get();
+      w.workCapacity = (int)workList_.size();
+      w.workCount = workCount_.get();
+      w.workHit = workHit_.get();
+      w.hitGeom = hitGeom_.get();
+      w.hitCapacity = hitCapacity_;
+      w.hitCount = hitCount_.get();
+      fdmCheck(fdm_launchNarrowPhaseSets(na, ns, &w, stream_), "narrow phase");
     }
     if (prime) {
       // no forces in the pre-loop build: drop the mirror entries
@@ -1023,7 +1035,9 @@ class LSSolver {
   DeviceBuffer<fdm_grid> instPGrid_, instWGrid_;
   DeviceBuffer<long long> instPOff_, instWOff_;
   DeviceBuffer<fdm_periodic> instPeriodic_;
-  DeviceBuffer<int> workList_, workCount_;
+  DeviceBuffer<int> workList_, workCount_, workHit_, hitCount_;
+  DeviceBuffer<double> hitGeom_;
+  int hitCapacity_ = 0;
   DeviceBuffer<double> disp_;
   double skinRequest_ = -1.0, skin_ = 0.0;
   double maxRadius_ = 0.0, maxBoxReach_ = 0.0, rhoMax_ = 0.0;

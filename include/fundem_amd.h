@@ -477,18 +477,32 @@ typedef struct {
   double* dbgOverlap;
 } fdm_narrowphase_args;
 int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
+/* Scratch of fdm_launchNarrowPhaseSets. */
+typedef struct {
+  int* workList;               /* [workCapacity] kept pairs, (pair << 3) | set */
+  int workCapacity;            /* at least the sum of the sets' pair capacities */
+  int* workCount;              /* [FDM_REC_COUNTERS] kept-pair counter (entry 0) */
+  int* workHit;                /* [workCapacity] first staged contact of each kept pair */
+  double* hitGeom;             /* [8 * hitCapacity] staged contacts */
+  int hitCapacity;             /* staged contacts, split into FDM_REC_POOLS pools */
+  int* hitCount;               /* [FDM_REC_COUNTERS] staged contacts per pool */
+} fdm_narrow_workspace;
 /* No counterpart in the reference: the narrow phase of up to five sets at
- * once. A first kernel (one thread per pair) keeps the pairs whose bodies can
- * still touch at this step's positions (the broad-phase tests without the
- * skin) in workList and gives the others empty record ranges; a second runs
- * the wave-per-pair narrow phase over workList. Shared fields (bodies,
- * geometry, materials, pgrid, periodic, dt, updateSprings, status, dbg*) come
- * from sets[0]. workCapacity must cover the sum of the sets' pair capacities.
- * Unlike fdm_launchNarrowPhase it clears no counter: the sets' record
- * counters (pairs.recTotal) and *workCount must be zero at launch. Pair
- * work item w reserves its records from pool w % FDM_REC_POOLS. */
-int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets, int* workList,
-                              int workCapacity, int* workCount, hipStream_t stream);
+ * once, in three kernels. The first (one thread per pair) keeps the pairs
+ * whose bodies can still touch at this step's positions (the broad-phase
+ * tests without the skin) in workList and gives the others empty record
+ * ranges; the second (one wave per kept pair) sweeps the surface nodes,
+ * stages every contact's point, normal and overlap in hitGeom and reserves
+ * the pair's records; the third (one thread per kept pair) applies the
+ * contact law to the staged contacts in node order and writes the records
+ * and the pair sums. Shared fields (bodies, geometry, materials, pgrid,
+ * periodic, dt, updateSprings, status, dbg*) come from sets[0]. Unlike
+ * fdm_launchNarrowPhase it clears no counter: the sets' record counters
+ * (pairs.recTotal), *workCount and the hitCount pools must be zero at launch.
+ * Work item w reserves its records and staged contacts from pool
+ * w % FDM_REC_POOLS. */
+int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets,
+                              const fdm_narrow_workspace* ws, hipStream_t stream);
 
 /* Replaces launchAddBondedForceTorque (contactKernel.cuh:311): VBond forces,
  * stresses and one-way failure; results per bond. */
