@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../include/fundem/LSAdjoint.h"
 #include "../include/fundem/LSSolver.h"
 
 namespace {
@@ -64,6 +65,81 @@ int fdmx_add_material(void* s, double kn, double ks, double mu, double e) {
 }
 int fdmx_add_wall_material(void* s, double mu, double e) {
   return guarded([&] { static_cast<BoundSolver*>(s)->addWallMaterial(mu, e); });
+}
+int fdmx_add_material_damping(void* s, double kn, double ks, double mu, double zeta) {
+  return guarded(
+      [&] { static_cast<BoundSolver*>(s)->addParticleMaterialDampingRatio(kn, ks, mu, zeta); });
+}
+int fdmx_add_wall_material_damping(void* s, double mu, double zeta) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->addWallMaterialDampingRatio(mu, zeta); });
+}
+int fdmx_num_materials(void* s, int* particle, int* wall) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    *particle = (int)S->getLSParticle().numMaterial();
+    *wall = (int)S->getWall().numMaterial();
+  });
+}
+
+// Adjoint (fundem::LSAdjoint) of a solver; the solver must outlive it.
+void* fdmx_adjoint_create(void* s) {
+  try {
+    return new fundem::LSAdjoint(*static_cast<BoundSolver*>(s));
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+void fdmx_adjoint_destroy(void* a) { delete static_cast<fundem::LSAdjoint*>(a); }
+int fdmx_adjoint_set_contacts(void* a, int perParticle) {
+  return guarded([&] { static_cast<fundem::LSAdjoint*>(a)->setContactsPerParticle(perParticle); });
+}
+int fdmx_adjoint_forward(void* a, long long steps, long long every) {
+  return guarded([&] { static_cast<fundem::LSAdjoint*>(a)->forward((size_t)steps, (size_t)every); });
+}
+// Adjoints of the final state in (AoS: 3 or 4 per particle); gradients out:
+// initial state (AoS) and per material (particle: kn, ks, mu, zeta; wall:
+// mu, zeta). Any output may be NULL.
+int fdmx_adjoint_backward(void* a, int n, const double* xb, const double* qb, const double* vb,
+                          const double* wb, double* x0, double* q0, double* v0, double* w0,
+                          double* kn, double* ks, double* mu, double* zeta, double* wmu,
+                          double* wzeta) {
+  return guarded([&] {
+    std::vector<double3> X(n), V(n), Wv(n);
+    std::vector<quaternion> Q(n);
+    for (int i = 0; i < n; ++i) {
+      X[i] = d3(xb + 3 * i);
+      V[i] = d3(vb + 3 * i);
+      Wv[i] = d3(wb + 3 * i);
+      Q[i] = make_quaternion(qb[4 * i], qb[4 * i + 1], qb[4 * i + 2], qb[4 * i + 3]);
+    }
+    const fundem::AdjointGradients g = static_cast<fundem::LSAdjoint*>(a)->backward(X, Q, V, Wv);
+    for (int i = 0; i < n; ++i) {
+      if (x0) { x0[3 * i] = g.position[i].x; x0[3 * i + 1] = g.position[i].y; x0[3 * i + 2] = g.position[i].z; }
+      if (v0) { v0[3 * i] = g.velocity[i].x; v0[3 * i + 1] = g.velocity[i].y; v0[3 * i + 2] = g.velocity[i].z; }
+      if (w0) {
+        w0[3 * i] = g.angularVelocity[i].x;
+        w0[3 * i + 1] = g.angularVelocity[i].y;
+        w0[3 * i + 2] = g.angularVelocity[i].z;
+      }
+      if (q0) {
+        q0[4 * i] = g.orientation[i].q0;
+        q0[4 * i + 1] = g.orientation[i].q1;
+        q0[4 * i + 2] = g.orientation[i].q2;
+        q0[4 * i + 3] = g.orientation[i].q3;
+      }
+    }
+    for (size_t m = 0; m < g.kn.size(); ++m) {
+      if (kn) kn[m] = g.kn[m];
+      if (ks) ks[m] = g.ks[m];
+      if (mu) mu[m] = g.mu[m];
+      if (zeta) zeta[m] = g.dampingRatio[m];
+    }
+    for (size_t m = 0; m < g.wallMu.size(); ++m) {
+      if (wmu) wmu[m] = g.wallMu[m];
+      if (wzeta) wzeta[m] = g.wallDampingRatio[m];
+    }
+  });
 }
 int fdmx_add_geometry(void* s, double mass, const double* I6, const double* o3, double h,
                       const int* sz, const double* sdf, const double* nodes, int nn) {

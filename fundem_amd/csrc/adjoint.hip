@@ -1,0 +1,619 @@
+// Adjoint timestep: the contact tape of a checkpoint segment (append and
+// restore) and one step in reverse. The reverse step is four kernels:
+//   kAdjKick     thread per particle: reverse of the second half-kick, which
+//                yields the adjoint of every particle's contact force and
+//                torque sums;
+//   kAdjContacts thread per node-contact: the hand-written VJP of the contact
+//                geometry, the contact law and the torque arms (adjoint_math.hpp),
+//                writing each contact's share for A and for its target and the
+//                adjoint of its previous spring into the previous block;
+//   kAdjGather   thread per particle: its shares in a fixed order (as A among
+//                particle pairs, as A among wall pairs, as target by the key
+//                order), then the reverse of the drift and the first half-kick;
+//   kAdjWalls    thread per wall: the wall-material parameter shares.
+// Every sum runs in a fixed order, so the adjoint is deterministic.
+#include <algorithm>
+
+#include "fdm_device.hpp"
+#include "fdm_launch.hpp"
+#include "adjoint_math.hpp"
+
+namespace fdm {
+namespace {
+
+constexpr int kStride = 1024;  // blocks of the grid-stride loops over device counts
+constexpr int kShare = 17;     // x 3, q 4, v 3, w 3, kn ks mu zeta
+
+__device__ __forceinline__ adj::V toV(V3 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ adj::Q toQ(Q4 a) { return {a.q0, a.q1, a.q2, a.q3}; }
+__device__ __forceinline__ adj::S toS(const double* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
+__device__ __forceinline__ adj::V ldV(const double* p, int n, int i) { return {p[i], p[n + i], p[2 * n + i]}; }
+__device__ __forceinline__ adj::Q ldQ(const double* p, int n, int i) {
+  return {p[i], p[n + i], p[2 * n + i], p[3 * n + i]};
+}
+__device__ __forceinline__ void stV(double* p, int n, int i, adj::V v) {
+  p[i] = v.x;
+  p[n + i] = v.y;
+  p[2 * n + i] = v.z;
+}
+__device__ __forceinline__ void stQ(double* p, int n, int i, adj::Q q) {
+  p[i] = q.q0;
+  p[n + i] = q.q1;
+  p[2 * n + i] = q.q2;
+  p[3 * n + i] = q.q3;
+}
+__device__ __forceinline__ adj::V f32(adj::V v) { return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z}; }
+
+// ------------------------------------------------------------ tape append
+__global__ void kTapeCount(fdm_pairs s, int* cnt) {
+  const int total = min(*s.total, s.capacity);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < s.capacity; p += gridDim.x * blockDim.x)
+    cnt[p] = p < total ? s.recCnt[p] : 0;
+}
+
+__global__ void kTapeBlock(fdm_contact_tape t, int block, const int* totals, int maxBlock) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int pp = totals[0], pw = totals[1];
+  const int base = *t.head;
+  if (pp + pw > maxBlock || (long long)base + pp + pw > (long long)t.capacity) {
+    atomicOr(t.status, FDM_STATUS_RECORD_OVERFLOW);
+    t.blockStart[block] = base;
+    t.blockPP[block] = 0;
+    t.blockCount[block] = 0;
+    return;
+  }
+  t.blockStart[block] = base;
+  t.blockPP[block] = pp;
+  t.blockCount[block] = pp + pw;
+  *t.head = base + pp + pw;
+}
+
+__global__ void kTapeScatter(fdm_pairs s, int wallPart, const int* off, const int* totals,
+                             fdm_contact_tape t, int block) {
+  if (t.blockCount[block] != totals[0] + totals[1]) return;  // block did not fit
+  const int total = min(*s.total, s.capacity);
+  const int base = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+    const int n = s.recCnt[p];
+    if (n <= 0) continue;
+    const int d0 = base + off[p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
+    for (int r = 0; r < n; ++r) {
+      const float4 rec = s.rec[r0 + r];
+      const int c = d0 + r;
+      t.A[c] = A;
+      t.T[c] = T;
+      t.node[c] = __float_as_int(rec.w);
+      t.spring[3 * c] = rec.x;
+      t.spring[3 * c + 1] = rec.y;
+      t.spring[3 * c + 2] = rec.z;
+    }
+  }
+}
+
+__device__ __forceinline__ bool keyLess(const fdm_contact_tape& t, int c, int A, int T, int k) {
+  if (t.A[c] != A) return t.A[c] < A;
+  if (t.T[c] != T) return t.T[c] < T;
+  return t.node[c] < k;
+}
+
+// the same (A, target, node) in the previous block's part of the same kind
+__global__ void kTapePrev(fdm_contact_tape t, int block, int prevBlock) {
+  const int start = t.blockStart[block], count = t.blockCount[block], pp = t.blockPP[block];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const int c = start + i;
+    int found = -1;
+    if (prevBlock >= 0) {
+      const int ps = t.blockStart[prevBlock], ppp = t.blockPP[prevBlock];
+      int lo = i < pp ? ps : ps + ppp;
+      int hi = i < pp ? ps + ppp : ps + t.blockCount[prevBlock];
+      const int A = t.A[c], T = t.T[c], k = t.node[c];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keyLess(t, mid, A, T, k)) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < (i < pp ? ps + ppp : ps + t.blockCount[prevBlock]) && t.A[lo] == A && t.T[lo] == T &&
+          t.node[lo] == k)
+        found = lo;
+    }
+    t.prev[c] = found;
+  }
+}
+
+__global__ void kTapeKeys(fdm_contact_tape t, int block, int nParticles, unsigned* key, int* val) {
+  const int start = t.blockStart[block], count = t.blockCount[block], pp = t.blockPP[block];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    key[i] = (unsigned)t.T[start + i] + (i < pp ? 0u : (unsigned)nParticles);
+    val[i] = i;
+  }
+}
+
+__global__ void kTapeKeysStore(fdm_contact_tape t, int block, const unsigned* key, const int* val) {
+  const int start = t.blockStart[block], count = t.blockCount[block];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    t.key[start + i] = key[i];
+    t.order[start + i] = val[i];
+  }
+}
+
+// ----------------------------------------------------------- tape restore
+// pair starts of a block part; zeros up to maxBlock so that the scan total counts pairs
+__global__ void kRestoreFlags(fdm_contact_tape t, int block, int wallPart, int* flag, int maxBlock) {
+  const int start = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
+  const int count = wallPart ? t.blockCount[block] - t.blockPP[block] : t.blockPP[block];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < maxBlock; i += gridDim.x * blockDim.x) {
+    const int c = start + i;
+    flag[i] = i < count && (i == 0 || t.A[c] != t.A[c - 1] || t.T[c] != t.T[c - 1]) ? 1 : 0;
+  }
+}
+
+__global__ void kRestoreScatter(fdm_contact_tape t, int block, int wallPart, const int* flag,
+                                const int* idx, const int* numPairs, fdm_pairs s) {
+  const int start = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
+  const int count = wallPart ? t.blockCount[block] - t.blockPP[block] : t.blockPP[block];
+  if (*numPairs > s.capacity || count > s.recCapacity) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(t.status, FDM_STATUS_PAIR_OVERFLOW);
+    return;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const int c = start + i;
+    s.rec[i] = make_float4(t.spring[3 * c], t.spring[3 * c + 1], t.spring[3 * c + 2],
+                           __int_as_float(t.node[c]));
+    if (!flag[i]) continue;
+    int j = i + 1;
+    while (j < count && t.A[start + j] == t.A[c] && t.T[start + j] == t.T[c]) ++j;
+    const int p = idx[i];
+    s.pairA[p] = t.A[c];
+    s.pairT[p] = t.T[c];
+    s.pairG[p] = 0;
+    s.recOff[p] = i;
+    s.recCnt[p] = j - i;
+  }
+}
+
+// CSR starts by A over the restored pairs
+__global__ void kRestoreCount(fdm_pairs s, int nParticles, const int* numPairs) {
+  const int P = min(*numPairs, s.capacity);
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a <= nParticles; a += gridDim.x * blockDim.x) {
+    int lo = 0, hi = P;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s.pairA[mid] < a) lo = mid + 1;
+      else hi = mid;
+    }
+    s.count[a] = lo;
+    if (a == 0) *s.total = P;
+  }
+}
+
+// --------------------------------------------------------- reverse step
+struct AdjWork {
+  double* vh;      // [3n] v after the first half-kick
+  double* wh;      // [3n]
+  double* Fc;      // [3n] adjoint of the contact force sum
+  double* Tc;      // [3n]
+  double* shareA;  // [kShare * maxBlock]
+  double* shareB;  // [kShare * maxBlock]
+};
+
+__device__ __forceinline__ void kickPrimal(const fdm_adjoint_step_args& a, const fdm_bodies& b,
+                                           const fdm_template& t, int i, adj::V& v, adj::V& w) {
+  const int n = b.n;
+  v = toV(ld3(b.v, n, i));
+  w = toV(ld3(b.w, n, i));
+  if (nearZero(t.invMass)) return;
+  const adj::V F = toV(ld3(b.F, n, i)), T = toV(ld3(b.T, n, i));
+  const adj::V g = {a.gravity.x, a.gravity.y, a.gravity.z};
+  const double h = 0.5 * a.dt;
+  v = f32(adj::add(v, adj::mul(adj::add(adj::mul(F, t.invMass), g), h)));
+  w = f32(adj::spin(w, toQ(ld4(b.q, n, i)), T, toS(t.inertia), toS(t.invInertia), h));
+}
+
+// reverse of the second half-kick, v' = vh + h (F' invM + g), w' = spin(wh, q', T')
+__global__ void __launch_bounds__(256) kAdjKick(fdm_adjoint_step_args a, AdjWork k) {
+  const int n = a.before.n;
+  const double h = 0.5 * a.dt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const fdm_template& t = a.pgeo.info[a.before.geo[i]];
+    adj::V vh, wh;
+    kickPrimal(a, a.before, t, i, vh, wh);
+    stV(k.vh, n, i, vh);
+    stV(k.wh, n, i, wh);
+    const adj::V vb = ldV(a.vBar, n, i), wb = ldV(a.wBar, n, i);
+    adj::V Fc = ldV(a.FBar, n, i), Tc = ldV(a.TBar, n, i);
+    if (!nearZero(t.invMass)) {
+      adj::acc(Fc, adj::mul(vb, h * t.invMass));
+      adj::V whb = {0., 0., 0.}, tqb = {0., 0., 0.};
+      adj::Q qb = ldQ(a.qBar, n, i);
+      adj::spinVjp(wh, toQ(ld4(a.after.q, n, i)), toV(ld3(a.after.T, n, i)), toS(t.inertia),
+                   toS(t.invInertia), h, wb, whb, qb, tqb);
+      adj::acc(Tc, tqb);
+      stQ(a.qBar, n, i, qb);
+      stV(a.wBar, n, i, whb);
+    }
+    stV(k.Fc, n, i, Fc);
+    stV(k.Tc, n, i, Tc);
+  }
+}
+
+__device__ __forceinline__ void putShare(double* s, const adj::BodyBar& b, double kn, double ks,
+                                         double mu, double zeta) {
+  s[0] = b.x.x; s[1] = b.x.y; s[2] = b.x.z;
+  s[3] = b.q.q0; s[4] = b.q.q1; s[5] = b.q.q2; s[6] = b.q.q3;
+  s[7] = b.v.x; s[8] = b.v.y; s[9] = b.v.z;
+  s[10] = b.w.x; s[11] = b.w.y; s[12] = b.w.z;
+  s[13] = kn; s[14] = ks; s[15] = mu; s[16] = zeta;
+}
+
+__global__ void __launch_bounds__(256) kAdjContacts(fdm_adjoint_step_args a, AdjWork k) {
+  const fdm_contact_tape& tp = a.tape;
+  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
+  const int n = a.before.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const int c = start + i;
+    const bool wall = i >= npp;
+    const int A = tp.A[c], T = tp.T[c], node = tp.node[c];
+    const fdm_template& tA = a.pgeo.info[a.after.geo[A]];
+    const fdm_material& Mi = a.pmat[a.after.mat[A]];
+    const adj::V xA = toV(ldp(a.after.x, n, A));
+    const adj::Q qA = toQ(ld4(a.after.q, n, A));
+    const adj::V vA = ldV(k.vh, n, A), wA = ldV(k.wh, n, A);
+    adj::V xB, vB, wB;
+    adj::Q qB;
+    const fdm_template* tB;
+    const float* sdf;
+    adj::LawConst lc;
+    lc.dt = a.dt;
+    double shKnA = 0., shKnB = 0., shKsA = 0., shKsB = 0.;
+    bool muToA = true, zetaToA = true;
+    if (wall) {
+      const int m = a.walls.n;
+      xB = toV(ldp(a.walls.x, m, T));
+      qB = toQ(ld4(a.walls.q, m, T));
+      vB = toV(ld3(a.walls.v, m, T));
+      wB = toV(ld3(a.walls.w, m, T));
+      tB = &a.wgeo.info[a.walls.geo[T]];
+      sdf = a.wgeo.sdf;
+      const fdm_material& Mw = a.wmat[a.walls.mat[T]];
+      lc.effM = tA.invMass > 0. ? 1. / tA.invMass : 0.;
+      lc.mu = fmin(Mi.mu, Mw.mu);
+      lc.zeta = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
+      lc.kn = Mi.kn;
+      lc.ks = Mi.ks;
+      shKnA = 1.;
+      shKsA = 1.;
+      muToA = Mi.mu <= Mw.mu;
+      zetaToA = Mi.e <= Mw.e;
+    } else {
+      xB = toV(ldp(a.after.x, n, T));
+      qB = toQ(ld4(a.after.q, n, T));
+      vB = ldV(k.vh, n, T);
+      wB = ldV(k.wh, n, T);
+      tB = &a.pgeo.info[a.after.geo[T]];
+      sdf = a.pgeo.sdf;
+      const fdm_material& Mj = a.pmat[a.after.mat[T]];
+      const double invMj = tB->invMass;
+      lc.kn = 0.;
+      lc.ks = 0.;
+      if (Mi.kn > 0. && Mj.kn > 0.) {
+        lc.kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
+        const double s2 = (Mi.kn + Mj.kn) * (Mi.kn + Mj.kn);
+        shKnA = Mj.kn * Mj.kn / s2;
+        shKnB = Mi.kn * Mi.kn / s2;
+      }
+      if (Mi.ks > 0. && Mj.ks > 0.) {
+        lc.ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
+        const double s2 = (Mi.ks + Mj.ks) * (Mi.ks + Mj.ks);
+        shKsA = Mj.ks * Mj.ks / s2;
+        shKsB = Mi.ks * Mi.ks / s2;
+      }
+      lc.effM = (tA.invMass > 0. || invMj > 0.) ? 1. / (tA.invMass + invMj) : 0.;
+      lc.zeta = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
+      lc.mu = fmin(Mi.mu, Mj.mu);
+      muToA = Mi.mu <= Mj.mu;
+      zetaToA = Mi.e <= Mj.e;
+    }
+    // the node's SDF cell at the linearisation point
+    adj::GeoConst g;
+    const float* nd = a.pgeo.nodes + 3 * ((size_t)tA.nodeStart + node);
+    g.s = {(double)nd[0], (double)nd[1], (double)nd[2]};
+    g.origin = {tB->origin[0], tB->origin[1], tB->origin[2]};
+    g.ih = tB->invSpacing;
+    const adj::V p = adj::add(adj::qrot(qA, g.s), xA);
+    const adj::V l = adj::qrotT(adj::sub(p, xB), qB);
+    const int ci = toInt(floor((l.x - g.origin.x) * g.ih));
+    const int cj = toInt(floor((l.y - g.origin.y) * g.ih));
+    const int ck = toInt(floor((l.z - g.origin.z) * g.ih));
+    adj::BodyBar Ab = {{0., 0., 0.}, {0., 0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    adj::BodyBar Bb = Ab;
+    adj::V spb = {0., 0., 0.};
+    adj::LawBar pb = {0., 0., 0., 0.};
+    const int pc = tp.prev[c];
+    if (ci >= 0 && cj >= 0 && ck >= 0 && ci < tB->size[0] - 1 && cj < tB->size[1] - 1 &&
+        ck < tB->size[2] - 1) {
+      g.cell[0] = ci;
+      g.cell[1] = cj;
+      g.cell[2] = ck;
+      const int nx = tB->size[0], nxy = tB->size[0] * tB->size[1];
+      const float* b = sdf + tB->sdfStart + ((size_t)ck * nxy + (size_t)cj * nx + ci);
+      g.c[0] = b[0];
+      g.c[1] = b[1];
+      g.c[2] = b[nx];
+      g.c[3] = b[nx + 1];
+      g.c[4] = b[nxy];
+      g.c[5] = b[nxy + 1];
+      g.c[6] = b[nxy + nx];
+      g.c[7] = b[nxy + nx + 1];
+      const adj::V sp = pc >= 0 ? adj::V{(double)tp.spring[3 * pc], (double)tp.spring[3 * pc + 1],
+                                         (double)tp.spring[3 * pc + 2]}
+                                : adj::V{0., 0., 0.};
+      adj::V Fb = ldV(k.Fc, n, A);
+      adj::V TBb = {0., 0., 0.};
+      if (!wall) {
+        Fb = adj::sub(Fb, ldV(k.Fc, n, T));
+        TBb = ldV(k.Tc, n, T);
+      }
+      const adj::V sb = {a.springBar[3 * (size_t)c], a.springBar[3 * (size_t)c + 1],
+                         a.springBar[3 * (size_t)c + 2]};
+      adj::contactVjp(xA, qA, vA, wA, xB, qB, vB, wB, sp, g, lc, Fb, ldV(k.Tc, n, A), TBb, sb, Ab,
+                      Bb, spb, pb);
+    }
+    if (pc >= 0) {
+      a.springBar[3 * (size_t)pc] = spb.x;
+      a.springBar[3 * (size_t)pc + 1] = spb.y;
+      a.springBar[3 * (size_t)pc + 2] = spb.z;
+    }
+    putShare(k.shareA + (size_t)kShare * i, Ab, pb.kn * shKnA, pb.ks * shKsA, muToA ? pb.mu : 0.,
+             zetaToA ? pb.zeta : 0.);
+    putShare(k.shareB + (size_t)kShare * i, Bb, pb.kn * shKnB, pb.ks * shKsB, muToA ? 0. : pb.mu,
+             zetaToA ? 0. : pb.zeta);
+  }
+}
+
+__device__ __forceinline__ void addShare(double* acc, const double* s) {
+#pragma unroll
+  for (int j = 0; j < kShare; ++j) acc[j] += s[j];
+}
+
+// first index in [lo, hi) whose tape A is >= v
+__device__ __forceinline__ int lowerA(const int* A, int lo, int hi, int v) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int lowerKey(const unsigned* K, int lo, int hi, unsigned v) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (K[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) kAdjGather(fdm_adjoint_step_args a, AdjWork k) {
+  const fdm_contact_tape& tp = a.tape;
+  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
+  const int n = a.before.n;
+  const double dt = a.dt, h = 0.5 * a.dt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s[kShare];
+#pragma unroll
+    for (int j = 0; j < kShare; ++j) s[j] = 0.;
+    for (int c = lowerA(tp.A, start, start + npp, i); c < start + npp && tp.A[c] == i; ++c)
+      addShare(s, k.shareA + (size_t)kShare * (c - start));
+    for (int c = lowerA(tp.A, start + npp, start + count, i); c < start + count && tp.A[c] == i; ++c)
+      addShare(s, k.shareA + (size_t)kShare * (c - start));
+    for (int e = lowerKey(tp.key, start, start + npp, (unsigned)i); e < start + npp && tp.key[e] == (unsigned)i; ++e)
+      addShare(s, k.shareB + (size_t)kShare * tp.order[e]);
+    double* pbar = a.paramBar + 4 * (size_t)i;
+    pbar[0] += s[13];
+    pbar[1] += s[14];
+    pbar[2] += s[15];
+    pbar[3] += s[16];
+    // adjoint of the state after the drift
+    adj::V xb = ldV(a.xBar, n, i);
+    adj::Q qb = ldQ(a.qBar, n, i);
+    adj::V vhb = ldV(a.vBar, n, i), whb = ldV(a.wBar, n, i);
+    adj::acc(xb, {s[0], s[1], s[2]});
+    adj::accq(qb, {s[3], s[4], s[5], s[6]});
+    adj::acc(vhb, {s[7], s[8], s[9]});
+    adj::acc(whb, {s[10], s[11], s[12]});
+    // drift: x' = x + dt vh, q' = qint(q, wh, dt)
+    adj::acc(vhb, adj::mul(xb, dt));
+    const adj::Q q = toQ(ld4(a.before.q, n, i));
+    adj::Q q0b = {0., 0., 0., 0.};
+    adj::qintVjp(q, ldV(k.wh, n, i), dt, qb, q0b, whb);
+    // first half-kick
+    const fdm_template& t = a.pgeo.info[a.before.geo[i]];
+    adj::V wb = whb, Fb = {0., 0., 0.}, Tb = {0., 0., 0.};
+    if (!nearZero(t.invMass)) {
+      Fb = adj::mul(vhb, h * t.invMass);
+      wb = {0., 0., 0.};
+      adj::spinVjp(toV(ld3(a.before.w, n, i)), q, toV(ld3(a.before.T, n, i)), toS(t.inertia),
+                   toS(t.invInertia), h, whb, wb, q0b, Tb);
+    }
+    stV(a.xBar, n, i, xb);
+    stQ(a.qBar, n, i, q0b);
+    stV(a.vBar, n, i, vhb);
+    stV(a.wBar, n, i, wb);
+    stV(a.FBar, n, i, Fb);
+    stV(a.TBar, n, i, Tb);
+  }
+}
+
+__global__ void kAdjWalls(fdm_adjoint_step_args a, AdjWork k) {
+  const fdm_contact_tape& tp = a.tape;
+  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
+  const unsigned base = (unsigned)a.before.n;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < a.walls.n; w += gridDim.x * blockDim.x) {
+    double mu = 0., zeta = 0.;
+    for (int e = lowerKey(tp.key, start + npp, start + count, base + w);
+         e < start + count && tp.key[e] == base + (unsigned)w; ++e) {
+      const double* s = k.shareB + (size_t)kShare * tp.order[e];
+      mu += s[15];
+      zeta += s[16];
+    }
+    a.wallParamBar[2 * w] += mu;
+    a.wallParamBar[2 * w + 1] += zeta;
+  }
+}
+
+__global__ void kClearSpringBar(fdm_contact_tape t, int block, double* springBar) {
+  const int start = t.blockStart[block], count = t.blockCount[block];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * count; i += gridDim.x * blockDim.x)
+    springBar[3 * (size_t)start + i] = 0.;
+}
+
+// workspace carving
+size_t up(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct TapeWs {
+  int *cntPP, *cntPW, *offPP, *offPW, *totals, *flag, *idx, *val;
+  unsigned* key;
+  void* sub;
+  size_t subBytes;
+};
+size_t tapeSubBytes(int maxBlock, int capPP, int capPW) {
+  const size_t m = std::max<size_t>({(size_t)maxBlock, (size_t)capPP, (size_t)capPW, 64});
+  return std::max(scanWorkspace(m), radixWorkspace(std::max(maxBlock, 64)));
+}
+size_t tapeBytes(int maxBlock, int capPP, int capPW, TapeWs* w, char* base) {
+  const size_t m = std::max(maxBlock, 64);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + o : nullptr;
+    o += up(bytes);
+    return p;
+  };
+  int* cntPP = (int*)take(sizeof(int) * std::max(capPP, 1));
+  int* cntPW = (int*)take(sizeof(int) * std::max(capPW, 1));
+  int* offPP = (int*)take(sizeof(int) * std::max(capPP, 1));
+  int* offPW = (int*)take(sizeof(int) * std::max(capPW, 1));
+  int* totals = (int*)take(sizeof(int) * 8);
+  int* flag = (int*)take(sizeof(int) * m);
+  int* idx = (int*)take(sizeof(int) * m);
+  int* val = (int*)take(sizeof(int) * m);
+  unsigned* key = (unsigned*)take(sizeof(unsigned) * m);
+  const size_t sb = tapeSubBytes(maxBlock, capPP, capPW);
+  void* sub = take(sb);
+  if (w) *w = {cntPP, cntPW, offPP, offPW, totals, flag, idx, val, key, sub, sb};
+  return o;
+}
+
+}  // namespace
+}  // namespace fdm
+
+using namespace fdm;
+
+extern "C" size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, int pairCapacityPP, int pairCapacityPW) {
+  return tapeBytes(maxBlockContacts, pairCapacityPP, pairCapacityPW, nullptr, nullptr);
+}
+
+extern "C" int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, int nParticles, int nWalls,
+                                    fdm_contact_tape tape, int block, int prevBlock,
+                                    int maxBlockContacts, void* ws, size_t wsBytes, hipStream_t stream) {
+  if (!pp || block < 0 || block >= tape.maxBlocks || prevBlock >= block || nParticles < 0 ||
+      nWalls < 0 || maxBlockContacts < 1)
+    return FDM_EINVAL;
+  const int capPW = pw ? pw->capacity : 1;
+  TapeWs w;
+  if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
+  tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws);
+  hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pp->capacity, 256), 4096u)), dim3(256), 0,
+                     stream, *pp, w.cntPP);
+  int r = scanInts(w.cntPP, w.offPP, pp->capacity, false, w.totals, w.sub, w.subBytes, stream);
+  if (r != FDM_OK) return r;
+  if (pw) {
+    hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pw->capacity, 256), 4096u)), dim3(256), 0,
+                       stream, *pw, w.cntPW);
+    r = scanInts(w.cntPW, w.offPW, pw->capacity, false, w.totals + 1, w.sub, w.subBytes, stream);
+    if (r != FDM_OK) return r;
+  } else {
+    r = fillInts(w.totals + 1, 1, 0, stream);
+    if (r != FDM_OK) return r;
+  }
+  hipLaunchKernelGGL(kTapeBlock, dim3(1), dim3(64), 0, stream, tape, block, (const int*)w.totals,
+                     maxBlockContacts);
+  hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pp, 0, (const int*)w.offPP,
+                     (const int*)w.totals, tape, block);
+  if (pw)
+    hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pw, 1,
+                       (const int*)w.offPW, (const int*)w.totals, tape, block);
+  hipLaunchKernelGGL(kTapePrev, dim3(kStride), dim3(256), 0, stream, tape, block, prevBlock);
+  hipLaunchKernelGGL(kTapeKeys, dim3(kStride), dim3(256), 0, stream, tape, block, nParticles, w.key,
+                     w.val);
+  const int bits = keyBitsFor((long long)nParticles + nWalls + 1);
+  r = radixSort(w.key, w.val, (size_t)maxBlockContacts, tape.blockCount + block, bits, w.sub,
+                w.subBytes, stream);
+  if (r != FDM_OK) return r;
+  hipLaunchKernelGGL(kTapeKeysStore, dim3(kStride), dim3(256), 0, stream, tape, block,
+                     (const unsigned*)w.key, (const int*)w.val);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* pp,
+                                     const fdm_pairs* pw, int nParticles, int maxBlockContacts,
+                                     void* ws, size_t wsBytes, hipStream_t stream) {
+  if (!pp || block < 0 || block >= tape.maxBlocks || nParticles < 0 || maxBlockContacts < 1)
+    return FDM_EINVAL;
+  const int capPW = pw ? pw->capacity : 1;
+  TapeWs w;
+  if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
+  tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws);
+  for (int part = 0; part < 2; ++part) {
+    const fdm_pairs* s = part == 0 ? pp : pw;
+    if (!s) continue;
+    hipLaunchKernelGGL(kRestoreFlags, dim3(kStride), dim3(256), 0, stream, tape, block, part, w.flag,
+                       maxBlockContacts);
+    int r = scanInts(w.flag, w.idx, (size_t)maxBlockContacts, false, w.totals + 2 + part, w.sub,
+                     w.subBytes, stream);
+    if (r != FDM_OK) return r;
+    hipLaunchKernelGGL(kRestoreScatter, dim3(kStride), dim3(256), 0, stream, tape, block, part,
+                       (const int*)w.flag, (const int*)w.idx, (const int*)(w.totals + 2 + part), *s);
+    hipLaunchKernelGGL(kRestoreCount, dim3(std::min(blocksFor((long long)nParticles + 1, 256), 4096u)),
+                       dim3(256), 0, stream, *s, nParticles, (const int*)(w.totals + 2 + part));
+  }
+  return launchStatus();
+}
+
+extern "C" size_t fdm_adjointWorkspaceBytes(int nParticles, int maxBlockContacts) {
+  const size_t n = std::max(nParticles, 1), m = std::max(maxBlockContacts, 1);
+  return up(4 * 3 * n * sizeof(double)) + 2 * up(kShare * m * sizeof(double));
+}
+
+extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
+                                     hipStream_t stream) {
+  if (!args || args->block < 1 || args->block >= args->tape.maxBlocks ||
+      args->prevBlock >= args->block || args->before.n != args->after.n || !args->xBar ||
+      !args->qBar || !args->vBar || !args->wBar || !args->FBar || !args->TBar ||
+      !args->springBar || !args->paramBar || (args->walls.n > 0 && !args->wallParamBar))
+    return FDM_EINVAL;
+  const int n = args->before.n;
+  if (wsBytes < fdm_adjointWorkspaceBytes(n, args->maxBlockContacts)) return FDM_EINVAL;
+  char* b = (char*)ws;
+  AdjWork k;
+  const size_t v3 = 3 * (size_t)std::max(n, 1) * sizeof(double);
+  k.vh = (double*)b;
+  k.wh = (double*)(b + v3);
+  k.Fc = (double*)(b + 2 * v3);
+  k.Tc = (double*)(b + 3 * v3);
+  b += up(4 * v3);
+  k.shareA = (double*)b;
+  b += up(kShare * (size_t)std::max(args->maxBlockContacts, 1) * sizeof(double));
+  k.shareB = (double*)b;
+  const fdm_adjoint_step_args a = *args;
+  const unsigned pb = std::max(1u, std::min(blocksFor(n, 256), 4096u));
+  if (a.prevBlock >= 0)
+    hipLaunchKernelGGL(kClearSpringBar, dim3(kStride), dim3(256), 0, stream, a.tape, a.prevBlock,
+                       a.springBar);
+  hipLaunchKernelGGL(kAdjKick, dim3(pb), dim3(256), 0, stream, a, k);
+  hipLaunchKernelGGL(kAdjContacts, dim3(kStride), dim3(256), 0, stream, a, k);
+  hipLaunchKernelGGL(kAdjGather, dim3(pb), dim3(256), 0, stream, a, k);
+  if (a.walls.n > 0)
+    hipLaunchKernelGGL(kAdjWalls, dim3(std::max(1u, blocksFor(a.walls.n, 64))), dim3(64), 0, stream,
+                       a, k);
+  return launchStatus();
+}

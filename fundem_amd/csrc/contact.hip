@@ -350,17 +350,24 @@ __device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const Narr
   return f;
 }
 
-// The reference's node predicate for one (node, target) candidate.
+// The reference's node predicate for one (node, target) candidate: the
+// node's cell is in the target's stencil, the node is within RB of a
+// particle target, inside the target's SDF box, and phi <= 0 there. The
+// conjunction is evaluated with the cheapest and most selective tests first
+// (the cell test divides); `tcell` is the target's decoded cell.
 template <int SET>
 __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const fdm_grid& g,
                                         double RB, Sample& s) {
-  if (!inStencil(tg.hash, clampedCell(p, g), g)) return false;
-  (void)tcell;
+  if (tg.hash < 0 || (long long)tg.hash >= g.num) return false;
   const V3 rel = vsub(p, tg.x);
   if (SET == FDM_SET_PARTICLE) {
     if (vlen2(rel) > RB * RB) return false;
   }
-  if (!sampleSDF(qrotateT(rel, tg.q), *tg.t, tg.sdf, s)) return false;
+  int base;
+  if (!sdfLocate(qrotateT(rel, tg.q), *tg.t, s, base)) return false;
+  const I3 c = clampedCell(p, g);
+  if (abs(tcell.x - c.x) > 1 || abs(tcell.y - c.y) > 1 || abs(tcell.z - c.z) > 1) return false;
+  sdfCorners(tg.sdf + base, *tg.t, s);
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
   return ov >= 0.;
 }

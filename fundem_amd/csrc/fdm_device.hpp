@@ -275,10 +275,10 @@ struct Sample {
   double x, y, z;  // fractional position in the SDF cell
   double c[8];     // corner values phi000, phi100, phi010, phi110, phi001, phi101, phi011, phi111
 };
-// Grid coordinates, SDF box test and corner fetch of the detection kernels
-// (LSParticleContactDetectionKernel.cu:72-105). Returns false outside the box.
-__device__ __forceinline__ bool sampleSDF(V3 local, const fdm_template& t, const float* sdf,
-                                          Sample& s) {
+// Grid coordinates and SDF box test of the detection kernels
+// (LSParticleContactDetectionKernel.cu:72-105): fractional position in s and
+// the offset of the cell's first corner in `base`. False outside the box.
+__device__ __forceinline__ bool sdfLocate(V3 local, const fdm_template& t, Sample& s, int& base) {
   const double ih = t.invSpacing;
   const double gx = (local.x - t.origin[0]) * ih;
   const double gy = (local.y - t.origin[1]) * ih;
@@ -289,8 +289,12 @@ __device__ __forceinline__ bool sampleSDF(V3 local, const fdm_template& t, const
   s.x = gx - (double)i0;
   s.y = gy - (double)j0;
   s.z = gz - (double)k0;
+  base = t.sdfStart + (k0 * t.size[0] * t.size[1] + j0 * t.size[0] + i0);
+  return true;
+}
+// The eight corner values of a located cell.
+__device__ __forceinline__ void sdfCorners(const float* b, const fdm_template& t, Sample& s) {
   const int nx = t.size[0], nxy = t.size[0] * t.size[1];
-  const float* b = sdf + t.sdfStart + (k0 * nxy + j0 * nx + i0);
   s.c[0] = b[0];
   s.c[1] = b[1];
   s.c[2] = b[nx];
@@ -299,6 +303,12 @@ __device__ __forceinline__ bool sampleSDF(V3 local, const fdm_template& t, const
   s.c[5] = b[nxy + 1];
   s.c[6] = b[nxy + nx];
   s.c[7] = b[nxy + nx + 1];
+}
+__device__ __forceinline__ bool sampleSDF(V3 local, const fdm_template& t, const float* sdf,
+                                          Sample& s) {
+  int base;
+  if (!sdfLocate(local, t, s, base)) return false;
+  sdfCorners(sdf + base, t, s);
   return true;
 }
 

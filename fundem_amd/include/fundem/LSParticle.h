@@ -91,6 +91,20 @@ struct LSParticle {
     m.dissipation = -logRes / (double)std::sqrt((float)(logRes * logRes + pi() * pi()));
     materials_.push_back(m);
   }
+  // A material given by its damping ratio zeta; e is the restitution of a
+  // linear dashpot with that ratio, exp(-pi zeta / sqrt(1 - zeta^2)), which
+  // only orders materials when a pair picks whose damping applies.
+  void addMaterialDampingRatio(double kn, double ks, double mu, double zeta) {
+    fdm_material m{};
+    m.kn = kn > 0. ? kn : 0.0;
+    m.ks = ks > 0. ? ks : 0.0;
+    m.mu = mu > 0. ? mu : 0.0;
+    m.dissipation = zeta > 0. ? zeta : 0.0;
+    m.e = m.dissipation <= 0. ? 1.0
+          : m.dissipation >= 1. ? 0.0
+                                : std::exp(-pi() * m.dissipation / std::sqrt(1.0 - m.dissipation * m.dissipation));
+    materials_.push_back(m);
+  }
 
   // LSParticle::addGeometry (LSParticle.h:242-290). SDF values and nodes are
   // stored FP32 on the device; the template constants stay FP64.

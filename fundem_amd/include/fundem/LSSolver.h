@@ -31,7 +31,11 @@ inline void fdmCheck(int r, const char* what) {
   if (r != FDM_OK) throw std::runtime_error(std::string(what) + " failed with code " + std::to_string(r));
 }
 
+class LSAdjoint;
+
 class LSSolver {
+  friend class LSAdjoint;
+
  public:
   explicit LSSolver(const std::string dir = "Problem", hipStream_t stream = 0,
                     const size_t maxGPUThread = 256, const int device = 0)
@@ -87,6 +91,16 @@ class LSSolver {
   }
   void addWallMaterial(double frictionCoefficient, double restitutionCoefficient) {
     walls_.addMaterial(0.0, 0.0, frictionCoefficient, restitutionCoefficient);
+  }
+  // Additive: materials given by their damping ratio zeta (the contact
+  // dashpots are 2 zeta sqrt(m k)) instead of a restitution coefficient.
+  void addParticleMaterialDampingRatio(double normalStiffness, double shearStiffness,
+                                       double frictionCoefficient, double dampingRatio) {
+    particles_.addMaterialDampingRatio(normalStiffness, shearStiffness, frictionCoefficient,
+                                       dampingRatio);
+  }
+  void addWallMaterialDampingRatio(double frictionCoefficient, double dampingRatio) {
+    walls_.addMaterialDampingRatio(0.0, 0.0, frictionCoefficient, dampingRatio);
   }
   void addWallGeometry(double3 gridNodeOrigin, double gridNodeSpacing, int3 gridNodeSize,
                        const std::vector<double>& gridNodeSignedDistance,

@@ -41,6 +41,15 @@ def solver_lib():
         "fdmx_destroy": (None, [P]),
         "fdmx_add_material": (I, [P, D, D, D, D]),
         "fdmx_add_wall_material": (I, [P, D, D]),
+        "fdmx_add_material_damping": (I, [P, D, D, D, D]),
+        "fdmx_add_wall_material_damping": (I, [P, D, D]),
+        "fdmx_num_materials": (I, [P, C.POINTER(I), C.POINTER(I)]),
+        "fdmx_adjoint_create": (P, [P]),
+        "fdmx_adjoint_destroy": (None, [P]),
+        "fdmx_adjoint_set_contacts": (I, [P, I]),
+        "fdmx_adjoint_forward": (I, [P, LL, LL]),
+        "fdmx_adjoint_backward": (I, [P, I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, _dp, _dp]),
         "fdmx_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
         "fdmx_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
         "fdmx_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I]),
@@ -119,6 +128,18 @@ class Solver:
 
     def add_wall_material(self, mu, e):
         self._ok(self._lib.fdmx_add_wall_material(self.h, mu, e))
+
+    def add_material_damping(self, kn, ks, mu, zeta):
+        """A particle material given by its damping ratio instead of restitution."""
+        self._ok(self._lib.fdmx_add_material_damping(self.h, kn, ks, mu, zeta))
+
+    def add_wall_material_damping(self, mu, zeta):
+        self._ok(self._lib.fdmx_add_wall_material_damping(self.h, mu, zeta))
+
+    def num_materials(self):
+        a, b = C.c_int(0), C.c_int(0)
+        self._ok(self._lib.fdmx_num_materials(self.h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def add_geometry(self, mass, inertia6, t):
         self._ok(self._lib.fdmx_add_geometry(
@@ -273,3 +294,48 @@ class Solver:
                                          calls.ctypes.data_as(C.POINTER(C.c_longlong)), 64)
         keys = names.value.decode().split("\n")[:cnt]
         return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(keys)}
+
+
+class Adjoint:
+    """Reverse-mode gradients of a scalar outcome of a solver's run
+    (fundem::LSAdjoint): forward() advances the solver keeping checkpoints,
+    backward() takes the adjoint of the final state (dJ/dx, dJ/dq, dJ/dv,
+    dJ/dw per particle) and returns the gradients with respect to the initial
+    state and the material parameters."""
+
+    def __init__(self, solver: Solver, contacts_per_particle: int = 16):
+        self._s = solver
+        self._lib = solver._lib
+        self.h = self._lib.fdmx_adjoint_create(solver.h)
+        if not self.h:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+        solver._ok(self._lib.fdmx_adjoint_set_contacts(self.h, int(contacts_per_particle)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fdmx_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def forward(self, steps: int, checkpoint_every: int):
+        self._s._ok(self._lib.fdmx_adjoint_forward(self.h, int(steps), int(checkpoint_every)))
+
+    def backward(self, x_bar=None, q_bar=None, v_bar=None, w_bar=None):
+        n = self._s.n
+        pm, wm = self._s.num_materials()
+
+        def arr(a, k):
+            return np.zeros(n * k) if a is None else _d(a)
+
+        xb, qb, vb, wb = arr(x_bar, 3), arr(q_bar, 4), arr(v_bar, 3), arr(w_bar, 3)
+        x0, v0, w0 = (np.zeros((n, 3)) for _ in range(3))
+        q0 = np.zeros((n, 4))
+        kn, ks, mu, zeta = (np.zeros(max(pm, 1)) for _ in range(4))
+        wmu, wzeta = np.zeros(max(wm, 1)), np.zeros(max(wm, 1))
+        self._s._ok(self._lib.fdmx_adjoint_backward(
+            self.h, n, _dp_(xb), _dp_(qb), _dp_(vb), _dp_(wb), _dp_(x0), _dp_(q0), _dp_(v0),
+            _dp_(w0), _dp_(kn), _dp_(ks), _dp_(mu), _dp_(zeta), _dp_(wmu), _dp_(wzeta)))
+        return dict(x=x0, q=q0, v=v0, w=w0, kn=kn[:pm], ks=ks[:pm], mu=mu[:pm], zeta=zeta[:pm],
+                    wall_mu=wmu[:wm], wall_zeta=wzeta[:wm])

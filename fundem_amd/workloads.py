@@ -49,6 +49,13 @@ class Scenario:
     cell_size: float = 0.0
     bonds: list = field(default_factory=list)  # (i, j, point, normal, R, L, E, nu, st, c, mub)
     steps: int = 1
+    # materials given as (kn, ks, mu, zeta) and wall materials as (mu, zeta)
+    damping_ratio: bool = False
+    # templates added with zero mass (fixed particles: invM = 0)
+    fixed_templates: tuple = ()
+    # explicit (mass, inertia6) per template instead of the density path
+    # (which also shifts each template by its centroid)
+    mass_properties: list = None
 
     @property
     def n(self) -> int:
@@ -58,12 +65,17 @@ class Scenario:
 def apply(s: Scenario, t) -> None:
     """Feed a scenario into a solver-like object."""
     for m in s.materials:
-        t.add_material(*m)
+        (t.add_material_damping if s.damping_ratio else t.add_material)(*m)
     dens = s.density if isinstance(s.density, (list, tuple)) else [s.density] * len(s.templates)
-    for d, tp in zip(dens, s.templates):
-        t.add_geometry_density(d, tp)
+    for g, (d, tp) in enumerate(zip(dens, s.templates)):
+        if g in s.fixed_templates:
+            t.add_geometry(0.0, np.zeros(6), tp)
+        elif s.mass_properties is not None:
+            t.add_geometry(s.mass_properties[g][0], s.mass_properties[g][1], tp)
+        else:
+            t.add_geometry_density(d, tp)
     for m in s.wall_materials:
-        t.add_wall_material(*m)
+        (t.add_wall_material_damping if s.damping_ratio else t.add_wall_material)(*m)
     for tp in s.wall_templates:
         t.add_wall_geometry(tp)
     t.add_particles(s.x, s.v, s.w, s.q, s.mat, s.geo)

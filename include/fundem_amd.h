@@ -547,6 +547,84 @@ int fdm_fillInt(int* ptr, size_t n, int value, hipStream_t stream);
 /* Device properties used by the launchers (number of CUs); 0 on success. */
 int fdm_deviceInfo(int* numCU, int* warpSize);
 
+/* ------------------------------------------------------------------ adjoint
+ * No counterpart in the reference: the reverse (adjoint) timestep behind
+ * fundem::LSAdjoint, for gradients of a scalar outcome of the final state
+ * with respect to the initial state and the contact parameters. Supported:
+ * particle-particle and particle-wall contacts (walls with prescribed
+ * motion); no periodic ghosts and no bonds. */
+
+/* Contacts of consecutive steps, block by block: block b holds a step's
+ * particle-pair contacts in canonical order (A, target, node), then its
+ * particle-wall contacts in the same order. */
+typedef struct {
+  int capacity;        /* contacts over all blocks */
+  int maxBlocks;
+  int* head;           /* [1] contacts used */
+  int* blockStart;     /* [maxBlocks] first contact of the block */
+  int* blockPP;        /* [maxBlocks] its particle-pair contacts */
+  int* blockCount;     /* [maxBlocks] all its contacts */
+  int* A;              /* [capacity] */
+  int* T;              /* [capacity] target particle or wall */
+  int* node;           /* [capacity] node of A */
+  float* spring;       /* [3 * capacity] tangential spring after the step, xyz per contact */
+  int* prev;           /* [capacity] the same (A, target, node) in the previous block, or -1 */
+  unsigned* key;       /* [capacity] per block, ascending: target (particle pairs), then
+                          nParticles + wall (wall pairs) */
+  int* order;          /* [capacity] per block, the contact (block-local) of each key */
+  unsigned* status;    /* FDM_STATUS_RECORD_OVERFLOW when a block does not fit */
+} fdm_contact_tape;
+
+/* Workspace of the tape routines for blocks of at most maxBlockContacts
+ * contacts and sets of the given pair capacities. */
+size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, int pairCapacityPP, int pairCapacityPW);
+/* Append the contacts of the latest step (the sets' current lists and
+ * results) as block `block`, the next unused one. prevBlock: the block the
+ * step's history came from, -1 for none. pw may be NULL (no walls). */
+int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, int nParticles, int nWalls,
+                         fdm_contact_tape tape, int block, int prevBlock, int maxBlockContacts,
+                         void* ws, size_t wsBytes, hipStream_t stream);
+/* Write block `block` into the sets' lists and results (pp, pw: the
+ * generations a following step reads as its history; their count arrays hold
+ * nParticles + 1 entries). The following step must rebuild its lists. */
+int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* pp,
+                          const fdm_pairs* pw, int nParticles, int maxBlockContacts, void* ws,
+                          size_t wsBytes, hipStream_t stream);
+
+/* One step in reverse. The state before the step (x, q, v, w and the F, T
+ * of the previous step) and after it (x, q, F, T) come from the tape; the
+ * adjoint arrays hold the adjoint of the state after the step on entry and
+ * of the state before it on return. */
+typedef struct {
+  fdm_bodies before;           /* particles at the start of the step */
+  fdm_bodies after;            /* particles at its end */
+  fdm_geometry pgeo;
+  const fdm_material* pmat;
+  fdm_bodies walls;            /* walls during the step (after their drift) */
+  fdm_geometry wgeo;
+  const fdm_material* wmat;
+  fdm_double3 gravity;
+  double dt;
+  fdm_contact_tape tape;
+  int block;                   /* the step's contacts */
+  int prevBlock;               /* the block its history came from (-1: none) */
+  int maxBlockContacts;
+  double* xBar;                /* [3n] planar */
+  double* qBar;                /* [4n] */
+  double* vBar;                /* [3n] */
+  double* wBar;                /* [3n] */
+  double* FBar;                /* [3n] adjoint of the force the step starts with */
+  double* TBar;                /* [3n] */
+  double* springBar;           /* [3 * tape.capacity] adjoint of each contact's spring: read for
+                                  `block`, written for prevBlock */
+  double* paramBar;            /* [4n] accumulated: each particle's share of the kn, ks, mu,
+                                  zeta adjoints, for its own material */
+  double* wallParamBar;        /* [2 * walls.n] accumulated mu, zeta adjoints per wall */
+} fdm_adjoint_step_args;
+size_t fdm_adjointWorkspaceBytes(int nParticles, int maxBlockContacts);
+int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
+                          hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

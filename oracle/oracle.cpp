@@ -211,8 +211,11 @@ struct Geoms {
   std::vector<V3> nodes;
   int num() const { return (int)radius.size(); }
 };
+// diss: the damping ratio of the contact dashpots, -ln e / sqrtf(ln^2 e + pi^2)
+// as contactKernel.cuh:50-51 evaluates it, or given directly (damping-ratio
+// materials); a pair uses the one of its material with the smaller e.
 struct Materials {
-  std::vector<double> kn, ks, mu, e;
+  std::vector<double> kn, ks, mu, e, diss;
 };
 
 // ------------------------------------------------ level-set sampling ----
@@ -281,11 +284,13 @@ static V3 integrateSlidingSpring(V3 prev, V3 vt, V3 n, V3 Fn, double mu, double 
   return spring;
 }
 // contactKernel.cuh:36-77 (the FP32 square roots are the reference's sqrtf).
+static double dissipationOf(double e) {
+  const double logRes = std::log(e);
+  return -logRes / (double)sqrtf((float)(logRes * logRes + kPi * kPi));
+}
 static void linearContact(V3& F, V3& spring, V3 v, V3 n, double delta, double dt, double kn,
-                          double ks, double mu, double res, double effM) {
+                          double ks, double mu, double diss, double effM) {
   if (delta > 0.) {
-    const double logRes = std::log(res);
-    const double diss = -logRes / (double)sqrtf((float)(logRes * logRes + kPi * kPi));
     const double cn = 2. * diss * (double)sqrtf((float)(effM * kn));
     const V3 vn = scl(n, dot(v, n));
     const V3 Fn = sub(scl(n, kn * delta), scl(vn, cn));
@@ -533,13 +538,13 @@ struct OracleSim {
       const double invMi = pgeo.invMass[P.geo[i]];
       const int mi = P.mat[i];
       V3 rj, vj, wj;
-      double kn = 0., ks = 0., effM = 0., res, mu;
+      double kn = 0., ks = 0., effM = 0., diss, mu;
       if (set == SET_PW) {
         rj = W.x[j]; vj = W.v[j]; wj = W.w[j];
         const int mj = W.mat[j];
         if (invMi > 0.) effM = 1. / invMi;
         mu = std::fmin(pmat.mu[mi], wmat.mu[mj]);
-        res = std::fmin(pmat.e[mi], wmat.e[mj]);
+        diss = pmat.e[mi] <= wmat.e[mj] ? pmat.diss[mi] : wmat.diss[mj];
         kn = pmat.kn[mi];
         ks = pmat.ks[mi];
       } else {
@@ -552,13 +557,13 @@ struct OracleSim {
         if (kni > 0. && knj > 0.) kn = kni * knj / (kni + knj);
         if (ksi > 0. && ksj > 0.) ks = ksi * ksj / (ksi + ksj);
         if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
-        res = std::fmin(pmat.e[mi], pmat.e[mj]);
+        diss = pmat.e[mi] <= pmat.e[mj] ? pmat.diss[mi] : pmat.diss[mj];
         mu = std::fmin(pmat.mu[mi], pmat.mu[mj]);
       }
       const V3 vc = sub(add(vi, cross(wi, sub(c.point, ri))), add(vj, cross(wj, sub(c.point, rj))));
       V3 F = {0., 0., 0.};
       V3 s = c.spring;
-      linearContact(F, s, vc, c.normal, c.overlap, dt, kn, ks, mu, res, effM);
+      linearContact(F, s, vc, c.normal, c.overlap, dt, kn, ks, mu, diss, effM);
       c.spring = r32(s, f32);
       c.F = F;
     }
@@ -741,13 +746,37 @@ int osim_add_material(OracleSim* s, double kn, double ks, double mu, double e) {
   s->pmat.ks.push_back(ks > 0. ? ks : 0.0);
   s->pmat.mu.push_back(mu > 0. ? mu : 0.0);
   s->pmat.e.push_back(e > 0. && e < 1.0 ? e : 1.0);
+  s->pmat.diss.push_back(dissipationOf(s->pmat.e.back()));
   return (int)s->pmat.kn.size() - 1;
+}
+// Additive: a material given by its damping ratio (e orders materials only).
+static double restitutionOfDamping(double zeta) {
+  if (zeta <= 0.) return 1.0;
+  if (zeta >= 1.) return 0.0;
+  return std::exp(-kPi * zeta / std::sqrt(1.0 - zeta * zeta));
+}
+int osim_add_material_damping(OracleSim* s, double kn, double ks, double mu, double zeta) {
+  s->pmat.kn.push_back(kn > 0. ? kn : 0.0);
+  s->pmat.ks.push_back(ks > 0. ? ks : 0.0);
+  s->pmat.mu.push_back(mu > 0. ? mu : 0.0);
+  s->pmat.diss.push_back(zeta > 0. ? zeta : 0.0);
+  s->pmat.e.push_back(restitutionOfDamping(s->pmat.diss.back()));
+  return (int)s->pmat.kn.size() - 1;
+}
+int osim_add_wall_material_damping(OracleSim* s, double mu, double zeta) {
+  s->wmat.kn.push_back(0.0);
+  s->wmat.ks.push_back(0.0);
+  s->wmat.mu.push_back(mu > 0. ? mu : 0.0);
+  s->wmat.diss.push_back(zeta > 0. ? zeta : 0.0);
+  s->wmat.e.push_back(restitutionOfDamping(s->wmat.diss.back()));
+  return (int)s->wmat.kn.size() - 1;
 }
 int osim_add_wall_material(OracleSim* s, double mu, double e) {
   s->wmat.kn.push_back(0.0);
   s->wmat.ks.push_back(0.0);
   s->wmat.mu.push_back(mu > 0. ? mu : 0.0);
   s->wmat.e.push_back(e > 0. && e < 1.0 ? e : 1.0);
+  s->wmat.diss.push_back(dissipationOf(s->wmat.e.back()));
   return (int)s->wmat.kn.size() - 1;
 }
 // geometryInfo::pushHost (LSParticle.h:74-117) after LSParticle::addGeometry's checks.

@@ -63,6 +63,11 @@ void osim_destroy(OracleSim* s);
 int osim_add_material(OracleSim* s, double kn, double ks, double mu, double e);
 /* LSSolver::addWallMaterial (LSSolver.h:175-183): kn = ks = 0. */
 int osim_add_wall_material(OracleSim* s, double mu, double e);
+/* Additive damping-ratio materials: the dashpots use zeta directly; e =
+ * exp(-pi zeta / sqrt(1 - zeta^2)) only decides which material of a pair
+ * applies (the one with the smaller e, as for restitution materials). */
+int osim_add_material_damping(OracleSim* s, double kn, double ks, double mu, double zeta);
+int osim_add_wall_material_damping(OracleSim* s, double mu, double zeta);
 /* LSParticle::addGeometry (LSParticle.h:242-290). sdf: nx*ny*nz (x fastest),
  * nodes: nnodes*3. Returns the geometry id or -1. */
 int osim_add_geometry(OracleSim* s, double mass, const double* inertia6, const double* origin3,

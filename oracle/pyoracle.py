@@ -40,6 +40,8 @@ def _load():
         "osim_destroy": (V, [P]),
         "osim_add_material": (I, [P, D, D, D, D]),
         "osim_add_wall_material": (I, [P, D, D]),
+        "osim_add_material_damping": (I, [P, D, D, D, D]),
+        "osim_add_wall_material_damping": (I, [P, D, D]),
         "osim_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
         "osim_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
         "osim_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I]),
@@ -146,6 +148,12 @@ class OracleSim:
 
     def add_wall_material(self, mu, e):
         return lib().osim_add_wall_material(self.h, mu, e)
+
+    def add_material_damping(self, kn, ks, mu, zeta):
+        return lib().osim_add_material_damping(self.h, kn, ks, mu, zeta)
+
+    def add_wall_material_damping(self, mu, zeta):
+        return lib().osim_add_wall_material_damping(self.h, mu, zeta)
 
     def add_geometry(self, mass, inertia6, t):
         return lib().osim_add_geometry(self.h, mass, dptr(d3(inertia6)), dptr(d3(t.origin)), t.spacing,
