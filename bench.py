@@ -45,7 +45,10 @@ def parse():
                    help="candidate-list skin in m (negative: a tenth of the largest radius)")
     p.add_argument("--copy-probe", action="store_true",
                    help="first run a plain 1 GiB device copy (counter scaling under rocprofv3)")
-    p.add_argument("--workload", choices=("pour", "batch"), default="pour",
+    p.add_argument("--settle", type=int, default=20000,
+                   help="column: plain steps that pack the column before the timed runs")
+    p.add_argument("--checkpoint", type=int, default=100, help="column: steps per checkpoint")
+    p.add_argument("--workload", choices=("pour", "batch", "column"), default="pour",
                    help="pour: the 1M-particle pour; batch: 512 packings in one solver against "
                         "plain runs of the same packings one after another")
     p.add_argument("--batch-sample", type=int, default=32,
@@ -204,8 +207,76 @@ def batch_main(a):
     print(json.dumps(out), flush=True)
 
 
+def column_main(a):
+    """The differentiated column collapse: one adjoint step (the segment's
+    recomputation, its contact tape and the reverse step) against one plain
+    forward step, both eager, from a column packed by a settling run."""
+    import dataclasses
+    import numpy as np
+    from fundem_amd import Adjoint, Solver
+    from fundem_amd import workloads as W
+    sc = W.column()
+    s0 = Solver(0)
+    W.apply(sc, s0)
+    t0 = time.perf_counter()
+    s0.step(a.settle)
+    st = s0.state()
+    log(f"column of {sc.n} bodies settled for {a.settle} steps in {time.perf_counter() - t0:.1f} s; "
+        f"contacts {[int(s0.contacts(k)['A'].size) for k in (0, 1)]}")
+    del s0
+    sc = dataclasses.replace(sc, x=st["x"], q=st["q"], v=st["v"], w=st["w"])
+    steps = a.steps
+    f = Solver(0)
+    W.apply(sc, f)
+    fwd = run_steps(f, steps, a.warmup, False) / steps
+    contacts = [int(f.contacts(k)["A"].size) for k in (0, 1)]
+    del f
+    g = Solver(0)
+    W.apply(sc, g)
+    adj = Adjoint(g, contacts_per_particle=16)
+    g.sync()
+    t0 = time.perf_counter()
+    adj.forward(steps, a.checkpoint)
+    g.sync()
+    t1 = time.perf_counter()
+    fin = g.state()
+    masses = []
+    for i in range(sc.n):
+        gi = int(sc.geo[i])
+        masses.append(0.0 if gi in sc.fixed_templates else 1.0)
+    vbar = np.asarray(masses)[:, None] * fin["v"]
+    t2 = time.perf_counter()
+    grad = adj.backward(None, None, vbar, None)
+    t3 = time.perf_counter()
+    fwd_ckpt = (t1 - t0) / steps
+    back = (t3 - t2) / steps
+    log(f"forward {1e3 * fwd:.3f} ms/step, forward with checkpoints {1e3 * fwd_ckpt:.3f} ms/step, "
+        f"adjoint {1e3 * back:.3f} ms/step (ratio {back / fwd:.2f}); contacts at the last step "
+        f"{contacts}; dJ/dkn {grad['kn'][0]:.4e} dJ/dzeta {grad['zeta'][0]:.4e} dJ/dmu {grad['mu'][0]:.4e}")
+    out = {
+        "metric": "adjoint step cost / forward step cost (column collapse, one GPU)",
+        "value": back / fwd, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * back, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"column of {sc.n - len([1 for m in masses if m == 0.0])} "
+                               "superellipsoids (8 templates, 32^3 SDF, 800 nodes) on a rough floor "
+                               f"({len([1 for m in masses if m == 0.0])} fixed bodies), packed by "
+                               f"{a.settle} settling steps, dt 1e-6 s, kn 1e4, zeta 0.3, mu 0.5; "
+                               f"checkpoint every {a.checkpoint} steps; J = sum of |v|^2 / 2 over the moving bodies",
+                   "forward_ms_per_step": 1e3 * fwd, "forward_with_checkpoints_ms_per_step":
+                   1e3 * fwd_ckpt, "contacts_last_step": contacts},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.workload == "column":
+        if a.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        column_main(a)
+        return
     if a.workload == "batch":
         if a.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)

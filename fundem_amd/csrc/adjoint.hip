@@ -8,9 +8,10 @@
 //                writing each contact's share for A and for its target and the
 //                adjoint of its previous spring into the previous block;
 //   kAdjGather   thread per particle: its shares in a fixed order (as A among
-//                particle pairs, as A among wall pairs, as target by the key
-//                order), then the reverse of the drift and the first half-kick;
-//   kAdjWalls    thread per wall: the wall-material parameter shares.
+//                particle pairs, as A among wall pairs, as target by ascending
+//                contact), then the reverse of the drift and the first half-kick;
+//                the wall-material shares of its wall contacts are kept per
+//                particle (at most two wall materials) and summed on the host.
 // Every sum runs in a fixed order, so the adjoint is deterministic.
 #include <algorithm>
 
@@ -23,6 +24,7 @@ namespace {
 
 constexpr int kStride = 1024;  // blocks of the grid-stride loops over device counts
 constexpr int kShare = 17;     // x 3, q 4, v 3, w 3, kn ks mu zeta
+constexpr int kBucket = 64;    // particle-pair contacts per target particle and step
 
 __device__ __forceinline__ adj::V toV(V3 a) { return {a.x, a.y, a.z}; }
 __device__ __forceinline__ adj::Q toQ(Q4 a) { return {a.q0, a.q1, a.q2, a.q3}; }
@@ -45,49 +47,30 @@ __device__ __forceinline__ void stQ(double* p, int n, int i, adj::Q q) {
 __device__ __forceinline__ adj::V f32(adj::V v) { return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z}; }
 
 // ------------------------------------------------------------ tape append
-__global__ void kTapeCount(fdm_pairs s, int* cnt) {
-  const int total = min(*s.total, s.capacity);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < s.capacity; p += gridDim.x * blockDim.x)
-    cnt[p] = p < total ? s.recCnt[p] : 0;
+// per-pair contact counts of both sets, particle pairs first
+__global__ void kTapeCount(fdm_pairs pp, fdm_pairs pw, int capPW, int* cnt) {
+  const int tp = min(*pp.total, pp.capacity);
+  const int tw = capPW > 0 ? min(*pw.total, pw.capacity) : 0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < pp.capacity + capPW; p += gridDim.x * blockDim.x) {
+    if (p < pp.capacity) cnt[p] = p < tp ? pp.recCnt[p] : 0;
+    else cnt[p] = p - pp.capacity < tw ? pw.recCnt[p - pp.capacity] : 0;
+  }
 }
 
-__global__ void kTapeBlock(fdm_contact_tape t, int block, const int* totals, int maxBlock) {
+__global__ void kTapeBlock(fdm_contact_tape t, int block, const int* total, const int* off, int capPP,
+                           int capPW, int maxBlock) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int pp = totals[0], pw = totals[1];
-  const int base = *t.head;
-  if (pp + pw > maxBlock || (long long)base + pp + pw > (long long)t.capacity) {
+  const int all = *total;
+  const int pp = capPW > 0 ? off[capPP] : all;
+  t.blockStart[block] = block * maxBlock;
+  if (all > maxBlock) {
     atomicOr(t.status, FDM_STATUS_RECORD_OVERFLOW);
-    t.blockStart[block] = base;
     t.blockPP[block] = 0;
     t.blockCount[block] = 0;
     return;
   }
-  t.blockStart[block] = base;
   t.blockPP[block] = pp;
-  t.blockCount[block] = pp + pw;
-  *t.head = base + pp + pw;
-}
-
-__global__ void kTapeScatter(fdm_pairs s, int wallPart, const int* off, const int* totals,
-                             fdm_contact_tape t, int block) {
-  if (t.blockCount[block] != totals[0] + totals[1]) return;  // block did not fit
-  const int total = min(*s.total, s.capacity);
-  const int base = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-    const int n = s.recCnt[p];
-    if (n <= 0) continue;
-    const int d0 = base + off[p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
-    for (int r = 0; r < n; ++r) {
-      const float4 rec = s.rec[r0 + r];
-      const int c = d0 + r;
-      t.A[c] = A;
-      t.T[c] = T;
-      t.node[c] = __float_as_int(rec.w);
-      t.spring[3 * c] = rec.x;
-      t.spring[3 * c + 1] = rec.y;
-      t.spring[3 * c + 2] = rec.z;
-    }
-  }
+  t.blockCount[block] = all;
 }
 
 __device__ __forceinline__ bool keyLess(const fdm_contact_tape& t, int c, int A, int T, int k) {
@@ -97,42 +80,46 @@ __device__ __forceinline__ bool keyLess(const fdm_contact_tape& t, int c, int A,
 }
 
 // the same (A, target, node) in the previous block's part of the same kind
-__global__ void kTapePrev(fdm_contact_tape t, int block, int prevBlock) {
-  const int start = t.blockStart[block], count = t.blockCount[block], pp = t.blockPP[block];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    const int c = start + i;
-    int found = -1;
-    if (prevBlock >= 0) {
-      const int ps = t.blockStart[prevBlock], ppp = t.blockPP[prevBlock];
-      int lo = i < pp ? ps : ps + ppp;
-      int hi = i < pp ? ps + ppp : ps + t.blockCount[prevBlock];
-      const int A = t.A[c], T = t.T[c], k = t.node[c];
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keyLess(t, mid, A, T, k)) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lo < (i < pp ? ps + ppp : ps + t.blockCount[prevBlock]) && t.A[lo] == A && t.T[lo] == T &&
-          t.node[lo] == k)
-        found = lo;
+__device__ __forceinline__ int prevOf(const fdm_contact_tape& t, int prevBlock, bool wall, int A, int T,
+                                      int k) {
+  if (prevBlock < 0) return -1;
+  const int ps = t.blockStart[prevBlock], ppp = t.blockPP[prevBlock];
+  int lo = wall ? ps + ppp : ps;
+  const int end = wall ? ps + t.blockCount[prevBlock] : ps + ppp;
+  int hi = end;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keyLess(t, mid, A, T, k)) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && t.A[lo] == A && t.T[lo] == T && t.node[lo] == k) ? lo : -1;
+}
+
+// contacts, their history links and their target keys (thread per pair)
+__global__ void kTapeScatter(fdm_pairs pp, fdm_pairs pw, int capPW, const int* off, const int* total,
+                             fdm_contact_tape t, int block, int prevBlock, int nParticles) {
+  if (t.blockCount[block] != *total) return;  // block did not fit
+  const int start = t.blockStart[block];
+  const int tp = min(*pp.total, pp.capacity);
+  const int tw = capPW > 0 ? min(*pw.total, pw.capacity) : 0;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < tp + tw; g += gridDim.x * blockDim.x) {
+    const bool wall = g >= tp;
+    const fdm_pairs& s = wall ? pw : pp;
+    const int p = wall ? g - tp : g;
+    const int n = s.recCnt[p];
+    if (n <= 0) continue;
+    const int i0 = off[wall ? pp.capacity + p : p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
+    for (int r = 0; r < n; ++r) {
+      const float4 rec = s.rec[r0 + r];
+      const int i = i0 + r, c = start + i, k = __float_as_int(rec.w);
+      t.A[c] = A;
+      t.T[c] = T;
+      t.node[c] = k;
+      t.spring[3 * (size_t)c] = rec.x;
+      t.spring[3 * (size_t)c + 1] = rec.y;
+      t.spring[3 * (size_t)c + 2] = rec.z;
+      t.prev[c] = prevOf(t, prevBlock, wall, A, T, k);
     }
-    t.prev[c] = found;
-  }
-}
-
-__global__ void kTapeKeys(fdm_contact_tape t, int block, int nParticles, unsigned* key, int* val) {
-  const int start = t.blockStart[block], count = t.blockCount[block], pp = t.blockPP[block];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    key[i] = (unsigned)t.T[start + i] + (i < pp ? 0u : (unsigned)nParticles);
-    val[i] = i;
-  }
-}
-
-__global__ void kTapeKeysStore(fdm_contact_tape t, int block, const unsigned* key, const int* val) {
-  const int start = t.blockStart[block], count = t.blockCount[block];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    t.key[start + i] = key[i];
-    t.order[start + i] = val[i];
   }
 }
 
@@ -194,6 +181,15 @@ struct AdjWork {
   double* Tc;      // [3n]
   double* shareA;  // [kShare * maxBlock]
   double* shareB;  // [kShare * maxBlock]
+  // half-open block-local ranges per particle, written by kAdjContacts at
+  // run boundaries (empty unless written): contacts with A = i among
+  // particle pairs and among wall pairs
+  int* runPP;      // [2n]
+  int* runPW;      // [2n]
+  // particle-pair contacts whose target is particle i (unordered)
+  int* bucketCount;  // [n]
+  int* bucket;       // [n * kBucket]
+  unsigned* status;
 };
 
 __device__ __forceinline__ void kickPrimal(const fdm_adjoint_step_args& a, const fdm_bodies& b,
@@ -213,6 +209,16 @@ __device__ __forceinline__ void kickPrimal(const fdm_adjoint_step_args& a, const
 __global__ void __launch_bounds__(256) kAdjKick(fdm_adjoint_step_args a, AdjWork k) {
   const int n = a.before.n;
   const double h = 0.5 * a.dt;
+  if (a.prevBlock >= 0) {  // the previous block's spring adjoints are written by kAdjContacts
+    const size_t s0 = 3 * (size_t)a.tape.blockStart[a.prevBlock];
+    const int m = 3 * a.tape.blockCount[a.prevBlock];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x)
+      a.springBar[s0 + j] = 0.;
+  }
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < 2 * n; j += gridDim.x * blockDim.x) {
+    k.runPP[j] = k.runPW[j] = 0;
+    if (j < n) k.bucketCount[j] = 0;
+  }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const fdm_template& t = a.pgeo.info[a.before.geo[i]];
     adj::V vh, wh;
@@ -253,6 +259,18 @@ __global__ void __launch_bounds__(256) kAdjContacts(fdm_adjoint_step_args a, Adj
     const int c = start + i;
     const bool wall = i >= npp;
     const int A = tp.A[c], T = tp.T[c], node = tp.node[c];
+    {
+      // run boundaries: by A within the part, and by key over the block
+      int* run = wall ? k.runPW : k.runPP;
+      const int lo = wall ? npp : 0, hi = wall ? count : npp;
+      if (i == lo || tp.A[c - 1] != A) run[2 * A] = i;
+      if (i == hi - 1 || tp.A[c + 1] != A) run[2 * A + 1] = i + 1;
+      if (!wall) {
+        const int pos = atomicAdd(k.bucketCount + T, 1);
+        if (pos < kBucket) k.bucket[(size_t)T * kBucket + pos] = i;
+        else atomicOr(k.status, FDM_STATUS_BUCKET_OVERFLOW);
+      }
+    }
     const fdm_template& tA = a.pgeo.info[a.after.geo[A]];
     const fdm_material& Mi = a.pmat[a.after.mat[A]];
     const adj::V xA = toV(ldp(a.after.x, n, A));
@@ -375,39 +393,52 @@ __device__ __forceinline__ void addShare(double* acc, const double* s) {
   for (int j = 0; j < kShare; ++j) acc[j] += s[j];
 }
 
-// first index in [lo, hi) whose tape A is >= v
-__device__ __forceinline__ int lowerA(const int* A, int lo, int hi, int v) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (A[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-__device__ __forceinline__ int lowerKey(const unsigned* K, int lo, int hi, unsigned v) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (K[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 __global__ void __launch_bounds__(256) kAdjGather(fdm_adjoint_step_args a, AdjWork k) {
   const fdm_contact_tape& tp = a.tape;
-  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
+  const int start = tp.blockStart[a.block];
   const int n = a.before.n;
   const double dt = a.dt, h = 0.5 * a.dt;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double s[kShare];
 #pragma unroll
     for (int j = 0; j < kShare; ++j) s[j] = 0.;
-    for (int c = lowerA(tp.A, start, start + npp, i); c < start + npp && tp.A[c] == i; ++c)
-      addShare(s, k.shareA + (size_t)kShare * (c - start));
-    for (int c = lowerA(tp.A, start + npp, start + count, i); c < start + count && tp.A[c] == i; ++c)
-      addShare(s, k.shareA + (size_t)kShare * (c - start));
-    for (int e = lowerKey(tp.key, start, start + npp, (unsigned)i); e < start + npp && tp.key[e] == (unsigned)i; ++e)
-      addShare(s, k.shareB + (size_t)kShare * tp.order[e]);
+    for (int c = k.runPP[2 * i]; c < k.runPP[2 * i + 1]; ++c) addShare(s, k.shareA + (size_t)kShare * c);
+    for (int c = k.runPW[2 * i]; c < k.runPW[2 * i + 1]; ++c) addShare(s, k.shareA + (size_t)kShare * c);
+    {
+      // as target: ascending contact order, selected from the bucket
+      const int m = min(k.bucketCount[i], kBucket);
+      const int* bk = k.bucket + (size_t)i * kBucket;
+      int last = -1;
+      for (int r = 0; r < m; ++r) {
+        int next = INT_MAX;
+        for (int u = 0; u < m; ++u) {
+          const int c = bk[u];
+          if (c > last && c < next) next = c;
+        }
+        last = next;
+        addShare(s, k.shareB + (size_t)kShare * next);
+      }
+    }
+    // wall-material shares of the wall contacts, per wall material
+    for (int c = k.runPW[2 * i]; c < k.runPW[2 * i + 1]; ++c) {
+      const double* sb = k.shareB + (size_t)kShare * c;
+      if (sb[15] == 0. && sb[16] == 0.) continue;
+      const int wm = a.walls.mat[tp.T[start + c]];
+      int slot = -1;
+      for (int u = 0; u < 2 && slot < 0; ++u) {
+        if (a.wallMat[2 * i + u] == wm) slot = u;
+        else if (a.wallMat[2 * i + u] < 0) {
+          a.wallMat[2 * i + u] = wm;
+          slot = u;
+        }
+      }
+      if (slot < 0) {
+        atomicOr(k.status, FDM_STATUS_BUCKET_OVERFLOW);
+        continue;
+      }
+      a.wallShare[4 * i + 2 * slot] += sb[15];
+      a.wallShare[4 * i + 2 * slot + 1] += sb[16];
+    }
     double* pbar = a.paramBar + 4 * (size_t)i;
     pbar[0] += s[13];
     pbar[1] += s[14];
@@ -444,62 +475,46 @@ __global__ void __launch_bounds__(256) kAdjGather(fdm_adjoint_step_args a, AdjWo
   }
 }
 
-__global__ void kAdjWalls(fdm_adjoint_step_args a, AdjWork k) {
-  const fdm_contact_tape& tp = a.tape;
-  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
-  const unsigned base = (unsigned)a.before.n;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < a.walls.n; w += gridDim.x * blockDim.x) {
-    double mu = 0., zeta = 0.;
-    for (int e = lowerKey(tp.key, start + npp, start + count, base + w);
-         e < start + count && tp.key[e] == base + (unsigned)w; ++e) {
-      const double* s = k.shareB + (size_t)kShare * tp.order[e];
-      mu += s[15];
-      zeta += s[16];
+// state copy of a set of bodies
+__global__ void kCopyBodies(fdm_bodies dst, fdm_bodies src) {
+  const int n = src.n;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < 4 * n; j += gridDim.x * blockDim.x) {
+    dst.q[j] = src.q[j];
+    if (j < 3 * n) {
+      dst.x[j] = src.x[j];
+      dst.v[j] = src.v[j];
+      dst.w[j] = src.w[j];
+      dst.F[j] = src.F[j];
+      dst.T[j] = src.T[j];
     }
-    a.wallParamBar[2 * w] += mu;
-    a.wallParamBar[2 * w + 1] += zeta;
   }
-}
-
-__global__ void kClearSpringBar(fdm_contact_tape t, int block, double* springBar) {
-  const int start = t.blockStart[block], count = t.blockCount[block];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * count; i += gridDim.x * blockDim.x)
-    springBar[3 * (size_t)start + i] = 0.;
 }
 
 // workspace carving
 size_t up(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct TapeWs {
-  int *cntPP, *cntPW, *offPP, *offPW, *totals, *flag, *idx, *val;
-  unsigned* key;
+  int *cnt, *off, *total, *flag, *idx;
   void* sub;
   size_t subBytes;
 };
-size_t tapeSubBytes(int maxBlock, int capPP, int capPW) {
-  const size_t m = std::max<size_t>({(size_t)maxBlock, (size_t)capPP, (size_t)capPW, 64});
-  return std::max(scanWorkspace(m), radixWorkspace(std::max(maxBlock, 64)));
-}
 size_t tapeBytes(int maxBlock, int capPP, int capPW, TapeWs* w, char* base) {
   const size_t m = std::max(maxBlock, 64);
+  const size_t pairs = std::max<size_t>((size_t)capPP + (size_t)std::max(capPW, 0), 64);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     char* p = base ? base + o : nullptr;
     o += up(bytes);
     return p;
   };
-  int* cntPP = (int*)take(sizeof(int) * std::max(capPP, 1));
-  int* cntPW = (int*)take(sizeof(int) * std::max(capPW, 1));
-  int* offPP = (int*)take(sizeof(int) * std::max(capPP, 1));
-  int* offPW = (int*)take(sizeof(int) * std::max(capPW, 1));
-  int* totals = (int*)take(sizeof(int) * 8);
+  int* cnt = (int*)take(sizeof(int) * pairs);
+  int* off = (int*)take(sizeof(int) * pairs);
+  int* total = (int*)take(sizeof(int) * 8);
   int* flag = (int*)take(sizeof(int) * m);
   int* idx = (int*)take(sizeof(int) * m);
-  int* val = (int*)take(sizeof(int) * m);
-  unsigned* key = (unsigned*)take(sizeof(unsigned) * m);
-  const size_t sb = tapeSubBytes(maxBlock, capPP, capPW);
+  const size_t sb = scanWorkspace(std::max(pairs, m));
   void* sub = take(sb);
-  if (w) *w = {cntPP, cntPW, offPP, offPW, totals, flag, idx, val, key, sub, sb};
+  if (w) *w = {cnt, off, total, flag, idx, sub, sb};
   return o;
 }
 
@@ -516,41 +531,22 @@ extern "C" int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, in
                                     fdm_contact_tape tape, int block, int prevBlock,
                                     int maxBlockContacts, void* ws, size_t wsBytes, hipStream_t stream) {
   if (!pp || block < 0 || block >= tape.maxBlocks || prevBlock >= block || nParticles < 0 ||
-      nWalls < 0 || maxBlockContacts < 1)
+      nWalls < 0 || maxBlockContacts < 1 ||
+      (long long)tape.maxBlocks * maxBlockContacts > (long long)tape.capacity)
     return FDM_EINVAL;
-  const int capPW = pw ? pw->capacity : 1;
+  const int capPW = pw ? pw->capacity : 0;
   TapeWs w;
   if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
-  tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws);
-  hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pp->capacity, 256), 4096u)), dim3(256), 0,
-                     stream, *pp, w.cntPP);
-  int r = scanInts(w.cntPP, w.offPP, pp->capacity, false, w.totals, w.sub, w.subBytes, stream);
+  const fdm_pairs pwv = pw ? *pw : *pp;
+  const long long pairs = (long long)pp->capacity + capPW;
+  hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pairs, 256), 4096u)), dim3(256), 0, stream,
+                     *pp, pwv, capPW, w.cnt);
+  int r = scanInts(w.cnt, w.off, (size_t)pairs, false, w.total, w.sub, w.subBytes, stream);
   if (r != FDM_OK) return r;
-  if (pw) {
-    hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pw->capacity, 256), 4096u)), dim3(256), 0,
-                       stream, *pw, w.cntPW);
-    r = scanInts(w.cntPW, w.offPW, pw->capacity, false, w.totals + 1, w.sub, w.subBytes, stream);
-    if (r != FDM_OK) return r;
-  } else {
-    r = fillInts(w.totals + 1, 1, 0, stream);
-    if (r != FDM_OK) return r;
-  }
-  hipLaunchKernelGGL(kTapeBlock, dim3(1), dim3(64), 0, stream, tape, block, (const int*)w.totals,
-                     maxBlockContacts);
-  hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pp, 0, (const int*)w.offPP,
-                     (const int*)w.totals, tape, block);
-  if (pw)
-    hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pw, 1,
-                       (const int*)w.offPW, (const int*)w.totals, tape, block);
-  hipLaunchKernelGGL(kTapePrev, dim3(kStride), dim3(256), 0, stream, tape, block, prevBlock);
-  hipLaunchKernelGGL(kTapeKeys, dim3(kStride), dim3(256), 0, stream, tape, block, nParticles, w.key,
-                     w.val);
-  const int bits = keyBitsFor((long long)nParticles + nWalls + 1);
-  r = radixSort(w.key, w.val, (size_t)maxBlockContacts, tape.blockCount + block, bits, w.sub,
-                w.subBytes, stream);
-  if (r != FDM_OK) return r;
-  hipLaunchKernelGGL(kTapeKeysStore, dim3(kStride), dim3(256), 0, stream, tape, block,
-                     (const unsigned*)w.key, (const int*)w.val);
+  hipLaunchKernelGGL(kTapeBlock, dim3(1), dim3(64), 0, stream, tape, block, (const int*)w.total,
+                     (const int*)w.off, pp->capacity, capPW, maxBlockContacts);
+  hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pp, pwv, capPW,
+                     (const int*)w.off, (const int*)w.total, tape, block, prevBlock, nParticles);
   return launchStatus();
 }
 
@@ -559,29 +555,29 @@ extern "C" int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm
                                      void* ws, size_t wsBytes, hipStream_t stream) {
   if (!pp || block < 0 || block >= tape.maxBlocks || nParticles < 0 || maxBlockContacts < 1)
     return FDM_EINVAL;
-  const int capPW = pw ? pw->capacity : 1;
+  const int capPW = pw ? pw->capacity : 0;
   TapeWs w;
   if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
-  tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws);
   for (int part = 0; part < 2; ++part) {
     const fdm_pairs* s = part == 0 ? pp : pw;
     if (!s) continue;
     hipLaunchKernelGGL(kRestoreFlags, dim3(kStride), dim3(256), 0, stream, tape, block, part, w.flag,
                        maxBlockContacts);
-    int r = scanInts(w.flag, w.idx, (size_t)maxBlockContacts, false, w.totals + 2 + part, w.sub,
+    int r = scanInts(w.flag, w.idx, (size_t)maxBlockContacts, false, w.total + 2 + part, w.sub,
                      w.subBytes, stream);
     if (r != FDM_OK) return r;
     hipLaunchKernelGGL(kRestoreScatter, dim3(kStride), dim3(256), 0, stream, tape, block, part,
-                       (const int*)w.flag, (const int*)w.idx, (const int*)(w.totals + 2 + part), *s);
+                       (const int*)w.flag, (const int*)w.idx, (const int*)(w.total + 2 + part), *s);
     hipLaunchKernelGGL(kRestoreCount, dim3(std::min(blocksFor((long long)nParticles + 1, 256), 4096u)),
-                       dim3(256), 0, stream, *s, nParticles, (const int*)(w.totals + 2 + part));
+                       dim3(256), 0, stream, *s, nParticles, (const int*)(w.total + 2 + part));
   }
   return launchStatus();
 }
 
 extern "C" size_t fdm_adjointWorkspaceBytes(int nParticles, int maxBlockContacts) {
   const size_t n = std::max(nParticles, 1), m = std::max(maxBlockContacts, 1);
-  return up(4 * 3 * n * sizeof(double)) + 2 * up(kShare * m * sizeof(double));
+  return up(4 * 3 * n * sizeof(double)) + 2 * up(kShare * m * sizeof(double)) +
+         2 * up(2 * n * sizeof(int)) + up(n * sizeof(int)) + up(n * kBucket * sizeof(int));
 }
 
 extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
@@ -589,7 +585,7 @@ extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws
   if (!args || args->block < 1 || args->block >= args->tape.maxBlocks ||
       args->prevBlock >= args->block || args->before.n != args->after.n || !args->xBar ||
       !args->qBar || !args->vBar || !args->wBar || !args->FBar || !args->TBar ||
-      !args->springBar || !args->paramBar || (args->walls.n > 0 && !args->wallParamBar))
+      !args->springBar || !args->paramBar || !args->wallShare || !args->wallMat || !args->status)
     return FDM_EINVAL;
   const int n = args->before.n;
   if (wsBytes < fdm_adjointWorkspaceBytes(n, args->maxBlockContacts)) return FDM_EINVAL;
@@ -604,16 +600,27 @@ extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws
   k.shareA = (double*)b;
   b += up(kShare * (size_t)std::max(args->maxBlockContacts, 1) * sizeof(double));
   k.shareB = (double*)b;
+  b += up(kShare * (size_t)std::max(args->maxBlockContacts, 1) * sizeof(double));
+  const size_t r2 = up(2 * (size_t)std::max(n, 1) * sizeof(int));
+  k.runPP = (int*)b;
+  k.runPW = (int*)(b + r2);
+  b += 2 * r2;
+  k.bucketCount = (int*)b;
+  b += up((size_t)std::max(n, 1) * sizeof(int));
+  k.bucket = (int*)b;
+  k.status = args->status;
   const fdm_adjoint_step_args a = *args;
   const unsigned pb = std::max(1u, std::min(blocksFor(n, 256), 4096u));
-  if (a.prevBlock >= 0)
-    hipLaunchKernelGGL(kClearSpringBar, dim3(kStride), dim3(256), 0, stream, a.tape, a.prevBlock,
-                       a.springBar);
-  hipLaunchKernelGGL(kAdjKick, dim3(pb), dim3(256), 0, stream, a, k);
+  hipLaunchKernelGGL(kAdjKick, dim3(std::max(pb, 64u)), dim3(256), 0, stream, a, k);
   hipLaunchKernelGGL(kAdjContacts, dim3(kStride), dim3(256), 0, stream, a, k);
   hipLaunchKernelGGL(kAdjGather, dim3(pb), dim3(256), 0, stream, a, k);
-  if (a.walls.n > 0)
-    hipLaunchKernelGGL(kAdjWalls, dim3(std::max(1u, blocksFor(a.walls.n, 64))), dim3(64), 0, stream,
-                       a, k);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchCopyBodies(fdm_bodies dst, fdm_bodies src, hipStream_t stream) {
+  if (src.n != dst.n || src.n < 0) return FDM_EINVAL;
+  if (src.n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kCopyBodies, dim3(std::min(blocksFor(4ll * src.n, 256), 4096u)), dim3(256), 0,
+                     stream, dst, src);
   return launchStatus();
 }

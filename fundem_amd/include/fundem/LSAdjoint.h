@@ -99,14 +99,14 @@ class LSAdjoint {
     zero(FBar_, st);
     zero(TBar_, st);
     zero(paramBar_, st);
-    zero(wallParamBar_, st);
+    zero(wallShare_, st);
+    hipCheck(hipMemsetAsync(wallMat_.get(), 0xff, wallMat_.size() * sizeof(int), st), "clear");
     std::vector<double> boundary;  // spring adjoints of the first block of the later segment
     for (size_t seg = checkpoints_.size(); seg-- > 0;) {
       const size_t n0 = seg * every_, n1 = std::min(n0 + every_, steps_);
       const int len = (int)(n1 - n0);
       restore(checkpoints_[seg]);
       // record the segment
-      zeroInt(tape_.head, st);
       copyState(slots_[0], true);
       appendBlock(0, -1);
       for (int j = 1; j <= len; ++j) {
@@ -139,6 +139,9 @@ class LSAdjoint {
       }
     }
     restore(final_);
+    if (s_.status() & FDM_STATUS_BUCKET_OVERFLOW)
+      throw std::runtime_error("more than 64 contacts on one target in a step, or contacts with "
+                               "walls of more than two materials on one particle");
     // results
     AdjointGradients g;
     std::vector<double> x = xBar_.toHost(st, 3 * N), v = vBar_.toHost(st, 3 * N),
@@ -171,12 +174,15 @@ class LSAdjoint {
     g.wallMu.assign(WM, 0.0);
     g.wallDampingRatio.assign(WM, 0.0);
     if (W > 0) {
-      std::vector<double> wp = wallParamBar_.toHost(st, 2 * W);
-      const std::vector<int>& wmat = s_.walls_.materialIDHostRef();
-      for (size_t k = 0; k < W; ++k) {
-        g.wallMu[wmat[k]] += wp[2 * k];
-        g.wallDampingRatio[wmat[k]] += wp[2 * k + 1];
-      }
+      const std::vector<double> ws = wallShare_.toHost(st, 4 * N);
+      const std::vector<int> wm = wallMat_.toHost(st, 2 * N);
+      for (size_t i = 0; i < N; ++i)
+        for (int u = 0; u < 2; ++u) {
+          const int m = wm[2 * i + u];
+          if (m < 0) continue;
+          g.wallMu[m] += ws[4 * i + 2 * u];
+          g.wallDampingRatio[m] += ws[4 * i + 2 * u + 1];
+        }
     }
     return g;
   }
@@ -187,17 +193,15 @@ class LSAdjoint {
  private:
   struct StateBuf {
     DeviceBuffer<double> x, wx;
-    DeviceBuffer<float> q, v, w, F, T, wq;
+    DeviceBuffer<float> q, v, w, F, T, wq, wv, ww, wF, wT;
   };
   struct TapeBuf {
     int capacity = 0, maxBlocks = 0;
-    DeviceBuffer<int> head, blockStart, blockPP, blockCount, A, T, node, prev, order;
+    DeviceBuffer<int> blockStart, blockPP, blockCount, A, T, node, prev;
     DeviceBuffer<float> spring;
-    DeviceBuffer<unsigned> key;
     void allocate(int cap, int blocks) {
       capacity = std::max(cap, 1);
       maxBlocks = std::max(blocks, 1);
-      head.allocate(1);
       blockStart.allocate(maxBlocks);
       blockPP.allocate(maxBlocks);
       blockCount.allocate(maxBlocks);
@@ -205,15 +209,12 @@ class LSAdjoint {
       T.allocate(capacity, false);
       node.allocate(capacity, false);
       prev.allocate(capacity, false);
-      order.allocate(capacity, false);
-      key.allocate(capacity, false);
       spring.allocate(3 * (size_t)capacity, false);
     }
     fdm_contact_tape view(unsigned* status) {
       fdm_contact_tape t;
       t.capacity = capacity;
       t.maxBlocks = maxBlocks;
-      t.head = head.get();
       t.blockStart = blockStart.get();
       t.blockPP = blockPP.get();
       t.blockCount = blockCount.get();
@@ -222,8 +223,6 @@ class LSAdjoint {
       t.node = node.get();
       t.spring = spring.get();
       t.prev = prev.get();
-      t.key = key.get();
-      t.order = order.get();
       t.status = status;
       return t;
     }
@@ -271,7 +270,8 @@ class LSAdjoint {
     FBar_.allocate(3 * N_);
     TBar_.allocate(3 * N_);
     paramBar_.allocate(4 * N_);
-    wallParamBar_.allocate(std::max<size_t>(2 * W_, 1));
+    wallShare_.allocate(4 * N_);
+    wallMat_.allocate(2 * N_);
     springBar_.allocate(3 * (size_t)tape_.capacity);
   }
   void allocState(StateBuf& b) {
@@ -283,29 +283,25 @@ class LSAdjoint {
     b.T.allocate(3 * N_, false);
     b.wx.allocate(std::max<size_t>(3 * W_, 1), false);
     b.wq.allocate(std::max<size_t>(4 * W_, 1), false);
+    b.wv.allocate(std::max<size_t>(3 * W_, 1), false);
+    b.ww.allocate(std::max<size_t>(3 * W_, 1), false);
+    b.wF.allocate(std::max<size_t>(3 * W_, 1), false);
+    b.wT.allocate(std::max<size_t>(3 * W_, 1), false);
   }
   // solver state to a slot (out) or back (!out)
   void copyState(StateBuf& b, bool out) {
     hipStream_t st = s_.stream_;
-    fdm_bodies P = s_.particles_.bodies(), Wb = s_.walls_.bodies();
-    if (out) {
-      copy(b.x.get(), P.x, 3 * N_, st);
-      copy(b.q.get(), P.q, 4 * N_, st);
-      copy(b.v.get(), P.v, 3 * N_, st);
-      copy(b.w.get(), P.w, 3 * N_, st);
-      copy(b.F.get(), P.F, 3 * N_, st);
-      copy(b.T.get(), P.T, 3 * N_, st);
-      copy(b.wx.get(), Wb.x, 3 * W_, st);
-      copy(b.wq.get(), Wb.q, 4 * W_, st);
-    } else {
-      copy(P.x, b.x.get(), 3 * N_, st);
-      copy(P.q, b.q.get(), 4 * N_, st);
-      copy(P.v, b.v.get(), 3 * N_, st);
-      copy(P.w, b.w.get(), 3 * N_, st);
-      copy(P.F, b.F.get(), 3 * N_, st);
-      copy(P.T, b.T.get(), 3 * N_, st);
-      copy(Wb.x, b.wx.get(), 3 * W_, st);
-      copy(Wb.q, b.wq.get(), 4 * W_, st);
+    const fdm_bodies P = s_.particles_.bodies(), S = slotView(b);
+    fdmCheck(out ? fdm_launchCopyBodies(S, P, st) : fdm_launchCopyBodies(P, S, st), "state copy");
+    if (W_ > 0) {
+      fdm_bodies Wb = s_.walls_.bodies(), Ws = Wb;
+      Ws.x = b.wx.get();
+      Ws.q = b.wq.get();
+      Ws.v = b.wv.get();
+      Ws.w = b.ww.get();
+      Ws.F = b.wF.get();
+      Ws.T = b.wT.get();
+      fdmCheck(out ? fdm_launchCopyBodies(Ws, Wb, st) : fdm_launchCopyBodies(Wb, Ws, st), "state copy");
     }
   }
   fdm_bodies slotView(StateBuf& b) {
@@ -337,7 +333,6 @@ class LSAdjoint {
     hipStream_t st = s_.stream_;
     allocState(c.state);
     copyState(c.state, true);
-    zeroInt(staging_.head, st);
     appendTo(staging_, 0, -1);
     checkTape();
     c.pp = staging_.blockPP.toHost(st, 1)[0];
@@ -401,7 +396,9 @@ class LSAdjoint {
     a.TBar = TBar_.get();
     a.springBar = springBar_.get();
     a.paramBar = paramBar_.get();
-    a.wallParamBar = wallParamBar_.get();
+    a.wallMat = wallMat_.get();
+    a.wallShare = wallShare_.get();
+    a.status = s_.status_.get();
     fdmCheck(fdm_launchAdjointStep(&a, adjWs_.get(), adjWs_.size(), s_.stream_), "adjoint step");
   }
 
@@ -414,7 +411,8 @@ class LSAdjoint {
   std::vector<StateBuf> slots_;
   TapeBuf tape_, staging_;
   DeviceBuffer<char> tapeWs_, adjWs_;
-  DeviceBuffer<double> xBar_, qBar_, vBar_, wBar_, FBar_, TBar_, paramBar_, wallParamBar_, springBar_;
+  DeviceBuffer<double> xBar_, qBar_, vBar_, wBar_, FBar_, TBar_, paramBar_, wallShare_, springBar_;
+  DeviceBuffer<int> wallMat_;
 };
 
 }  // namespace fundem

@@ -294,6 +294,54 @@ def batch_packings(count: int = 512, seed: int = 5, n_range=(500, 20000), templa
     return out
 
 
+def column(counts=(8, 8, 64), seed: int = 4, n_templates: int = 8, n_nodes: int = 800,
+           kn: float = 1e4, zeta: float = 0.3, mu: float = 0.5, dt: float = 1e-6,
+           v_sd: float = 0.01, floor_side: int = 16, steps: int = 2000) -> Scenario:
+    """The differentiated column collapse: counts[0] x counts[1] x counts[2]
+    superellipsoids (8 templates with semi-axes U[0.8, 1.2] mm and powers
+    U[2, 8], 32^3 SDF, 800 nodes) on a jittered lattice (spacing 1.05
+    bounding diameters, no initial overlap) released above a rough floor: a
+    floor wall carrying a floor_side^2 layer of fixed particles (the same
+    shapes with zero mass, so invM = 0). Materials by damping ratio: kn,
+    ks = 0.4 kn, mu, zeta for particles and the wall; initial velocities
+    N(0, v_sd) per component; dt = 1e-6 s."""
+    rng = np.random.default_rng(seed)
+    tmpl = superellipsoid_family(n_templates, seed + 1, n_nodes=n_nodes)
+    rmax = max(gen.template_radius(t) for t in tmpl)
+    sp = 2.0 * rmax * 1.05
+    nx, ny, nz = counts
+    F = floor_side
+    L = F * sp
+    x0 = 0.5 * (L - nx * sp)
+    y0 = 0.5 * (L - ny * sp)
+    zf = rmax  # centres of the fixed layer
+    cols = gen.jittered_lattice((nx, ny, nz), sp, (x0 + 0.5 * sp, y0 + 0.5 * sp, zf + sp * 1.05),
+                                0.01 * sp, rng)
+    n = len(cols)
+    fx = np.array([[(i + 0.5) * sp, (j + 0.5) * sp, zf] for j in range(F) for i in range(F)])
+    x = np.vstack([cols, fx])
+    q = gen.shoemake_quaternions(rng, len(x))
+    geo = np.concatenate([rng.integers(0, n_templates, size=n),
+                          n_templates + rng.integers(0, n_templates, size=len(fx))]).astype(np.int32)
+    for i in range(n, len(x)):  # fixed particles rest on the floor
+        R = _rotmat(q[i])
+        x[i, 2] = -float(np.min((tmpl[geo[i] - n_templates].nodes.astype(np.float64) @ R.T)[:, 2]))
+    v = np.zeros((len(x), 3))
+    v[:n] = rng.normal(0.0, v_sd, size=(n, 3))
+    H = cols[:, 2].max() + 2 * sp
+    dmin = np.array([0.0, 0.0, -sp])
+    dmax = np.array([L, L, H])
+    wt, wall = floor_wall([0.0, 0.0], [L, L])
+    return Scenario(
+        name=f"column-{n}", templates=tmpl + tmpl, density=2500.0,
+        materials=[(kn, 0.4 * kn, mu, zeta)],
+        x=x, v=v, w=np.zeros((len(x), 3)), q=q, mat=np.zeros(len(x), np.int32), geo=geo,
+        domain_min=dmin, domain_max=dmax, gravity=np.array([0.0, 0.0, -9.81]), dt=dt,
+        wall_templates=[wt], wall_materials=[(mu, zeta)], walls=[wall],
+        damping_ratio=True, fixed_templates=tuple(range(n_templates, 2 * n_templates)),
+        cell_size=0.0, steps=steps)
+
+
 def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_templates: int = 3,
             periodic=(False, False), walls: bool = True, scale: float = 1e-3,
             spacing: float = 3.0, speed: float = 0.5, spin: float = 20.0, dt: float = 1e-6,

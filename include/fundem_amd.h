@@ -556,12 +556,12 @@ int fdm_deviceInfo(int* numCU, int* warpSize);
 
 /* Contacts of consecutive steps, block by block: block b holds a step's
  * particle-pair contacts in canonical order (A, target, node), then its
- * particle-wall contacts in the same order. */
+ * particle-wall contacts in the same order, in the slot
+ * [b * maxBlockContacts, (b + 1) * maxBlockContacts) of every array. */
 typedef struct {
-  int capacity;        /* contacts over all blocks */
+  int capacity;        /* contacts over all slots: at least maxBlocks * maxBlockContacts */
   int maxBlocks;
-  int* head;           /* [1] contacts used */
-  int* blockStart;     /* [maxBlocks] first contact of the block */
+  int* blockStart;     /* [maxBlocks] first contact of the block (b * maxBlockContacts) */
   int* blockPP;        /* [maxBlocks] its particle-pair contacts */
   int* blockCount;     /* [maxBlocks] all its contacts */
   int* A;              /* [capacity] */
@@ -569,18 +569,15 @@ typedef struct {
   int* node;           /* [capacity] node of A */
   float* spring;       /* [3 * capacity] tangential spring after the step, xyz per contact */
   int* prev;           /* [capacity] the same (A, target, node) in the previous block, or -1 */
-  unsigned* key;       /* [capacity] per block, ascending: target (particle pairs), then
-                          nParticles + wall (wall pairs) */
-  int* order;          /* [capacity] per block, the contact (block-local) of each key */
   unsigned* status;    /* FDM_STATUS_RECORD_OVERFLOW when a block does not fit */
 } fdm_contact_tape;
 
 /* Workspace of the tape routines for blocks of at most maxBlockContacts
  * contacts and sets of the given pair capacities. */
 size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, int pairCapacityPP, int pairCapacityPW);
-/* Append the contacts of the latest step (the sets' current lists and
- * results) as block `block`, the next unused one. prevBlock: the block the
- * step's history came from, -1 for none. pw may be NULL (no walls). */
+/* Write the contacts of the latest step (the sets' current lists and
+ * results) as block `block`. prevBlock: the block the step's history came
+ * from, -1 for none. pw may be NULL (no walls). */
 int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, int nParticles, int nWalls,
                          fdm_contact_tape tape, int block, int prevBlock, int maxBlockContacts,
                          void* ws, size_t wsBytes, hipStream_t stream);
@@ -619,9 +616,15 @@ typedef struct {
                                   `block`, written for prevBlock */
   double* paramBar;            /* [4n] accumulated: each particle's share of the kn, ks, mu,
                                   zeta adjoints, for its own material */
-  double* wallParamBar;        /* [2 * walls.n] accumulated mu, zeta adjoints per wall */
+  int* wallMat;                /* [2n] the wall materials each particle's wall contacts used
+                                  (at most two; -1 before the first) */
+  double* wallShare;           /* [4n] accumulated mu, zeta adjoints of those two materials */
+  unsigned* status;            /* FDM_STATUS_BUCKET_OVERFLOW: more than 64 particle-pair contacts
+                                  on one target in a step, or a third wall material */
 } fdm_adjoint_step_args;
 size_t fdm_adjointWorkspaceBytes(int nParticles, int maxBlockContacts);
+/* Copy the state arrays (x, q, v, w, F, T) of n bodies from src to dst. */
+int fdm_launchCopyBodies(fdm_bodies dst, fdm_bodies src, hipStream_t stream);
 int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
                           hipStream_t stream);
 
