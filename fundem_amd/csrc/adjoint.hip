@@ -25,6 +25,7 @@ namespace {
 constexpr int kStride = 1024;  // blocks of the grid-stride loops over device counts
 constexpr int kShare = 17;     // x 3, q 4, v 3, w 3, kn ks mu zeta
 constexpr int kBucket = 64;    // particle-pair contacts per target particle and step
+constexpr int kParts = 5;      // sets in tape order: particle, wall, ghost X, Y, XY
 
 __device__ __forceinline__ adj::V toV(V3 a) { return {a.x, a.y, a.z}; }
 __device__ __forceinline__ adj::Q toQ(Q4 a) { return {a.q0, a.q1, a.q2, a.q3}; }
@@ -47,29 +48,44 @@ __device__ __forceinline__ void stQ(double* p, int n, int i, adj::Q q) {
 __device__ __forceinline__ adj::V f32(adj::V v) { return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z}; }
 
 // ------------------------------------------------------------ tape append
-// per-pair contact counts of both sets, particle pairs first
-__global__ void kTapeCount(fdm_pairs pp, fdm_pairs pw, int capPW, int* cnt) {
-  const int tp = min(*pp.total, pp.capacity);
-  const int tw = capPW > 0 ? min(*pw.total, pw.capacity) : 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < pp.capacity + capPW; p += gridDim.x * blockDim.x) {
-    if (p < pp.capacity) cnt[p] = p < tp ? pp.recCnt[p] : 0;
-    else cnt[p] = p - pp.capacity < tw ? pw.recCnt[p - pp.capacity] : 0;
+// The sets of a step in tape order: particle, wall, ghost X, Y, XY. An
+// absent set has capacity 0.
+struct TapeSets {
+  fdm_pairs s[kParts];
+  int capOff[kParts + 1];  // prefix of the pair capacities
+};
+
+__device__ __forceinline__ int setTotal(const TapeSets& S, int p) {
+  return S.capOff[p + 1] > S.capOff[p] ? min(*S.s[p].total, S.s[p].capacity) : 0;
+}
+
+// per-pair contact counts of all sets, set after set
+__global__ void kTapeCount(TapeSets S, int* cnt) {
+  int tot[kParts];
+#pragma unroll
+  for (int p = 0; p < kParts; ++p) tot[p] = setTotal(S, p);
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < S.capOff[kParts]; g += gridDim.x * blockDim.x) {
+    int p = 0;
+    while (g >= S.capOff[p + 1]) ++p;
+    const int q = g - S.capOff[p];
+    cnt[g] = q < tot[p] ? S.s[p].recCnt[q] : 0;
   }
 }
 
-__global__ void kTapeBlock(fdm_contact_tape t, int block, const int* total, const int* off, int capPP,
-                           int capPW, int maxBlock) {
+__global__ void kTapeBlock(fdm_contact_tape t, int block, TapeSets S, const int* total, const int* off,
+                           int maxBlock) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int all = *total;
-  const int pp = capPW > 0 ? off[capPP] : all;
+  int* part = t.blockPart + (size_t)(kParts + 1) * block;
   t.blockStart[block] = block * maxBlock;
   if (all > maxBlock) {
     atomicOr(t.status, FDM_STATUS_RECORD_OVERFLOW);
-    t.blockPP[block] = 0;
+    for (int p = 0; p <= kParts; ++p) part[p] = 0;
     t.blockCount[block] = 0;
     return;
   }
-  t.blockPP[block] = pp;
+  for (int p = 0; p < kParts; ++p) part[p] = S.capOff[p] < S.capOff[kParts] ? off[S.capOff[p]] : all;
+  part[kParts] = all;
   t.blockCount[block] = all;
 }
 
@@ -79,13 +95,14 @@ __device__ __forceinline__ bool keyLess(const fdm_contact_tape& t, int c, int A,
   return t.node[c] < k;
 }
 
-// the same (A, target, node) in the previous block's part of the same kind
-__device__ __forceinline__ int prevOf(const fdm_contact_tape& t, int prevBlock, bool wall, int A, int T,
+// the same (A, target, node) in the previous block's part of the same set
+__device__ __forceinline__ int prevOf(const fdm_contact_tape& t, int prevBlock, int set, int A, int T,
                                       int k) {
   if (prevBlock < 0) return -1;
-  const int ps = t.blockStart[prevBlock], ppp = t.blockPP[prevBlock];
-  int lo = wall ? ps + ppp : ps;
-  const int end = wall ? ps + t.blockCount[prevBlock] : ps + ppp;
+  const int ps = t.blockStart[prevBlock];
+  const int* part = t.blockPart + (size_t)(kParts + 1) * prevBlock;
+  int lo = ps + part[set];
+  const int end = ps + part[set + 1];
   int hi = end;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -95,49 +112,55 @@ __device__ __forceinline__ int prevOf(const fdm_contact_tape& t, int prevBlock, 
   return (lo < end && t.A[lo] == A && t.T[lo] == T && t.node[lo] == k) ? lo : -1;
 }
 
-// contacts, their history links and their target keys (thread per pair)
-__global__ void kTapeScatter(fdm_pairs pp, fdm_pairs pw, int capPW, const int* off, const int* total,
-                             fdm_contact_tape t, int block, int prevBlock, int nParticles) {
+// contacts and their history links (thread per pair)
+__global__ void kTapeScatter(TapeSets S, const int* off, const int* total, fdm_contact_tape t, int block,
+                             int prevBlock) {
   if (t.blockCount[block] != *total) return;  // block did not fit
   const int start = t.blockStart[block];
-  const int tp = min(*pp.total, pp.capacity);
-  const int tw = capPW > 0 ? min(*pw.total, pw.capacity) : 0;
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < tp + tw; g += gridDim.x * blockDim.x) {
-    const bool wall = g >= tp;
-    const fdm_pairs& s = wall ? pw : pp;
-    const int p = wall ? g - tp : g;
+  int tot[kParts], pre[kParts + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int p = 0; p < kParts; ++p) {
+    tot[p] = setTotal(S, p);
+    pre[p + 1] = pre[p] + tot[p];
+  }
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < pre[kParts]; g += gridDim.x * blockDim.x) {
+    int set = 0;
+    while (g >= pre[set + 1]) ++set;
+    const fdm_pairs& s = S.s[set];
+    const int p = g - pre[set];
     const int n = s.recCnt[p];
     if (n <= 0) continue;
-    const int i0 = off[wall ? pp.capacity + p : p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
+    const int i0 = off[S.capOff[set] + p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
     for (int r = 0; r < n; ++r) {
       const float4 rec = s.rec[r0 + r];
-      const int i = i0 + r, c = start + i, k = __float_as_int(rec.w);
+      const int c = start + i0 + r, k = __float_as_int(rec.w);
       t.A[c] = A;
       t.T[c] = T;
       t.node[c] = k;
       t.spring[3 * (size_t)c] = rec.x;
       t.spring[3 * (size_t)c + 1] = rec.y;
       t.spring[3 * (size_t)c + 2] = rec.z;
-      t.prev[c] = prevOf(t, prevBlock, wall, A, T, k);
+      t.prev[c] = prevOf(t, prevBlock, set, A, T, k);
     }
   }
 }
 
 // ----------------------------------------------------------- tape restore
 // pair starts of a block part; zeros up to maxBlock so that the scan total counts pairs
-__global__ void kRestoreFlags(fdm_contact_tape t, int block, int wallPart, int* flag, int maxBlock) {
-  const int start = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
-  const int count = wallPart ? t.blockCount[block] - t.blockPP[block] : t.blockPP[block];
+__global__ void kRestoreFlags(fdm_contact_tape t, int block, int set, int* flag, int maxBlock) {
+  const int* part = t.blockPart + (size_t)(kParts + 1) * block;
+  const int start = t.blockStart[block] + part[set], count = part[set + 1] - part[set];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < maxBlock; i += gridDim.x * blockDim.x) {
     const int c = start + i;
     flag[i] = i < count && (i == 0 || t.A[c] != t.A[c - 1] || t.T[c] != t.T[c - 1]) ? 1 : 0;
   }
 }
 
-__global__ void kRestoreScatter(fdm_contact_tape t, int block, int wallPart, const int* flag,
-                                const int* idx, const int* numPairs, fdm_pairs s) {
-  const int start = t.blockStart[block] + (wallPart ? t.blockPP[block] : 0);
-  const int count = wallPart ? t.blockCount[block] - t.blockPP[block] : t.blockPP[block];
+__global__ void kRestoreScatter(fdm_contact_tape t, int block, int set, const int* flag, const int* idx,
+                                const int* numPairs, fdm_pairs s) {
+  const int* part = t.blockPart + (size_t)(kParts + 1) * block;
+  const int start = t.blockStart[block] + part[set], count = part[set + 1] - part[set];
   if (*numPairs > s.capacity || count > s.recCapacity) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(t.status, FDM_STATUS_PAIR_OVERFLOW);
     return;
@@ -181,12 +204,11 @@ struct AdjWork {
   double* Tc;      // [3n]
   double* shareA;  // [kShare * maxBlock]
   double* shareB;  // [kShare * maxBlock]
-  // half-open block-local ranges per particle, written by kAdjContacts at
-  // run boundaries (empty unless written): contacts with A = i among
-  // particle pairs and among wall pairs
-  int* runPP;      // [2n]
-  int* runPW;      // [2n]
-  // particle-pair contacts whose target is particle i (unordered)
+  // half-open block-local ranges per particle and set, written by
+  // kAdjContacts at run boundaries (empty unless written): contacts with
+  // A = i in each set
+  int* run;        // [kParts * 2n]
+  // contacts whose target is particle i, from every particle-target set (unordered)
   int* bucketCount;  // [n]
   int* bucket;       // [n * kBucket]
   unsigned* status;
@@ -215,8 +237,8 @@ __global__ void __launch_bounds__(256) kAdjKick(fdm_adjoint_step_args a, AdjWork
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x)
       a.springBar[s0 + j] = 0.;
   }
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < 2 * n; j += gridDim.x * blockDim.x) {
-    k.runPP[j] = k.runPW[j] = 0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < kParts * 2 * n; j += gridDim.x * blockDim.x) {
+    k.run[j] = 0;
     if (j < n) k.bucketCount[j] = 0;
   }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -253,18 +275,20 @@ __device__ __forceinline__ void putShare(double* s, const adj::BodyBar& b, doubl
 
 __global__ void __launch_bounds__(256) kAdjContacts(fdm_adjoint_step_args a, AdjWork k) {
   const fdm_contact_tape& tp = a.tape;
-  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block], npp = tp.blockPP[a.block];
+  const int start = tp.blockStart[a.block], count = tp.blockCount[a.block];
+  const int* part = tp.blockPart + (size_t)(kParts + 1) * a.block;
   const int n = a.before.n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const int c = start + i;
-    const bool wall = i >= npp;
+    int set = 0;
+    while (i >= part[set + 1]) ++set;
+    const bool wall = set == FDM_SET_WALL;
     const int A = tp.A[c], T = tp.T[c], node = tp.node[c];
     {
-      // run boundaries: by A within the part, and by key over the block
-      int* run = wall ? k.runPW : k.runPP;
-      const int lo = wall ? npp : 0, hi = wall ? count : npp;
-      if (i == lo || tp.A[c - 1] != A) run[2 * A] = i;
-      if (i == hi - 1 || tp.A[c + 1] != A) run[2 * A + 1] = i + 1;
+      // run boundaries by A within the set, and the target's bucket
+      int* run = k.run + (size_t)set * 2 * n;
+      if (i == part[set] || tp.A[c - 1] != A) run[2 * A] = i;
+      if (i == part[set + 1] - 1 || tp.A[c + 1] != A) run[2 * A + 1] = i + 1;
       if (!wall) {
         const int pos = atomicAdd(k.bucketCount + T, 1);
         if (pos < kBucket) k.bucket[(size_t)T * kBucket + pos] = i;
@@ -304,6 +328,13 @@ __global__ void __launch_bounds__(256) kAdjContacts(fdm_adjoint_step_args a, Adj
       zetaToA = Mi.e <= Mw.e;
     } else {
       xB = toV(ldp(a.after.x, n, T));
+      if (set >= FDM_SET_GHOST) {
+        // the ghost: the real particle shifted by its set's periods
+        const fdm_grid& g = a.inst.count > 0 ? a.inst.pgrid[a.inst.particleInstance[T]] : a.pgrid;
+        const int mask = set - FDM_SET_GHOST + 1;
+        if (mask & 1) xB.x += g.max.x - g.min.x;
+        if (mask & 2) xB.y += g.max.y - g.min.y;
+      }
       qB = toQ(ld4(a.after.q, n, T));
       vB = ldV(k.vh, n, T);
       wB = ldV(k.wh, n, T);
@@ -402,8 +433,10 @@ __global__ void __launch_bounds__(256) kAdjGather(fdm_adjoint_step_args a, AdjWo
     double s[kShare];
 #pragma unroll
     for (int j = 0; j < kShare; ++j) s[j] = 0.;
-    for (int c = k.runPP[2 * i]; c < k.runPP[2 * i + 1]; ++c) addShare(s, k.shareA + (size_t)kShare * c);
-    for (int c = k.runPW[2 * i]; c < k.runPW[2 * i + 1]; ++c) addShare(s, k.shareA + (size_t)kShare * c);
+    for (int set = 0; set < kParts; ++set) {
+      const int* run = k.run + (size_t)set * 2 * n + 2 * i;
+      for (int c = run[0]; c < run[1]; ++c) addShare(s, k.shareA + (size_t)kShare * c);
+    }
     {
       // as target: ascending contact order, selected from the bucket
       const int m = min(k.bucketCount[i], kBucket);
@@ -420,7 +453,8 @@ __global__ void __launch_bounds__(256) kAdjGather(fdm_adjoint_step_args a, AdjWo
       }
     }
     // wall-material shares of the wall contacts, per wall material
-    for (int c = k.runPW[2 * i]; c < k.runPW[2 * i + 1]; ++c) {
+    const int* wrun = k.run + (size_t)FDM_SET_WALL * 2 * n + 2 * i;
+    for (int c = wrun[0]; c < wrun[1]; ++c) {
       const double* sb = k.shareB + (size_t)kShare * c;
       if (sb[15] == 0. && sb[16] == 0.) continue;
       const int wm = a.walls.mat[tp.T[start + c]];
@@ -498,9 +532,9 @@ struct TapeWs {
   void* sub;
   size_t subBytes;
 };
-size_t tapeBytes(int maxBlock, int capPP, int capPW, TapeWs* w, char* base) {
+size_t tapeBytes(int maxBlock, long long pairCapacity, TapeWs* w, char* base) {
   const size_t m = std::max(maxBlock, 64);
-  const size_t pairs = std::max<size_t>((size_t)capPP + (size_t)std::max(capPW, 0), 64);
+  const size_t pairs = std::max<size_t>((size_t)pairCapacity, 64);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     char* p = base ? base + o : nullptr;
@@ -509,7 +543,7 @@ size_t tapeBytes(int maxBlock, int capPP, int capPW, TapeWs* w, char* base) {
   };
   int* cnt = (int*)take(sizeof(int) * pairs);
   int* off = (int*)take(sizeof(int) * pairs);
-  int* total = (int*)take(sizeof(int) * 8);
+  int* total = (int*)take(sizeof(int) * 16);
   int* flag = (int*)take(sizeof(int) * m);
   int* idx = (int*)take(sizeof(int) * m);
   const size_t sb = scanWorkspace(std::max(pairs, m));
@@ -518,58 +552,72 @@ size_t tapeBytes(int maxBlock, int capPP, int capPW, TapeWs* w, char* base) {
   return o;
 }
 
+bool makeSets(const fdm_pairs* const* sets, TapeSets& S, long long& cap) {
+  if (!sets || !sets[0]) return false;
+  S.capOff[0] = 0;
+  cap = 0;
+  for (int p = 0; p < kParts; ++p) {
+    S.s[p] = sets[p] ? *sets[p] : *sets[0];
+    const int c = sets[p] ? sets[p]->capacity : 0;
+    if (c < 0) return false;
+    cap += c;
+    if (cap >= (1ll << 31)) return false;
+    S.capOff[p + 1] = (int)cap;
+  }
+  return true;
+}
+
 }  // namespace
 }  // namespace fdm
 
 using namespace fdm;
 
-extern "C" size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, int pairCapacityPP, int pairCapacityPW) {
-  return tapeBytes(maxBlockContacts, pairCapacityPP, pairCapacityPW, nullptr, nullptr);
+extern "C" size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, long long pairCapacity) {
+  return tapeBytes(maxBlockContacts, pairCapacity, nullptr, nullptr);
 }
 
-extern "C" int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, int nParticles, int nWalls,
-                                    fdm_contact_tape tape, int block, int prevBlock,
-                                    int maxBlockContacts, void* ws, size_t wsBytes, hipStream_t stream) {
-  if (!pp || block < 0 || block >= tape.maxBlocks || prevBlock >= block || nParticles < 0 ||
-      nWalls < 0 || maxBlockContacts < 1 ||
-      (long long)tape.maxBlocks * maxBlockContacts > (long long)tape.capacity)
+extern "C" int fdm_launchTapeAppend(const fdm_pairs* const* sets, fdm_contact_tape tape, int block,
+                                    int prevBlock, int maxBlockContacts, void* ws, size_t wsBytes,
+                                    hipStream_t stream) {
+  TapeSets S;
+  long long cap = 0;
+  if (!makeSets(sets, S, cap) || block < 0 || block >= tape.maxBlocks || prevBlock >= block ||
+      maxBlockContacts < 1 || (long long)tape.maxBlocks * maxBlockContacts > (long long)tape.capacity)
     return FDM_EINVAL;
-  const int capPW = pw ? pw->capacity : 0;
   TapeWs w;
-  if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
-  const fdm_pairs pwv = pw ? *pw : *pp;
-  const long long pairs = (long long)pp->capacity + capPW;
-  hipLaunchKernelGGL(kTapeCount, dim3(std::min(blocksFor(pairs, 256), 4096u)), dim3(256), 0, stream,
-                     *pp, pwv, capPW, w.cnt);
-  int r = scanInts(w.cnt, w.off, (size_t)pairs, false, w.total, w.sub, w.subBytes, stream);
+  if (wsBytes < tapeBytes(maxBlockContacts, cap, &w, (char*)ws)) return FDM_EINVAL;
+  hipLaunchKernelGGL(kTapeCount, dim3(std::max(1u, std::min(blocksFor(cap, 256), 4096u))), dim3(256), 0,
+                     stream, S, w.cnt);
+  int r = scanInts(w.cnt, w.off, (size_t)std::max(cap, 1ll), false, w.total, w.sub, w.subBytes, stream);
   if (r != FDM_OK) return r;
-  hipLaunchKernelGGL(kTapeBlock, dim3(1), dim3(64), 0, stream, tape, block, (const int*)w.total,
-                     (const int*)w.off, pp->capacity, capPW, maxBlockContacts);
-  hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, *pp, pwv, capPW,
-                     (const int*)w.off, (const int*)w.total, tape, block, prevBlock, nParticles);
+  hipLaunchKernelGGL(kTapeBlock, dim3(1), dim3(64), 0, stream, tape, block, S, (const int*)w.total,
+                     (const int*)w.off, maxBlockContacts);
+  hipLaunchKernelGGL(kTapeScatter, dim3(kStride), dim3(256), 0, stream, S, (const int*)w.off,
+                     (const int*)w.total, tape, block, prevBlock);
   return launchStatus();
 }
 
-extern "C" int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* pp,
-                                     const fdm_pairs* pw, int nParticles, int maxBlockContacts,
-                                     void* ws, size_t wsBytes, hipStream_t stream) {
-  if (!pp || block < 0 || block >= tape.maxBlocks || nParticles < 0 || maxBlockContacts < 1)
+extern "C" int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* const* sets,
+                                     int nParticles, int maxBlockContacts, void* ws, size_t wsBytes,
+                                     hipStream_t stream) {
+  TapeSets S;
+  long long cap = 0;
+  if (!makeSets(sets, S, cap) || block < 0 || block >= tape.maxBlocks || nParticles < 0 ||
+      maxBlockContacts < 1)
     return FDM_EINVAL;
-  const int capPW = pw ? pw->capacity : 0;
   TapeWs w;
-  if (wsBytes < tapeBytes(maxBlockContacts, pp->capacity, capPW, &w, (char*)ws)) return FDM_EINVAL;
-  for (int part = 0; part < 2; ++part) {
-    const fdm_pairs* s = part == 0 ? pp : pw;
-    if (!s) continue;
-    hipLaunchKernelGGL(kRestoreFlags, dim3(kStride), dim3(256), 0, stream, tape, block, part, w.flag,
+  if (wsBytes < tapeBytes(maxBlockContacts, cap, &w, (char*)ws)) return FDM_EINVAL;
+  for (int set = 0; set < kParts; ++set) {
+    if (!sets[set]) continue;
+    hipLaunchKernelGGL(kRestoreFlags, dim3(kStride), dim3(256), 0, stream, tape, block, set, w.flag,
                        maxBlockContacts);
-    int r = scanInts(w.flag, w.idx, (size_t)maxBlockContacts, false, w.total + 2 + part, w.sub,
+    int r = scanInts(w.flag, w.idx, (size_t)maxBlockContacts, false, w.total + 2 + set, w.sub,
                      w.subBytes, stream);
     if (r != FDM_OK) return r;
-    hipLaunchKernelGGL(kRestoreScatter, dim3(kStride), dim3(256), 0, stream, tape, block, part,
-                       (const int*)w.flag, (const int*)w.idx, (const int*)(w.total + 2 + part), *s);
+    hipLaunchKernelGGL(kRestoreScatter, dim3(kStride), dim3(256), 0, stream, tape, block, set,
+                       (const int*)w.flag, (const int*)w.idx, (const int*)(w.total + 2 + set), *sets[set]);
     hipLaunchKernelGGL(kRestoreCount, dim3(std::min(blocksFor((long long)nParticles + 1, 256), 4096u)),
-                       dim3(256), 0, stream, *s, nParticles, (const int*)(w.total + 2 + part));
+                       dim3(256), 0, stream, *sets[set], nParticles, (const int*)(w.total + 2 + set));
   }
   return launchStatus();
 }
@@ -577,7 +625,7 @@ extern "C" int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm
 extern "C" size_t fdm_adjointWorkspaceBytes(int nParticles, int maxBlockContacts) {
   const size_t n = std::max(nParticles, 1), m = std::max(maxBlockContacts, 1);
   return up(4 * 3 * n * sizeof(double)) + 2 * up(kShare * m * sizeof(double)) +
-         2 * up(2 * n * sizeof(int)) + up(n * sizeof(int)) + up(n * kBucket * sizeof(int));
+         up(kParts * 2 * n * sizeof(int)) + up(n * sizeof(int)) + up(n * kBucket * sizeof(int));
 }
 
 extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
@@ -601,10 +649,8 @@ extern "C" int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws
   b += up(kShare * (size_t)std::max(args->maxBlockContacts, 1) * sizeof(double));
   k.shareB = (double*)b;
   b += up(kShare * (size_t)std::max(args->maxBlockContacts, 1) * sizeof(double));
-  const size_t r2 = up(2 * (size_t)std::max(n, 1) * sizeof(int));
-  k.runPP = (int*)b;
-  k.runPW = (int*)(b + r2);
-  b += 2 * r2;
+  k.run = (int*)b;
+  b += up(kParts * 2 * (size_t)std::max(n, 1) * sizeof(int));
   k.bucketCount = (int*)b;
   b += up((size_t)std::max(n, 1) * sizeof(int));
   k.bucket = (int*)b;

@@ -15,11 +15,12 @@
 // stick or slip); FP32 storage of velocities and springs is differentiated
 // as the identity.
 //
-// Supported: particle-particle and particle-wall contacts, walls with
-// prescribed motion, several independent systems in one solver. Not
-// supported: periodic boundaries, bonds, external-force hooks.
+// Supported: particle-particle, particle-wall (walls with prescribed motion)
+// and periodic ghost contacts, several independent systems in one solver.
+// Not supported: bonds, external-force hooks.
 #pragma once
 #include <algorithm>
+#include <array>
 #include <stdexcept>
 #include <vector>
 
@@ -48,7 +49,6 @@ class LSAdjoint {
   void forward(size_t steps, size_t checkpointEvery) {
     if (steps < 1) throw std::invalid_argument("forward needs at least one step");
     if (s_.particles_.num_device() == 0) throw std::logic_error("initialise the solver first");
-    if (s_.periodic_.isActived()) throw std::invalid_argument("periodic boundaries are not differentiated");
     if (s_.bonds_.view().n > 0) throw std::invalid_argument("bonds are not differentiated");
     steps_ = steps;
     every_ = std::max<size_t>(1, std::min(checkpointEvery, steps));
@@ -197,13 +197,13 @@ class LSAdjoint {
   };
   struct TapeBuf {
     int capacity = 0, maxBlocks = 0;
-    DeviceBuffer<int> blockStart, blockPP, blockCount, A, T, node, prev;
+    DeviceBuffer<int> blockStart, blockPart, blockCount, A, T, node, prev;
     DeviceBuffer<float> spring;
     void allocate(int cap, int blocks) {
       capacity = std::max(cap, 1);
       maxBlocks = std::max(blocks, 1);
       blockStart.allocate(maxBlocks);
-      blockPP.allocate(maxBlocks);
+      blockPart.allocate(6 * (size_t)maxBlocks);
       blockCount.allocate(maxBlocks);
       A.allocate(capacity, false);
       T.allocate(capacity, false);
@@ -216,7 +216,7 @@ class LSAdjoint {
       t.capacity = capacity;
       t.maxBlocks = maxBlocks;
       t.blockStart = blockStart.get();
-      t.blockPP = blockPP.get();
+      t.blockPart = blockPart.get();
       t.blockCount = blockCount.get();
       t.A = A.get();
       t.T = T.get();
@@ -229,7 +229,8 @@ class LSAdjoint {
   };
   struct Checkpoint {
     StateBuf state;
-    int pp = 0, count = 0;
+    int part[6] = {0, 0, 0, 0, 0, 0};
+    int count = 0;
     DeviceBuffer<int> A, T, node;
     DeviceBuffer<float> spring;
   };
@@ -248,9 +249,15 @@ class LSAdjoint {
   void setup() {
     N_ = s_.particles_.num();
     W_ = s_.walls_.num();
+    long long recs = 0;
+    pairCap_ = 0;
+    for (const fdm_pairs& p : currentSets(setBuf_))
+      if (p.recTotal) {
+        recs += p.recCapacity;
+        pairCap_ += p.capacity;
+      }
     maxBlock_ = (int)std::min<long long>((long long)contactsPerParticle_ * (long long)std::max<size_t>(N_, 1),
-                                         (long long)s_.pp_.current().recCapacity +
-                                             (W_ ? (long long)s_.pw_.current().recCapacity : 0));
+                                         recs);
     maxBlock_ = std::max(maxBlock_, 64);
     const int len = (int)every_;
     slots_.resize(len + 1);
@@ -259,9 +266,7 @@ class LSAdjoint {
     if (cap >= (1ll << 31)) throw std::invalid_argument("contact tape too large");
     tape_.allocate((int)cap, len + 1);
     staging_.allocate(maxBlock_, 1);
-    const fdm_pairs pp = s_.pp_.current();
-    const int capPW = W_ ? s_.pw_.current().capacity : 1;
-    tapeWs_.allocate(fdm_tapeWorkspaceBytes(maxBlock_, pp.capacity, capPW), false);
+    tapeWs_.allocate(fdm_tapeWorkspaceBytes(maxBlock_, pairCap_), false);
     adjWs_.allocate(fdm_adjointWorkspaceBytes((int)N_, maxBlock_), false);
     xBar_.allocate(3 * N_);
     qBar_.allocate(4 * N_);
@@ -314,12 +319,25 @@ class LSAdjoint {
     v.T = b.T.get();
     return v;
   }
+  // the current generation of every set, in tape order (absent: recTotal NULL)
+  std::array<fdm_pairs, 5>& currentSets(std::array<fdm_pairs, 5>& out) {
+    for (auto& p : out) p = fdm_pairs{};
+    out[0] = s_.pp_.current();
+    if (W_ || s_.walls_.num()) out[1] = s_.pw_.current();
+    for (int k = 0; k < 3; ++k)
+      if (s_.periodic_.active(k)) out[2 + k] = s_.periodic_.set(k).interaction.current();
+    return out;
+  }
+  std::array<const fdm_pairs*, 5> setPointers(std::array<fdm_pairs, 5>& sets) {
+    std::array<const fdm_pairs*, 5> ptr;
+    currentSets(sets);
+    for (int k = 0; k < 5; ++k) ptr[k] = sets[k].recTotal ? &sets[k] : nullptr;
+    return ptr;
+  }
   void appendTo(TapeBuf& t, int block, int prevBlock) {
-    const fdm_pairs pp = s_.pp_.current();
-    const fdm_pairs pw = s_.pw_.current();
-    fdmCheck(fdm_launchTapeAppend(&pp, W_ ? &pw : nullptr, (int)N_, (int)W_, t.view(s_.status_.get()),
-                                  block, prevBlock, maxBlock_, tapeWs_.get(), tapeWs_.size(),
-                                  s_.stream_),
+    const auto ptr = setPointers(setBuf_);
+    fdmCheck(fdm_launchTapeAppend(ptr.data(), t.view(s_.status_.get()), block, prevBlock, maxBlock_,
+                                  tapeWs_.get(), tapeWs_.size(), s_.stream_),
              "contact tape");
   }
   void appendBlock(int block, int prevBlock) { appendTo(tape_, block, prevBlock); }
@@ -335,7 +353,8 @@ class LSAdjoint {
     copyState(c.state, true);
     appendTo(staging_, 0, -1);
     checkTape();
-    c.pp = staging_.blockPP.toHost(st, 1)[0];
+    const std::vector<int> part = staging_.blockPart.toHost(st, 6);
+    for (int k = 0; k < 6; ++k) c.part[k] = part[k];
     c.count = staging_.blockCount.toHost(st, 1)[0];
     const size_t m = std::max(c.count, 1);
     c.A.allocate(m, false);
@@ -357,14 +376,15 @@ class LSAdjoint {
     copy(staging_.T.get(), c.T.get(), c.count, st);
     copy(staging_.node.get(), c.node.get(), c.count, st);
     copy(staging_.spring.get(), c.spring.get(), 3 * (size_t)c.count, st);
-    const int h[3] = {0, c.pp, c.count};
-    hipCheck(hipMemcpyAsync(staging_.blockStart.get(), &h[0], sizeof(int), hipMemcpyHostToDevice, st), "restore");
-    hipCheck(hipMemcpyAsync(staging_.blockPP.get(), &h[1], sizeof(int), hipMemcpyHostToDevice, st), "restore");
-    hipCheck(hipMemcpyAsync(staging_.blockCount.get(), &h[2], sizeof(int), hipMemcpyHostToDevice, st), "restore");
-    const fdm_pairs pp = s_.pp_.current();
-    const fdm_pairs pw = s_.pw_.current();
-    fdmCheck(fdm_launchTapeRestore(staging_.view(s_.status_.get()), 0, &pp, W_ ? &pw : nullptr, (int)N_,
-                                   maxBlock_, tapeWs_.get(), tapeWs_.size(), st),
+    const int zero = 0;
+    hipCheck(hipMemcpyAsync(staging_.blockStart.get(), &zero, sizeof(int), hipMemcpyHostToDevice, st), "restore");
+    hipCheck(hipMemcpyAsync(staging_.blockPart.get(), c.part, 6 * sizeof(int), hipMemcpyHostToDevice, st),
+             "restore");
+    hipCheck(hipMemcpyAsync(staging_.blockCount.get(), &c.count, sizeof(int), hipMemcpyHostToDevice, st),
+             "restore");
+    const auto ptr = setPointers(setBuf_);
+    fdmCheck(fdm_launchTapeRestore(staging_.view(s_.status_.get()), 0, ptr.data(), (int)N_, maxBlock_,
+                                   tapeWs_.get(), tapeWs_.size(), st),
              "history restore");
     const int one = 1;
     hipCheck(hipMemcpyAsync(s_.ctl_.get() + FDM_CTL_FORCE, &one, sizeof(int), hipMemcpyHostToDevice, st),
@@ -384,6 +404,8 @@ class LSAdjoint {
     a.wmat = s_.walls_.deviceMaterials();
     a.gravity = {s_.gravity_.x, s_.gravity_.y, s_.gravity_.z};
     a.dt = s_.dt_;
+    a.pgrid = s_.particles_.grid();
+    a.inst = s_.inst_;
     a.tape = tape_.view(s_.status_.get());
     a.block = j;
     a.prevBlock = j - 1;
@@ -406,6 +428,8 @@ class LSAdjoint {
   int contactsPerParticle_ = 16;
   size_t steps_ = 0, every_ = 1, N_ = 0, W_ = 0;
   int maxBlock_ = 0;
+  long long pairCap_ = 0;
+  std::array<fdm_pairs, 5> setBuf_{};
   std::vector<Checkpoint> checkpoints_;
   Checkpoint final_;
   std::vector<StateBuf> slots_;

@@ -551,18 +551,18 @@ int fdm_deviceInfo(int* numCU, int* warpSize);
  * No counterpart in the reference: the reverse (adjoint) timestep behind
  * fundem::LSAdjoint, for gradients of a scalar outcome of the final state
  * with respect to the initial state and the contact parameters. Supported:
- * particle-particle and particle-wall contacts (walls with prescribed
- * motion); no periodic ghosts and no bonds. */
+ * particle-particle, particle-wall (walls with prescribed motion) and
+ * periodic ghost contacts, several systems in one solver; no bonds. */
 
 /* Contacts of consecutive steps, block by block: block b holds a step's
- * particle-pair contacts in canonical order (A, target, node), then its
- * particle-wall contacts in the same order, in the slot
+ * contacts set by set (particle pairs, wall pairs, ghost X, Y, XY pairs),
+ * each set in canonical order (A, target, node), in the slot
  * [b * maxBlockContacts, (b + 1) * maxBlockContacts) of every array. */
 typedef struct {
   int capacity;        /* contacts over all slots: at least maxBlocks * maxBlockContacts */
   int maxBlocks;
   int* blockStart;     /* [maxBlocks] first contact of the block (b * maxBlockContacts) */
-  int* blockPP;        /* [maxBlocks] its particle-pair contacts */
+  int* blockPart;      /* [6 * maxBlocks] block-local start of each set, then the count */
   int* blockCount;     /* [maxBlocks] all its contacts */
   int* A;              /* [capacity] */
   int* T;              /* [capacity] target particle or wall */
@@ -573,20 +573,20 @@ typedef struct {
 } fdm_contact_tape;
 
 /* Workspace of the tape routines for blocks of at most maxBlockContacts
- * contacts and sets of the given pair capacities. */
-size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, int pairCapacityPP, int pairCapacityPW);
-/* Write the contacts of the latest step (the sets' current lists and
- * results) as block `block`. prevBlock: the block the step's history came
- * from, -1 for none. pw may be NULL (no walls). */
-int fdm_launchTapeAppend(const fdm_pairs* pp, const fdm_pairs* pw, int nParticles, int nWalls,
-                         fdm_contact_tape tape, int block, int prevBlock, int maxBlockContacts,
-                         void* ws, size_t wsBytes, hipStream_t stream);
-/* Write block `block` into the sets' lists and results (pp, pw: the
- * generations a following step reads as its history; their count arrays hold
+ * contacts and sets whose pair capacities sum to pairCapacity. */
+size_t fdm_tapeWorkspaceBytes(int maxBlockContacts, long long pairCapacity);
+/* Write the contacts of the latest step as block `block`. sets: the five
+ * sets' current lists and results in tape order (particle, wall, ghost X,
+ * Y, XY), NULL for an absent set (sets[0] is required). prevBlock: the block
+ * the step's history came from, -1 for none. */
+int fdm_launchTapeAppend(const fdm_pairs* const* sets, fdm_contact_tape tape, int block, int prevBlock,
+                         int maxBlockContacts, void* ws, size_t wsBytes, hipStream_t stream);
+/* Write block `block` into the sets' lists and results (the generations a
+ * following step reads as its history; their count arrays hold
  * nParticles + 1 entries). The following step must rebuild its lists. */
-int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* pp,
-                          const fdm_pairs* pw, int nParticles, int maxBlockContacts, void* ws,
-                          size_t wsBytes, hipStream_t stream);
+int fdm_launchTapeRestore(fdm_contact_tape tape, int block, const fdm_pairs* const* sets,
+                          int nParticles, int maxBlockContacts, void* ws, size_t wsBytes,
+                          hipStream_t stream);
 
 /* One step in reverse. The state before the step (x, q, v, w and the F, T
  * of the previous step) and after it (x, q, F, T) come from the tape; the
@@ -602,6 +602,8 @@ typedef struct {
   const fdm_material* wmat;
   fdm_double3 gravity;
   double dt;
+  fdm_grid pgrid;              /* particle grid (ghost periods) */
+  fdm_instances inst;          /* count 0: one system on pgrid */
   fdm_contact_tape tape;
   int block;                   /* the step's contacts */
   int prevBlock;               /* the block its history came from (-1: none) */

@@ -53,11 +53,14 @@ def _templates():
     return _TM["v"]
 
 
-def _scenario(tm, x, q, v, w, p):
+L = 0.08
+
+
+def _scenario(tm, x, q, v, w, p, periodic=False):
     tm, props = tm
     n = len(x)
-    L = 0.08
-    wt, wall = W.floor_wall([-L, -L], [L, L])
+    # the floor reaches past the box, under bodies that straddle a periodic face
+    wt, wall = W.floor_wall([-L - 0.02, -L - 0.02], [L + 0.02, L + 0.02])
     return W.Scenario(
         name="adjoint-pile", templates=tm, density=2500.0,
         materials=[(p["kn"], p["ks"], p["mu"], p["zeta"])],
@@ -67,7 +70,7 @@ def _scenario(tm, x, q, v, w, p):
         domain_min=np.array([-L, -L, -L]), domain_max=np.array([L, L, L]),
         gravity=np.array([0.0, 0.0, -9.81]), dt=DT,
         wall_templates=[wt], wall_materials=[(p["wall_mu"], p["wall_zeta"])], walls=[wall],
-        damping_ratio=True, mass_properties=props)
+        periodic=(bool(periodic), False), damping_ratio=True, mass_properties=props)
 
 
 def _rot(q):
@@ -78,25 +81,27 @@ def _rot(q):
         [2 * (q1 * q3 - q0 * q2), 2 * (q2 * q3 + q0 * q1), 1 - 2 * (q1 * q1 + q2 * q2)]])
 
 
-def _pp_contacts(tm, x, q):
+def _pair_contacts(tm, x, q, periodic):
     o = po.OracleSim(False, True)
-    W.apply(_scenario(tm, x, q, np.zeros((len(x), 3)), np.zeros((len(x), 3)), PARAMS), o)
-    return o.contacts(po.SET_PP)["A"].size
+    W.apply(_scenario(tm, x, q, np.zeros((len(x), 3)), np.zeros((len(x), 3)), PARAMS, periodic), o)
+    return o.contacts(po.SET_GX if periodic else po.SET_PP)["A"].size
 
 
 _PILE = {}
 
 
-def resting_pile():
+def resting_pile(periodic=False):
     """Particle 0 rests on the floor, particle 1 rests on top of particle 0,
     particle 2 sits on the floor apart from them. Each floor overlap is a
     third of the one that carries the weight above it on a single node, and
     particle 1 is lowered onto particle 0 by a third of its own such
     overlap from the height where the two first touch (found by bisection
     on the oracle's detection), so every contact stays closed while the
-    bodies oscillate about equilibrium over the test window."""
-    if _PILE:
-        return _PILE["v"]
+    bodies oscillate about equilibrium over the test window. Periodic in x:
+    particle 0 sits just inside the -x face and particle 1 just inside the
+    +x face, on particle 0's periodic image (a ghost-set contact)."""
+    if periodic in _PILE:
+        return _PILE[periodic]
     tmp = _templates()
     tm, props = tmp
     rng = np.random.default_rng(11)
@@ -109,18 +114,21 @@ def resting_pile():
     d0 = (m[0] + m[1]) * gz / PARAMS["kn"] / 3.0
     d2 = m[2] * gz / PARAMS["kn"] / 3.0
     x = np.array([[0.0, 0.0, -lowest[0] - d0], [0.0005, -0.0003, 0.05], [0.02, 0.01, -lowest[2] - d2]])
+    if periodic:
+        x[0, 0] = -L + 0.0002
+        x[1, 0] = L - 0.0003
     lo, hi = x[0, 2], x[0, 2] + 0.03  # particle 1 height: touching at lo, apart at hi
     for _ in range(40):
         mid = 0.5 * (lo + hi)
         x[1, 2] = mid
-        if _pp_contacts(tmp, x, q) > 0:
+        if _pair_contacts(tmp, x, q, periodic) > 0:
             lo = mid
         else:
             hi = mid
     d1 = m[1] * gz / (0.5 * PARAMS["kn"]) / 3.0
     x[1, 2] = hi - d1
-    _PILE["v"] = (tmp, x, q)
-    return _PILE["v"]
+    _PILE[periodic] = (tmp, x, q)
+    return _PILE[periodic]
 
 
 def objectives(state, masses, coef):
@@ -138,17 +146,17 @@ def objective_adjoints(state, masses, coef):
             dict(x=coef["x"], q=coef["q"], v=np.zeros_like(state["v"]), w=coef["w"])]
 
 
-def run_oracle(tm, x, q, v, w, p, steps):
+def run_oracle(tm, x, q, v, w, p, steps, periodic=False):
     o = po.OracleSim(False, True)
-    W.apply(_scenario(tm, x, q, v, w, p), o)
+    W.apply(_scenario(tm, x, q, v, w, p, periodic), o)
     o.step(steps)
     return o.state(), o
 
 
-@pytest.mark.parametrize("steps,every", [(60, 20)])
-def test_adjoint_matches_finite_differences(steps, every):
+@pytest.mark.parametrize("steps,every,periodic", [(60, 20, False), (60, 20, True)])
+def test_adjoint_matches_finite_differences(steps, every, periodic):
     from fundem_amd import Adjoint, Solver
-    tm, x, q = resting_pile()
+    tm, x, q = resting_pile(periodic)
     n = len(x)
     rng = np.random.default_rng(5)
     v0 = rng.normal(0.0, 1e-5, size=(n, 3))
@@ -158,18 +166,19 @@ def test_adjoint_matches_finite_differences(steps, every):
                 q=rng.normal(0, 1e-8, size=(n, 4)))
     # GPU run with its adjoint
     g = Solver()
-    W.apply(_scenario(tm, x, q, v0, w0, PARAMS), g)
+    W.apply(_scenario(tm, x, q, v0, w0, PARAMS, periodic), g)
     adj = Adjoint(g)
     adj.forward(steps, every)
     assert g.status() == 0
-    assert g.contacts(po.SET_PP)["A"].size > 0 and g.contacts(po.SET_PW)["A"].size > 0
+    pair_set = po.SET_GX if periodic else po.SET_PP
+    assert g.contacts(pair_set)["A"].size > 0 and g.contacts(po.SET_PW)["A"].size > 0
     fin = g.state()
-    ref, o = run_oracle(tm, x, q, v0, w0, PARAMS, steps)
+    ref, o = run_oracle(tm, x, q, v0, w0, PARAMS, steps, periodic)
     masses = np.array([tm[1][k][0] for k in (np.arange(n) % len(tm[0]))])
     grads = [adj.backward(a["x"], a["q"], a["v"], a["w"]) for a in objective_adjoints(fin, masses, coef)]
     # the same adjoint without intermediate checkpoints, bit for bit
     g2 = Solver()
-    W.apply(_scenario(tm, x, q, v0, w0, PARAMS), g2)
+    W.apply(_scenario(tm, x, q, v0, w0, PARAMS, periodic), g2)
     adj2 = Adjoint(g2)
     adj2.forward(steps, steps)
     a0 = objective_adjoints(g2.state(), masses, coef)[0]
@@ -196,7 +205,7 @@ def test_adjoint_matches_finite_differences(steps, every):
             ww[spec[1], spec[2]] += delta
         else:
             xx[spec[1], spec[2]] += delta
-        st, _ = run_oracle(tm, xx, q, vv, ww, p, steps)
+        st, _ = run_oracle(tm, xx, q, vv, ww, p, steps, periodic)
         return np.array(objectives(st, masses, coef))
 
     base = np.array(objectives(ref, masses, coef))
