@@ -76,16 +76,22 @@ def test_inclusive_scan():
 
 # -------------------------------------------------------- contact mechanics
 def _sphere_probe(e, kn, mu=0.0, ks=0.0, v0=(0, 0, -1.0), dt=1e-6, node=(0, 0, -0.01), steps=200000,
-                  gravity=(0, 0, 0), rotate=True):
+                  gravity=(0, 0, 0), rotate=True, zeta=None):
     r, rho = 0.01, 2000.0
     m = 4 / 3 * math.pi * r ** 3 * rho
     I = 0.4 * m * r * r if rotate else 0.0
     s = po.OracleSim(False, False)
-    s.add_material(kn, ks, mu, e)
+    if zeta is None:
+        s.add_material(kn, ks, mu, e)
+    else:
+        s.add_material_damping(kn, ks, mu, zeta)
     t = gen.sphere_template(r, n_grid=13, n_nodes=400)
     t.nodes = np.array([node], np.float32)
     s.add_geometry(m, [I, I, I, 0, 0, 0], t)
-    s.add_wall_material(mu, e)
+    if zeta is None:
+        s.add_wall_material(mu, e)
+    else:
+        s.add_wall_material_damping(mu, zeta)
     s.add_wall_geometry(gen.plane_wall([-0.5, -0.5], [0.5, 0.5]))
     s.add_particle([0, 0, r + 1e-5], v0, [0, 0, 0], [1, 0, 0, 0], 0, 0)
     s.add_wall([0, 0, 0], [0, 0, 0], [0, 0, 0], [1, 0, 0, 0], 0, 0)
@@ -93,11 +99,15 @@ def _sphere_probe(e, kn, mu=0.0, ks=0.0, v0=(0, 0, -1.0), dt=1e-6, node=(0, 0, -
     return s, m
 
 
-@pytest.mark.parametrize("e", [0.3, 0.6, 0.9])
-def test_single_contact_restitution_and_duration(e):
-    """Linear spring-dashpot: rebound ratio e and contact time pi/omega_d."""
+@pytest.mark.parametrize("e,damping", [(0.3, False), (0.6, False), (0.9, False), (0.3, True), (0.6, True)])
+def test_single_contact_restitution_and_duration(e, damping):
+    """Linear spring-dashpot: rebound ratio e and contact time pi/omega_d,
+    for a restitution material and for a damping-ratio material with the
+    matching zeta = -ln e / sqrt(ln^2 e + pi^2) (whose dashpot then gives
+    back e)."""
     kn = 1e5
-    s, m = _sphere_probe(e, kn)
+    z = -math.log(e) / math.sqrt(math.log(e) ** 2 + math.pi ** 2)
+    s, m = _sphere_probe(e, kn, zeta=z if damping else None)
     started, n_in = False, 0
     for _ in range(20000):
         s.step(1)
