@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../include/fundem/LSAdjoint.h"
+#include "../include/fundem/LSAutotune.h"
 #include "../include/fundem/LSSolver.h"
 
 namespace {
@@ -78,6 +79,31 @@ int fdmx_num_materials(void* s, int* particle, int* wall) {
     auto* S = static_cast<BoundSolver*>(s);
     *particle = (int)S->getLSParticle().numMaterial();
     *wall = (int)S->getWall().numMaterial();
+  });
+}
+
+// Autotuner (fundem::LSAutotuner): out7 = skin, cell size, sweep waves,
+// sweep waves per CU, law block, reduce block, ms per step; log receives
+// one line per timed candidate (truncated at logCap).
+int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log, int logCap) {
+  return guarded([&] {
+    fundem::LSAutotuner t(*static_cast<BoundSolver*>(s));
+    const fundem::TuneChoice c = t.run((size_t)std::max(trialSteps, 4), tuneCell != 0);
+    if (out7) {
+      out7[0] = c.skin;
+      out7[1] = c.cellSize;
+      out7[2] = c.launch.sweepWaves;
+      out7[3] = c.launch.sweepWavesPerCU;
+      out7[4] = c.launch.lawBlock;
+      out7[5] = c.launch.reduceBlock;
+      out7[6] = c.msPerStep;
+    }
+    if (log && logCap > 0) {
+      std::string all;
+      for (const auto& l : c.trials) all += l + "\n";
+      std::strncpy(log, all.c_str(), (size_t)logCap - 1);
+      log[logCap - 1] = 0;
+    }
   });
 }
 

@@ -1050,23 +1050,25 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
   }
 }
 
-// One wave per kept pair of the work list.
-// One wave per kept pair; one-wave workgroups, so a finished wave frees its
-// slot at once.
-__global__ void __launch_bounds__(64) kNarrowSweep(NarrowMulti a, NarrowScratch x, const int* work,
-                                                   const int* workCount) {
+// One wave per kept pair of the work list, WAVES waves per workgroup (one
+// wave per workgroup frees a finished wave's slot at once; more waves per
+// workgroup launch fewer workgroups).
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) kNarrowSweep(NarrowMulti a, NarrowScratch x,
+                                                           const int* work, const int* workCount) {
   if (gateOff(a.gate)) return;
-  __shared__ HitSlots slots[1];
+  __shared__ HitSlots slots[WAVES];
   const int lane = laneId();
+  const int wave = threadIdx.x >> 6;
   const int total = *workCount;
-  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+  for (int w = blockIdx.x * WAVES + wave; w < total; w += gridDim.x * WAVES) {
     const int code = work[w];
     const NarrowSet& s = a.s[code & 7];
     const int p = code >> 3;
     const int pool = w % FDM_REC_POOLS;
-    if (s.kind == FDM_SET_PARTICLE) sweepPair<FDM_SET_PARTICLE>(a.c, s, x, w, p, lane, slots[0], pool);
-    else if (s.kind == FDM_SET_WALL) sweepPair<FDM_SET_WALL>(a.c, s, x, w, p, lane, slots[0], pool);
-    else sweepPair<FDM_SET_GHOST>(a.c, s, x, w, p, lane, slots[0], pool);
+    if (s.kind == FDM_SET_PARTICLE) sweepPair<FDM_SET_PARTICLE>(a.c, s, x, w, p, lane, slots[wave], pool);
+    else if (s.kind == FDM_SET_WALL) sweepPair<FDM_SET_WALL>(a.c, s, x, w, p, lane, slots[wave], pool);
+    else sweepPair<FDM_SET_GHOST>(a.c, s, x, w, p, lane, slots[wave], pool);
   }
 }
 
@@ -1089,6 +1091,13 @@ __global__ void kResetCounter(int* c) {
   for (int i = threadIdx.x; i < FDM_REC_COUNTERS; i += blockDim.x) c[i] = 0;
 }
 
+}  // namespace fdm
+
+namespace fdm {
+namespace {
+fdm_tuning g_tuning = {1, 32, 256, 256};
+}
+const fdm_tuning& tuning() { return g_tuning; }
 }  // namespace fdm
 
 using namespace fdm;
@@ -1276,9 +1285,36 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   const unsigned cullBlocks = std::max<unsigned>(1, std::min<unsigned>(blocksFor(cap, 256), 2048));
   hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, ws->workList,
                      ws->workCount);
-  hipLaunchKernelGGL(kNarrowSweep, dim3(persistentBlocks(1)), dim3(64), 0, stream, k, x,
-                     (const int*)ws->workList, (const int*)ws->workCount);
-  hipLaunchKernelGGL(kContactLaw, dim3(persistentBlocks(4)), dim3(256), 0, stream, k, x,
-                     (const int*)ws->workList, (const int*)ws->workCount);
+  const fdm_tuning& tu = tuning();
+  const int sweepBlocks = std::max(1, persistentBlocks(tu.sweepWaves) * tu.sweepWavesPerCU / 32);
+  const int* wl = ws->workList;
+  const int* wc = ws->workCount;
+  if (tu.sweepWaves == 4)
+    hipLaunchKernelGGL(kNarrowSweep<4>, dim3(sweepBlocks), dim3(256), 0, stream, k, x, wl, wc);
+  else if (tu.sweepWaves == 2)
+    hipLaunchKernelGGL(kNarrowSweep<2>, dim3(sweepBlocks), dim3(128), 0, stream, k, x, wl, wc);
+  else
+    hipLaunchKernelGGL(kNarrowSweep<1>, dim3(sweepBlocks), dim3(64), 0, stream, k, x, wl, wc);
+  hipLaunchKernelGGL(kContactLaw, dim3(persistentBlocks(tu.lawBlock / 64)), dim3(tu.lawBlock), 0,
+                     stream, k, x, wl, wc);
   return launchStatus();
+}
+
+extern "C" int fdm_setTuning(const fdm_tuning* t) {
+  if (!t) return FDM_EINVAL;
+  fdm_tuning n = g_tuning;
+  if (t->sweepWaves) n.sweepWaves = t->sweepWaves;
+  if (t->sweepWavesPerCU) n.sweepWavesPerCU = t->sweepWavesPerCU;
+  if (t->lawBlock) n.lawBlock = t->lawBlock;
+  if (t->reduceBlock) n.reduceBlock = t->reduceBlock;
+  if (!(n.sweepWaves == 1 || n.sweepWaves == 2 || n.sweepWaves == 4)) return FDM_EINVAL;
+  if (n.sweepWavesPerCU < 4 || n.sweepWavesPerCU > 128) return FDM_EINVAL;
+  if (!(n.lawBlock == 64 || n.lawBlock == 128 || n.lawBlock == 256)) return FDM_EINVAL;
+  if (n.reduceBlock < 64 || n.reduceBlock > 1024 || n.reduceBlock % 64) return FDM_EINVAL;
+  g_tuning = n;
+  return FDM_OK;
+}
+
+extern "C" void fdm_getTuning(fdm_tuning* t) {
+  if (t) *t = g_tuning;
 }

@@ -40,6 +40,8 @@ struct Carve {
 
 // Number of persistent workgroups for grid-stride kernels.
 int persistentBlocks(int wavesPerBlock);
+// Launch shapes chosen by fdm_setTuning (contact.hip).
+const fdm_tuning& tuning();
 
 // A device flag that switches a kernel off: the kernel returns at once unless
 // the flag is null or *flag != 0. Steps captured in a graph launch every

@@ -250,6 +250,7 @@ extern "C" int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStrea
     if (g.stepCounter) hipLaunchKernelGGL(kIncrement, dim3(1), dim3(64), 0, stream, g.stepCounter);
     return launchStatus();
   }
-  hipLaunchKernelGGL(kReduce, dim3(blocksFor(g.particles.n, 256)), dim3(256), 0, stream, k);
+  const int rb = tuning().reduceBlock;
+  hipLaunchKernelGGL(kReduce, dim3(blocksFor(g.particles.n, rb)), dim3(rb), 0, stream, k);
   return launchStatus();
 }

@@ -190,6 +190,18 @@ class LSAdjoint {
   size_t steps() const { return steps_; }
   size_t checkpoints() const { return checkpoints_.size(); }
 
+  // A point the solver can return to: mark() saves its state and contacts,
+  // rollback() restores them (with a forced list rebuild), after which the
+  // solver continues exactly as it would have from the mark.
+  void mark() {
+    if (slots_.empty()) {
+      every_ = 1;
+      setup();
+    }
+    save(mark_);
+  }
+  void rollback() { restore(mark_); }
+
  private:
   struct StateBuf {
     DeviceBuffer<double> x, wx;
@@ -431,7 +443,7 @@ class LSAdjoint {
   long long pairCap_ = 0;
   std::array<fdm_pairs, 5> setBuf_{};
   std::vector<Checkpoint> checkpoints_;
-  Checkpoint final_;
+  Checkpoint final_, mark_;
   std::vector<StateBuf> slots_;
   TapeBuf tape_, staging_;
   DeviceBuffer<char> tapeWs_, adjWs_;

@@ -32,9 +32,11 @@ inline void fdmCheck(int r, const char* what) {
 }
 
 class LSAdjoint;
+class LSAutotuner;
 
 class LSSolver {
   friend class LSAdjoint;
+  friend class LSAutotuner;
 
  public:
   explicit LSSolver(const std::string dir = "Problem", hipStream_t stream = 0,
@@ -246,6 +248,8 @@ class LSSolver {
     }
     gravity_ = gravity;
     dt_ = timeStep;
+    domainMin_ = minDomain;
+    domainMax_ = maxDomain;
     if (systems_.empty()) {
       particles_.initialize(minDomain, maxDomain, cellSize_, stream_);
       walls_.initialize(minDomain, maxDomain, 0.0, stream_);
@@ -1054,6 +1058,7 @@ class LSSolver {
   int hitCapacity_ = 0;
   DeviceBuffer<double> disp_;
   double skinRequest_ = -1.0, skin_ = 0.0;
+  double3 domainMin_{}, domainMax_{};
   double maxRadius_ = 0.0, maxBoxReach_ = 0.0, rhoMax_ = 0.0;
   bool useGraph_ = false, timing_ = false;
   bool baseHookCalled_ = false;

@@ -44,6 +44,7 @@ def solver_lib():
         "fdmx_add_material_damping": (I, [P, D, D, D, D]),
         "fdmx_add_wall_material_damping": (I, [P, D, D]),
         "fdmx_num_materials": (I, [P, C.POINTER(I), C.POINTER(I)]),
+        "fdmx_autotune": (I, [P, I, I, _dp, C.c_char_p, I]),
         "fdmx_adjoint_create": (P, [P]),
         "fdmx_adjoint_destroy": (None, [P]),
         "fdmx_adjoint_set_contacts": (I, [P, I]),
@@ -135,6 +136,17 @@ class Solver:
 
     def add_wall_material_damping(self, mu, zeta):
         self._ok(self._lib.fdmx_add_wall_material_damping(self.h, mu, zeta))
+
+    def autotune(self, trial_steps=20, tune_cell=True):
+        """Pick skin, cell size and launch shapes for the fastest step on this
+        system (results do not change); the solver is left where it was."""
+        out = np.zeros(7)
+        buf = C.create_string_buffer(16384)
+        self._ok(self._lib.fdmx_autotune(self.h, int(trial_steps), int(bool(tune_cell)), _dp_(out), buf,
+                                         len(buf)))
+        return dict(skin=out[0], cell_size=out[1], sweep_waves=int(out[2]),
+                    sweep_waves_per_cu=int(out[3]), law_block=int(out[4]), reduce_block=int(out[5]),
+                    ms_per_step=out[6], trials=buf.value.decode().splitlines())
 
     def num_materials(self):
         a, b = C.c_int(0), C.c_int(0)

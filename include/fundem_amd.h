@@ -544,6 +544,19 @@ int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream)
  * correctly inside captured graphs). */
 int fdm_fillInt(int* ptr, size_t n, int value, hipStream_t stream);
 
+/* No counterpart in the reference: launch shapes, picked by the autotuner
+ * (fundem::LSAutotuner). Results do not depend on them. In fdm_setTuning a
+ * zero field keeps the current value. The setting is process-wide; graphs
+ * captured before a change keep the shapes they were captured with. */
+typedef struct {
+  int sweepWaves;       /* node-sweep waves (kept pairs in flight) per workgroup: 1, 2 or 4 */
+  int sweepWavesPerCU;  /* node-sweep grid size, in waves per CU: 4..128 (default 32) */
+  int lawBlock;         /* contact-law threads per workgroup: 64, 128 or 256 */
+  int reduceBlock;      /* reduction threads per workgroup: a multiple of 64 up to 1024 */
+} fdm_tuning;
+int fdm_setTuning(const fdm_tuning* t);
+void fdm_getTuning(fdm_tuning* t);
+
 /* Device properties used by the launchers (number of CUs); 0 on success. */
 int fdm_deviceInfo(int* numCU, int* warpSize);
 
