@@ -45,6 +45,9 @@ def parse():
                    help="candidate-list skin in m (negative: a tenth of the largest radius)")
     p.add_argument("--copy-probe", action="store_true",
                    help="first run a plain 1 GiB device copy (counter scaling under rocprofv3)")
+    p.add_argument("--autotune", type=int, default=1,
+                   help="pick skin, cell size and launch shapes with the autotuner first (results "
+                        "do not change)")
     p.add_argument("--settle", type=int, default=20000,
                    help="column: plain steps that pack the column before the timed runs")
     p.add_argument("--checkpoint", type=int, default=100, help="column: steps per checkpoint")
@@ -132,6 +135,22 @@ def copy_probe():
     log("copy probe: 3 copies of 1 GiB")
 
 
+def autotune(s, a):
+    """Run the autotuner from the solver's current state (it returns the
+    solver to that state) and report its choice."""
+    if not a.autotune:
+        return None
+    t0 = time.perf_counter()
+    c = s.autotune(trial_steps=20)
+    for line in c["trials"]:
+        log("  tune " + line)
+    log(f"autotuned in {time.perf_counter() - t0:.1f} s: skin {c['skin']:.3e} m, cell "
+        f"{c['cell_size']:.3e} m (0: the cell rule), sweep {c['sweep_waves']} waves/workgroup, "
+        f"{c['sweep_waves_per_cu']} waves/CU, law block {c['law_block']}, reduce block "
+        f"{c['reduce_block']} ({c['ms_per_step']:.3f} ms/step in the trials)")
+    return {k: (float(v) if not isinstance(v, int) else v) for k, v in c.items() if k != "trials"}
+
+
 def run_steps(s, steps, warmup, graph):
     s.use_graph(bool(graph))
     s.step(1)
@@ -160,12 +179,17 @@ def batch_main(a):
     t0 = time.perf_counter()
     W.apply_batch(scns, s)
     log(f"batch uploaded and initialised in {time.perf_counter() - t0:.1f} s")
+    s.use_graph(bool(a.graph))
+    s.step(1 + a.warmup)
+    tuning = autotune(s, a)
     s.set_timing(not a.graph)
-    elapsed = run_steps(s, a.steps, a.warmup, a.graph)
+    b0, _, _ = s.list_builds()
+    elapsed = run_steps(s, a.steps, 0, a.graph)
     if s.status() != 0:
         raise RuntimeError(f"capacity exceeded in the batch (status {s.status()})")
     batched = n_total * a.steps / elapsed
     b, gb, skin = s.list_builds()
+    b -= b0
     log(f"batched: {1e3 * elapsed / a.steps:.3f} ms/step, {batched:.3e} particle-steps/s; "
         f"candidate pairs {s.pair_totals()}, {b} list builds")
     if not a.graph:
@@ -198,7 +222,7 @@ def batch_main(a):
                                f"(log-uniform, {n_total} in total), 16 shared templates (32^3 "
                                "SDF, 1000 nodes), per-packing kn in [5e3, 2e4], damping ratio "
                                "in [0.1, 0.5], mu in [0.2, 0.8], dt 5e-7 s",
-                   "graph": bool(a.graph), "skin_m": skin, "list_builds": b},
+                   "graph": bool(a.graph), "skin_m": skin, "list_builds": b, "tuning": tuning},
         "sequential": {"value": sequential, "unit": "particle-steps/s",
                        "sample": f"{len(pick)} of the {len(scns)} packings (seeded choice), each "
                                  f"in a solver of its own, {a.steps} steps each, one after another"},
@@ -305,6 +329,7 @@ def main():
     s.step(1)
     s.step(a.warmup)
     s.sync()
+    tuning = autotune(s, a)
     if s.status() != 0:
         raise RuntimeError(f"capacity exceeded during warm-up (status {s.status()})")
     s.set_timing(not a.graph)
@@ -355,7 +380,7 @@ def main():
                                "positions, FP32 v/w/q/F/T, FP64 arithmetic",
                    "particles": n, "contacts_last_step": contacts, "candidate_pairs": pairs,
                    "graph": bool(a.graph), "skin_m": skin, "list_builds": b1 - b0,
-                   "ghost_list_builds": gb1 - gb0},
+                   "ghost_list_builds": gb1 - gb0, "tuning": tuning},
         "step_hbm_fraction": step_bytes * a.steps / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
     }

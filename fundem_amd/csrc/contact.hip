@@ -1059,7 +1059,8 @@ __global__ void __launch_bounds__(64 * WAVES) kNarrowSweep(NarrowMulti a, Narrow
   if (gateOff(a.gate)) return;
   __shared__ HitSlots slots[WAVES];
   const int lane = laneId();
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform (scalar), so the pair's data loads stay scalar loads
+  const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = *workCount;
   for (int w = blockIdx.x * WAVES + wave; w < total; w += gridDim.x * WAVES) {
     const int code = work[w];

@@ -12,9 +12,11 @@
 // Each candidate is timed from the same saved point (LSAdjoint::mark /
 // rollback) one knob at a time, keeping the best value of each knob before
 // the next; the solver is left at the saved point with the best settings.
-// The cost of a candidate is the time of a list-building step plus the
-// time of kept-list steps, amortised over the rebuild interval that the
-// measured motion gives for its skin.
+// The cost of a candidate is its mean step time from the saved point (one
+// forced list build, then the solver's own decisions), the better of two
+// timings; the skin is timed over a longer window that holds real list
+// rebuilds. A candidate replaces the current choice only when it is 2 %
+// faster.
 #pragma once
 #include <chrono>
 #include <cstdio>
@@ -46,44 +48,53 @@ class LSAutotuner {
     fdm_getTuning(&best.launch);
     best.skin = s_.skin_;
     best.cellSize = s_.cellSize_;
-    double bestCost = cost(trialSteps, best, "start");
+    // the skin trades list builds for longer lists: time it over a window
+    // that holds real rebuilds
+    const size_t window = std::max<size_t>(5 * trialSteps, 60);
     const double R = s_.maxRadius_;
-    for (double f : {0.0, 0.05, 0.1, 0.2, 0.35}) {
+    std::vector<TuneChoice> group;
+    for (double f : {0.05, 0.1, 0.2, 0.35}) {
       TuneChoice c = best;
       c.skin = f * R;
-      tryCandidate(c, trialSteps, best, bestCost, "skin");
+      group.push_back(c);
     }
+    double bestCost = pick(group, window, best, "skin");
     const bool cellOk = tuneCellSize && s_.systems_.empty() && !s_.periodic_.isActived() &&
                         s_.bonds_.view().n == 0;
     if (cellOk) {
       const fdm_grid g = s_.particles_.grid();
       const double base = s_.cellSize_ > 0.0 ? s_.cellSize_ : 1.0 / g.invCell.x;
+      group.clear();
       for (double f : {1.25, 1.5, 2.0}) {
         TuneChoice c = best;
         c.cellSize = f * base;
-        tryCandidate(c, trialSteps, best, bestCost, "cell");
+        group.push_back(c);
       }
+      bestCost = pick(group, window, best, "cell");
     }
-    for (int w : {1, 2, 4}) {
-      TuneChoice c = best;
-      c.launch.sweepWaves = w;
-      tryCandidate(c, trialSteps, best, bestCost, "sweepWaves");
-    }
-    for (int w : {16, 32, 64}) {
-      TuneChoice c = best;
-      c.launch.sweepWavesPerCU = w;
-      tryCandidate(c, trialSteps, best, bestCost, "sweepWavesPerCU");
-    }
+    group.clear();
+    for (int w : {1, 2, 4})
+      for (int pc : {16, 32, 64}) {
+        TuneChoice c = best;
+        c.launch.sweepWaves = w;
+        c.launch.sweepWavesPerCU = pc;
+        group.push_back(c);
+      }
+    bestCost = pick(group, trialSteps, best, "sweep");
+    group.clear();
     for (int b : {64, 128, 256}) {
       TuneChoice c = best;
       c.launch.lawBlock = b;
-      tryCandidate(c, trialSteps, best, bestCost, "lawBlock");
+      group.push_back(c);
     }
+    bestCost = pick(group, trialSteps, best, "lawBlock");
+    group.clear();
     for (int b : {128, 256, 512}) {
       TuneChoice c = best;
       c.launch.reduceBlock = b;
-      tryCandidate(c, trialSteps, best, bestCost, "reduceBlock");
+      group.push_back(c);
     }
+    bestCost = pick(group, trialSteps, best, "reduceBlock");
     apply(best);
     snap_.rollback();
     best.msPerStep = bestCost;
@@ -103,13 +114,24 @@ class LSAutotuner {
       s_.skin_ = c.skin;  // initialize derives the skin from the request, which is c.skin
     }
   }
-  void tryCandidate(const TuneChoice& c, size_t steps, TuneChoice& best, double& bestCost,
-                    const char* knob) {
-    const double t = cost(steps, c, knob);
-    if (t < bestCost) {
-      bestCost = t;
-      best = c;
+  // the current choice against a group of candidates, each timed twice
+  // (the smaller time counts); a candidate must be 2 % faster to be taken
+  double pick(const std::vector<TuneChoice>& group, size_t steps, TuneChoice& best, const char* knob) {
+    double bestCost = std::min(cost(steps, best, knob), cost(steps, best, knob));
+    TuneChoice choice = best;
+    double choiceCost = bestCost;
+    for (const TuneChoice& c : group) {
+      const double t = std::min(cost(steps, c, knob), cost(steps, c, knob));
+      if (t < choiceCost) {
+        choiceCost = t;
+        choice = c;
+      }
     }
+    if (choiceCost < 0.98 * bestCost) {
+      best = choice;
+      bestCost = choiceCost;
+    }
+    return bestCost;
   }
   // ms per step of a candidate, amortising its list builds
   double cost(size_t steps, const TuneChoice& c, const char* knob) {
@@ -137,26 +159,18 @@ class LSAutotuner {
     const double build = std::chrono::duration<double, std::milli>(t1 - t0).count();
     const double rest = std::chrono::duration<double, std::milli>(t2 - t1).count() / double(steps - 1);
     double total;
-    char line[256];
+    char line[320];
     if (status != 0) {
       total = std::numeric_limits<double>::infinity();
       hipCheck(hipMemsetAsync(s_.status_.get(), 0, sizeof(unsigned), st), "tune");
-    } else if (c.skin <= 0.0) {
-      total = (build + rest * double(steps - 1)) / double(steps);
     } else {
-      // steps until the displacement bound reaches half the skin, at the
-      // motion measured since the last build of the trial
-      const long kept = std::max<long>(1, (long)(steps - 1) - (long)(b1 - b0));
-      const double perStep = disp / double(kept);
-      const double interval = perStep > 0.0 ? std::max(1.0, 0.5 * c.skin / perStep) : 1e6;
-      total = (build + rest * (interval - 1.0)) / interval;
-      if (b1 > b0) total = (build + rest * double(steps - 1)) / double(steps);
+      total = (build + rest * double(steps - 1)) / double(steps);
     }
     std::snprintf(line, sizeof line,
-                  "%-16s skin %.3e cell %.3e sweep %d/%d law %d reduce %d: build %.3f ms, step %.3f ms"
-                  " -> %.4f ms/step",
+                  "%-12s skin %.3e cell %.3e sweep %d/%d law %d reduce %d: first %.3f ms, then %.3f "
+                  "ms/step, %ld builds, displacement bound %.2e -> %.4f ms/step",
                   knob, c.skin, c.cellSize, c.launch.sweepWaves, c.launch.sweepWavesPerCU,
-                  c.launch.lawBlock, c.launch.reduceBlock, build, rest, total);
+                  c.launch.lawBlock, c.launch.reduceBlock, build, rest, (long)(b1 - b0), disp, total);
     log_.push_back(line);
     return total;
   }
