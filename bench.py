@@ -12,9 +12,10 @@ the device's list-rebuild decision, list builds when needed (sort, broad
 phase, ghost sets) or the kept lists, the narrow phase with contact forces
 for the particle, wall and ghost sets, the reduction and the second kick.
 The timed region covers exactly --steps steps with the device synchronised
-on both sides. Every stage is bracketed by HIP events on the solver's
-stream (the stream its kernels run on); their average per launch gives the
-roofline of the dominant kernel. The CPU baseline is the serial FP64 oracle
+on both sides and no instrumentation. A separate pass of --stage-steps
+steps brackets every stage with HIP events on the solver's stream (the
+stream its kernels run on); their average per launch gives the roofline of
+the dominant kernel. The CPU baseline is the serial FP64 oracle
 on a 512-particle sample of the same workload.
 
 Prints one JSON line.
@@ -45,6 +46,8 @@ def parse():
                    help="candidate-list skin in m (negative: a tenth of the largest radius)")
     p.add_argument("--copy-probe", action="store_true",
                    help="first run a plain 1 GiB device copy (counter scaling under rocprofv3)")
+    p.add_argument("--stage-steps", type=int, default=200,
+                   help="pour: steps of the separate stage-timed pass (roofline)")
     p.add_argument("--autotune", type=int, default=1,
                    help="pick skin, cell size and launch shapes with the autotuner first (results "
                         "do not change)")
@@ -182,7 +185,6 @@ def batch_main(a):
     s.use_graph(bool(a.graph))
     s.step(1 + a.warmup)
     tuning = autotune(s, a)
-    s.set_timing(not a.graph)
     b0, _, _ = s.list_builds()
     elapsed = run_steps(s, a.steps, 0, a.graph)
     if s.status() != 0:
@@ -192,9 +194,14 @@ def batch_main(a):
     b -= b0
     log(f"batched: {1e3 * elapsed / a.steps:.3f} ms/step, {batched:.3e} particle-steps/s; "
         f"candidate pairs {s.pair_totals()}, {b} list builds")
-    if not a.graph:
-        for k, (ms, calls) in sorted(s.stage_times().items(), key=lambda kv: -kv[1][0]):
-            log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches")
+    # stage timing in a separate eager pass
+    s.use_graph(False)
+    s.set_timing(True)
+    s.step(min(a.stage_steps, 50))
+    s.sync()
+    for k, (ms, calls) in sorted(s.stage_times().items(), key=lambda kv: -kv[1][0]):
+        log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches (stage-timed pass)")
+    s.set_timing(False)
     del s
     rng = np.random.default_rng(17)
     pick = sorted(rng.choice(len(scns), size=min(a.batch_sample, len(scns)), replace=False))
@@ -332,7 +339,6 @@ def main():
     tuning = autotune(s, a)
     if s.status() != 0:
         raise RuntimeError(f"capacity exceeded during warm-up (status {s.status()})")
-    s.set_timing(not a.graph)
     s.sync()
     b0, gb0, skin = s.list_builds()
     t0 = time.perf_counter()
@@ -343,7 +349,14 @@ def main():
     status = s.status()
     if status != 0:
         raise RuntimeError(f"capacity exceeded during the timed steps (status {status})")
-    stages = s.stage_times() if not a.graph else {}
+    # stage timing (HIP events around every stage) in a separate eager pass,
+    # so that the timed steps above carry no instrumentation
+    s.use_graph(False)
+    s.set_timing(True)
+    s.step(a.stage_steps)
+    s.sync()
+    stages = s.stage_times()
+    s.set_timing(False)
     contacts = {k: int(s.contacts(k)["A"].size) for k in range(5)}
     pairs = dict(enumerate(s.pair_totals()))
     n = scn.n
@@ -354,7 +367,7 @@ def main():
         f"{gb1 - gb0} ghost-list builds in {a.steps} steps")
     for k, (ms, calls) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
         log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches  "
-            f"{ms / a.steps * 1e3:8.1f} us/step")
+            f"{ms / a.stage_steps * 1e3:8.1f} us/step (stage-timed pass)")
 
     roof = None
     modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}

@@ -353,8 +353,10 @@ __device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const Narr
 // The reference's node predicate for one (node, target) candidate: the
 // node's cell is in the target's stencil, the node is within RB of a
 // particle target, inside the target's SDF box, and phi <= 0 there. The
-// conjunction is evaluated with the cheapest and most selective tests first
-// (the cell test divides); `tcell` is the target's decoded cell.
+// conjunction is evaluated with the cheapest and most selective tests first:
+// the sphere, the SDF box, phi, and last the cell test (three divisions),
+// which then runs for the few nodes inside the target; `tcell` is the
+// target's decoded cell.
 template <int SET>
 __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const fdm_grid& g,
                                         double RB, Sample& s) {
@@ -365,11 +367,11 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
   }
   int base;
   if (!sdfLocate(qrotateT(rel, tg.q), *tg.t, s, base)) return false;
-  const I3 c = clampedCell(p, g);
-  if (abs(tcell.x - c.x) > 1 || abs(tcell.y - c.y) > 1 || abs(tcell.z - c.z) > 1) return false;
   sdfCorners(tg.sdf + base, *tg.t, s);
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
-  return ov >= 0.;
+  if (!(ov >= 0.)) return false;
+  const I3 c = clampedCell(p, g);
+  return abs(tcell.x - c.x) <= 1 && abs(tcell.y - c.y) <= 1 && abs(tcell.z - c.z) <= 1;
 }
 
 __device__ __forceinline__ int findPrevPair(const NarrowSet& a, int p, int A, int T, int& off) {
