@@ -82,6 +82,35 @@ int fdmx_num_materials(void* s, int* particle, int* wall) {
   });
 }
 
+// Contact geometry of the latest step's contacts of a set (host recomputed):
+// returns the count (outputs truncated at cap) or -1.
+long long fdmx_contact_geometry(void* s, int set, int* A, int* T, int* k, double* point,
+                                double* normal, double* overlap, long long cap) {
+  long long n = -1;
+  guarded([&] {
+    const auto c = static_cast<BoundSolver*>(s)->contactGeometry(set);
+    n = (long long)c.size();
+    for (long long i = 0; i < n && i < cap; ++i) {
+      if (A) A[i] = c[i].A;
+      if (T) T[i] = c[i].target;
+      if (k) k[i] = c[i].node;
+      if (point) { point[3 * i] = c[i].point.x; point[3 * i + 1] = c[i].point.y; point[3 * i + 2] = c[i].point.z; }
+      if (normal) { normal[3 * i] = c[i].normal.x; normal[3 * i + 1] = c[i].normal.y; normal[3 * i + 2] = c[i].normal.z; }
+      if (overlap) overlap[i] = c[i].overlap;
+    }
+  });
+  return n;
+}
+int fdmx_create_bonds_from_contacts(void* s, double R, double L, double E, double nu, double st,
+                                    double coh, double mub) {
+  return guarded([&] {
+    static_cast<BoundSolver*>(s)->createBondsFromLSInteractions(R, L, E, nu, st, coh, mub);
+  });
+}
+long long fdmx_num_bonds(void* s) {
+  return (long long)static_cast<BoundSolver*>(s)->getLSParticleBondedInteraction().numPair();
+}
+
 // Autotuner (fundem::LSAutotuner): out7 = skin, cell size, sweep waves,
 // sweep waves per CU, law block, reduce block, ms per step; log receives
 // one line per timed candidate (truncated at logCap).

@@ -417,6 +417,9 @@ struct LSParticle {
   size_t numMaterial() const { return materials_.size(); }
   const fdm_template& templateInfo(size_t g) const { return templates_[g]; }
   const std::vector<double3>& templateNodes(size_t g) const { return nodeHost_[g]; }
+  // concatenated FP32 SDF values and node triples, as uploaded
+  const std::vector<float>& sdfHost() const { return sdf_; }
+  const std::vector<float>& nodesHost() const { return nodes_; }
   long long totalNodes() const {
     long long s = 0;
     for (int g : geometryID_) s += templates_[g].nodeCount;

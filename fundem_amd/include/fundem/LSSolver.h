@@ -400,6 +400,89 @@ class LSSolver {
     return {};
   }
 
+  // One node-contact of the latest step with its geometry.
+  struct ContactGeometry {
+    int A, target, node;
+    double3 point, normal;
+    double overlap;
+    double3 spring;
+  };
+  // Contact points, normals and overlaps of the latest step's contacts of a
+  // set (0 particle, 1 wall, 2/3/4 ghost X/Y/XY), recomputed on the host from
+  // the downloaded state and the templates with the detection kernels'
+  // arithmetic (LSParticleContactDetectionKernel.cu:128-305): the node in the
+  // target's frame, the trilinear value and int-x gradient of its SDF cell,
+  // the normal and the point p + n delta / 2. The reference keeps these per
+  // contact (SolidInteraction contactPoint / contactNormal / overlap).
+  // Synchronises.
+  std::vector<ContactGeometry> contactGeometry(int set) {
+    download();
+    const std::vector<ContactRecord> recs = contacts(set);
+    std::vector<ContactGeometry> out;
+    out.reserve(recs.size());
+    const auto& px = particles_.positionHostRef();
+    const auto& pq = particles_.orientationHostRef();
+    const auto& geo = particles_.geometryIDHostRef();
+    const bool wall = set == FDM_SET_WALL;
+    LSParticle& tb = wall ? walls_ : particles_;
+    for (const ContactRecord& r : recs) {
+      const fdm_template& tA = particles_.templateInfo(geo[r.A]);
+      const float* nd = particles_.nodesHost().data() + 3 * ((size_t)tA.nodeStart + r.node);
+      const double3 s = make_double3(nd[0], nd[1], nd[2]);
+      const double3 p = vec::add(vec::rotate(pq[r.A], s), px[r.A]);
+      double3 xB = tb.positionHostRef()[r.target];
+      const quaternion qB = tb.orientationHostRef()[r.target];
+      if (set >= FDM_SET_GHOST) {
+        const fdm_grid& g = systems_.empty() ? particles_.grid() : sysGrid_[particles_.systemOfBody()[r.target]];
+        const int mask = set - FDM_SET_GHOST + 1;
+        if (mask & 1) xB.x += g.max.x - g.min.x;
+        if (mask & 2) xB.y += g.max.y - g.min.y;
+      }
+      const fdm_template& tB = tb.templateInfo(tb.geometryIDHostRef()[r.target]);
+      const double3 l = vec::rotateT(vec::sub(p, xB), qB);
+      const double gx = (l.x - tB.origin[0]) * tB.invSpacing, gy = (l.y - tB.origin[1]) * tB.invSpacing,
+                   gz = (l.z - tB.origin[2]) * tB.invSpacing;
+      const int i0 = (int)std::floor(gx), j0 = (int)std::floor(gy), k0 = (int)std::floor(gz);
+      ContactGeometry c{r.A, r.target, r.node, make_double3(0, 0, 0), make_double3(0, 0, 0), 0.0, r.spring};
+      if (i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 < tB.size[0] - 1 && j0 < tB.size[1] - 1 && k0 < tB.size[2] - 1) {
+        const double x = gx - (double)i0, y = gy - (double)j0, z = gz - (double)k0;
+        const int nx = tB.size[0], nxy = tB.size[0] * tB.size[1];
+        const float* b = tb.sdfHost().data() + tB.sdfStart + ((size_t)k0 * nxy + (size_t)j0 * nx + i0);
+        const double cc[8] = {b[0], b[1], b[nx], b[nx + 1], b[nxy], b[nxy + 1], b[nxy + nx], b[nxy + nx + 1]};
+        const double wx0 = 1.0 - x, wx1 = x, wy0 = 1.0 - y, wy1 = y, wz0 = 1.0 - z, wz1 = z;
+        const double phi = cc[0] * wx0 * wy0 * wz0 + cc[1] * wx1 * wy0 * wz0 + cc[2] * wx0 * wy1 * wz0 +
+                           cc[3] * wx1 * wy1 * wz0 + cc[4] * wx0 * wy0 * wz1 + cc[5] * wx1 * wy0 * wz1 +
+                           cc[6] * wx0 * wy1 * wz1 + cc[7] * wx1 * wy1 * wz1;
+        // the int-x gradient weights (LSParticleContactDetectionKernel.cuh:34-77)
+        const double ix0 = 1.0, ix1 = 0.0;
+        const double3 grad = make_double3(
+            (cc[1] - cc[0]) * wy0 * wz0 + (cc[3] - cc[2]) * wy1 * wz0 + (cc[5] - cc[4]) * wy0 * wz1 +
+                (cc[7] - cc[6]) * wy1 * wz1,
+            (cc[2] - cc[0]) * ix0 * wz0 + (cc[3] - cc[1]) * ix1 * wz0 + (cc[6] - cc[4]) * ix0 * wz1 +
+                (cc[7] - cc[5]) * ix1 * wz1,
+            (cc[4] - cc[0]) * ix0 * wy0 + (cc[5] - cc[1]) * ix1 * wy0 + (cc[6] - cc[2]) * ix0 * wy1 +
+                (cc[7] - cc[3]) * ix1 * wy1);
+        c.overlap = -phi;
+        c.normal = vec::unit(vec::rotate(qB, grad));
+        c.point = vec::add(p, vec::mul(c.normal, 0.5 * c.overlap));
+      }
+      out.push_back(c);
+    }
+    return out;
+  }
+
+  // LSSolver::createBondsFromLSInteractions (LSSolver.h:347-380): one bond per
+  // particle-particle node-contact of the latest step, between its particles,
+  // at its contact point with its normal. The bonds join the next initialise.
+  void createBondsFromLSInteractions(const double radius, const double initialLength,
+                                     const double YoungsModulus, const double poissonRatio,
+                                     const double tensileStrength = 0., const double cohesion = 0.,
+                                     const double frictionCoefficient = 0.) {
+    for (const ContactGeometry& c : contactGeometry(FDM_SET_PARTICLE))
+      addBondedInteraction(c.A, c.target, c.point, c.normal, radius, initialLength, YoungsModulus,
+                           poissonRatio, tensileStrength, cohesion, frictionCoefficient);
+  }
+
   LSParticle& getLSParticle() { return particles_; }
   LSParticle& getWall() { return walls_; }
   SolidInteraction& getLSParticleInteraction() { return pp_; }
@@ -617,6 +700,7 @@ class LSSolver {
     instWall_.upload(ws, stream_);
     if (ws.empty()) instWall_.allocate(1);
     instWallStart_.upload(wallStart, stream_);
+    sysGrid_ = pg;
     instPGrid_.upload(pg, stream_);
     instWGrid_.upload(wg, stream_);
     instPOff_.upload(po, stream_);
@@ -1048,6 +1132,7 @@ class LSSolver {
   DeviceBuffer<int> stepCounter_;
   DeviceBuffer<int> ctl_;
   std::vector<SystemDesc> systems_;
+  std::vector<fdm_grid> sysGrid_;  // particle grid of each system (host copy)
   fdm_instances inst_{};
   DeviceBuffer<int> instParticle_, instWall_, instWallStart_;
   DeviceBuffer<fdm_grid> instPGrid_, instWGrid_;

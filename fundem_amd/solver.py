@@ -45,6 +45,9 @@ def solver_lib():
         "fdmx_add_wall_material_damping": (I, [P, D, D]),
         "fdmx_num_materials": (I, [P, C.POINTER(I), C.POINTER(I)]),
         "fdmx_autotune": (I, [P, I, I, _dp, C.c_char_p, I]),
+        "fdmx_contact_geometry": (LL, [P, I, _ip, _ip, _ip, _dp, _dp, _dp, LL]),
+        "fdmx_create_bonds_from_contacts": (I, [P, D, D, D, D, D, D, D]),
+        "fdmx_num_bonds": (LL, [P]),
         "fdmx_adjoint_create": (P, [P]),
         "fdmx_adjoint_destroy": (None, [P]),
         "fdmx_adjoint_set_contacts": (I, [P, I]),
@@ -136,6 +139,26 @@ class Solver:
 
     def add_wall_material_damping(self, mu, zeta):
         self._ok(self._lib.fdmx_add_wall_material_damping(self.h, mu, zeta))
+
+    def contact_geometry(self, set_id):
+        """Contact points, normals and overlaps of the latest step's contacts."""
+        n = self._lib.fdmx_contact_geometry(self.h, set_id, None, None, None, None, None, None, 0)
+        if n < 0:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+        a, b, k = (np.zeros(n, np.int32) for _ in range(3))
+        pt, nr, ov = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        self._lib.fdmx_contact_geometry(self.h, set_id, _ip_(a), _ip_(b), _ip_(k), _dp_(pt), _dp_(nr),
+                                        _dp_(ov), n)
+        return dict(A=a, B=b, k=k, point=pt, normal=nr, overlap=ov)
+
+    def create_bonds_from_contacts(self, radius, length, youngs, poisson, tensile=0.0, cohesion=0.0,
+                                   friction=0.0):
+        self._ok(self._lib.fdmx_create_bonds_from_contacts(self.h, radius, length, youngs, poisson,
+                                                           tensile, cohesion, friction))
+
+    @property
+    def num_bonds(self) -> int:
+        return int(self._lib.fdmx_num_bonds(self.h))
 
     def autotune(self, trial_steps=20, tune_cell=True):
         """Pick skin, cell size and launch shapes for the fastest step on this
