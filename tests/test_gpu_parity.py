@@ -94,6 +94,23 @@ def test_cluster_bitexact_any_skin(name, skin):
     assert contacts > 0, "scenario produced no contacts"
 
 
+@pytest.mark.parametrize("skin", [0.0, None])
+def test_moving_wall_bitexact(skin):
+    """A floor with a prescribed velocity and spin (setFixedVelocityToWall,
+    setFixedAngularVelocityToWall: the wall drifts, is never kicked, and its
+    velocity enters the contact law) under a cluster, bit for bit."""
+    import dataclasses
+    scn = W.cluster(n=27, speed=1.0, scale=1e-2, dt=1e-5, kn=1e5, spin=5.0)
+    x, v, w, q, m, g = scn.walls[0]
+    # bring the bottom layer down to the floor
+    px = scn.x.copy()
+    px[:, 2] -= px[:, 2].min() - 0.011
+    scn = dataclasses.replace(scn, x=px, walls=[(x, np.array([0.02, -0.01, 0.0]),
+                                                 np.array([0.0, 0.0, 0.8]), q, m, g)])
+    contacts = run_bitexact(scn, steps=400, every=20, skin=skin)
+    assert contacts > 0
+
+
 def test_list_reuse_slow_pour_bitexact():
     """A slow, settling pour (periodic, wall, many steps): lists are reused
     for many steps, particles wrap and ghosts come and go, and every checked
