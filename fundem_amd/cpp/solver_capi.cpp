@@ -19,11 +19,19 @@ class BoundSolver : public fundem::LSSolver {
  public:
   explicit BoundSolver(int device) : fundem::LSSolver("fundem", 0, 256, device) {}
   void addGeometryDensity(double3& off, double density, double3 origin, double h, int3 size,
-                          const std::vector<double>& sdf, const std::vector<double3>& nodes) {
-    addParticleGeometry(off, density, origin, h, size, sdf, nodes, {});
+                          const std::vector<double>& sdf, const std::vector<double3>& nodes,
+                          const std::vector<int3>& tris) {
+    addParticleGeometry(off, density, origin, h, size, sdf, nodes, tris);
   }
 };
 
+// tris: ntri triangles of node indices (the surface mesh the frame files
+// draw); may be null with ntri = 0.
+static std::vector<int3> i3v(const int* t, int n) {
+  std::vector<int3> out((size_t)std::max(n, 0));
+  for (int k = 0; k < n; ++k) out[k] = make_int3(t[3 * k], t[3 * k + 1], t[3 * k + 2]);
+  return out;
+}
 template <class F>
 int guarded(F f) {
   try {
@@ -106,6 +114,10 @@ int fdmx_create_bonds_from_contacts(void* s, double R, double L, double E, doubl
   return guarded([&] {
     static_cast<BoundSolver*>(s)->createBondsFromLSInteractions(R, L, E, nu, st, coh, mub);
   });
+}
+// The frame files of one output frame (LSSolver::output) under dir.
+int fdmx_write_vtu(void* s, const char* dir, long long frame, long long step, double time) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->writeVTU(dir, (size_t)frame, (size_t)step, time); });
 }
 long long fdmx_num_bonds(void* s) {
   return (long long)static_cast<BoundSolver*>(s)->getLSParticleBondedInteraction().numPair();
@@ -197,34 +209,38 @@ int fdmx_adjoint_backward(void* a, int n, const double* xb, const double* qb, co
   });
 }
 int fdmx_add_geometry(void* s, double mass, const double* I6, const double* o3, double h,
-                      const int* sz, const double* sdf, const double* nodes, int nn) {
+                      const int* sz, const double* sdf, const double* nodes, int nn, const int* tris,
+                      int ntri) {
   return guarded([&] {
     const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
     static_cast<BoundSolver*>(s)->addParticleGeometry(
         mass, symMatrix(I6[0], I6[1], I6[2], I6[3], I6[4], I6[5]), d3(o3), h,
-        make_int3(sz[0], sz[1], sz[2]), std::vector<double>(sdf, sdf + ns), d3v(nodes, nn), {});
+        make_int3(sz[0], sz[1], sz[2]), std::vector<double>(sdf, sdf + ns), d3v(nodes, nn),
+        i3v(tris, ntri));
   });
 }
 int fdmx_add_geometry_density(void* s, double density, const double* o3, double h, const int* sz,
-                              const double* sdf, const double* nodes, int nn, double* off3) {
+                              const double* sdf, const double* nodes, int nn, double* off3,
+                              const int* tris, int ntri) {
   return guarded([&] {
     const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
     double3 off;
     static_cast<BoundSolver*>(s)->addGeometryDensity(off, density, d3(o3), h,
                                                      make_int3(sz[0], sz[1], sz[2]),
-                                                     std::vector<double>(sdf, sdf + ns), d3v(nodes, nn));
+                                                     std::vector<double>(sdf, sdf + ns), d3v(nodes, nn),
+                                                     i3v(tris, ntri));
     off3[0] = off.x;
     off3[1] = off.y;
     off3[2] = off.z;
   });
 }
 int fdmx_add_wall_geometry(void* s, const double* o3, double h, const int* sz, const double* sdf,
-                           const double* nodes, int nn) {
+                           const double* nodes, int nn, const int* tris, int ntri) {
   return guarded([&] {
     const size_t ns = size_t(sz[0]) * sz[1] * sz[2];
     static_cast<BoundSolver*>(s)->addWallGeometry(d3(o3), h, make_int3(sz[0], sz[1], sz[2]),
                                                   std::vector<double>(sdf, sdf + ns), d3v(nodes, nn),
-                                                  {});
+                                                  i3v(tris, ntri));
   });
 }
 int fdmx_add_particles(void* s, int n, const double* x, const double* v, const double* w,

@@ -28,6 +28,22 @@ class Template:
     size: tuple  # (nx, ny, nz)
     sdf: np.ndarray  # (nx*ny*nz,) float32, x fastest
     nodes: np.ndarray  # (M, 3) float32
+    tris: np.ndarray | None = None  # (T, 3) int32 node ids of the surface triangles, outward
+
+
+def _sphere_triangles(dirs: np.ndarray, order: np.ndarray) -> np.ndarray:
+    """Triangles of unit directions (their convex hull), wound outward, with
+    node ids renumbered for the node order `order` (new position of old node
+    i is where i appears in `order`)."""
+    from scipy.spatial import ConvexHull
+
+    tri = ConvexHull(dirs).simplices.astype(np.int64)
+    a, b, c = dirs[tri[:, 0]], dirs[tri[:, 1]], dirs[tri[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0.0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    return inv[tri].astype(np.int32)
 
 
 def _safe_pow(base: np.ndarray, exponent: float) -> np.ndarray:
@@ -122,10 +138,12 @@ def superellipsoid_template(a: float, b: float, c: float, p1: float, p2: float,
     f, g = superellipsoid_value_grad(pts, a, b, c, ee, en)
     gn = np.linalg.norm(g, axis=1)
     phi = np.where(gn > 1e-12, f / np.where(gn > 1e-12, gn, 1.0), np.where(f <= 0.0, -0.5 * h, 0.5 * h))
-    nodes = _radial_surface_points(fibonacci_directions(n_nodes), a, b, c, ee, en)
-    nodes = nodes[chunk_order(nodes)]
+    dirs = fibonacci_directions(n_nodes)
+    nodes = _radial_surface_points(dirs, a, b, c, ee, en)
+    order = chunk_order(nodes)
     return Template(origin=origin, spacing=float(h), size=(n_grid, n_grid, n_grid),
-                    sdf=phi.astype(np.float32), nodes=nodes.astype(np.float32))
+                    sdf=phi.astype(np.float32), nodes=nodes[order].astype(np.float32),
+                    tris=_sphere_triangles(dirs, order))
 
 
 def sphere_template(r: float, n_grid: int = 16, n_nodes: int = 400, pad: int = 2) -> Template:
@@ -136,10 +154,12 @@ def sphere_template(r: float, n_grid: int = 16, n_nodes: int = 400, pad: int = 2
     zz, yy, xx = np.meshgrid(ax, ax, ax, indexing="ij")
     pts = np.stack([xx, yy, zz], axis=-1).reshape(-1, 3)
     phi = np.linalg.norm(pts, axis=1) - r
-    nodes = fibonacci_directions(n_nodes) * r
-    nodes = nodes[chunk_order(nodes)]
+    dirs = fibonacci_directions(n_nodes)
+    nodes = dirs * r
+    order = chunk_order(nodes)
     return Template(origin=np.array([o, o, o]), spacing=float(h), size=(n_grid,) * 3,
-                    sdf=phi.astype(np.float32), nodes=nodes.astype(np.float32))
+                    sdf=phi.astype(np.float32), nodes=nodes[order].astype(np.float32),
+                    tris=_sphere_triangles(dirs, order))
 
 
 def plane_wall(xy_min, xy_max) -> Template:
@@ -162,7 +182,8 @@ def plane_wall(xy_min, xy_max) -> Template:
     nodes = np.array([[xy_min[0], xy_min[1], 0.0], [xy_max[0], xy_min[1], 0.0],
                       [xy_max[0], xy_max[1], 0.0], [xy_min[0], xy_max[1], 0.0]])
     return Template(origin=origin, spacing=s, size=(2, 2, 2), sdf=sdf.astype(np.float32),
-                    nodes=nodes.astype(np.float32))
+                    nodes=nodes.astype(np.float32),
+                    tris=np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32))
 
 
 def shoemake_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:

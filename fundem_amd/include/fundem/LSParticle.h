@@ -417,6 +417,8 @@ struct LSParticle {
   size_t numMaterial() const { return materials_.size(); }
   const fdm_template& templateInfo(size_t g) const { return templates_[g]; }
   const std::vector<double3>& templateNodes(size_t g) const { return nodeHost_[g]; }
+  const std::vector<int3>& templateConnectivity(size_t g) const { return connectivity_[g]; }
+  const fdm_material& materialInfo(size_t m) const { return materials_[m]; }
   // concatenated FP32 SDF values and node triples, as uploaded
   const std::vector<float>& sdfHost() const { return sdf_; }
   const std::vector<float>& nodesHost() const { return nodes_; }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <filesystem>
 #include <cstdio>
 #include <iostream>
 #include <map>
@@ -23,6 +24,7 @@
 #include "PeriodicBoundary.h"
 #include "SolidInteraction.h"
 #include "VBondedInteraction.h"
+#include "VTUWriter.h"
 #include "types.h"
 
 namespace fundem {
@@ -204,11 +206,13 @@ class LSSolver {
   void solve(const double3 minDomain, const double3 maxDomain, const double3 gravity,
              const double timeStep, const double maximumTime, const size_t numFrame, const int argc,
              char** argv) {
-    (void)argc;
-    (void)argv;
     if (!checkSolverInput(minDomain, maxDomain, timeStep, maximumTime)) return;
     if (!activateGPUDevice()) return;
     phase_ += 1;
+    if (writeVTU_) {
+      updateDir(argc, argv);
+      removeFrameFiles();
+    }
     initialize(minDomain, maxDomain, gravity, timeStep);
     const size_t numStep = size_t(maximumTime / timeStep) + 1;
     size_t frameInterval = numStep;
@@ -417,6 +421,10 @@ class LSSolver {
   // Synchronises.
   std::vector<ContactGeometry> contactGeometry(int set) {
     download();
+    return contactGeometryHost(set);
+  }
+  // The same from the state last downloaded.
+  std::vector<ContactGeometry> contactGeometryHost(int set) {
     const std::vector<ContactRecord> recs = contacts(set);
     std::vector<ContactGeometry> out;
     out.reserve(recs.size());
@@ -482,6 +490,35 @@ class LSSolver {
       addBondedInteraction(c.A, c.target, c.point, c.normal, radius, initialLength, YoungsModulus,
                            poissonRatio, tensileStrength, cohesion, frictionCoefficient);
   }
+
+  // The frame files of LSSolver::output (LSSolver.h:717-745) for the current
+  // state, under `dir`:
+  //   LSParticle/LSObjectMesh_NNNN.vtu         surface nodes and triangles of the particles
+  //   LSParticle/Particle_NNNN.vtu             one point per particle
+  //   LSParticleInteraction/LSInteraction_NNNN.vtu       particle-particle node contacts
+  //   LSParticleInteraction/VBondedInteraction_NNNN.vtu  bonds, when there are any
+  //   Wall/LSObjectMesh_NNNN.vtu               wall surfaces
+  //   LSParticle-WallInteraction/LSInteraction_NNNN.vtu  particle-wall node contacts
+  // Contacts are those of the latest step (none before the first). Ghost
+  // contacts of periodic sets are not written, as in the reference.
+  // Synchronises.
+  void writeVTU(const std::string& dir, size_t iFrame, size_t iStep, double time) {
+    download();
+    vtu::makeDirs(dir);
+    const std::string dp = dir + "/LSParticle", di = dir + "/LSParticleInteraction",
+                      dw = dir + "/Wall", dpw = dir + "/LSParticle-WallInteraction";
+    writeObjectMesh(particles_, dp, iFrame, iStep, time);
+    writeParticlePoints(dp, iFrame, iStep, time);
+    writeInteractions(FDM_SET_PARTICLE, di, iFrame, iStep, time);
+    writeBonds(di, iFrame, iStep, time);
+    writeObjectMesh(walls_, dw, iFrame, iStep, time);
+    writeInteractions(FDM_SET_WALL, dpw, iFrame, iStep, time);
+  }
+  // The directory LSSolver::output writes a frame to: the solver's directory
+  // with the phase appended (LSSolver.h:732).
+  std::string frameDirectory() const { return dir_ + "_phase" + std::to_string(phase_); }
+  // Write (default) or skip the frame files at each output frame of solve().
+  void setWriteVTU(bool on) { writeVTU_ = on; }
 
   LSParticle& getLSParticle() { return particles_; }
   LSParticle& getWall() { return walls_; }
@@ -586,6 +623,229 @@ class LSSolver {
  private:
   enum HookMode { HookUnknown, HookNone, HookUser };
 
+  // LSParticle::outputVTU (LSParticle.h:426-600): every surface node at
+  // x + R(q) s with its rigid-body velocity v + w x (p - x) and its particle
+  // id; one triangle per template triangle (VTK_TRIANGLE), node ids offset by
+  // the nodes of the bodies before.
+  void writeObjectMesh(const LSParticle& P, const std::string& dir, size_t iFrame, size_t iStep,
+                       double time) {
+    const size_t n = P.num();
+    if (n == 0) return;
+    const auto& x = P.positionHostRef();
+    const auto& v = P.velocityHostRef();
+    const auto& w = P.angularVelocityHostRef();
+    const auto& q = P.orientationHostRef();
+    const auto& geo = P.geometryIDHostRef();
+    size_t N = 0, M = 0;
+    for (size_t i = 0; i < n; ++i) {
+      N += P.templateNodes(geo[i]).size();
+      M += P.templateConnectivity(geo[i]).size();
+    }
+    std::vector<float> pts(3 * N), vel(3 * N);
+    std::vector<int32_t> pid(N), conn(3 * M), offs(M);
+    std::vector<uint8_t> types(M, 5);
+    size_t a = 0, c = 0;
+    for (size_t i = 0; i < n; ++i) {
+      for (const int3& t : P.templateConnectivity(geo[i])) {
+        conn[3 * c + 0] = int32_t(a + t.x);
+        conn[3 * c + 1] = int32_t(a + t.y);
+        conn[3 * c + 2] = int32_t(a + t.z);
+        offs[c] = int32_t(3 * (c + 1));
+        ++c;
+      }
+      for (const double3& s : P.templateNodes(geo[i])) {
+        const double3 pw = vec::add(x[i], vec::rotate(q[i], s));
+        const double3 vw = vec::add(v[i], vec::cross(w[i], vec::sub(pw, x[i])));
+        put3(pts, a, pw);
+        put3(vel, a, vw);
+        pid[a] = int32_t(i);
+        ++a;
+      }
+    }
+    vtu::makeDirs(dir);
+    vtu::Piece f(time, iStep, N, M);
+    f.points(pts);
+    f.cells(conn, offs, types);
+    f.beginPointData("velocity");
+    f.pointArray("particleID", pid, 1);
+    f.pointArray("velocity", vel, 3);
+    f.write(vtu::framePath(dir, "LSObjectMesh", iFrame));
+  }
+  // LSParticle::outputParticleVTU (LSParticle.h:602-756): one point per
+  // particle with velocity, angular velocity and orientation (q0 q1 q2 q3).
+  void writeParticlePoints(const std::string& dir, size_t iFrame, size_t iStep, double time) {
+    const size_t n = particles_.num();
+    if (n == 0) return;
+    std::vector<float> pts(3 * n), vel(3 * n), ang(3 * n), ori(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+      put3(pts, i, particles_.positionHostRef()[i]);
+      put3(vel, i, particles_.velocityHostRef()[i]);
+      put3(ang, i, particles_.angularVelocityHostRef()[i]);
+      const quaternion& q = particles_.orientationHostRef()[i];
+      ori[4 * i + 0] = float(q.q0);
+      ori[4 * i + 1] = float(q.q1);
+      ori[4 * i + 2] = float(q.q2);
+      ori[4 * i + 3] = float(q.q3);
+    }
+    vtu::makeDirs(dir);
+    vtu::Piece f(time, iStep, n, 0);
+    f.points(pts);
+    f.noCells();
+    f.beginPointData("velocity");
+    f.pointArray("velocity", vel, 3);
+    f.pointArray("angularVelocity", ang, 3);
+    f.pointArray("orientation", ori, 4);
+    f.write(vtu::framePath(dir, "Particle", iFrame));
+  }
+  // SolidInteraction::outputVTU (SolidInteraction.h:153-316): one vertex per
+  // node-contact at its contact point, with the normal, the direction of the
+  // sliding spring and the elastic energies 0.5 kn delta^2 and
+  // 0.5 ks |spring|^2 (contactKernel.cu:118-119 for particle pairs, with the
+  // series stiffnesses; :201-202 for walls, with the particle's own).
+  void writeInteractions(int set, const std::string& dir, size_t iFrame, size_t iStep, double time) {
+    const std::vector<ContactGeometry> cg =
+        particles_.num_device() > 0 ? contactGeometryHost(set) : std::vector<ContactGeometry>{};
+    const size_t N = cg.size();
+    std::vector<float> pts(3 * N), nrm(3 * N), dir3(3 * N), eN(N), eS(N);
+    std::vector<int32_t> conn(N), offs(N);
+    std::vector<uint8_t> types(N, 1);
+    const auto& matA = particles_.materialIDHostRef();
+    for (size_t k = 0; k < N; ++k) {
+      const ContactGeometry& c = cg[k];
+      const fdm_material& mi = particles_.materialInfo(matA[c.A]);
+      double kn = mi.kn, ks = mi.ks;
+      if (set != FDM_SET_WALL) {
+        const fdm_material& mj = particles_.materialInfo(matA[c.target]);
+        kn = (mi.kn > 0. && mj.kn > 0.) ? mi.kn * mj.kn / (mi.kn + mj.kn) : 0.;
+        ks = (mi.ks > 0. && mj.ks > 0.) ? mi.ks * mj.ks / (mi.ks + mj.ks) : 0.;
+      }
+      put3(pts, k, c.point);
+      put3(nrm, k, c.normal);
+      put3(dir3, k, vec::unit(c.spring));
+      eN[k] = float(0.5 * c.overlap * c.overlap * kn);
+      eS[k] = float(0.5 * vec::dot(c.spring, c.spring) * ks);
+      conn[k] = int32_t(k);
+      offs[k] = int32_t(k + 1);
+    }
+    vtu::makeDirs(dir);
+    vtu::Piece f(time, iStep, N, N);
+    f.points(pts);
+    f.cells(conn, offs, types);
+    f.beginPointData("");
+    f.pointArray("contactNormal", nrm, 3);
+    f.pointArray("slidingDirection", dir3, 3);
+    f.pointArray("normalElasticEnergy", eN, 1);
+    f.pointArray("slidingElasticEnergy", eS, 1);
+    f.write(vtu::framePath(dir, "LSInteraction", iFrame));
+  }
+  // VBondedInteraction::outputVTU (VBondedInteraction.h:262-448): one vertex
+  // per bond at the midpoint of its two bond points, with its ends, state,
+  // the elastic energies of VBond (contactKernel.cuh:170-173) evaluated on
+  // the downloaded state from the stored FP32 frames, and the last step's
+  // stresses (those of the failing step for a bond that just failed, zero
+  // after). A failed bond reports zero energies; its point is the midpoint
+  // of its bond points in the current state.
+  void writeBonds(const std::string& dir, size_t iFrame, size_t iStep, double time) {
+    const size_t N = bonds_.numPair();
+    if (N == 0) return;
+    const auto& x = particles_.positionHostRef();
+    const auto& q = particles_.orientationHostRef();
+    const auto& act = bonds_.activatedHostRef();
+    const bool onDevice = bonds_.numPair_device() > 0;
+    const std::vector<double> sig = onDevice ? bonds_.maxNormalStress(stream_) : std::vector<double>(N, 0.);
+    const std::vector<double> tau = onDevice ? bonds_.maxShearStress(stream_) : std::vector<double>(N, 0.);
+    std::vector<float> pts(3 * N), un(N), us(N), ub(N), ut(N), sn(N), ss(N);
+    std::vector<int32_t> conn(N), offs(N), mid(N), sid(N), on(N);
+    std::vector<uint8_t> types(N, 1);
+    for (size_t k = 0; k < N; ++k) {
+      const int i = bonds_.masterObjectIDHostRef()[k], j = bonds_.slaveObjectIDHostRef()[k];
+      const float* L = bonds_.localHost().data() + 24 * k;
+      const float* C = bonds_.coefHost().data() + 9 * k;
+      auto lv = [&](int o) { return make_double3(L[o], L[o + 1], L[o + 2]); };
+      const double3 n1i = vec::rotate(q[i], lv(0)), n2i = vec::rotate(q[i], lv(3)),
+                    n3i = vec::rotate(q[i], lv(6));
+      const double3 n1j = vec::rotate(q[j], lv(12)), n2j = vec::rotate(q[j], lv(15)),
+                    n3j = vec::rotate(q[j], lv(18));
+      const double3 rbi = vec::add(vec::rotate(q[i], lv(9)), x[i]);
+      const double3 rbj = vec::add(vec::rotate(q[j], lv(21)), x[j]);
+      put3(pts, k, vec::mul(vec::add(rbi, rbj), 0.5));
+      const int active = k < act.size() ? act[k] : 1;
+      if (active) {
+        const double B1 = C[0], B2 = C[1], B3 = C[2], B4 = C[3], D0 = C[5];
+        const double3 rij = vec::sub(rbi, rbj);
+        const double D = vec::len(rij);
+        if (!isZero(D)) {
+          const double3 e = vec::neg(vec::unit(rij));
+          const double c11 = vec::dot(n1i, n1j);
+          un[k] = float(0.5 * B1 * (D - D0) * (D - D0));
+          us[k] = float(B2 * (0.5 * vec::dot(vec::sub(n1j, n1i), e) - 0.25 * c11 + 0.75));
+          ub[k] = float((0.25 * B2 + B3 + 0.5 * B4) * (c11 + 1.));
+          ut[k] = float(-0.5 * B4 * (c11 + vec::dot(n2i, n2j) + vec::dot(n3i, n3j) - 1.));
+        }
+      }
+      sn[k] = float(sig[k]);
+      ss[k] = float(tau[k]);
+      conn[k] = int32_t(k);
+      offs[k] = int32_t(k + 1);
+      mid[k] = i;
+      sid[k] = j;
+      on[k] = active;
+    }
+    vtu::makeDirs(dir);
+    vtu::Piece f(time, iStep, N, N);
+    f.points(pts);
+    f.cells(conn, offs, types);
+    f.beginPointData("");
+    f.pointArray("masterObjectID", mid, 1);
+    f.pointArray("slaveObjectID", sid, 1);
+    f.pointArray("activated", on, 1);
+    f.pointArray("normalElasticEnergy", un, 1);
+    f.pointArray("shearElasticEnergy", us, 1);
+    f.pointArray("bendingElasticEnergy", ub, 1);
+    f.pointArray("torsionElasticEnergy", ut, 1);
+    f.pointArray("maxNormalStress", sn, 1);
+    f.pointArray("maxShearStress", ss, 1);
+    f.write(vtu::framePath(dir, "VBondedInteraction", iFrame));
+  }
+  static void put3(std::vector<float>& a, size_t i, const double3& v) {
+    a[3 * i + 0] = float(v.x);
+    a[3 * i + 1] = float(v.y);
+    a[3 * i + 2] = float(v.z);
+  }
+  // LSSolver::updateDir (LSSolver.h:611-644): a relative output directory is
+  // taken from the nearest enclosing folder named "build" of the executable,
+  // else the executable's folder (the working directory without argv).
+  void updateDir(int argc, char** argv) {
+    namespace fs = std::filesystem;
+    const fs::path p(dir_);
+    if (p.is_absolute()) return;
+    fs::path base = fs::current_path();
+    if (argc > 0 && argv && argv[0]) {
+      const fs::path exe = fs::weakly_canonical(fs::absolute(fs::path(argv[0])));
+      fs::path d = exe.parent_path();
+      base = d;
+      while (!d.empty() && d.filename() != "build") {
+        const fs::path up = d.parent_path();
+        if (up == d) break;
+        d = up;
+      }
+      if (!d.empty() && d.filename() == "build") base = d;
+    }
+    dir_ = (base / p).string();
+  }
+  // LSSolver::removeFiles (LSSolver.h:678-690): old frames of this phase.
+  void removeFrameFiles() {
+    namespace fs = std::filesystem;
+    const std::string dir = frameDirectory();
+    vtu::makeDirs(dir);
+    for (const char* sub : {"/LSParticle", "/LSParticleInteraction", "/Wall", "/LSParticle-WallInteraction"}) {
+      std::error_code ec;
+      if (!fs::is_directory(dir + sub, ec)) continue;
+      for (const auto& e : fs::directory_iterator(dir + sub, ec))
+        if (e.is_regular_file() && e.path().extension() == ".vtu") fs::remove(e.path(), ec);
+    }
+  }
+
   bool activateGPUDevice() {
     if (hipSetDevice(device_) != hipSuccess) {
       std::cout << "hipSetDevice( " << device_ << " ) failed! Do you have a HIP-capable GPU installed?"
@@ -617,6 +877,11 @@ class LSSolver {
     download();
     checkStatus();
     outputFrame(iFrame, iStep, time);
+    if (writeVTU_) {
+      std::cout << "[Solver] Outputting... " << std::endl;
+      writeVTU(frameDirectory(), iFrame, iStep, time);
+      std::cout << "[Solver] Output Completed" << std::endl;
+    }
   }
 
   // Brackets a stage with two events on the solver stream; no host
@@ -1115,6 +1380,7 @@ class LSSolver {
 
   std::string dir_;
   size_t phase_ = 0;
+  bool writeVTU_ = true;
   hipStream_t stream_;
   hipStream_t ownStream_ = nullptr;
   size_t maxGPUThread_;

@@ -148,6 +148,10 @@ struct VBondedInteraction {
   const std::vector<int>& breakStepHostRef() const { return breakStep_; }
   const std::vector<int>& masterObjectIDHostRef() const { return master_; }
   const std::vector<int>& slaveObjectIDHostRef() const { return slave_; }
+  // per bond: B1 B2 B3 B4 R L0 tensile cohesion friction; local frames n1 n2 n3 p of
+  // master then slave (FP32, as uploaded)
+  const std::vector<float>& coefHost() const { return coef_; }
+  const std::vector<float>& localHost() const { return local_; }
   std::vector<double> maxNormalStress(hipStream_t s) { return sigma_.toHost(s, master_.size()); }
   std::vector<double> maxShearStress(hipStream_t s) { return tau_.toHost(s, master_.size()); }
   double deviceMemoryGB() const {

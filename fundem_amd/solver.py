@@ -54,9 +54,10 @@ def solver_lib():
         "fdmx_adjoint_forward": (I, [P, LL, LL]),
         "fdmx_adjoint_backward": (I, [P, I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp]),
-        "fdmx_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
-        "fdmx_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
-        "fdmx_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I]),
+        "fdmx_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I, _ip, I]),
+        "fdmx_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp, _ip, I]),
+        "fdmx_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I, _ip, I]),
+        "fdmx_write_vtu": (I, [P, C.c_char_p, LL, LL, D]),
         "fdmx_add_particles": (I, [P, I, _dp, _dp, _dp, _dp, _ip, _ip]),
         "fdmx_add_wall": (I, [P, _dp, _dp, _dp, _dp, I, I]),
         "fdmx_add_bond": (I, [P, I, I, _dp, _dp, D, D, D, D, D, D, D]),
@@ -101,6 +102,16 @@ def _dp_(a: np.ndarray):
 
 def _ip_(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def _tris(t):
+    """(pointer, count) of a template's surface triangles, or (None, 0)."""
+    tris = getattr(t, "tris", None)
+    if tris is None or len(tris) == 0:
+        return None, 0
+    a = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+    t._tris_keep = a  # kept alive for the call
+    return _ip_(a), len(a)
 
 
 class Solver:
@@ -156,6 +167,14 @@ class Solver:
         self._ok(self._lib.fdmx_create_bonds_from_contacts(self.h, radius, length, youngs, poisson,
                                                            tensile, cohesion, friction))
 
+    def write_vtu(self, directory, frame=0, step=0, time=0.0):
+        """Write one output frame's .vtu files (LSSolver::output, LSSolver.h:717-745)
+        under `directory`: LSParticle/LSObjectMesh_NNNN.vtu, LSParticle/Particle_NNNN.vtu,
+        LSParticleInteraction/LSInteraction_NNNN.vtu (+ VBondedInteraction_NNNN.vtu),
+        Wall/LSObjectMesh_NNNN.vtu, LSParticle-WallInteraction/LSInteraction_NNNN.vtu."""
+        self._ok(self._lib.fdmx_write_vtu(self.h, str(directory).encode(), int(frame), int(step),
+                                          float(time)))
+
     @property
     def num_bonds(self) -> int:
         return int(self._lib.fdmx_num_bonds(self.h))
@@ -179,19 +198,20 @@ class Solver:
     def add_geometry(self, mass, inertia6, t):
         self._ok(self._lib.fdmx_add_geometry(
             self.h, mass, _dp_(_d(inertia6)), _dp_(_d(t.origin)), t.spacing,
-            _ip_(np.asarray(t.size, np.int32)), _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes)))
+            _ip_(np.asarray(t.size, np.int32)), _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes),
+            *_tris(t)))
 
     def add_geometry_density(self, density, t):
         off = np.zeros(3)
         self._ok(self._lib.fdmx_add_geometry_density(
             self.h, density, _dp_(_d(t.origin)), t.spacing, _ip_(np.asarray(t.size, np.int32)),
-            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes), _dp_(off)))
+            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes), _dp_(off), *_tris(t)))
         return off
 
     def add_wall_geometry(self, t):
         self._ok(self._lib.fdmx_add_wall_geometry(
             self.h, _dp_(_d(t.origin)), t.spacing, _ip_(np.asarray(t.size, np.int32)),
-            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes)))
+            _dp_(_d(t.sdf)), _dp_(_d(t.nodes)), len(t.nodes), *_tris(t)))
 
     def add_particles(self, x, v, w, q, mat, geo):
         x, v, w, q = _d(x), _d(v), _d(w), _d(q)
