@@ -9,6 +9,8 @@
 #include "../include/fundem/LSAdjoint.h"
 #include "../include/fundem/LSAutotune.h"
 #include "../include/fundem/LSSolver.h"
+#include "../include/fundem/LevelSetObjectGenerator.h"
+#include "../include/fundem/OBJLoader.h"
 
 namespace {
 
@@ -453,4 +455,86 @@ int fdmx_stage_times(void* s, char* names, int namesCap, double* ms, long long* 
   return count;
 }
 
+}  // extern "C"
+
+
+// OBJ meshes (OBJLoader::loadOBJMesh) and mesh level sets
+// (LevelSetObject::TriangleMeshParticle).
+namespace {
+struct ObjMesh {
+  std::vector<double3> v;
+  std::vector<int3> f;
+};
+}  // namespace
+extern "C" {
+void* fdmx_obj_load(const char* path, int verbose) {
+  auto* m = new ObjMesh;
+  if (!OBJLoader::loadOBJMesh(path, m->v, m->f, 0, nullptr, verbose != 0)) {
+    delete m;
+    g_error = std::string("cannot load OBJ mesh ") + path;
+    return nullptr;
+  }
+  return m;
+}
+long long fdmx_obj_num_vertices(void* h) { return (long long)static_cast<ObjMesh*>(h)->v.size(); }
+long long fdmx_obj_num_faces(void* h) { return (long long)static_cast<ObjMesh*>(h)->f.size(); }
+void fdmx_obj_get(void* h, double* v, int* f) {
+  const ObjMesh& m = *static_cast<ObjMesh*>(h);
+  for (size_t i = 0; i < m.v.size(); ++i) {
+    v[3 * i] = m.v[i].x;
+    v[3 * i + 1] = m.v[i].y;
+    v[3 * i + 2] = m.v[i].z;
+  }
+  for (size_t i = 0; i < m.f.size(); ++i) {
+    f[3 * i] = m.f[i].x;
+    f[3 * i + 1] = m.f[i].y;
+    f[3 * i + 2] = m.f[i].z;
+  }
+}
+void fdmx_obj_free(void* h) { delete static_cast<ObjMesh*>(h); }
+
+void* fdmx_mesh_create(const double* v, int nv, const int* f, int nf, int positiveInside) {
+  try {
+    auto* m = new LevelSetObject::TriangleMeshParticle(d3v(v, nv), i3v(f, nf));
+    m->setLevelSetPositiveInside(positiveInside != 0);
+    return m;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+void fdmx_mesh_destroy(void* h) { delete static_cast<LevelSetObject::TriangleMeshParticle*>(h); }
+int fdmx_mesh_is_valid(void* h) { return static_cast<LevelSetObject::TriangleMeshParticle*>(h)->isValid() ? 1 : 0; }
+int fdmx_mesh_build_grid(void* h, double spacing, int padding) {
+  return guarded([&] { static_cast<LevelSetObject::TriangleMeshParticle*>(h)->buildGrid(spacing, padding); });
+}
+int fdmx_mesh_build_grid_by_resolution(void* h, int nodesPerDiameter, int padding) {
+  return guarded([&] {
+    static_cast<LevelSetObject::TriangleMeshParticle*>(h)->buildGridByResolution(nodesPerDiameter, padding);
+  });
+}
+// origin3, size3, spacing of the built grid (size 0 when none is built)
+void fdmx_mesh_grid_info(void* h, double* origin3, int* size3, double* spacing) {
+  const auto& g = static_cast<LevelSetObject::TriangleMeshParticle*>(h)->gridInfo();
+  origin3[0] = g.gridOrigin.x;
+  origin3[1] = g.gridOrigin.y;
+  origin3[2] = g.gridOrigin.z;
+  const bool built = !g.gridNodeSignedDistance.empty();
+  size3[0] = built ? g.gridNodeSize.x : 0;
+  size3[1] = built ? g.gridNodeSize.y : 0;
+  size3[2] = built ? g.gridNodeSize.z : 0;
+  *spacing = g.gridNodeSpacing;
+}
+void fdmx_mesh_grid_values(void* h, double* out) {
+  const auto& g = static_cast<LevelSetObject::TriangleMeshParticle*>(h)->gridInfo();
+  std::memcpy(out, g.gridNodeSignedDistance.data(), g.gridNodeSignedDistance.size() * sizeof(double));
+}
+int fdmx_mesh_evaluate(void* h, const double* points, int n, double* values, double* grads) {
+  return guarded([&] {
+    static_cast<LevelSetObject::TriangleMeshParticle*>(h)->evaluatePoints(points, n, values, grads);
+  });
+}
+int fdmx_mesh_output_grid_vtu(void* h, const char* prefix) {
+  return guarded([&] { static_cast<LevelSetObject::TriangleMeshParticle*>(h)->outputGridVTU(prefix); });
+}
 }  // extern "C"

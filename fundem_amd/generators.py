@@ -213,3 +213,163 @@ def template_box_reach(t: Template) -> float:
     hi = t.origin + t.spacing * (np.asarray(t.size) - 1)
     corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
     return float(np.max(np.linalg.norm(corners, axis=1)))
+
+
+# ------------------------------------------------------------ meshes ----
+def load_obj(path, verbose: bool = False):
+    """Vertices (n, 3) float64 and triangles (m, 3) int32 of a Wavefront OBJ
+    file, read by the native OBJLoader::loadOBJMesh (OBJLoader.h): polygons
+    fan-split, 1-based and negative indices resolved. Raises OSError when the
+    file cannot be read or is malformed."""
+    import ctypes as C
+
+    from .solver import solver_lib
+
+    lib = solver_lib()
+    h = lib.fdmx_obj_load(str(path).encode(), int(verbose))
+    if not h:
+        raise OSError(lib.fdmx_last_error().decode())
+    try:
+        v = np.zeros((lib.fdmx_obj_num_vertices(h), 3))
+        f = np.zeros((lib.fdmx_obj_num_faces(h), 3), np.int32)
+        lib.fdmx_obj_get(h, v.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_int)))
+    finally:
+        lib.fdmx_obj_free(h)
+    return v, f
+
+
+class MeshParticle:
+    """A closed triangle mesh as a level set (LevelSetObject::
+    TriangleMeshParticle, LevelSetObjectGenerator.h:382-951): the grid of
+    signed distances is sampled on the GPU."""
+
+    def __init__(self, vertices, triangles, positive_inside: bool = False):
+        import ctypes as C
+
+        from .solver import solver_lib
+
+        self._C = C
+        self._lib = solver_lib()
+        self.vertices = np.ascontiguousarray(np.asarray(vertices, np.float64).reshape(-1, 3))
+        self.triangles = np.ascontiguousarray(np.asarray(triangles, np.int32).reshape(-1, 3))
+        self.h = self._lib.fdmx_mesh_create(self._dp(self.vertices), len(self.vertices),
+                                            self._ip(self.triangles), len(self.triangles),
+                                            int(positive_inside))
+        if not self.h:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+
+    def _dp(self, a):
+        return a.ctypes.data_as(self._C.POINTER(self._C.c_double))
+
+    def _ip(self, a):
+        return a.ctypes.data_as(self._C.POINTER(self._C.c_int))
+
+    def _ok(self, r):
+        if r != 0:
+            raise RuntimeError(self._lib.fdmx_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fdmx_mesh_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def valid(self) -> bool:
+        return bool(self._lib.fdmx_mesh_is_valid(self.h))
+
+    def build_grid(self, spacing: float, padding: int = 2):
+        self._ok(self._lib.fdmx_mesh_build_grid(self.h, float(spacing), int(padding)))
+        return self.grid()
+
+    def build_grid_by_resolution(self, nodes_per_diameter: int = 50, padding: int = 2):
+        self._ok(self._lib.fdmx_mesh_build_grid_by_resolution(self.h, int(nodes_per_diameter),
+                                                                int(padding)))
+        return self.grid()
+
+    def grid(self):
+        """(origin (3,), spacing, size (3,), values (nx*ny*nz,) x fastest), or
+        None before a grid is built (or for an invalid mesh)."""
+        o, sz, h = np.zeros(3), np.zeros(3, np.int32), np.zeros(1)
+        self._lib.fdmx_mesh_grid_info(self.h, self._dp(o), self._ip(sz), self._dp(h))
+        if sz.prod() == 0:
+            return None
+        vals = np.zeros(int(np.prod(sz.astype(np.int64))))
+        self._lib.fdmx_mesh_grid_values(self.h, self._dp(vals))
+        return o, float(h[0]), tuple(int(x) for x in sz), vals
+
+    def evaluate(self, points):
+        """Signed distance (negative inside) and gradient at points (n, 3)."""
+        p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 3))
+        val, grad = np.zeros(len(p)), np.zeros((len(p), 3))
+        self._ok(self._lib.fdmx_mesh_evaluate(self.h, self._dp(p), len(p), self._dp(val),
+                                              self._dp(grad)))
+        return val, grad
+
+    def output_grid_vtu(self, prefix: str):
+        self._ok(self._lib.fdmx_mesh_output_grid_vtu(self.h, str(prefix).encode()))
+
+
+def mesh_template(vertices, triangles, spacing: float = None, nodes_per_diameter: int = 50,
+                  padding: int = 2) -> Template:
+    """Template of a closed triangle mesh: its level-set grid (spacing, or the
+    largest box extent over nodes_per_diameter) sampled on the GPU, and its
+    vertices as surface nodes (ordered into compact 64-node chunks, the
+    triangles renumbered to match), as the reference's tutorial feeds a
+    TriangleMeshParticle to addParticle (apps/tutorial/LSParticleFalling.cpp:
+    57-103)."""
+    m = MeshParticle(vertices, triangles)
+    try:
+        g = m.build_grid(spacing, padding) if spacing else m.build_grid_by_resolution(
+            nodes_per_diameter, padding)
+    finally:
+        m.close()
+    if g is None:
+        raise ValueError("invalid mesh: no level-set grid")
+    origin, h, size, vals = g
+    v = np.asarray(vertices, np.float64).reshape(-1, 3)
+    order = chunk_order(v)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    tris = inv[np.asarray(triangles, np.int64).reshape(-1, 3)].astype(np.int32)
+    return Template(origin=origin, spacing=h, size=size, sdf=vals.astype(np.float32),
+                    nodes=v[order].astype(np.float32), tris=tris)
+
+
+def icosphere(radius: float = 1.0, subdivisions: int = 2):
+    """Vertices and outward triangles of a subdivided icosahedron on a sphere."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts) * radius, np.array(faces, np.int32)
+
+
+def box_mesh(lx: float, ly: float, lz: float):
+    """The 8 corners and 12 outward triangles of an axis-aligned box centred
+    at the origin."""
+    hx, hy, hz = 0.5 * lx, 0.5 * ly, 0.5 * lz
+    v = np.array([[x, y, z] for z in (-hz, hz) for y in (-hy, hy) for x in (-hx, hx)])
+    f = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
+                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]], np.int32)
+    return v, f

@@ -10,6 +10,8 @@
 #include <utility>
 #include <vector>
 
+#include "../../../include/fundem_amd.h"
+
 namespace fundem {
 
 inline void hipCheck(hipError_t e, const char* what) {
@@ -17,6 +19,11 @@ inline void hipCheck(hipError_t e, const char* what) {
     (void)hipGetLastError();  // do not leave the error for the next launch to report
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
   }
+}
+
+// A launcher's return code (FDM_OK or an error code) as an exception.
+inline void fdmCheck(int r, const char* what) {
+  if (r != FDM_OK) throw std::runtime_error(std::string(what) + " failed with code " + std::to_string(r));
 }
 
 template <class T>

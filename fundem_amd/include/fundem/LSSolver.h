@@ -29,9 +29,6 @@
 
 namespace fundem {
 
-inline void fdmCheck(int r, const char* what) {
-  if (r != FDM_OK) throw std::runtime_error(std::string(what) + " failed with code " + std::to_string(r));
-}
 
 class LSAdjoint;
 class LSAutotuner;

@@ -127,6 +127,10 @@ class Piece {
   void beginPointData(const char* vectors) {
     text_ += vectors && *vectors ? std::string("<PointData Vectors=\"") + vectors + "\">\n" : "<PointData>\n";
   }
+  // Opens PointData with `scalars` as the active scalar array.
+  void beginPointDataScalars(const char* scalars) {
+    text_ += std::string("<PointData Scalars=\"") + scalars + "\">\n";
+  }
   template <class T>
   void pointArray(const char* name, const std::vector<T>& values, int components) {
     check(values.size(), points_ * components, name);

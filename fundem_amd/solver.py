@@ -58,6 +58,20 @@ def solver_lib():
         "fdmx_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp, _ip, I]),
         "fdmx_add_wall_geometry": (I, [P, _dp, D, _ip, _dp, _dp, I, _ip, I]),
         "fdmx_write_vtu": (I, [P, C.c_char_p, LL, LL, D]),
+        "fdmx_obj_load": (P, [C.c_char_p, I]),
+        "fdmx_obj_num_vertices": (LL, [P]),
+        "fdmx_obj_num_faces": (LL, [P]),
+        "fdmx_obj_get": (None, [P, _dp, _ip]),
+        "fdmx_obj_free": (None, [P]),
+        "fdmx_mesh_create": (P, [_dp, I, _ip, I, I]),
+        "fdmx_mesh_destroy": (None, [P]),
+        "fdmx_mesh_is_valid": (I, [P]),
+        "fdmx_mesh_build_grid": (I, [P, D, I]),
+        "fdmx_mesh_build_grid_by_resolution": (I, [P, I, I]),
+        "fdmx_mesh_grid_info": (None, [P, _dp, _ip, _dp]),
+        "fdmx_mesh_grid_values": (None, [P, _dp]),
+        "fdmx_mesh_evaluate": (I, [P, _dp, I, _dp, _dp]),
+        "fdmx_mesh_output_grid_vtu": (I, [P, C.c_char_p]),
         "fdmx_add_particles": (I, [P, I, _dp, _dp, _dp, _dp, _ip, _ip]),
         "fdmx_add_wall": (I, [P, _dp, _dp, _dp, _dp, I, I]),
         "fdmx_add_bond": (I, [P, I, I, _dp, _dp, D, D, D, D, D, D, D]),

@@ -643,6 +643,28 @@ int fdm_launchCopyBodies(fdm_bodies dst, fdm_bodies src, hipStream_t stream);
 int fdm_launchAdjointStep(const fdm_adjoint_step_args* args, void* ws, size_t wsBytes,
                           hipStream_t stream);
 
+/* ---------------------------------------------------------------------- */
+/* Mesh signed distance (input preparation)                               */
+
+/* Replaces the grid loop of LevelSetObject::Base::buildGrid
+ * (LevelSetObjectGenerator.h:186-223) for a TriangleMeshParticle
+ * (LevelSetObjectGenerator.h:888-914): the level-set value of a closed
+ * triangle mesh at every node origin + (i, j, k) * spacing of a
+ * size[0] x size[1] x size[2] grid (x fastest), written to sdf. triangles:
+ * [9 * numTriangles] device FP64 vertex coordinates a b c per triangle.
+ * Distance: the minimum over triangles (closestPointOnTriangle, :454-495);
+ * sign: the solid-angle sum over triangles (:758-800); positiveInside flips
+ * it (setLevelSetPositiveInside). origin and size are host arrays. */
+int fdm_launchMeshSignedDistanceGrid(const double* triangles, int numTriangles,
+                                     const double* origin, double spacing, const int* size,
+                                     int positiveInside, double* sdf, hipStream_t stream);
+/* TriangleMeshParticle::evaluateImplicitFunctionValueAndGradient
+ * (LevelSetObjectGenerator.h:888-914) at numPoints device points [3*numPoints]:
+ * value [numPoints] and gradient [3*numPoints]. */
+int fdm_launchMeshSignedDistancePoints(const double* triangles, int numTriangles,
+                                       const double* points, int numPoints, int positiveInside,
+                                       double* value, double* grad, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

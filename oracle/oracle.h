@@ -132,6 +132,16 @@ void osim_get_bonds(const OracleSim* s, int* active, double* max_normal_stress,
  * returns 1 if the key was present. Used by tests to seed histories. */
 int osim_set_spring(OracleSim* s, int set, int a, int target, int k, const double* spring3);
 
+/* Mesh level sets (meshsdf.cpp; LevelSetObjectGenerator.h:122-224,454-914).
+ * v: [3*nv] vertices, f: [3*nf] triangles. Points: value and gradient at np
+ * points (value 1, gradient 0 for an invalid mesh). Grid: Base::buildGrid's
+ * placement (origin3, size3) and, when sdf is not null, its values; returns
+ * the node count (0 for an invalid mesh or spacing). */
+void oracle_mesh_points(const double* v, int nv, const int* f, int nf, const double* pts, int np,
+                        int positiveInside, double* value, double* grad);
+long long oracle_mesh_grid(const double* v, int nv, const int* f, int nf, double h, int padding,
+                           int positiveInside, double* origin3, int* size3, double* sdf);
+
 #ifdef __cplusplus
 }
 #endif

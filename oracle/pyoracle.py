@@ -41,6 +41,8 @@ def _load():
         "osim_add_material": (I, [P, D, D, D, D]),
         "osim_add_wall_material": (I, [P, D, D]),
         "osim_add_material_damping": (I, [P, D, D, D, D]),
+        "oracle_mesh_points": (V, [_dp, I, _ip, I, _dp, I, I, _dp, _dp]),
+        "oracle_mesh_grid": (LL, [_dp, I, _ip, I, D, I, I, _dp, _ip, _dp]),
         "osim_add_wall_material_damping": (I, [P, D, D]),
         "osim_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
         "osim_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
@@ -275,3 +277,30 @@ class OracleSim:
 
     def set_spring(self, set_id, a, target, k, spring):
         return lib().osim_set_spring(self.h, set_id, a, target, k, dptr(d3(spring)))
+
+
+def mesh_points(vertices, triangles, points, positive_inside=False):
+    """Oracle signed distance and gradient of a closed mesh at points (n, 3)."""
+    v = np.ascontiguousarray(np.asarray(vertices, np.float64).reshape(-1))
+    f = np.ascontiguousarray(np.asarray(triangles, np.int32).reshape(-1))
+    p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 3))
+    val, grad = np.zeros(len(p)), np.zeros((len(p), 3))
+    lib().oracle_mesh_points(v.ctypes.data_as(_dp), len(v) // 3, f.ctypes.data_as(_ip), len(f) // 3,
+                             p.ctypes.data_as(_dp), len(p), int(positive_inside),
+                             val.ctypes.data_as(_dp), grad.ctypes.data_as(_dp))
+    return val, grad
+
+
+def mesh_grid(vertices, triangles, spacing, padding=2, positive_inside=False):
+    """Oracle Base::buildGrid of a closed mesh: (origin, size, values) or None."""
+    v = np.ascontiguousarray(np.asarray(vertices, np.float64).reshape(-1))
+    f = np.ascontiguousarray(np.asarray(triangles, np.int32).reshape(-1))
+    o, sz = np.zeros(3), np.zeros(3, np.int32)
+    args = (v.ctypes.data_as(_dp), len(v) // 3, f.ctypes.data_as(_ip), len(f) // 3, float(spacing),
+            int(padding), int(positive_inside), o.ctypes.data_as(_dp), sz.ctypes.data_as(_ip))
+    n = lib().oracle_mesh_grid(*args, None)
+    if n == 0:
+        return None
+    vals = np.zeros(n)
+    lib().oracle_mesh_grid(*args, vals.ctypes.data_as(_dp))
+    return o, tuple(int(x) for x in sz), vals
