@@ -54,9 +54,10 @@ def parse():
     p.add_argument("--settle", type=int, default=20000,
                    help="column: plain steps that pack the column before the timed runs")
     p.add_argument("--checkpoint", type=int, default=100, help="column: steps per checkpoint")
-    p.add_argument("--workload", choices=("pour", "batch", "column"), default="pour",
+    p.add_argument("--workload", choices=("pour", "batch", "column", "meshsdf"), default="pour",
                    help="pour: the 1M-particle pour; batch: 512 packings in one solver against "
-                        "plain runs of the same packings one after another")
+                        "plain runs of the same packings one after another; column: adjoint "
+                        "against forward step cost; meshsdf: level-set grid of a closed mesh")
     p.add_argument("--batch-sample", type=int, default=32,
                    help="packings timed one by one for the sequential reference")
     return p.parse_args()
@@ -300,8 +301,58 @@ def column_main(a):
     print(json.dumps(out), flush=True)
 
 
+def meshsdf_main(a):
+    """Level-set grid of a closed triangle mesh (TriangleMeshParticle::
+    buildGridByResolution at its default 50 nodes per diameter): a torus of
+    5120 triangles (about the reference tutorial's bunny, 4968), timed as whole
+    grid builds (triangle upload, kernel, values back). The oracle's serial
+    build of a 20-node-per-diameter grid of the same mesh is the CPU baseline."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from fundem_amd import generators as G
+    v, f = G.torus_mesh(1.0, 0.4, 80, 32)
+    m = G.MeshParticle(v, f)
+    for _ in range(max(1, a.warmup)):
+        g = m.build_grid_by_resolution(50, 2)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g = m.build_grid_by_resolution(50, 2)
+    dt = (time.perf_counter() - t0) / a.steps
+    nodes = g[3].size
+    pairs = nodes * len(f)
+    ext = (v.max(axis=0) - v.min(axis=0)).max()
+    c0 = time.perf_counter()
+    ref = po.mesh_grid(v, f, ext / 20, 2)
+    cdt = time.perf_counter() - c0
+    cpairs = ref[2].size * len(f)
+    sample = m.build_grid(ext / 20, 2)
+    assert np.array_equal(sample[3], ref[2]), "device grid differs from the oracle's"
+    log(f"mesh grid {g[2]} ({nodes} nodes) x {len(f)} triangles: {1e3 * dt:.2f} ms per grid; "
+        f"oracle {ref[1]} grid in {cdt:.2f} s")
+    out = {
+        "metric": "mesh level-set grid: node-triangle evaluations per second (one GPU)",
+        "value": pairs / dt, "unit": "node-triangle pairs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"closed torus mesh R 1, r 0.4, {len(v)} vertices, {len(f)} triangles; "
+                               f"grid of 50 nodes per diameter, 2 padding layers: {g[2]} = {nodes} "
+                               "nodes; one step = one whole grid build"},
+        "cpu_baseline": {"value": cpairs / cdt, "unit": "node-triangle pairs/s", "cores": 1,
+                         "kind": "port", "sample": f"oracle grid of 20 nodes per diameter, "
+                                                   f"{ref[1]} = {ref[2].size} nodes, {cdt:.1f} s"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.workload == "meshsdf":
+        if a.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        meshsdf_main(a)
+        return
     if a.workload == "column":
         if a.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
@@ -316,7 +367,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the one-GPU build "
-              f"does not shard; DESIGN.md describes the slab decomposition)", file=sys.stderr)
+              f"does not shard)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         copy_probe()

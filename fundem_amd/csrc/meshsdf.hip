@@ -5,16 +5,18 @@
 //
 // The reference walks a SAH-BVH for the closest point and sums the solid
 // angles of all triangles for the inside test, one point at a time on the
-// host. Here one thread takes one point and both reductions run over every
-// triangle: the workgroup stages 256 triangles at a time in LDS (SoA, read by
-// all lanes at the same address, i.e. broadcast) and every lane folds them in
-// triangle order. A triangle whose bounding box is farther than the current
-// best distance skips the closest-point test (the BVH's pruning, per lane);
-// the solid angle is needed for every triangle. The closest distance is the
+// host. Here both reductions run over every triangle: the workgroup stages
+// 256 triangles at a time in LDS (SoA) and eight lanes share a point, each
+// folding every eighth triangle, before the partials are combined: a grid of
+// 70k nodes then runs 560k lanes, enough to keep every SIMD of the chip busy.
+// A triangle whose bounding box is farther than the lane's best distance so
+// far skips the closest-point test (the BVH's pruning, per lane); the solid
+// angle is needed for every triangle. The closest distance is the
 // exact minimum over triangles of the FP64 point-triangle distance; ties go
 // to the lowest triangle index. Compute bound (FP64 VALU), not HBM: the
 // triangles are read once per workgroup from L2.
 #include <cfloat>
+#include <climits>
 
 #include "fdm_device.hpp"
 #include "fdm_launch.hpp"
@@ -79,19 +81,33 @@ __device__ __forceinline__ V3 pointOf(const MeshPoints& m, long long id) {
   return {m.ox + double(i) * m.h, m.oy + double(j) * m.h, m.oz + double(k) * m.h};
 }
 
-// One thread per point. sdf (grid mode): the level-set value of buildGrid,
-// value / |grad| or +-h/2 where the gradient vanishes. value/grad (point
-// mode): the implicit value and gradient of
+// Eight threads per point (kSlices), each folding every eighth triangle of a
+// staged tile; the partials are combined in LDS. sdf (grid mode): the
+// level-set value of buildGrid, value / |grad| or +-h/2 where the gradient
+// vanishes. value/grad (point mode): the implicit value and gradient of
 // evaluateImplicitFunctionValueAndGradient (LevelSetObjectGenerator.h:888-914).
+// The closest point is the first triangle (lowest index) at the minimum
+// squared distance, as a serial scan finds it. The solid angles are summed per
+// slice and the slices then in order, so the sum's rounding differs from a
+// serial sum; the inside decision |sum| / 4 pi > 1/2 only sees that for
+// points within rounding of the surface, where the value is +-0 either way.
+constexpr int kSlices = 8;
+constexpr int kPointsPerBlock = kMeshTile / kSlices;
+
 __global__ __launch_bounds__(kMeshTile) void kMeshDistance(const double* __restrict__ tri, int nt,
                                                             MeshPoints m, int positiveInside,
                                                             double* sdf, double* value,
                                                             double* grad) {
   __shared__ double sT[9][kMeshTile];
-  const long long id = (long long)blockIdx.x * kMeshTile + threadIdx.x;
+  __shared__ double sBest[kMeshTile], sSum[kMeshTile], sCp[3][kMeshTile];
+  __shared__ int sIdx[kMeshTile];
+  const int slice = threadIdx.x % kSlices;
+  const int local = threadIdx.x / kSlices;
+  const long long id = (long long)blockIdx.x * kPointsPerBlock + local;
   const bool active = id < m.count;
   const V3 p = active ? pointOf(m, id) : V3{0.0, 0.0, 0.0};
   double best = DBL_MAX;
+  int bestIdx = INT_MAX;
   V3 bestCp = {0.0, 0.0, 0.0};
   double sum = 0.0;
   for (int t0 = 0; t0 < nt; t0 += kMeshTile) {
@@ -104,7 +120,7 @@ __global__ __launch_bounds__(kMeshTile) void kMeshDistance(const double* __restr
     __syncthreads();
     if (!active) continue;
     const int cnt = min(kMeshTile, nt - t0);
-    for (int u = 0; u < cnt; ++u) {
+    for (int u = slice; u < cnt; u += kSlices) {
       const V3 a = {sT[0][u], sT[1][u], sT[2][u]};
       const V3 b = {sT[3][u], sT[4][u], sT[5][u]};
       const V3 c = {sT[6][u], sT[7][u], sT[8][u]};
@@ -117,15 +133,33 @@ __global__ __launch_bounds__(kMeshTile) void kMeshDistance(const double* __restr
       if (box <= best * (1.0 + 1e-9)) {
         const V3 cp = closestOnTriangle(p, a, b, c);
         const double d2 = vlen2(vsub(p, cp));
-        if (d2 < best) {
+        if (d2 < best) {  // triangles of a slice come in increasing index
           best = d2;
+          bestIdx = t0 + u;
           bestCp = cp;
         }
       }
       sum += solidAngle(p, a, b, c);
     }
   }
-  if (!active) return;
+  sBest[threadIdx.x] = best;
+  sIdx[threadIdx.x] = bestIdx;
+  sSum[threadIdx.x] = sum;
+  sCp[0][threadIdx.x] = bestCp.x;
+  sCp[1][threadIdx.x] = bestCp.y;
+  sCp[2][threadIdx.x] = bestCp.z;
+  __syncthreads();
+  if (!active || slice != 0) return;
+  for (int k = 1; k < kSlices; ++k) {
+    const int o = threadIdx.x + k;
+    const double b2 = sBest[o];
+    if (b2 < best || (b2 == best && sIdx[o] < bestIdx)) {
+      best = b2;
+      bestIdx = sIdx[o];
+      bestCp = {sCp[0][o], sCp[1][o], sCp[2][o]};
+    }
+    sum += sSum[o];
+  }
   // evaluateImplicitFunctionValueAndGradient
   const V3 dp = vsub(p, bestCp);
   const double d = vlen(dp);
@@ -174,8 +208,8 @@ extern "C" int fdm_launchMeshSignedDistanceGrid(const double* triangles, int num
   m.nz = size[2];
   m.points = nullptr;
   m.count = (long long)size[0] * size[1] * size[2];
-  hipLaunchKernelGGL(kMeshDistance, dim3(blocksFor(m.count, kMeshTile)), dim3(kMeshTile), 0, stream,
-                     triangles, numTriangles, m, positiveInside, sdf, nullptr, nullptr);
+  hipLaunchKernelGGL(kMeshDistance, dim3(blocksFor(m.count, kPointsPerBlock)), dim3(kMeshTile), 0,
+                     stream, triangles, numTriangles, m, positiveInside, sdf, nullptr, nullptr);
   return launchStatus();
 }
 
@@ -190,7 +224,7 @@ extern "C" int fdm_launchMeshSignedDistancePoints(const double* triangles, int n
   m.points = points;
   m.count = numPoints;
   m.h = 0.0;
-  hipLaunchKernelGGL(kMeshDistance, dim3(blocksFor(numPoints, kMeshTile)), dim3(kMeshTile), 0,
+  hipLaunchKernelGGL(kMeshDistance, dim3(blocksFor(numPoints, kPointsPerBlock)), dim3(kMeshTile), 0,
                      stream, triangles, numTriangles, m, positiveInside, nullptr, value, grad);
   return launchStatus();
 }

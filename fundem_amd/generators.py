@@ -373,3 +373,21 @@ def box_mesh(lx: float, ly: float, lz: float):
     f = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
                   [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]], np.int32)
     return v, f
+
+
+def torus_mesh(R: float = 1.0, r: float = 0.4, nu: int = 48, nv: int = 24):
+    """Vertices and outward triangles of a torus about z (tube radius r,
+    centre-line radius R) on an nu x nv parameter grid: a closed mesh that is
+    not convex."""
+    u = 2 * np.pi * np.arange(nu) / nu
+    w = 2 * np.pi * np.arange(nv) / nv
+    uu, ww = np.meshgrid(u, w, indexing="ij")
+    v = np.stack([(R + r * np.cos(ww)) * np.cos(uu), (R + r * np.cos(ww)) * np.sin(uu),
+                  r * np.sin(ww)], -1).reshape(-1, 3)
+    f = []
+    for i in range(nu):
+        for j in range(nv):
+            a, b = i * nv + j, ((i + 1) % nu) * nv + j
+            c, d = ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv
+            f += [(a, b, c), (a, c, d)]
+    return v, np.array(f, np.int32)

@@ -12,14 +12,13 @@ import pyoracle as po
 import vtu_read
 from fundem_amd import generators as G
 from fundem_amd import workloads as W
-from test_mesh_oracle import torus_mesh
 
 pytestmark = pytest.mark.gpu
 
 MESHES = {
     "box": lambda: G.box_mesh(1.0, 2.0, 3.0),
     "icosphere": lambda: G.icosphere(1.3, 3),
-    "torus": lambda: torus_mesh(),
+    "torus": lambda: G.torus_mesh(),
 }
 
 
@@ -65,7 +64,7 @@ def test_mesh_template_particles_bitexact():
     """Tori built from their mesh (SDF on the GPU, vertices as nodes) dropped
     as a small cluster onto a floor: device and oracle steps agree bit for bit."""
     from fundem_amd import Solver
-    v, f = torus_mesh(1.0e-3, 0.4e-3, 32, 16)
+    v, f = G.torus_mesh(1.0e-3, 0.4e-3, 32, 16)
     t = G.mesh_template(v, f, nodes_per_diameter=24)
     assert t.tris is not None and len(t.nodes) == len(v)
     scn = W.cluster(n=8, speed=1.0, templates=[t], spacing=2.6, steps=300)

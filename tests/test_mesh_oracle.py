@@ -20,20 +20,7 @@ import pytest
 import pyoracle as po
 from fundem_amd import generators as G
 
-
-def torus_mesh(R=1.0, r=0.4, nu=48, nv=24):
-    u = 2 * np.pi * np.arange(nu) / nu
-    w = 2 * np.pi * np.arange(nv) / nv
-    uu, ww = np.meshgrid(u, w, indexing="ij")
-    v = np.stack([(R + r * np.cos(ww)) * np.cos(uu), (R + r * np.cos(ww)) * np.sin(uu),
-                  r * np.sin(ww)], -1).reshape(-1, 3)
-    f = []
-    for i in range(nu):
-        for j in range(nv):
-            a, b = i * nv + j, ((i + 1) % nu) * nv + j
-            c, d = ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv
-            f += [(a, b, c), (a, c, d)]
-    return v, np.array(f, np.int32)
+torus_mesh = G.torus_mesh
 
 
 def box_sdf(p, half):
