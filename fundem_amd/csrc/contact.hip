@@ -115,6 +115,7 @@ struct BroadK {
   double maxReach;
   const int* gate;
   fdm_instances inst;
+  fdm_periodic periodic;
 };
 
 template <int SET>
@@ -162,11 +163,15 @@ __device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, d
 template <int SET, class Emit>
 __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emit) {
   const int n = a.P.n;
-  const V3 xA = ldp(a.P.x, n, A);
   const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
   // with several instances: A's own grid, and its instance's cells and walls
   const bool many = a.inst.count > 0;
   const int in = many ? a.inst.particleInstance[A] : 0;
+  // ghost contacts are detected after the periodic wrap
+  const V3 xA = SET == FDM_SET_GHOST
+                    ? wrapStages(ldp(a.P.x, n, A), many ? a.inst.pgrid[in] : a.grid,
+                                 many ? a.inst.periodic[in] : a.periodic).afterY
+                    : ldp(a.P.x, n, A);
   if (SET == FDM_SET_WALL) {
     const fdm_grid& wg = many ? a.inst.wgrid[in] : a.grid;
     const int w0 = many ? a.inst.wallStart[in] : 0;
@@ -350,6 +355,15 @@ __device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const Narr
   return f;
 }
 
+// A's position as the set's detection sees it: stored (before this step's
+// periodic wrap) for particle and wall pairs, wrapped for ghost pairs, which
+// the reference detects after updateGhostSpatialGrid (LSSolver.h:1168-1171).
+template <int SET>
+__device__ __forceinline__ V3 detectPosA(const NarrowCommon& C, const PairFrame& f, int A) {
+  const V3 x = ldp(C.P.x, C.P.n, A);
+  return SET == FDM_SET_GHOST ? wrappedArm(x, *f.pgrid, f.periodic) : x;
+}
+
 // The reference's node predicate for one (node, target) candidate: the
 // node's cell is in the target's stencil, the node is within RB of a
 // particle target, inside the target's SDF box, and phi <= 0 there. The
@@ -476,9 +490,9 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
-  const V3 xA = ldp(C.P.x, n, A);
   const Q4 qA = ld4(C.P.q, n, A);
   const PairFrame fr = pairFrame<SET>(C, S, A);
+  const V3 xA = detectPosA<SET>(C, fr, A);
   const fdm_grid& tgrid = *fr.tgrid;
   Target tg;
   V3 armB;
@@ -762,9 +776,9 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
-  const V3 xA = ldp(C.P.x, n, A);
   const Q4 qA = ld4(C.P.q, n, A);
   const PairFrame fr = pairFrame<SET>(C, S, A);
+  const V3 xA = detectPosA<SET>(C, fr, A);
   const fdm_grid& tgrid = *fr.tgrid;
   Target tg;
   V3 armB;
@@ -914,8 +928,8 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
-  const V3 xA = ldp(C.P.x, n, A);
   const PairFrame fr = pairFrame<SET>(C, S, A);
+  const V3 xA = detectPosA<SET>(C, fr, A);
   Target tg;
   V3 armB;
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
@@ -1015,7 +1029,7 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
       const int A = S.pairs.pairA[p];
       if (A >= 0) {
         const int T = S.pairs.pairT[p];
-        const V3 xA = ldp(a.c.P.x, a.c.P.n, A);
+        V3 xA = ldp(a.c.P.x, a.c.P.n, A);
         const double RA = tmplOf(a.c.pgeo, a.c.P.geo[A]).radius * kReach;
         Target tg;
         V3 armB;
@@ -1027,7 +1041,9 @@ __global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* w
           resolveTarget<FDM_SET_PARTICLE>(a.c, S, p, T, pairFrame<FDM_SET_PARTICLE>(a.c, S, A), tg, armB);
           reachT = tg.t->radius;
         } else {
-          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, pairFrame<FDM_SET_GHOST>(a.c, S, A), tg, armB);
+          const PairFrame fr = pairFrame<FDM_SET_GHOST>(a.c, S, A);
+          xA = wrappedArm(xA, *fr.pgrid, fr.periodic);  // ghost pairs: after the wrap
+          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, fr, tg, armB);
           reachT = tg.t->boxReach;
         }
         const V3 d = vsub(xA, tg.x);
@@ -1130,6 +1146,7 @@ extern "C" int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* works
   k.maxReach = g.maxReach;
   k.gate = g.gate;
   k.inst = g.inst;
+  k.periodic = g.periodic;
   if (g.set == FDM_SET_GHOST && !g.ghosts) return FDM_EINVAL;
   if (!(g.skin >= 0.0) || !(g.maxReach >= 0.0)) return FDM_EINVAL;
   const unsigned nb = blocksFor((long long)n + 1, 128);
@@ -1179,6 +1196,7 @@ extern "C" int fdm_launchBroadPhaseWrite(const fdm_broadphase_args* args, unsign
   k.maxReach = g.maxReach;
   k.gate = g.gate;
   k.inst = g.inst;
+  k.periodic = g.periodic;
   if (n <= 0) return FDM_OK;
   const unsigned nb = blocksFor(n, 128);
   switch (g.set) {

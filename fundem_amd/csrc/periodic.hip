@@ -127,7 +127,9 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
 }
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
-// ghosts of each set, then store the wrapped positions.
+// ghosts of each set. The wrapped positions are stored later, by
+// kCommitWrap after the step's detection: the reference detects particle and
+// wall contacts before the wrap (LSSolver.h:1168-1171).
 // With several instances (I.count > 0) each particle uses its instance's
 // grid and periodic flags; `per` then only says which sets exist.
 __global__ void kGhostScatter(double* pos, int n, fdm_grid g0, fdm_periodic per, GhostOut ox,
@@ -163,7 +165,23 @@ __global__ void kGhostScatter(double* pos, int n, fdm_grid g0, fdm_periodic per,
     }
   }
   if (atBuild) atBuild[i] = bits;
-  stp(pos, n, i, w.afterY);
+}
+
+// The periodic wrap of the stored positions (updateGhostSpatialGrid's
+// launchUpdatePositionOutOfBoundaryXD / YD, PeriodicBoundary.h:225-231), run
+// after the step's detection; gated by *gate (a ghost rebuild, which every
+// wrap causes).
+__global__ void kCommitWrap(double* pos, int n, fdm_grid g0, fdm_periodic per, fdm_instances I,
+                            const int* gate) {
+  if (gateOff(gate)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int in = I.count > 0 ? I.particleInstance[i] : 0;
+    const fdm_grid& g = I.count > 0 ? I.pgrid[in] : g0;
+    const fdm_periodic pi = I.count > 0 ? I.periodic[in] : per;
+    const V3 x = ldp(pos, n, i);
+    const V3 w = wrapStages(x, g, pi).afterY;
+    if (!sameV3(w, x)) stp(pos, n, i, w);
+  }
 }
 
 // Steps without a ghost rebuild: no particle wrapped and every visibility bit
@@ -313,8 +331,10 @@ extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm
   const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
   hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
                      grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag, nullptr, nullptr);
-  return buildGhosts(particles, grid, periodic, sets, on, nullptr, nullptr, status, workspace,
-                     workspaceBytes, stream);
+  const int r = buildGhosts(particles, grid, periodic, sets, on, nullptr, nullptr, status, workspace,
+                            workspaceBytes, stream);
+  if (r != FDM_OK) return r;
+  return fdm_launchPeriodicCommitWrap(particles, grid, periodic, nullptr, nullptr, stream);
 }
 
 extern "C" int fdm_launchPeriodicPrepare(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
@@ -354,6 +374,19 @@ extern "C" int fdm_launchPeriodicBuildInstances(fdm_bodies particles, fdm_grid g
   const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
   return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), nullptr,
                      status, workspace, workspaceBytes, stream, inst);
+}
+
+extern "C" int fdm_launchPeriodicCommitWrap(fdm_bodies particles, fdm_grid grid,
+                                            fdm_periodic periodic, const fdm_instances* inst,
+                                            const int* gate, hipStream_t stream) {
+  const int n = particles.n;
+  if (n < 0) return FDM_EINVAL;
+  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  fdm_instances none{};
+  const unsigned nb = std::min<unsigned>(blocksFor(n, 256), 2048);
+  hipLaunchKernelGGL(kCommitWrap, dim3(nb), dim3(256), 0, stream, particles.x, n, grid, periodic,
+                     inst ? *inst : none, gate);
+  return launchStatus();
 }
 
 extern "C" int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,

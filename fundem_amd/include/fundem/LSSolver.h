@@ -997,6 +997,7 @@ class LSSolver {
     fdm_broadphase_args b{};
     b.skin = skin_;
     b.maxReach = set == FDM_SET_PARTICLE ? maxRadius_ : set == FDM_SET_GHOST ? maxBoxReach_ : 0.0;
+    b.periodic = periodic_.flags();
     b.gate = nullptr;
     b.inst = inst_;
     b.set = set;
@@ -1220,6 +1221,14 @@ class LSSolver {
       w.hitCapacity = hitCapacity_;
       w.hitCount = hitCount_.get();
       fdmCheck(fdm_launchNarrowPhaseSets(na, ns, &w, stream_), "narrow phase");
+    }
+    if (periodic_.isActived() && !gate) {
+      // the wrap of this step's positions, after its detection (a step that
+      // keeps every list, the gated one, has no wrap)
+      fdmCheck(fdm_launchPeriodicCommitWrap(P, particles_.grid(), periodic_.flags(),
+                                            inst_.count > 0 ? &inst_ : nullptr,
+                                            ctl_.get() + FDM_CTL_REBUILD_GHOST, stream_),
+               "periodic wrap");
     }
     if (prime) {
       // no forces in the pre-loop build: drop the mirror entries

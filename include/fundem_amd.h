@@ -400,6 +400,15 @@ int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_periodic pe
 int fdm_launchPeriodicBuildInstances(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
                                      const fdm_instances* inst, fdm_ghosts* sets, unsigned* status,
                                      void* workspace, size_t workspaceBytes, hipStream_t stream);
+/* The wrap of the stored positions (launchUpdatePositionOutOfBoundaryXD/YD,
+ * mySpatialGrid.cu:28-56, in updateGhostSpatialGrid's order). Build and
+ * BuildInstances compute the ghosts from the wrapped positions but leave the
+ * stored positions unwrapped: the reference detects particle and wall contacts
+ * with the positions before the wrap and ghost contacts and forces after it
+ * (LSSolver.h:1168-1178), so the step calls this after its detection. Runs
+ * when *gate != 0 (gate NULL: always). */
+int fdm_launchPeriodicCommitWrap(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
+                                 const fdm_instances* inst, const int* gate, hipStream_t stream);
 int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic,
                               fdm_ghosts* sets, const int* gate, hipStream_t stream);
 
@@ -432,6 +441,8 @@ typedef struct {
   double maxReach;         /* largest target reach: particle radius (set 0) or
                               SDF box reach (set 2) over all templates */
   const int* gate;         /* run only when *gate != 0 (NULL: always) */
+  fdm_periodic periodic;   /* ghost sets: the wrap applied to A's stored position (ghost
+                              contacts are detected after the wrap, fdm_launchPeriodicCommitWrap) */
 } fdm_broadphase_args;
 int fdm_launchBroadPhase(const fdm_broadphase_args* args, void* workspace, size_t workspaceBytes,
                          hipStream_t stream);
