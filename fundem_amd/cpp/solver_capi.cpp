@@ -276,6 +276,13 @@ int fdmx_set_periodic(void* s, int x, int y) {
     if (y) S->addPeriodicBoundaryYD();
   });
 }
+int fdmx_set_sector(void* s, int on) {
+  return guarded([&] {
+    auto* S = static_cast<BoundSolver*>(s);
+    S->removePeriodicBoundary();
+    if (on) S->addPeriodicBoundarySector();
+  });
+}
 int fdmx_add_system(void* s, const double* mn, const double* mx, int px, int py) {
   int k = -1;
   const int r = guarded([&] {

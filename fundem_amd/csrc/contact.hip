@@ -169,8 +169,8 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
   const int in = many ? a.inst.particleInstance[A] : 0;
   // ghost contacts are detected after the periodic wrap
   const V3 xA = SET == FDM_SET_GHOST
-                    ? wrapStages(ldp(a.P.x, n, A), many ? a.inst.pgrid[in] : a.grid,
-                                 many ? a.inst.periodic[in] : a.periodic).afterY
+                    ? wrapPosition(ldp(a.P.x, n, A), many ? a.inst.pgrid[in] : a.grid,
+                                   many ? a.inst.periodic[in] : a.periodic)
                     : ldp(a.P.x, n, A);
   if (SET == FDM_SET_WALL) {
     const fdm_grid& wg = many ? a.inst.wgrid[in] : a.grid;
@@ -298,6 +298,8 @@ struct NarrowSet {
   int ghostFromReal;  // ghost sets: derive positions from the real particles
   const double* ghostPos;
   const int* ghostHash;
+  const double* ghostQ;    // sector images: orientation and velocity of each ghost
+  const double* ghostVel;
   fdm_grid grid;
   const int* targetHash;
   fdm_pairs pairs;
@@ -320,16 +322,8 @@ struct NarrowMulti {
 
 // Position used for force arms: the stored position after this step's
 // periodic wrap (the reference computes forces after updateGhostSpatialGrid).
-__device__ __forceinline__ V3 wrappedArm(V3 x, const fdm_grid& g, fdm_periodic per) {
-  if (per.x) {
-    if (x.x < g.min.x) x = vadd(x, {g.max.x - g.min.x, 0., 0.});
-    else if (x.x >= g.max.x) x = vsub(x, {g.max.x - g.min.x, 0., 0.});
-  }
-  if (per.y) {
-    if (x.y < g.min.y) x = vadd(x, {0., g.max.y - g.min.y, 0.});
-    else if (x.y >= g.max.y) x = vsub(x, {0., g.max.y - g.min.y, 0.});
-  }
-  return x;
+__device__ __forceinline__ V3 wrappedArm(V3 x, const fdm_grid& g, const fdm_periodic& per) {
+  return wrapPosition(x, g, per);
 }
 
 // The grids and periodic flags that apply to a pair whose A is `A`: its
@@ -362,6 +356,22 @@ template <int SET>
 __device__ __forceinline__ V3 detectPosA(const NarrowCommon& C, const PairFrame& f, int A) {
   const V3 x = ldp(C.P.x, C.P.n, A);
   return SET == FDM_SET_GHOST ? wrappedArm(x, *f.pgrid, f.periodic) : x;
+}
+// A's orientation as the set's detection sees it: a sector turn of this
+// step's wrap rotates it for ghost pairs.
+template <int SET>
+__device__ __forceinline__ Q4 detectQA(const NarrowCommon& C, const PairFrame& f, int A) {
+  const Q4 q = ld4(C.P.q, C.P.n, A);
+  if (SET != FDM_SET_GHOST || !f.periodic.sector) return q;
+  return sectorTurnQ(q, sectorWrap(ldp(C.P.x, C.P.n, A), *f.pgrid, f.periodic).turns, f.periodic);
+}
+// A particle's velocity as the contact forces see it: after this step's wrap,
+// which turns it in a sector (the force kernels run after
+// updateGhostSpatialGrid, LSSolver.h:1170-1175).
+__device__ __forceinline__ V3 forceVel(const NarrowCommon& C, const PairFrame& f, int i) {
+  const V3 v = ld3(C.P.v, C.P.n, i);
+  if (!f.periodic.sector) return v;
+  return sectorTurnV(v, sectorWrap(ldp(C.P.x, C.P.n, i), *f.pgrid, f.periodic).turns, f.periodic);
 }
 
 // The reference's node predicate for one (node, target) candidate: the
@@ -454,7 +464,12 @@ __device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const Narro
       tg.x = ldd3(S.ghostPos + 3 * G);
       tg.hash = S.ghostHash[G];
     }
-    tg.q = ld4(C.P.q, n, T);
+    if (S.ghostQ) {
+      const double* gq = S.ghostQ + 4 * (size_t)S.pairs.pairG[p];
+      tg.q = {gq[0], gq[1], gq[2], gq[3]};
+    } else {
+      tg.q = ld4(C.P.q, n, T);
+    }
     tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
     tg.sdf = C.pgeo.sdf;
     armB = tg.x;
@@ -490,8 +505,8 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
-  const Q4 qA = ld4(C.P.q, n, A);
   const PairFrame fr = pairFrame<SET>(C, S, A);
+  const Q4 qA = detectQA<SET>(C, fr, A);
   const V3 xA = detectPosA<SET>(C, fr, A);
   const fdm_grid& tgrid = *fr.tgrid;
   Target tg;
@@ -626,10 +641,12 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
       if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
       diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
       mu = fmin(Mi.mu, Mj.mu);
-      vB = ld3(C.P.v, n, T);
-      wB = ld3(C.P.w, n, T);
+      vB = SET == FDM_SET_GHOST && S.ghostVel
+               ? ldd3(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
+               : forceVel(C, fr, T);
+      wB = ld3(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
     }
-    const V3 vA = ld3(C.P.v, n, A);
+    const V3 vA = forceVel(C, fr, A);
     const V3 wA = ld3(C.P.w, n, A);
     // contact law of one contact: its spring is updated in place and stored
     // with its node id as record r
@@ -776,8 +793,8 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
-  const Q4 qA = ld4(C.P.q, n, A);
   const PairFrame fr = pairFrame<SET>(C, S, A);
+  const Q4 qA = detectQA<SET>(C, fr, A);
   const V3 xA = detectPosA<SET>(C, fr, A);
   const fdm_grid& tgrid = *fr.tgrid;
   Target tg;
@@ -958,10 +975,11 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
     if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
     diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
     mu = fmin(Mi.mu, Mj.mu);
-    vB = ld3(C.P.v, n, T);
-    wB = ld3(C.P.w, n, T);
+    vB = SET == FDM_SET_GHOST && S.ghostVel ? ldd3(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
+                                            : forceVel(C, fr, T);
+    wB = ld3(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
   }
-  const V3 vA = ld3(C.P.v, n, A);
+  const V3 vA = forceVel(C, fr, A);
   const V3 wA = ld3(C.P.w, n, A);
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   for (int h = 0; h < cnt; ++h) {
@@ -1241,6 +1259,8 @@ void fillSet(NarrowSet& k, const fdm_narrowphase_args& g) {
   k.ghostFromReal = g.ghostFromReal;
   k.ghostPos = g.ghosts ? g.ghosts->pos : nullptr;
   k.ghostHash = g.ghosts ? g.ghosts->table.hashValue : nullptr;
+  k.ghostQ = g.set == FDM_SET_GHOST && g.periodic.sector && g.ghosts ? g.ghosts->q : nullptr;
+  k.ghostVel = g.set == FDM_SET_GHOST && g.periodic.sector && g.ghosts ? g.ghosts->vel : nullptr;
   k.grid = g.grid;
   k.targetHash = g.targetHash;
   k.pairs = g.pairs;
@@ -1253,7 +1273,10 @@ bool validSet(const fdm_narrowphase_args& g) {
   if (g.set == FDM_SET_GHOST) {
     if (g.ghostMask < 1 || g.ghostMask > 3) return false;
     if (!g.ghostFromReal && !g.ghosts) return false;
+    // sector images carry their own orientation and velocity
+    if (g.periodic.sector && (g.ghostFromReal || !g.ghosts->q || !g.ghosts->vel)) return false;
   }
+  if (g.periodic.sector && (g.periodic.x || g.periodic.y || g.inst.count > 0)) return false;
   return true;
 }
 

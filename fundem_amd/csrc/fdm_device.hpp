@@ -205,6 +205,92 @@ __device__ __forceinline__ WrapStages wrapStages(V3 x, const fdm_grid& g, fdm_pe
   return w;
 }
 
+// ------------------------------------------------------ sector periodicity
+// PeriodicBoundarySector (PeriodicBoundary.h:319-426): the quarter plane
+// x >= min.x, y >= min.y turns about the z axis through min. Image k: 0 is
+// +pi/2 (R90), 1 is pi (R180), 2 is -pi/2 (R270).
+__host__ __device__ __forceinline__ bool anyPeriodic(const fdm_periodic& p) {
+  return p.x || p.y || p.sector;
+}
+// whether ghost set s (planar X, Y, XY or sector R90, R180, R270) exists
+__host__ __device__ __forceinline__ bool ghostSetOn(const fdm_periodic& p, int s) {
+  return p.sector ? true : s == 0 ? p.x != 0 : s == 1 ? p.y != 0 : (p.x != 0 && p.y != 0);
+}
+// rotateVector (myVec.h:143-181) by image k's angle, with its host-evaluated
+// coefficients
+__device__ __forceinline__ V3 sectorRotV(V3 v, const fdm_periodic& p, int k) {
+  const V3 om = {0.0, 0.0, p.angle[k]};
+  const V3 oxv = vcross(om, v);
+  const V3 oxoxv = vcross(om, oxv);
+  return vadd(vadd(v, vmul(oxv, p.A[k])), vmul(oxoxv, p.B[k]));
+}
+// rotateQuaternion (myQua.h:131-175): the turn's quaternion times q, normalised
+__device__ __forceinline__ Q4 sectorRotQ(Q4 q, const fdm_periodic& p, int k) {
+  const double th = p.angle[k];
+  const double theta = sqrt(0.0 * 0.0 + 0.0 * 0.0 + th * th);
+  const double inv = 1.0 / theta;
+  const double ux = 0.0 * inv, uy = 0.0 * inv, uz = th * inv;
+  const double r0 = p.halfCos[k], r1 = p.halfSin[k] * ux, r2 = p.halfSin[k] * uy,
+               r3 = p.halfSin[k] * uz;
+  Q4 r;
+  r.q0 = r0 * q.q0 - r1 * q.q1 - r2 * q.q2 - r3 * q.q3;
+  r.q1 = r0 * q.q1 + r1 * q.q0 + r2 * q.q3 - r3 * q.q2;
+  r.q2 = r0 * q.q2 - r1 * q.q3 + r2 * q.q0 + r3 * q.q1;
+  r.q3 = r0 * q.q3 + r1 * q.q2 - r2 * q.q1 + r3 * q.q0;
+  return qnormalize(r);
+}
+__device__ __forceinline__ V3 sectorImagePos(V3 x, const fdm_grid& g, const fdm_periodic& p, int k) {
+  const V3 mn = fromD3(g.min);
+  return vadd(mn, sectorRotV(vsub(x, mn), p, k));
+}
+// updateVelocityPositionOrientationOutOfSectorKernel (mySpatialGrid.cu:59-85):
+// the position after the sector wrap and which turns it took (bit 0: -pi/2
+// through the x = min face, bit 1: +pi/2 through the y = min face, in that
+// order, the second test on the turned position)
+struct SectorWrap {
+  V3 x;
+  int turns;
+};
+__device__ __forceinline__ SectorWrap sectorWrap(V3 x, const fdm_grid& g, const fdm_periodic& p) {
+  SectorWrap w{x, 0};
+  if (w.x.x < g.min.x && w.x.y >= g.min.y) {
+    w.x = sectorImagePos(w.x, g, p, 2);
+    w.turns |= 1;
+  }
+  if (w.x.y < g.min.y && w.x.x >= g.min.x) {
+    w.x = sectorImagePos(w.x, g, p, 0);
+    w.turns |= 2;
+  }
+  return w;
+}
+// The same turns applied to a velocity and an orientation; a turned value is
+// stored (FP32), so it is used rounded from the wrap on.
+__device__ __forceinline__ V3 sectorTurnV(V3 v, int turns, const fdm_periodic& p) {
+  if (!turns) return v;
+  if (turns & 1) v = sectorRotV(v, p, 2);
+  if (turns & 2) v = sectorRotV(v, p, 0);
+  return round32(v);
+}
+__device__ __forceinline__ Q4 sectorTurnQ(Q4 q, int turns, const fdm_periodic& p) {
+  if (!turns) return q;
+  if (turns & 1) q = sectorRotQ(q, p, 2);
+  if (turns & 2) q = sectorRotQ(q, p, 0);
+  return {(double)(float)q.q0, (double)(float)q.q1, (double)(float)q.q2, (double)(float)q.q3};
+}
+// calculateSectorGhostVelocityPositionOrientationKernel (mySpatialGrid.cu:154-212):
+// image k of a wrapped particle is turned when it lies in the first y row
+// (R90), first x column (R270) or both (R180); otherwise the set holds the
+// particle's own pose (a ghost only if that lies outside the domain)
+__device__ __forceinline__ bool sectorImageOn(V3 xw, const fdm_grid& g, int k) {
+  const I3 c = gridPos(xw, g);
+  return k == 0 ? c.y == 0 : k == 2 ? c.x == 0 : (c.x == 0 && c.y == 0);
+}
+// The position after this step's periodic wrap, planar or sector.
+__device__ __forceinline__ V3 wrapPosition(V3 x, const fdm_grid& g, const fdm_periodic& p) {
+  if (p.sector) return sectorWrap(x, g, p).x;
+  return wrapStages(x, g, p).afterY;
+}
+
 // Sort key of a body of one instance among several: 1 + global cell, or 0
 // when the body is outside its instance's grid (ids rounded up past the last
 // cell are kept out too, as the table never holds them).

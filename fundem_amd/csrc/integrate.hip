@@ -202,7 +202,19 @@ __device__ __forceinline__ void stepBeginBody(const fdm_step_begin_args& a, int 
   }
   const int st = bodyState(x, grid);
   if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
-  if (!WALL && (a.periodic.x || a.periodic.y)) {
+  if (!WALL && a.periodic.sector) {
+    // the sector's R90, R180, R270 images of the wrapped particle
+    const SectorWrap sw = sectorWrap(x, grid, a.periodic);
+    int bits = 0;
+    for (int k = 0; k < 3; ++k) {
+      const V3 gp = sectorImageOn(sw.x, grid, k) ? sectorImagePos(sw.x, grid, a.periodic, k) : sw.x;
+      const int f = ghostCellHash(gp, grid) >= 0 ? 1 : 0;
+      a.ghostFlag[k][j] = f;
+      bits |= f << k;
+    }
+    if (bits != a.ghostAtBuild[j]) flags |= 2;
+    if (sw.turns) flags |= 4;
+  } else if (!WALL && (a.periodic.x || a.periodic.y)) {
     // sets exist where any instance is periodic; an instance contributes
     // only to the sets of its own periodic axes
     const WrapStages w = wrapStages(x, grid, perI);
@@ -357,11 +369,11 @@ int stepBeginSetup(const fdm_step_begin_args* args, const fdm_skin* skin, void* 
   if (a.particles.n < 0 || a.walls.n < 0) return FDM_EINVAL;
   if (a.particles.n > 0 && !a.ptable.validAtBuild) return FDM_EINVAL;
   if (a.walls.n > 0 && !a.wtable.validAtBuild) return FDM_EINVAL;
-  if (a.periodic.x || a.periodic.y) {
+  if (anyPeriodic(a.periodic)) {
     if (!a.ghostAtBuild) return FDM_EINVAL;
-    if (a.periodic.x && !a.ghostFlag[0]) return FDM_EINVAL;
-    if (a.periodic.y && !a.ghostFlag[1]) return FDM_EINVAL;
-    if (a.periodic.x && a.periodic.y && !a.ghostFlag[2]) return FDM_EINVAL;
+    if (a.periodic.sector && (a.periodic.x || a.periodic.y || a.inst.count > 0)) return FDM_EINVAL;
+    for (int k = 0; k < 3; ++k)
+      if (ghostSetOn(a.periodic, k) && !a.ghostFlag[k]) return FDM_EINVAL;
   }
   const long long total = (long long)a.particles.n + a.walls.n;
   if (workspaceBytes < fdm_stepBeginWorkspaceBytes((size_t)total)) return FDM_EINVAL;

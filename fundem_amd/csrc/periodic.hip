@@ -8,6 +8,7 @@
 // node), compacts them in particle order and builds one hash table per set.
 #include <algorithm>
 #include <climits>
+#include <cmath>
 
 #include "fdm_device.hpp"
 #include "fdm_launch.hpp"
@@ -62,7 +63,20 @@ __global__ void kGhostFlags(const double* pos, int n, fdm_grid g, fdm_periodic p
     const V3 x = ldp(pos, n, i);
     const WrapStages w = wrapStages(x, g, per);
     int bits = 0;
-    if (per.x) {
+    if (per.sector) {
+      const SectorWrap sw = sectorWrap(x, g, per);
+      int* fl[3] = {fx, fy, fxy};
+      for (int k = 0; k < 3; ++k) {
+        const V3 gp = sectorImageOn(sw.x, g, k) ? sectorImagePos(sw.x, g, per, k) : sw.x;
+        const int f = ghostCellHash(gp, g) >= 0 ? 1 : 0;
+        fl[k][i] = f;
+        bits |= f << k;
+      }
+      if (ctl) {
+        wrapped = sw.turns != 0;
+        changed = atBuild && bits != atBuild[i];
+      }
+    } else if (per.x) {
       const int f = ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0 ? 1 : 0;
       fx[i] = f;
       bits |= f;
@@ -98,14 +112,16 @@ struct GhostOut {
   int* cellBox;
   int capacity;
   unsigned* status;
+  double* q;    // sector images: orientation and velocity
+  double* vel;
 };
 
-__device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const fdm_grid& g,
-                                          long long cellOffset) {
+__device__ __forceinline__ int emitGhost(const GhostOut& o, int i, V3 gp, const fdm_grid& g,
+                                         long long cellOffset) {
   const int s = o.slot[i];
   if (s >= o.capacity) {
     atomicOr(o.status, FDM_STATUS_GHOST_OVERFLOW);
-    return;
+    return -1;
   }
   const int h = ghostCellHash(gp, g);
   o.realId[s] = i;
@@ -124,6 +140,7 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
   atomicMax(&o.cellBox[3], c.x);
   atomicMax(&o.cellBox[4], c.y);
   atomicMax(&o.cellBox[5], c.z);
+  return s;
 }
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
@@ -132,11 +149,42 @@ __device__ __forceinline__ void emitGhost(const GhostOut& o, int i, V3 gp, const
 // wall contacts before the wrap (LSSolver.h:1168-1171).
 // With several instances (I.count > 0) each particle uses its instance's
 // grid and periodic flags; `per` then only says which sets exist.
-__global__ void kGhostScatter(double* pos, int n, fdm_grid g0, fdm_periodic per, GhostOut ox,
-                              GhostOut oy, GhostOut oxy, int* atBuild, fdm_instances I) {
+__global__ void kGhostScatter(const double* pos, const float* vel, const float* quat, int n,
+                              fdm_grid g0, fdm_periodic per, GhostOut ox, GhostOut oy, GhostOut oxy,
+                              int* atBuild, fdm_instances I) {
   if (gateOff(ox.gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (per.sector) {  // one system: the sector is not combined with instances
+    const SectorWrap sw = sectorWrap(ldp(pos, n, i), g0, per);
+    const V3 v = sectorTurnV(ld3(vel, n, i), sw.turns, per);
+    const Q4 q = sectorTurnQ(ld4(quat, n, i), sw.turns, per);
+    const GhostOut* o[3] = {&ox, &oy, &oxy};
+    int bits = 0;
+    for (int k = 0; k < 3; ++k) {
+      // a particle outside its image rows keeps its own pose in every set,
+      // and that copy is a ghost too when it lies outside the domain
+      const bool turn = sectorImageOn(sw.x, g0, k);
+      const V3 gp = turn ? sectorImagePos(sw.x, g0, per, k) : sw.x;
+      if (ghostCellHash(gp, g0) < 0) continue;
+      bits |= 1 << k;
+      const int s = emitGhost(*o[k], i, gp, g0, 0);
+      if (s < 0) continue;
+      const Q4 gq = turn ? sectorRotQ(q, per, k) : q;
+      const V3 gv = turn ? sectorRotV(v, per, k) : v;
+      double* oq = o[k]->q + 4 * (size_t)s;
+      oq[0] = gq.q0;
+      oq[1] = gq.q1;
+      oq[2] = gq.q2;
+      oq[3] = gq.q3;
+      double* ov = o[k]->vel + 3 * (size_t)s;
+      ov[0] = gv.x;
+      ov[1] = gv.y;
+      ov[2] = gv.z;
+    }
+    if (atBuild) atBuild[i] = bits;
+    return;
+  }
   const int in = I.count > 0 ? I.particleInstance[i] : 0;
   const fdm_grid& g = I.count > 0 ? I.pgrid[in] : g0;
   const fdm_periodic pi = I.count > 0 ? I.periodic[in] : per;
@@ -171,10 +219,18 @@ __global__ void kGhostScatter(double* pos, int n, fdm_grid g0, fdm_periodic per,
 // launchUpdatePositionOutOfBoundaryXD / YD, PeriodicBoundary.h:225-231), run
 // after the step's detection; gated by *gate (a ghost rebuild, which every
 // wrap causes).
-__global__ void kCommitWrap(double* pos, int n, fdm_grid g0, fdm_periodic per, fdm_instances I,
-                            const int* gate) {
+__global__ void kCommitWrap(double* pos, float* vel, float* quat, int n, fdm_grid g0,
+                            fdm_periodic per, fdm_instances I, const int* gate) {
   if (gateOff(gate)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (per.sector) {  // updateVelocityPositionOrientationOutOfSectorKernel
+      const SectorWrap sw = sectorWrap(ldp(pos, n, i), g0, per);
+      if (!sw.turns) continue;
+      stp(pos, n, i, sw.x);
+      st3(vel, n, i, sectorTurnV(ld3(vel, n, i), sw.turns, per));
+      st4(quat, n, i, sectorTurnQ(ld4(quat, n, i), sw.turns, per));
+      continue;
+    }
     const int in = I.count > 0 ? I.particleInstance[i] : 0;
     const fdm_grid& g = I.count > 0 ? I.pgrid[in] : g0;
     const fdm_periodic pi = I.count > 0 ? I.periodic[in] : per;
@@ -222,6 +278,29 @@ __global__ void kClampCount(int* count, int capacity, const int* gate) {
 }  // namespace fdm
 
 using namespace fdm;
+
+// The sector's rotation constants, evaluated as rotateVector's and
+// rotateQuaternion's host branches evaluate them (myVec.h:143-181,
+// myQua.h:131-175), for angles +pi/2, pi and -pi/2 about z.
+extern "C" void fdm_sectorPeriodic(fdm_periodic* p) {
+  if (!p) return;
+  *p = fdm_periodic{};
+  p->sector = 1;
+  const double ang[3] = {0.5 * kPi, kPi, -0.5 * kPi};
+  for (int k = 0; k < 3; ++k) {
+    const volatile double thv = ang[k];  // evaluated by the C library at run time
+    const double th = thv;
+    const double theta2 = 0.0 * 0.0 + 0.0 * 0.0 + th * th;
+    const double invTheta = 1.0 / std::sqrt(theta2);
+    const double theta = theta2 * invTheta;
+    p->angle[k] = th;
+    p->A[k] = std::sin(theta) * invTheta;
+    p->B[k] = (1.0 - std::cos(theta)) * (invTheta * invTheta);
+    const double half = 0.5 * std::sqrt(theta2);
+    p->halfCos[k] = std::cos(half);
+    p->halfSin[k] = std::sin(half);
+  }
+}
 
 extern "C" int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const double* position,
                                                        int n, fdm_grid grid, void* workspace,
@@ -306,9 +385,12 @@ int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_
     o[s].cellBox = sets[s].cellBox;
     o[s].capacity = sets[s].capacity;
     o[s].status = status;
+    o[s].q = sets[s].q;
+    o[s].vel = sets[s].vel;
+    if (periodic.sector && on[s] && (!o[s].q || !o[s].vel)) return FDM_EINVAL;
   }
-  hipLaunchKernelGGL(kGhostScatter, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
-                     grid, periodic, o[0], o[1], o[2], atBuild, I);
+  hipLaunchKernelGGL(kGhostScatter, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x,
+                     particles.v, particles.q, n, grid, periodic, o[0], o[1], o[2], atBuild, I);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
     hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity,
@@ -327,8 +409,9 @@ extern "C" int fdm_launchPeriodicGhosts(fdm_bodies particles, fdm_grid grid, fdm
                                         size_t workspaceBytes, hipStream_t stream) {
   const int n = particles.n;
   if (n < 0 || !sets || !status) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  if (periodic.sector && (periodic.x || periodic.y)) return FDM_EINVAL;
+  const bool on[3] = {ghostSetOn(periodic, 0), ghostSetOn(periodic, 1), ghostSetOn(periodic, 2)};
   hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
                      grid, periodic, sets[0].flag, sets[1].flag, sets[2].flag, nullptr, nullptr);
   const int r = buildGhosts(particles, grid, periodic, sets, on, nullptr, nullptr, status, workspace,
@@ -342,8 +425,9 @@ extern "C" int fdm_launchPeriodicPrepare(fdm_bodies particles, fdm_grid grid, fd
                                          hipStream_t stream) {
   const int n = particles.n;
   if (n < 0 || !sets || !skin || !skin->ctl) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  if (periodic.sector && (periodic.x || periodic.y)) return FDM_EINVAL;
+  const bool on[3] = {ghostSetOn(periodic, 0), ghostSetOn(periodic, 1), ghostSetOn(periodic, 2)};
   const int* atBuild = firstAtBuild(sets, on);
   if (!atBuild) return FDM_EINVAL;
   hipLaunchKernelGGL(kGhostFlags, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x, n,
@@ -357,8 +441,9 @@ extern "C" int fdm_launchPeriodicBuild(fdm_bodies particles, fdm_grid grid, fdm_
                                        hipStream_t stream) {
   const int n = particles.n;
   if (n < 0 || !sets || !status) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  if (periodic.sector && (periodic.x || periodic.y)) return FDM_EINVAL;
+  const bool on[3] = {ghostSetOn(periodic, 0), ghostSetOn(periodic, 1), ghostSetOn(periodic, 2)};
   return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), gate, status,
                      workspace, workspaceBytes, stream);
 }
@@ -370,8 +455,9 @@ extern "C" int fdm_launchPeriodicBuildInstances(fdm_bodies particles, fdm_grid g
                                                 hipStream_t stream) {
   const int n = particles.n;
   if (n < 0 || !sets || !status || !inst || inst->count <= 0) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  if (periodic.sector) return FDM_EINVAL;  // the sector runs as one system
+  const bool on[3] = {ghostSetOn(periodic, 0), ghostSetOn(periodic, 1), ghostSetOn(periodic, 2)};
   return buildGhosts(particles, grid, periodic, sets, on, (int*)firstAtBuild(sets, on), nullptr,
                      status, workspace, workspaceBytes, stream, inst);
 }
@@ -381,11 +467,12 @@ extern "C" int fdm_launchPeriodicCommitWrap(fdm_bodies particles, fdm_grid grid,
                                             const int* gate, hipStream_t stream) {
   const int n = particles.n;
   if (n < 0) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  if (periodic.sector && inst && inst->count > 0) return FDM_EINVAL;
   fdm_instances none{};
   const unsigned nb = std::min<unsigned>(blocksFor(n, 256), 2048);
-  hipLaunchKernelGGL(kCommitWrap, dim3(nb), dim3(256), 0, stream, particles.x, n, grid, periodic,
-                     inst ? *inst : none, gate);
+  hipLaunchKernelGGL(kCommitWrap, dim3(nb), dim3(256), 0, stream, particles.x, particles.v,
+                     particles.q, n, grid, periodic, inst ? *inst : none, gate);
   return launchStatus();
 }
 
@@ -393,8 +480,11 @@ extern "C" int fdm_launchPeriodicRefresh(fdm_bodies particles, fdm_grid grid, fd
                                          fdm_ghosts* sets, const int* gate, hipStream_t stream) {
   const int n = particles.n;
   if (n < 0 || !sets) return FDM_EINVAL;
-  if (n == 0 || (!periodic.x && !periodic.y)) return FDM_OK;
-  const bool on[3] = {periodic.x != 0, periodic.y != 0, periodic.x && periodic.y};
+  if (n == 0 || !anyPeriodic(periodic)) return FDM_OK;
+  // sector images are rebuilt every step (their orientation and velocity
+  // are not refreshed here)
+  if (periodic.sector) return FDM_EINVAL;
+  const bool on[3] = {ghostSetOn(periodic, 0), ghostSetOn(periodic, 1), ghostSetOn(periodic, 2)};
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
     const unsigned nb = std::min<unsigned>(blocksFor(sets[s].capacity, 256), 256);

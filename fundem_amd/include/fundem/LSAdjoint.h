@@ -17,7 +17,7 @@
 //
 // Supported: particle-particle, particle-wall (walls with prescribed motion)
 // and periodic ghost contacts, several independent systems in one solver.
-// Not supported: bonds, external-force hooks.
+// Not supported: bonds, external-force hooks, the sector periodic boundary.
 #pragma once
 #include <algorithm>
 #include <array>
@@ -50,6 +50,8 @@ class LSAdjoint {
     if (steps < 1) throw std::invalid_argument("forward needs at least one step");
     if (s_.particles_.num_device() == 0) throw std::logic_error("initialise the solver first");
     if (s_.bonds_.view().n > 0) throw std::invalid_argument("bonds are not differentiated");
+    if (s_.periodic_.isSector())
+      throw std::invalid_argument("the sector periodic boundary is not differentiated");
     steps_ = steps;
     every_ = std::max<size_t>(1, std::min(checkpointEvery, steps));
     setup();

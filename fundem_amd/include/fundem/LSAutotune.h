@@ -1,7 +1,8 @@
 // LSAutotuner: picks the settings that do not change results for the
 // fastest step on the solver's own system (no counterpart in the reference):
 //   - the candidate-list skin (rebuild distance), as a fraction of the
-//     largest particle radius;
+//     largest particle radius (not with the sector periodic boundary, whose
+//     images are rebuilt every step);
 //   - the particle grid's cell size, as a multiple of the configured one
 //     (only for one non-periodic system without bonds: there every candidate
 //     is at least the largest radius, so the 27-cell stencil finds every
@@ -53,12 +54,15 @@ class LSAutotuner {
     const size_t window = std::max<size_t>(5 * trialSteps, 60);
     const double R = s_.maxRadius_;
     std::vector<TuneChoice> group;
-    for (double f : {0.05, 0.1, 0.2, 0.35}) {
-      TuneChoice c = best;
-      c.skin = f * R;
-      group.push_back(c);
+    double bestCost = 0.0;
+    if (!s_.periodic_.isSector()) {  // sector images are rebuilt every step: no skin
+      for (double f : {0.05, 0.1, 0.2, 0.35}) {
+        TuneChoice c = best;
+        c.skin = f * R;
+        group.push_back(c);
+      }
+      bestCost = pick(group, window, best, "skin");
     }
-    double bestCost = pick(group, window, best, "skin");
     const bool cellOk = tuneCellSize && s_.systems_.empty() && !s_.periodic_.isActived() &&
                         s_.bonds_.view().n == 0;
     if (cellOk) {

@@ -140,7 +140,15 @@ class LSSolver {
   }
   void addPeriodicBoundaryXD() { periodic_.turnXOn(); }
   void addPeriodicBoundaryYD() { periodic_.turnYOn(); }
+  // LSSolver::addPeriodicBoundarySector (LSSolver.h:410-413): the quarter
+  // plane x >= min.x, y >= min.y of the domain is one sector of a body of
+  // revolution about the z axis through min; a particle leaving through one
+  // face re-enters through the other turned by -+pi/2, and R90, R180, R270
+  // images stand in for the neighbouring sectors. One system, not combined
+  // with planar periodicity; candidate lists are rebuilt every step.
+  void addPeriodicBoundarySector() { periodic_.turnSectorOn(); }
   void removePeriodicBoundary() {
+    periodic_.turnSectorOff();
     periodic_.turnXOff();
     periodic_.turnYOff();
   }
@@ -276,7 +284,12 @@ class LSSolver {
       const fdm_template& t = walls_.templateInfo(g);
       rhoMax_ = std::max({rhoMax_, t.radius, t.boxReach});
     }
+    if (periodic_.isSector() && periodic_.isPlanar())
+      throw std::invalid_argument("the sector periodic boundary excludes planar periodicity");
+    if (periodic_.isSector() && !systems_.empty())
+      throw std::invalid_argument("the sector periodic boundary runs as one system");
     skin_ = skinRequest_ >= 0.0 ? skinRequest_ : 0.1 * maxR;
+    if (periodic_.isSector()) skin_ = 0.0;  // images are rebuilt every step
     ctl_.allocate(FDM_CTL_WORDS);
     disp_.allocate(1);
     {
@@ -1302,7 +1315,8 @@ class LSSolver {
 
   void stepOnce(bool prime, double time) {
     phaseA(prime);
-    if (prime || hookMode_ != HookNone) {
+    // sector images are rebuilt every step: no step keeps its lists
+    if (prime || hookMode_ != HookNone || periodic_.isSector()) {
       phaseB(prime, time, waitDecision());
       return;
     }
@@ -1336,16 +1350,18 @@ class LSSolver {
     hipCheck(hipGraphLaunch(ga, stream_), "graph launch");
     const GenState before = saveGenerations();
     const Decision keep = {false, false};
-    hipGraphExec_t& gs = graphsKeep_[stepNext][pp_.listIndex()][ghostListIndex()];
-    if (!gs) {
-      gs = capture([&] {
+    if (!periodic_.isSector()) {  // sector images are rebuilt every step
+      hipGraphExec_t& gs = graphsKeep_[stepNext][pp_.listIndex()][ghostListIndex()];
+      if (!gs) {
+        gs = capture([&] {
+          beginSets(keep);
+          phaseBCommon(false, 0.0, keep, ctl_.get() + FDM_CTL_KEEP);
+        });
+      } else {
         beginSets(keep);
-        phaseBCommon(false, 0.0, keep, ctl_.get() + FDM_CTL_KEEP);
-      });
-    } else {
-      beginSets(keep);
+      }
+      hipCheck(hipGraphLaunch(gs, stream_), "graph launch");
     }
-    hipCheck(hipGraphLaunch(gs, stream_), "graph launch");
     const Decision d = waitDecision();
     if (!(d.rebuild || d.ghost)) return;
     restoreGenerations(before);

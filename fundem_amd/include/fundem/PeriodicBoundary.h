@@ -1,9 +1,12 @@
-// PeriodicBoundaryXY2D: planar periodicity in x and/or y (kernel/
-// PeriodicBoundary.h:178-317). The reference keeps a full ghost copy of every
-// particle per set (X, Y, XY) and hashes all of them; only ghosts whose ghost
-// hash is valid can ever be found by a node, so the device keeps just those,
-// compacted in particle order, with one hash table and one interaction set
-// per active ghost set.
+// Periodic boundaries: planar periodicity in x and/or y
+// (PeriodicBoundaryXY2D, kernel/PeriodicBoundary.h:178-317) or the
+// quarter-plane sector about the z axis through the domain's min corner
+// (PeriodicBoundarySector, :319-426), whose three ghost sets are the R90, R180
+// and R270 images with turned orientations and velocities. The reference
+// keeps a full ghost copy of every particle per set and hashes all of them;
+// only ghosts whose ghost hash is valid can ever be found by a node, so the
+// device keeps just those, compacted in particle order, with one hash table
+// and one interaction set per active ghost set.
 #pragma once
 #include <algorithm>
 
@@ -17,15 +20,21 @@ namespace fundem {
 struct GhostSet {
   int capacity = 0;
   DeviceBuffer<int> count, flag, realId, cellBox;
-  DeviceBuffer<double> pos;
+  DeviceBuffer<double> pos, q, vel;
   HashTableBuffers table;
   SolidInteraction interaction;
-  void allocate(size_t n, int cap, long long numGrid) {
+  bool posed = false;  // holds image orientations and velocities (sector)
+  void allocate(size_t n, int cap, long long numGrid, bool withPose) {
     capacity = std::max(cap, 1);
     count.allocate(1);
     flag.allocate(std::max<size_t>(n, 1));
     realId.allocate(capacity);
     pos.allocate(3 * (size_t)capacity);
+    if (withPose) {
+      q.allocate(4 * (size_t)capacity);
+      vel.allocate(3 * (size_t)capacity);
+    }
+    posed = withPose;
     cellBox.allocate(6);
     table.allocate(capacity, numGrid);
   }
@@ -37,6 +46,8 @@ struct GhostSet {
     g.flag = flag.get();
     g.realId = realId.get();
     g.pos = pos.get();
+    g.q = posed ? q.get() : nullptr;
+    g.vel = posed ? vel.get() : nullptr;
     g.table = table.view();
     g.cellBox = cellBox.get();
     return g;
@@ -49,10 +60,18 @@ struct PeriodicBoundaryXY2D {
   void turnYOn() { y_ = true; }
   void turnXOff() { x_ = false; }
   void turnYOff() { y_ = false; }
-  bool isActived() const { return x_ || y_; }
-  bool active(int set) const { return set == 0 ? x_ : set == 1 ? y_ : (x_ && y_); }
+  void turnSectorOn() {
+    sector_ = true;
+    fdm_sectorPeriodic(&sectorFlags_);
+  }
+  void turnSectorOff() { sector_ = false; }
+  bool isActived() const { return x_ || y_ || sector_; }
+  bool isSector() const { return sector_; }
+  bool isPlanar() const { return x_ || y_; }
+  bool active(int set) const { return sector_ ? true : set == 0 ? x_ : set == 1 ? y_ : (x_ && y_); }
   fdm_periodic flags() const {
-    fdm_periodic p;
+    fdm_periodic p{};
+    if (sector_) return sectorFlags_;
     p.x = x_ ? 1 : 0;
     p.y = y_ ? 1 : 0;
     return p;
@@ -62,8 +81,9 @@ struct PeriodicBoundaryXY2D {
                   int mirrorCapacity) {
     for (int s = 0; s < 3; ++s) {
       if (!active(s)) continue;
-      if (sets_[s].capacity != std::max(ghostCapacity, 1) || allocatedFor_[s] != P.num())
-        sets_[s].allocate(P.num(), ghostCapacity, P.grid().num);
+      if (sets_[s].capacity != std::max(ghostCapacity, 1) || allocatedFor_[s] != P.num() ||
+          sets_[s].posed != sector_)
+        sets_[s].allocate(P.num(), ghostCapacity, P.grid().num, sector_);
       allocatedFor_[s] = P.num();
       sets_[s].interaction.initialize(P.num(), pairCapacity, recordCapacity, mirrorCapacity);
     }
@@ -79,7 +99,8 @@ struct PeriodicBoundaryXY2D {
   }
 
  private:
-  bool x_ = false, y_ = false;
+  bool x_ = false, y_ = false, sector_ = false;
+  fdm_periodic sectorFlags_{};
   GhostSet sets_[3];
   DeviceBuffer<int> atBuild_;
   size_t allocatedFor_[3] = {(size_t)-1, (size_t)-1, (size_t)-1};

@@ -76,6 +76,7 @@ def solver_lib():
         "fdmx_add_wall": (I, [P, _dp, _dp, _dp, _dp, I, I]),
         "fdmx_add_bond": (I, [P, I, I, _dp, _dp, D, D, D, D, D, D, D]),
         "fdmx_set_periodic": (I, [P, I, I]),
+        "fdmx_set_sector": (I, [P, I]),
         "fdmx_set_cell_size": (I, [P, D]),
         "fdmx_set_capacity": (I, [P, I, I]),
         "fdmx_set_skin": (I, [P, D]),
@@ -244,6 +245,11 @@ class Solver:
                  cohesion=0.0, friction=0.0):
         self._ok(self._lib.fdmx_add_bond(self.h, i, j, _dp_(_d(point)), _dp_(_d(normal)), radius,
                                          length, youngs, poisson, tensile, cohesion, friction))
+
+    def set_sector(self, on=True):
+        """LSSolver::addPeriodicBoundarySector (LSSolver.h:410-413); off removes
+        every periodic boundary."""
+        self._ok(self._lib.fdmx_set_sector(self.h, int(on)))
 
     def set_periodic(self, x, y):
         self._ok(self._lib.fdmx_set_periodic(self.h, int(x), int(y)))

@@ -46,6 +46,7 @@ class Scenario:
     wall_materials: list = field(default_factory=list)  # (mu, e)
     walls: list = field(default_factory=list)  # (x, v, w, q, mat, geo)
     periodic: tuple = (False, False)
+    sector: bool = False  # the quarter-plane sector periodic boundary about min
     cell_size: float = 0.0
     bonds: list = field(default_factory=list)  # (i, j, point, normal, R, L, E, nu, st, c, mub)
     steps: int = 1
@@ -85,6 +86,8 @@ def apply(s: Scenario, t) -> None:
         t.add_bond(*b)
     if any(s.periodic):
         t.set_periodic(*s.periodic)
+    if s.sector:
+        t.set_sector(True)
     if s.cell_size > 0:
         t.set_cell_size(s.cell_size)
     t.initialize(s.domain_min, s.domain_max, s.gravity, s.dt)

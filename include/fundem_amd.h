@@ -199,6 +199,9 @@ typedef struct {
   int* flag;           /* [n] scan workspace */
   int* realId;         /* [capacity] real particle of each ghost, ascending */
   double* pos;         /* [3*capacity] ghost positions, AoS */
+  double* q;           /* [4*capacity] sector images: ghost orientations (NULL for planar sets,
+                          whose ghosts keep the real particle's) */
+  double* vel;         /* [3*capacity] sector images: ghost velocities (NULL for planar sets) */
   fdm_hash_table table; /* hashValue: the reference's ghost cell id (stencil rule);
                            sortKey: 1 + id of the clamped geometric cell (list builds) */
   int* cellBox;        /* [6] min/max occupied sort cell per axis */
@@ -227,10 +230,24 @@ typedef struct {
   const int* gate;         /* run only when *gate != 0 (NULL: always) */
 } fdm_bonds;
 
-/* Periodic configuration (PeriodicBoundaryXY2D, PeriodicBoundary.h:178-317). */
+/* Periodic configuration: planar x / y (PeriodicBoundaryXY2D,
+ * PeriodicBoundary.h:178-317), or the quarter-plane sector about the z axis
+ * through the grid's min corner (PeriodicBoundarySector, :319-426; then x and
+ * y are 0). Sector ghost sets 0, 1, 2 hold the R90, R180 and R270 images. The
+ * sector's rotations are given by fdm_sectorPeriodic, per image k (0: +pi/2,
+ * 1: pi, 2: -pi/2): the angle about z, rotateVector's coefficients
+ * A = sin(t)/t and B = (1 - cos t)/t^2 (t = |angle|; myVec.h:143-181) and
+ * rotateQuaternion's cos and sin of t/2 (myQua.h:131-175), evaluated on the
+ * host as the reference's host code does, so that the device and a host
+ * restatement rotate with the same constants. */
 typedef struct {
   int x, y;
+  int sector;
+  int pad_;
+  double angle[3], A[3], B[3], halfCos[3], halfSin[3];
 } fdm_periodic;
+/* Fill p for the sector boundary (sector = 1, x = y = 0, rotation constants). */
+void fdm_sectorPeriodic(fdm_periodic* p);
 
 /* Independent systems advanced together (no counterpart in the reference,
  * which runs one system per solver). Every body carries an instance id; each

@@ -377,6 +377,7 @@ struct OracleSim {
   Body P, W;
   Bonds bonds;
   bool perX = false, perY = false;
+  bool sector = false;  // PeriodicBoundarySector: ghost sets 0, 1, 2 are R90, R180, R270
   double cellSizeOverride = 0.0;
   Grid pgrid, wgrid;
   HashTable pht, wht;
@@ -519,7 +520,97 @@ struct OracleSim {
     G.q = P.q;
     G.vel = P.v;
   }
+  // rotateVector (myVec.h:143-181, host branch) by the angle th about z.
+  static V3 rotZ(V3 v, double angle) {
+    const volatile double thv = angle;  // sin and cos from the C library at run time
+    const double th = thv;
+    const V3 om = {0.0, 0.0, th};
+    const double theta2 = dot(om, om);
+    if (isZero(theta2)) return add(add(v, cross(om, v)), scl(cross(om, cross(om, v)), 0.5));
+    const double invTheta = 1.0 / std::sqrt(theta2);
+    const double theta = theta2 * invTheta;
+    const double s = std::sin(theta), c = std::cos(theta);
+    const double A = s * invTheta;
+    const double B = (1.0 - c) * (invTheta * invTheta);
+    const V3 oxv = cross(om, v);
+    const V3 oxoxv = cross(om, oxv);
+    return add(add(v, scl(oxv, A)), scl(oxoxv, B));
+  }
+  // rotateQuaternion (myQua.h:131-175): q rotated by th about z, normalised.
+  static Q4 rotQz(Q4 q, double angle) {
+    const volatile double thv = angle;
+    const double th = thv;
+    const double theta2 = 0.0 * 0.0 + 0.0 * 0.0 + th * th;
+    if (isZero(theta2)) return q;
+    const double theta = std::sqrt(theta2);
+    const double invTheta = 1.0 / theta;
+    const double ux = 0.0 * invTheta, uy = 0.0 * invTheta, uz = th * invTheta;
+    const double half = 0.5 * theta;
+    const double c = std::cos(half), s = std::sin(half);
+    const double r0 = c, r1 = s * ux, r2 = s * uy, r3 = s * uz;
+    Q4 r;
+    r.q0 = r0 * q.q0 - r1 * q.q1 - r2 * q.q2 - r3 * q.q3;
+    r.q1 = r0 * q.q1 + r1 * q.q0 + r2 * q.q3 - r3 * q.q2;
+    r.q2 = r0 * q.q2 - r1 * q.q3 + r2 * q.q0 + r3 * q.q1;
+    r.q3 = r0 * q.q3 + r1 * q.q2 - r2 * q.q1 + r3 * q.q0;
+    return qnormalize(r);
+  }
+  // updateVelocityPositionOrientationOutOfSectorKernel (mySpatialGrid.cu:59-85):
+  // a particle past the x = min face (with y >= min) turns by -pi/2 about the
+  // z axis through min, one past the y = min face (with x >= min) by +pi/2.
+  void sectorWrap() {
+    const V3 mn = pgrid.mn;
+    const double hp = 0.5 * kPi;
+    for (int i = 0; i < P.num(); ++i) {
+      V3 v = P.v[i], p = P.x[i];
+      Q4 q = P.q[i];
+      if (p.x < mn.x && p.y >= mn.y) {
+        v = rotZ(v, -hp);
+        p = add(mn, rotZ(sub(p, mn), -hp));
+        q = rotQz(q, -hp);
+      }
+      if (p.y < mn.y && p.x >= mn.x) {
+        v = rotZ(v, hp);
+        p = add(mn, rotZ(sub(p, mn), hp));
+        q = rotQz(q, hp);
+      }
+      P.v[i] = r32(v, f32);
+      P.x[i] = p;
+      P.q[i] = r32(q, f32);
+    }
+  }
+  // calculateSectorGhostVelocityPositionOrientationKernel (mySpatialGrid.cu:154-212):
+  // R90 images of the particles in the first y cell row, R270 of the first
+  // x column, R180 of their corner column; hashed by the ghost cell rule.
+  void sectorGhosts() {
+    const V3 mn = pgrid.mn;
+    const double ang[3] = {0.5 * kPi, kPi, -0.5 * kPi};
+    for (int s = 0; s < 3; ++s) {
+      GhostSet& G = gh[s];
+      G.pos = P.x;
+      G.vel = P.v;
+      G.q = P.q;
+      for (int i = 0; i < P.num(); ++i) {
+        const I3 c = gridPos(P.x[i], pgrid.mn, pgrid.inv);
+        const bool on = s == 0 ? c.y == 0 : s == 2 ? c.x == 0 : (c.x == 0 && c.y == 0);
+        if (!on) continue;
+        G.vel[i] = rotZ(P.v[i], ang[s]);
+        G.pos[i] = add(mn, rotZ(sub(P.x[i], mn), ang[s]));
+        G.q[i] = rotQz(P.q[i], ang[s]);
+      }
+      G.ht.hash.resize(P.num());
+      for (int i = 0; i < P.num(); ++i) G.ht.hash[i] = ghostHashOf(G.pos[i], pgrid);
+      sortStartEnd(G.ht.hash, G.ht.index, pgrid.num, G.ht.start, G.ht.end);
+    }
+  }
   void updateGhosts() {
+    if (sector) {  // PeriodicBoundarySector::updateGhostSpatialGrid, buildGhostInteraction
+      sectorWrap();
+      sectorGhosts();
+      const int sets[3] = {SET_GX, SET_GY, SET_GXY};
+      for (int s = 0; s < 3; ++s) detectSet(sets[s], pgrid, gh[s].ht, gh[s].pos, gh[s].q, P.geo, pgeo, false);
+      return;
+    }
     if (perX) { wrapAxis(0); ghostPositions(0); }
     if (perY) { wrapAxis(1); ghostPositions(1); }
     if (perX && perY) ghostPositions(2);
@@ -681,7 +772,7 @@ struct OracleSim {
     if (canon) foldCanonical(bo); else foldReference(bo);
     for (int i = 0; i < P.num(); ++i) { P.F[i] = r32(P.F[i], f32); P.T[i] = r32(P.T[i], f32); }
   }
-  bool periodic() const { return perX || perY; }
+  bool periodic() const { return perX || perY || sector; }
   // compute_addBondAndPeriodicBoundary loop body (LSSolver.h:1206-1220) and
   // its simpler variants, which are the same with empty ghost/bond sets.
   void step() {
@@ -933,6 +1024,7 @@ int osim_add_bond(OracleSim* s, int i, int j, const double* pt3, const double* n
   return b.num() - 1;
 }
 void osim_set_periodic(OracleSim* s, int x, int y) { s->perX = x != 0; s->perY = y != 0; }
+void osim_set_sector(OracleSim* s, int on) { s->sector = on != 0; }
 void osim_set_cell_size(OracleSim* s, double cs) { s->cellSizeOverride = cs; }
 
 int osim_initialize(OracleSim* s, const double* mn3, const double* mx3, const double* g3, double dt) {

@@ -93,6 +93,10 @@ int osim_add_bond(OracleSim* s, int master, int slave, const double* point3, con
                   double radius, double length, double youngs, double poisson, double tensile,
                   double cohesion, double friction);
 void osim_set_periodic(OracleSim* s, int x_on, int y_on);
+/* PeriodicBoundarySector (PeriodicBoundary.h:319-426): the quarter-plane
+ * sector x >= min.x, y >= min.y about the z axis through min; ghost sets
+ * 2, 3, 4 hold the R90, R180 and R270 images. Exclusive with the planar sets. */
+void osim_set_sector(OracleSim* s, int on);
 /* Additive: overrides the particle grid's 2*max radius cell size (<= 0 restores it). */
 void osim_set_cell_size(OracleSim* s, double cell_size);
 /* LSSolver::upload (LSSolver.h:692-703) then the pre-loop grid/contact build of

@@ -41,6 +41,7 @@ def _load():
         "osim_add_material": (I, [P, D, D, D, D]),
         "osim_add_wall_material": (I, [P, D, D]),
         "osim_add_material_damping": (I, [P, D, D, D, D]),
+        "osim_set_sector": (V, [P, I]),
         "oracle_mesh_points": (V, [_dp, I, _ip, I, _dp, I, I, _dp, _dp]),
         "oracle_mesh_grid": (LL, [_dp, I, _ip, I, D, I, I, _dp, _ip, _dp]),
         "osim_add_wall_material_damping": (I, [P, D, D]),
@@ -196,6 +197,9 @@ class OracleSim:
 
     def set_periodic(self, x, y):
         lib().osim_set_periodic(self.h, int(x), int(y))
+
+    def set_sector(self, on=True):
+        lib().osim_set_sector(self.h, int(on))
 
     def set_cell_size(self, cs):
         lib().osim_set_cell_size(self.h, float(cs))
