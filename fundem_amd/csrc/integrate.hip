@@ -119,21 +119,29 @@ __global__ void kCommitStateInstances(const double* x, int n, fdm_instances I, i
 // distance rho by well under 1e-6 rho). Ghost lists are also rebuilt after
 // a visibility change or a wrap; a wrap (positions jump by a period)
 // rebuilds the next step's lists as well. One thread of one workgroup runs
-// it, after every writer of the control words has finished.
-__device__ void decideStep(const fdm_skin& s) {
+// it, after every other writer of the control words has finished (earlier
+// kernels of the stream); `rate` and `flags` (bit 0 grid state changed, 1
+// visibility changed, 2 wrapped) add this kernel's own findings. The control
+// words are read together (plain loads: the cache is fresh at kernel start),
+// and the host's copy of the decision is one word, (number << 2) | rebuild
+// << 1 | ghost rebuild, so it needs no ordering fence.
+__device__ void decideStep(const fdm_skin& s, float rateIn, int flagsIn) {
   int* c = s.ctl;
-  auto rd = [&](int w) { return *(volatile int*)(c + w); };
-  const float rate = __int_as_float(rd(FDM_CTL_MAX_RATE));
-  double d = *s.disp + s.dt * (double)rate * (1.0 + 1e-6) + 1e-6 * s.rhoMax;
-  const bool rebuild = rd(FDM_CTL_FORCE) != 0 || rd(FDM_CTL_PENDING) != 0 ||
-                       rd(FDM_CTL_VALID_CHANGED) != 0 || !(2.0 * d <= s.skin);
-  const bool wrapped = rd(FDM_CTL_WRAPPED) != 0;
-  const bool ghost = rebuild || wrapped || rd(FDM_CTL_GHOST_CHANGED) != 0;
+  int w[FDM_CTL_WORDS];
+#pragma unroll
+  for (int k = 0; k < FDM_CTL_WORDS; ++k) w[k] = c[k];
+  const double disp = *s.disp;
+  const float rate = fmaxf(__int_as_float(w[FDM_CTL_MAX_RATE]), rateIn);
+  double d = disp + s.dt * (double)rate * (1.0 + 1e-6) + 1e-6 * s.rhoMax;
+  const bool rebuild = w[FDM_CTL_FORCE] != 0 || w[FDM_CTL_PENDING] != 0 ||
+                       w[FDM_CTL_VALID_CHANGED] != 0 || (flagsIn & 1) || !(2.0 * d <= s.skin);
+  const bool wrapped = w[FDM_CTL_WRAPPED] != 0 || (flagsIn & 4);
+  const bool ghost = rebuild || wrapped || w[FDM_CTL_GHOST_CHANGED] != 0 || (flagsIn & 2);
   if (rebuild) {
     d = 0.0;
-    c[FDM_CTL_COUNT] = rd(FDM_CTL_COUNT) + 1;
+    c[FDM_CTL_COUNT] = w[FDM_CTL_COUNT] + 1;
   }
-  if (ghost) c[FDM_CTL_GHOST_COUNT] = rd(FDM_CTL_GHOST_COUNT) + 1;
+  if (ghost) c[FDM_CTL_GHOST_COUNT] = w[FDM_CTL_GHOST_COUNT] + 1;
   *s.disp = d;
   c[FDM_CTL_REBUILD] = rebuild ? 1 : 0;
   c[FDM_CTL_REBUILD_GHOST] = ghost ? 1 : 0;
@@ -144,21 +152,15 @@ __device__ void decideStep(const fdm_skin& s) {
   c[FDM_CTL_VALID_CHANGED] = 0;
   c[FDM_CTL_WRAPPED] = 0;
   c[FDM_CTL_GHOST_CHANGED] = 0;
-  const int seq = rd(FDM_CTL_SEQ) + 1;
+  const int seq = w[FDM_CTL_SEQ] + 1;
   c[FDM_CTL_SEQ] = seq;
-  if (s.hostDecision) {
-    volatile int* h = s.hostDecision;
-    h[1] = rebuild ? 1 : 0;
-    h[2] = ghost ? 1 : 0;
-    __threadfence_system();
-    h[0] = seq;
-    __threadfence_system();
-  }
+  if (s.hostDecision)
+    *(volatile int*)s.hostDecision = (int)(((unsigned)seq << 2) | (rebuild ? 2u : 0u) | (ghost ? 1u : 0u));
 }
 
 __global__ void kSkinDecide(fdm_skin s) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  decideStep(s);
+  decideStep(s, 0.f, 0);
 }
 
 struct StepBeginK {
@@ -294,12 +296,7 @@ __global__ void __launch_bounds__(1024) kStepDecide(StepBeginK k, int nblocks) {
   }
   r = fmaxf(r, rmax[0]);
   f |= fl[0];
-  int* c = k.s.ctl;
-  if (r > 0.f) atomicMax(c + FDM_CTL_MAX_RATE, __float_as_int(r));
-  if (f & 1) c[FDM_CTL_VALID_CHANGED] = 1;
-  if (f & 2) c[FDM_CTL_GHOST_CHANGED] = 1;
-  if (f & 4) c[FDM_CTL_WRAPPED] = 1;
-  decideStep(k.s);
+  decideStep(k.s, r, f);
 }
 
 // Copies the candidate lists of the previous step when no rebuild happens.

@@ -1106,21 +1106,24 @@ class LSSolver {
   };
   // Spin on the host-visible decision of the phase A just launched.
   Decision waitDecision() {
-    const int seq = ++decisionSeq_;
+    const unsigned seq = (unsigned)++decisionSeq_ & 0x3fffffffu;
     volatile int* h = hostDecision_;
+    unsigned v = 0;
     for (long spin = 1;; ++spin) {
-      if (h[0] == seq) break;
+      v = (unsigned)h[0];
+      if ((v >> 2) == seq) break;
       if ((spin & 1023) == 0) {
         const hipError_t e = hipStreamQuery(stream_);
         if (e == hipSuccess) {
-          if (h[0] == seq) break;
+          v = (unsigned)h[0];
+          if ((v >> 2) == seq) break;
           throw std::runtime_error("list decision missing after the stream drained");
         }
         if (e != hipErrorNotReady) hipCheck(e, "step");
       }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
-    return {h[1] != 0, h[2] != 0};
+    return {(v & 2u) != 0, (v & 1u) != 0};
   }
 
   // New result generations for every set, new list generations where rebuilt.

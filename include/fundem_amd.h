@@ -161,9 +161,9 @@ typedef struct {
   double dt;
   int* ctl;            /* [FDM_CTL_WORDS] */
   double* disp;        /* [1] displacement bound accumulated since the last build */
-  int* hostDecision;   /* optional host-visible [4]: decision number, rebuild, ghost
-                          rebuild (written with system scope, for a host that launches
-                          only the work a step needs); NULL to skip */
+  int* hostDecision;   /* optional host-visible word: (decision number << 2) | rebuild << 1
+                          | ghost rebuild, written last by the decision (for a host that
+                          launches only the work a step needs); NULL to skip */
 } fdm_skin;
 
 /* Candidate pairs and tangential history of one interaction set. */
