@@ -20,11 +20,43 @@ thread_local std::string g_error;
 class BoundSolver : public fundem::LSSolver {
  public:
   explicit BoundSolver(int device) : fundem::LSSolver("fundem", 0, 256, device) {}
+  // The tutorials' external forces (apps/tutorial/LSParticleFalling.cpp:26-45):
+  // global damping with coefficient c (c < 0: off) and a constant force on
+  // one particle (index < 0: off).
+  void setGlobalDamping(double c) {
+    damping_ = c;
+    resetHookDetection();
+  }
+  void setConstantForce(int index, double3 F, double3 T) {
+    constIndex_ = index;
+    constF_ = F;
+    constT_ = T;
+    resetHookDetection();
+  }
+  void addExternalForceTorque(const double time) override {
+    if (damping_ < 0.0 && constIndex_ < 0) {
+      fundem::LSSolver::addExternalForceTorque(time);
+      return;
+    }
+    fdm_bodies b = getLSParticle().bodies();
+    if (damping_ >= 0.0)
+      fundem::fdmCheck(fdm_launchAddGlobalDampingForceTorque(b, damping_, stream()), "damping");
+    if (constIndex_ >= 0)
+      fundem::fdmCheck(fdm_launchAddGlobalConstantForceTorque(b, {constF_.x, constF_.y, constF_.z},
+                                                              {constT_.x, constT_.y, constT_.z},
+                                                              constIndex_, stream()),
+                       "constant force");
+  }
+
   void addGeometryDensity(double3& off, double density, double3 origin, double h, int3 size,
                           const std::vector<double>& sdf, const std::vector<double3>& nodes,
                           const std::vector<int3>& tris) {
     addParticleGeometry(off, density, origin, h, size, sdf, nodes, tris);
   }
+ private:
+  double damping_ = -1.0;
+  int constIndex_ = -1;
+  double3 constF_{0.0, 0.0, 0.0}, constT_{0.0, 0.0, 0.0};
 };
 
 // tris: ntri triangles of node indices (the surface mesh the frame files
@@ -275,6 +307,12 @@ int fdmx_set_periodic(void* s, int x, int y) {
     if (x) S->addPeriodicBoundaryXD();
     if (y) S->addPeriodicBoundaryYD();
   });
+}
+int fdmx_set_global_damping(void* s, double c) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setGlobalDamping(c); });
+}
+int fdmx_set_constant_force(void* s, int index, const double* F3, const double* T3) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setConstantForce(index, d3(F3), d3(T3)); });
 }
 int fdmx_set_sector(void* s, int on) {
   return guarded([&] {

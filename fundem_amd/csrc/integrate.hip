@@ -315,9 +315,45 @@ __global__ void kCarryPairs(fdm_pairs from, fdm_pairs to, int n, const int* gate
   }
 }
 
+// addGlobalDampingForceTorque (globalDamping.cu:4-16) on FP32 forces
+__global__ void kGlobalDamping(fdm_bodies b, double c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const V3 f = ld3(b.F, b.n, i), t = ld3(b.T, b.n, i);
+  st3(b.F, b.n, i, vsub(f, vmul(vnormalize(ld3(b.v, b.n, i)), c * vlen(f))));
+  st3(b.T, b.n, i, vsub(t, vmul(vnormalize(ld3(b.w, b.n, i)), c * vlen(t))));
+}
+
+// addGlobalConstantForceTorque (globalDamping.cu:38-49): both vectors go to
+// the force
+__global__ void kConstantForce(fdm_bodies b, V3 F, V3 T, int index) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  V3 f = ld3(b.F, b.n, index);
+  f = round32(vadd(f, F));
+  st3(b.F, b.n, index, vadd(f, T));
+}
+
 }  // namespace fdm
 
 using namespace fdm;
+
+extern "C" int fdm_launchAddGlobalDampingForceTorque(fdm_bodies particles, double c,
+                                                     hipStream_t stream) {
+  if (particles.n < 0) return FDM_EINVAL;
+  if (particles.n == 0) return FDM_OK;
+  hipLaunchKernelGGL(kGlobalDamping, dim3(blocksFor(particles.n, 256)), dim3(256), 0, stream,
+                     particles, c);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchAddGlobalConstantForceTorque(fdm_bodies particles, fdm_double3 F,
+                                                      fdm_double3 T, int index,
+                                                      hipStream_t stream) {
+  if (index < 0 || index >= particles.n) return FDM_EINVAL;
+  hipLaunchKernelGGL(kConstantForce, dim3(1), dim3(64), 0, stream, particles, fromD3(F), fromD3(T),
+                     index);
+  return launchStatus();
+}
 
 extern "C" int fdm_launchParticleVelocityAngularVelocityIntegration(fdm_bodies bodies,
                                                                     fdm_geometry geo,

@@ -623,6 +623,13 @@ class LSSolver {
     baseHookCalled_ = true;
   }
 
+  // Detect again, at the next step, whether addExternalForceTorque is
+  // overridden to act (for a subclass that turns its hook on or off).
+  void resetHookDetection() {
+    hookMode_ = HookUnknown;
+    destroyGraphs();
+  }
+
   // Called at every output frame after the state is downloaded.
   virtual void outputFrame(size_t iFrame, size_t iStep, double time) {
     (void)iFrame;

@@ -77,6 +77,8 @@ def solver_lib():
         "fdmx_add_bond": (I, [P, I, I, _dp, _dp, D, D, D, D, D, D, D]),
         "fdmx_set_periodic": (I, [P, I, I]),
         "fdmx_set_sector": (I, [P, I]),
+        "fdmx_set_global_damping": (I, [P, D]),
+        "fdmx_set_constant_force": (I, [P, I, _dp, _dp]),
         "fdmx_set_cell_size": (I, [P, D]),
         "fdmx_set_capacity": (I, [P, I, I]),
         "fdmx_set_skin": (I, [P, D]),
@@ -245,6 +247,18 @@ class Solver:
                  cohesion=0.0, friction=0.0):
         self._ok(self._lib.fdmx_add_bond(self.h, i, j, _dp_(_d(point)), _dp_(_d(normal)), radius,
                                          length, youngs, poisson, tensile, cohesion, friction))
+
+    def set_global_damping(self, c):
+        """The tutorials' external-force hook (apps/tutorial/globalDamping.cu:4-16):
+        F -= c |F| v/|v|, T -= c |T| w/|w| between the force reduction and the
+        second half-kick; None turns it off."""
+        self._ok(self._lib.fdmx_set_global_damping(self.h, -1.0 if c is None else float(c)))
+
+    def set_constant_force(self, index, force, torque):
+        """apps/tutorial/globalDamping.cu:38-49 on particle `index` (both vectors are
+        added to its force, as that kernel does); index None turns it off."""
+        self._ok(self._lib.fdmx_set_constant_force(self.h, -1 if index is None else int(index),
+                                                   _dp_(_d(force)), _dp_(_d(torque))))
 
     def set_sector(self, on=True):
         """LSSolver::addPeriodicBoundarySector (LSSolver.h:410-413); off removes

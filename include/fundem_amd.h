@@ -278,6 +278,18 @@ int fdm_launchParticleVelocityAngularVelocityIntegration(fdm_bodies bodies, fdm_
                                                          fdm_double3 gravity, double h,
                                                          hipStream_t stream);
 
+/* The tutorials' external-force hooks (apps/tutorial/globalDamping.cu:4-63),
+ * for LSSolver::addExternalForceTorque overrides on this layout. Global
+ * damping: F -= c |F| v/|v| and T -= c |T| w/|w| per particle (a zero
+ * velocity gives a zero direction). Constant force: adds F to the force of
+ * particle `index`, and then T to its force as well, as the tutorial's kernel
+ * does (its torque is left alone). */
+int fdm_launchAddGlobalDampingForceTorque(fdm_bodies particles, double dampingCoefficient,
+                                          hipStream_t stream);
+int fdm_launchAddGlobalConstantForceTorque(fdm_bodies particles, fdm_double3 constantForce,
+                                           fdm_double3 constantTorque, int index,
+                                           hipStream_t stream);
+
 /* Replaces launchParticlePositionOrientationIntegration
  * (particleIntegrationKernel.cuh:23): drift of x and q over dt. */
 int fdm_launchParticlePositionOrientationIntegration(fdm_bodies bodies, double dt,

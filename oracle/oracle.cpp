@@ -378,6 +378,9 @@ struct OracleSim {
   Bonds bonds;
   bool perX = false, perY = false;
   bool sector = false;  // PeriodicBoundarySector: ghost sets 0, 1, 2 are R90, R180, R270
+  double damping = -1.0;  // the tutorials' global damping (off when < 0)
+  int constIndex = -1;    // the tutorials' constant force (off when < 0)
+  V3 constF{0, 0, 0}, constT{0, 0, 0};
   double cellSizeOverride = 0.0;
   Grid pgrid, wgrid;
   HashTable pht, wht;
@@ -775,6 +778,23 @@ struct OracleSim {
   bool periodic() const { return perX || perY || sector; }
   // compute_addBondAndPeriodicBoundary loop body (LSSolver.h:1206-1220) and
   // its simpler variants, which are the same with empty ghost/bond sets.
+  // The tutorials' addExternalForceTorque overrides (apps/tutorial/
+  // globalDamping.cu:4-49), called between the forces and the second
+  // half-kick (LSSolver.h:1176): global damping F -= c |F| v/|v| (myVec.h:120-135
+  // normalize, zero for a zero vector), then the constant-force kernel, which
+  // adds both of its vectors to the particle's force.
+  void externalForces() {
+    if (damping >= 0.0)
+      for (int i = 0; i < P.num(); ++i) {
+        const V3 f = P.F[i], t = P.T[i];
+        P.F[i] = r32(sub(f, scl(normalize(P.v[i]), damping * len(f))), f32);
+        P.T[i] = r32(sub(t, scl(normalize(P.w[i]), damping * len(t))), f32);
+      }
+    if (constIndex >= 0 && constIndex < P.num()) {
+      P.F[constIndex] = r32(add(P.F[constIndex], constF), f32);
+      P.F[constIndex] = r32(add(P.F[constIndex], constT), f32);
+    }
+  }
   void step() {
     const double h = 0.5 * dt;
     kick(h);
@@ -784,6 +804,7 @@ struct OracleSim {
     buildGrid(W, wgrid, wht);
     detectAll(periodic());
     forcesAndFold();
+    externalForces();
     kick(h);
   }
 };
@@ -1025,6 +1046,12 @@ int osim_add_bond(OracleSim* s, int i, int j, const double* pt3, const double* n
 }
 void osim_set_periodic(OracleSim* s, int x, int y) { s->perX = x != 0; s->perY = y != 0; }
 void osim_set_sector(OracleSim* s, int on) { s->sector = on != 0; }
+void osim_set_global_damping(OracleSim* s, double c) { s->damping = c; }
+void osim_set_constant_force(OracleSim* s, int index, const double* F3, const double* T3) {
+  s->constIndex = index;
+  s->constF = ld3(F3);
+  s->constT = ld3(T3);
+}
 void osim_set_cell_size(OracleSim* s, double cs) { s->cellSizeOverride = cs; }
 
 int osim_initialize(OracleSim* s, const double* mn3, const double* mx3, const double* g3, double dt) {

@@ -97,6 +97,12 @@ void osim_set_periodic(OracleSim* s, int x_on, int y_on);
  * sector x >= min.x, y >= min.y about the z axis through min; ghost sets
  * 2, 3, 4 hold the R90, R180 and R270 images. Exclusive with the planar sets. */
 void osim_set_sector(OracleSim* s, int on);
+/* The tutorials' external forces (apps/tutorial/globalDamping.cu): global
+ * damping with coefficient c (c < 0: off) and the constant-force kernel on one
+ * particle (index < 0: off), applied between the forces and the second
+ * half-kick. */
+void osim_set_global_damping(OracleSim* s, double c);
+void osim_set_constant_force(OracleSim* s, int index, const double* F3, const double* T3);
 /* Additive: overrides the particle grid's 2*max radius cell size (<= 0 restores it). */
 void osim_set_cell_size(OracleSim* s, double cell_size);
 /* LSSolver::upload (LSSolver.h:692-703) then the pre-loop grid/contact build of

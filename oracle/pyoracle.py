@@ -42,6 +42,8 @@ def _load():
         "osim_add_wall_material": (I, [P, D, D]),
         "osim_add_material_damping": (I, [P, D, D, D, D]),
         "osim_set_sector": (V, [P, I]),
+        "osim_set_global_damping": (V, [P, D]),
+        "osim_set_constant_force": (V, [P, I, _dp, _dp]),
         "oracle_mesh_points": (V, [_dp, I, _ip, I, _dp, I, I, _dp, _dp]),
         "oracle_mesh_grid": (LL, [_dp, I, _ip, I, D, I, I, _dp, _ip, _dp]),
         "osim_add_wall_material_damping": (I, [P, D, D]),
@@ -200,6 +202,13 @@ class OracleSim:
 
     def set_sector(self, on=True):
         lib().osim_set_sector(self.h, int(on))
+
+    def set_global_damping(self, c):
+        lib().osim_set_global_damping(self.h, -1.0 if c is None else float(c))
+
+    def set_constant_force(self, index, force, torque):
+        f, t = d3(force), d3(torque)
+        lib().osim_set_constant_force(self.h, -1 if index is None else int(index), dptr(f), dptr(t))
 
     def set_cell_size(self, cs):
         lib().osim_set_cell_size(self.h, float(cs))

@@ -303,3 +303,32 @@ def test_sector_graph_replay_and_guards():
     W.apply(scn, g)
     with pytest.raises(RuntimeError, match="not differentiated"):
         Adjoint(g).forward(5, 5)
+
+
+def test_external_force_hooks_bitexact():
+    """The tutorials' addExternalForceTorque overrides (global damping and the
+    constant-force kernel, apps/tutorial/globalDamping.cu) run between the
+    force reduction and the second half-kick; states equal the oracle's."""
+    scn = W.cluster(n=27, periodic=(True, True), speed=1.0, scale=1e-2, dt=1e-5, kn=1e5, spin=5.0)
+    from fundem_amd import Solver
+    g = Solver()
+    g.set_global_damping(0.2)
+    g.set_constant_force(3, [1e-3, 0.0, 2e-3], [0.0, -1e-3, 0.0])
+    W.apply(scn, g)
+    o = po.OracleSim(True, True)
+    W.apply(scn, o)
+    o.set_global_damping(0.2)
+    o.set_constant_force(3, [1e-3, 0.0, 2e-3], [0.0, -1e-3, 0.0])
+    for i in range(1, 301):
+        g.step(1)
+        o.step(1)
+        if i % 10 == 0:
+            assert_states_equal(g.state(), o.state(), f"step {i}")
+            assert_contacts_equal(g, o, f"step {i}")
+    g.set_global_damping(None)
+    g.set_constant_force(None, [0, 0, 0], [0, 0, 0])
+    o.set_global_damping(None)
+    o.set_constant_force(None, [0, 0, 0], [0, 0, 0])
+    g.step(20)
+    o.step(20)
+    assert_states_equal(g.state(), o.state(), "hooks off")
