@@ -422,6 +422,16 @@ def main():
 
     roof = None
     modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}
+    # every modelled stage beside the dominant one: the streaming stages sit
+    # near the bandwidth roof, the narrow phase far below it (latency-bound)
+    stage_roof = []
+    for k, (ms, calls) in sorted(modeled.items(), key=lambda kv: -kv[1][0]):
+        nb = kernel_bytes(k, n, pairs, contacts)
+        avg_s = ms / max(calls, 1) * 1e-3
+        ach = nb / avg_s / 1e9 if avg_s > 0 else 0.0
+        stage_roof.append({"stage": k, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb,
+                           "achieved": ach, "frac": ach / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(k)})
     if modeled:
         dom, (ms, calls) = max(modeled.items(), key=lambda kv: kv[1][0])
         nb = kernel_bytes(dom, n, pairs, contacts)
@@ -447,6 +457,7 @@ def main():
                    "ghost_list_builds": gb1 - gb0, "tuning": tuning},
         "step_hbm_fraction": step_bytes * a.steps / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
+        "stage_roofline": stage_roof,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scn.templates, a.cpu_seconds)
