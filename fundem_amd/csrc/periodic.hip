@@ -109,7 +109,6 @@ struct GhostOut {
   int* hashValue;
   unsigned* sortKey;
   int* sortIndex;
-  int* cellBox;
   int capacity;
   unsigned* status;
   double* q;    // sector images: orientation and velocity
@@ -134,13 +133,58 @@ __device__ __forceinline__ int emitGhost(const GhostOut& o, int i, V3 gp, const 
   const I3 c = clampedCell(gp, g);
   o.sortKey[s] = (unsigned)(cellOffset + lin3(c, gsize(g)) + 1);
   o.sortIndex[s] = s;
-  atomicMin(&o.cellBox[0], c.x);
-  atomicMin(&o.cellBox[1], c.y);
-  atomicMin(&o.cellBox[2], c.z);
-  atomicMax(&o.cellBox[3], c.x);
-  atomicMax(&o.cellBox[4], c.y);
-  atomicMax(&o.cellBox[5], c.z);
-  return s;
+  return s;  // the set's cell box is folded afterwards by kGhostBox
+}
+
+constexpr int kBoxBlocks = 128;
+constexpr int kBoxThreads = 256;
+
+// Bounding box of the clamped cells of a set's stored ghosts (the cells
+// emitGhost keys them by), folded per workgroup and merged with one atomic
+// per bound and workgroup. Folding inside the scatter cost one atomic per
+// ghost on the same six words, which serialised at L2; min and max are
+// exact, so the box does not depend on the order of the merges.
+__global__ void __launch_bounds__(kBoxThreads) kGhostBox(const double* gpos, const int* realId,
+                                                         const int* count, int capacity,
+                                                         fdm_grid g0, fdm_instances I,
+                                                         int* cellBox, const int* gate) {
+  if (gateOff(gate)) return;
+  __shared__ int part[6][kBoxThreads / 64];
+  const int c = min(*count, capacity);
+  int b[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < c; k += gridDim.x * blockDim.x) {
+    const fdm_grid& g = I.count > 0 ? I.pgrid[I.particleInstance[realId[k]]] : g0;
+    const I3 cc = clampedCell(ldd3(gpos + 3 * (size_t)k), g);
+    b[0] = min(b[0], cc.x);
+    b[1] = min(b[1], cc.y);
+    b[2] = min(b[2], cc.z);
+    b[3] = max(b[3], cc.x);
+    b[4] = max(b[4], cc.y);
+    b[5] = max(b[5], cc.z);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b[d] = min(b[d], __shfl_xor(b[d], off, 64));
+      b[3 + d] = max(b[3 + d], __shfl_xor(b[3 + d], off, 64));
+    }
+  }
+  const int wave = threadIdx.x >> 6;
+  if (laneId() == 0)
+    for (int d = 0; d < 6; ++d) part[d][wave] = b[d];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < kBoxThreads / 64; ++w)
+    for (int d = 0; d < 3; ++d) {
+      b[d] = min(b[d], part[d][w]);
+      b[3 + d] = max(b[3 + d], part[3 + d][w]);
+    }
+  if (b[0] > b[3]) return;  // no ghost in this workgroup
+  for (int d = 0; d < 3; ++d) {
+    atomicMin(&cellBox[d], b[d]);
+    atomicMax(&cellBox[3 + d], b[3 + d]);
+  }
 }
 
 // Pass 2 (after the flags were scanned into slots): scatter the visible
@@ -382,7 +426,6 @@ int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_
     o[s].hashValue = sets[s].table.hashValue;
     o[s].sortKey = sets[s].table.sortKey;
     o[s].sortIndex = sets[s].table.sortIndex;
-    o[s].cellBox = sets[s].cellBox;
     o[s].capacity = sets[s].capacity;
     o[s].status = status;
     o[s].q = sets[s].q;
@@ -393,6 +436,9 @@ int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_
                      particles.v, particles.q, n, grid, periodic, o[0], o[1], o[2], atBuild, I);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
+    hipLaunchKernelGGL(kGhostBox, dim3(kBoxBlocks), dim3(kBoxThreads), 0, stream, sets[s].pos,
+                       sets[s].realId, sets[s].count, sets[s].capacity, grid, I, sets[s].cellBox,
+                       gate);
     hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, stream, sets[s].count, sets[s].capacity,
                        gate);
     int r = buildStartEnd(sets[s].table, sets[s].capacity, sets[s].count, grid, workspace,
