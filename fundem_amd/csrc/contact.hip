@@ -200,24 +200,34 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
   }
   const I3 sz = gsize(grid);
   for (int cz = b0.z; cz <= b1.z; ++cz)
-    for (int cy = b0.y; cy <= b1.y; ++cy)
-      for (int cx = b0.x; cx <= b1.x; ++cx) {
-        const long long h = cellOffset + lin3({cx, cy, cz}, sz);
-        const int s = a.table.cellStart[h];
-        if (s < 0) continue;
-        const int e = a.table.cellEnd[h];
-        for (int m = s; m < e; ++m) {
-          const int idx = a.table.sortIndex[m];
-          if (SET == FDM_SET_PARTICLE) {
-            if (idx <= A) continue;
-            if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ldp(a.P.x, n, idx))) emit(idx, idx);
-          } else {
-            const int j = a.ghostReal[idx];
-            if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3(a.ghostPos + 3 * idx)))
-              emit(j, idx);
-          }
+    for (int cy = b0.y; cy <= b1.y; ++cy) {
+      // The cells of an x-row have consecutive ids and the table is sorted by
+      // cell id, so the row's entries form one range: the bounds of its
+      // cells are independent loads, and the range is visited in the same
+      // order as cell by cell.
+      const long long h0 = cellOffset + lin3({b0.x, cy, cz}, sz);
+      int s = INT_MAX, e = 0;
+#pragma unroll 4
+      for (int k = 0; k <= b1.x - b0.x; ++k) {
+        const int cs = a.table.cellStart[h0 + k];
+        const int ce = a.table.cellEnd[h0 + k];
+        if (cs >= 0) {
+          s = min(s, cs);
+          e = max(e, ce);
         }
       }
+      for (int m = s; m < e; ++m) {
+        const int idx = a.table.sortIndex[m];
+        if (SET == FDM_SET_PARTICLE) {
+          if (idx <= A) continue;
+          if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ldp(a.P.x, n, idx))) emit(idx, idx);
+        } else {
+          const int j = a.ghostReal[idx];
+          if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3(a.ghostPos + 3 * idx)))
+            emit(j, idx);
+        }
+      }
+    }
 }
 
 template <int SET>
