@@ -239,27 +239,38 @@ static inline V3 gradientQ1(double xf, double y, double z, const double* c) {
               (c[7] - c[3]) * wx1 * wy1;
   return {dx, dy, dz};
 }
-// Shared body of the four detection kernels from the local-frame position on:
-// LSParticleContactDetectionKernel.cu:72-119 (count) and :218-287 (write).
-// Returns true on contact (-phi >= 0) and fills overlap and the world normal.
-static bool sampleContact(V3 local, Q4 qB, int geoB, const Geoms& G, double& overlap, V3& normal) {
-  const V3 o = G.origin[geoB];
-  const double ih = G.invh[geoB];
-  const I3 sz = G.size[geoB];
+// Cell lookup of a local-frame point on one grid (LSParticleContactDetectionKernel.cu:72-92):
+// the fractional position in the cell and its eight corner values; false when
+// the cell is not inside the grid.
+struct CellSample {
+  double x, y, z, c[8];
+};
+static bool cellSample(V3 local, V3 o, double ih, I3 sz, const double* s, CellSample& cs) {
   const double gx = (local.x - o.x) * ih, gy = (local.y - o.y) * ih, gz = (local.z - o.z) * ih;
   const int i0 = toInt(std::floor(gx)), j0 = toInt(std::floor(gy)), k0 = toInt(std::floor(gz));
   if (i0 < 0 || j0 < 0 || k0 < 0) return false;
   if (i0 >= sz.x - 1 || j0 >= sz.y - 1 || k0 >= sz.z - 1) return false;
   const int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
-  const double x = gx - (double)i0, y = gy - (double)j0, z = gz - (double)k0;
-  const double* s = G.sdf.data() + G.sdfStart[geoB];
-  const double c[8] = {s[lin3({i0, j0, k0}, sz)], s[lin3({i1, j0, k0}, sz)], s[lin3({i0, j1, k0}, sz)],
-                       s[lin3({i1, j1, k0}, sz)], s[lin3({i0, j0, k1}, sz)], s[lin3({i1, j0, k1}, sz)],
-                       s[lin3({i0, j1, k1}, sz)], s[lin3({i1, j1, k1}, sz)]};
-  const double ov = -trilinear(x, y, z, c);
+  cs.x = gx - (double)i0;
+  cs.y = gy - (double)j0;
+  cs.z = gz - (double)k0;
+  const I3 corner[8] = {{i0, j0, k0}, {i1, j0, k0}, {i0, j1, k0}, {i1, j1, k0},
+                        {i0, j0, k1}, {i1, j0, k1}, {i0, j1, k1}, {i1, j1, k1}};
+  for (int q = 0; q < 8; ++q) cs.c[q] = s[lin3(corner[q], sz)];
+  return true;
+}
+// Shared body of the four detection kernels from the local-frame position on:
+// LSParticleContactDetectionKernel.cu:72-119 (count) and :218-287 (write).
+// Returns true on contact (-phi >= 0) and fills overlap and the world normal.
+static bool sampleContact(V3 local, Q4 qB, int geoB, const Geoms& G, double& overlap, V3& normal) {
+  CellSample cs;
+  if (!cellSample(local, G.origin[geoB], G.invh[geoB], G.size[geoB],
+                  G.sdf.data() + G.sdfStart[geoB], cs))
+    return false;
+  const double ov = -trilinear(cs.x, cs.y, cs.z, cs.c);
   if (!(ov >= 0.)) return false;
   overlap = ov;
-  normal = normalize(rotate(qB, gradientQ1(x, y, z, c)));
+  normal = normalize(rotate(qB, gradientQ1(cs.x, cs.y, cs.z, cs.c)));
   return true;
 }
 
@@ -1202,3 +1213,25 @@ int osim_set_spring(OracleSim* s, int set, int a, int tg, int k, const double* s
 }
 
 }  // extern "C"
+
+// The level-set sampling of the detection kernels on one grid, for the tests:
+// trilinear value and Q1 gradient (per cell, before rotation and
+// normalisation) at local-frame points; inside[i] = 0 and zeros where the
+// point's cell is not inside the grid.
+extern "C" void oracle_sample_level_set(const double* sdf, const int* size3, const double* origin3,
+                                        double spacing, const double* pts, int np, double* value,
+                                        double* grad, int* inside) {
+  const I3 sz{size3[0], size3[1], size3[2]};
+  const V3 o{origin3[0], origin3[1], origin3[2]};
+  const double ih = 1.0 / spacing;
+  for (int i = 0; i < np; ++i) {
+    CellSample cs;
+    const bool in = cellSample({pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, o, ih, sz, sdf, cs);
+    const V3 g = in ? gradientQ1(cs.x, cs.y, cs.z, cs.c) : V3{0., 0., 0.};
+    value[i] = in ? trilinear(cs.x, cs.y, cs.z, cs.c) : 0.;
+    grad[3 * i] = g.x;
+    grad[3 * i + 1] = g.y;
+    grad[3 * i + 2] = g.z;
+    inside[i] = in ? 1 : 0;
+  }
+}

@@ -152,6 +152,14 @@ void oracle_mesh_points(const double* v, int nv, const int* f, int nf, const dou
 long long oracle_mesh_grid(const double* v, int nv, const int* f, int nf, double h, int padding,
                            int positiveInside, double* origin3, int* size3, double* sdf);
 
+/* Level-set sampling of the detection kernels (LSParticleContactDetectionKernel.cu:72-119,
+ * .cuh:4-77) on one grid of size3 nodes (x fastest) at origin3 with the given spacing:
+ * trilinear value and the Q1 gradient per cell at np local-frame points; inside[i] = 0
+ * (value and gradient 0) where the point's cell is not inside the grid. */
+void oracle_sample_level_set(const double* sdf, const int* size3, const double* origin3,
+                             double spacing, const double* pts, int np, double* value,
+                             double* grad, int* inside);
+
 #ifdef __cplusplus
 }
 #endif

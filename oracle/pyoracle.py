@@ -46,6 +46,7 @@ def _load():
         "osim_set_constant_force": (V, [P, I, _dp, _dp]),
         "oracle_mesh_points": (V, [_dp, I, _ip, I, _dp, I, I, _dp, _dp]),
         "oracle_mesh_grid": (LL, [_dp, I, _ip, I, D, I, I, _dp, _ip, _dp]),
+        "oracle_sample_level_set": (V, [_dp, _ip, _dp, D, _dp, I, _dp, _dp, _ip]),
         "osim_add_wall_material_damping": (I, [P, D, D]),
         "osim_add_geometry": (I, [P, D, _dp, _dp, D, _ip, _dp, _dp, I]),
         "osim_add_geometry_density": (I, [P, D, _dp, D, _ip, _dp, _dp, I, _dp]),
@@ -317,3 +318,20 @@ def mesh_grid(vertices, triangles, spacing, padding=2, positive_inside=False):
     vals = np.zeros(n)
     lib().oracle_mesh_grid(*args, vals.ctypes.data_as(_dp))
     return o, tuple(int(x) for x in sz), vals
+
+
+def sample_level_set(sdf, size, origin, spacing, points):
+    """Oracle trilinear value, Q1 gradient (per cell) and inside flag at local points (n, 3);
+    sdf holds size[0]*size[1]*size[2] node values, x fastest."""
+    g = np.ascontiguousarray(np.asarray(sdf, np.float64).reshape(-1))
+    sz = np.ascontiguousarray(np.asarray(size, np.int32))
+    o = np.ascontiguousarray(np.asarray(origin, np.float64))
+    p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 3))
+    if g.size != int(np.prod(sz)):
+        raise ValueError("sdf size does not match the grid")
+    val, grad, ins = np.zeros(len(p)), np.zeros((len(p), 3)), np.zeros(len(p), np.int32)
+    lib().oracle_sample_level_set(g.ctypes.data_as(_dp), sz.ctypes.data_as(_ip), o.ctypes.data_as(_dp),
+                                  float(spacing), p.ctypes.data_as(_dp), len(p),
+                                  val.ctypes.data_as(_dp), grad.ctypes.data_as(_dp),
+                                  ins.ctypes.data_as(_ip))
+    return val, grad, ins.astype(bool)
