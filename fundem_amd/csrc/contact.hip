@@ -796,7 +796,9 @@ __device__ __forceinline__ int reservePooled(int* counters, int capacity, int po
 // processPair; the contacts' geometry is staged in node order (through LDS,
 // or recomputed chunk by chunk past kMaxHits) for the contact-law kernel,
 // and the pair's record range is reserved.
-template <int SET>
+// CULL: take the pre-cull's test (kPreCull) first, from the same values, and
+// leave empty record ranges when it rejects the pair.
+template <int SET, bool CULL = false>
 __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
                           int p, int lane, HitSlots& hs, int pool) {
   const int A = S.pairs.pairA[p];
@@ -810,6 +812,24 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   Target tg;
   V3 armB;
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
+  if (CULL) {
+    const double RA = tA.radius * kReach;
+    const double reachT = SET == FDM_SET_PARTICLE ? tg.t->radius : tg.t->boxReach;
+    const V3 d = vsub(xA, tg.x);
+    const double rs = RA + reachT * kReach;
+    bool keep = vlen2(d) <= rs * rs;
+    if (keep) {
+      const V3 local = qrotateT(d, tg.q);
+      keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
+    }
+    if (!keep) {
+      if (lane == 0) {
+        S.pairs.recOff[p] = 0;
+        S.pairs.recCnt[p] = 0;
+      }
+      return;
+    }
+  }
   const double RB = tg.t->radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
   const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
@@ -1033,104 +1053,79 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
 // thread per pair): the pair is kept only if a node of A can still reach
 // the target (the broad-phase tests without the skin). Culled pairs get
 // empty record ranges; kept pairs go to the work list as (p << 3 | set).
-__global__ void __launch_bounds__(256) kPreCull(NarrowMulti a, int* work, int* workCount) {
-  if (gateOff(a.gate)) return;
-  int start[kMaxSets + 1];
-  start[0] = 0;
+// Candidate pairs of all sets in one index space: set k holds the indices
+// [start[k], start[k + 1]).
+struct SetStarts {
+  int v[kMaxSets + 1];
+};
+__device__ __forceinline__ SetStarts setStarts(const NarrowMulti& a) {
+  SetStarts st;
+  st.v[0] = 0;
 #pragma unroll
   for (int k = 0; k < kMaxSets; ++k)
-    start[k + 1] = start[k] + (k < a.nsets ? min(*a.s[k].pairs.total, a.s[k].pairs.capacity) : 0);
-  const int lane = laneId();
-  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const int stride = gridDim.x * blockDim.x;
-  // whole waves iterate together so the append below stays wave-uniform
-  for (int g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < start[kMaxSets]; g0 += stride) {
-    const int g = g0 + lane;
-    bool keep = false;
-    int code = 0;
-    if (g < start[kMaxSets]) {
-      int k = 0;
-      while (g >= start[k + 1]) ++k;
-      const NarrowSet& S = a.s[k];
-      const int p = g - start[k];
-      code = (p << 3) | k;
-      const int A = S.pairs.pairA[p];
-      if (A >= 0) {
-        const int T = S.pairs.pairT[p];
-        V3 xA = ldp(a.c.P.x, a.c.P.n, A);
-        const double RA = tmplOf(a.c.pgeo, a.c.P.geo[A]).radius * kReach;
-        Target tg;
-        V3 armB;
-        double reachT;
-        if (S.kind == FDM_SET_WALL) {
-          resolveTarget<FDM_SET_WALL>(a.c, S, p, T, pairFrame<FDM_SET_WALL>(a.c, S, A), tg, armB);
-          reachT = tg.t->boxReach;
-        } else if (S.kind == FDM_SET_PARTICLE) {
-          resolveTarget<FDM_SET_PARTICLE>(a.c, S, p, T, pairFrame<FDM_SET_PARTICLE>(a.c, S, A), tg, armB);
-          reachT = tg.t->radius;
-        } else {
-          const PairFrame fr = pairFrame<FDM_SET_GHOST>(a.c, S, A);
-          xA = wrappedArm(xA, *fr.pgrid, fr.periodic);  // ghost pairs: after the wrap
-          resolveTarget<FDM_SET_GHOST>(a.c, S, p, T, fr, tg, armB);
-          reachT = tg.t->boxReach;
-        }
-        const V3 d = vsub(xA, tg.x);
-        const double rs = RA + reachT * kReach;
-        if (vlen2(d) <= rs * rs) {
-          const V3 local = qrotateT(d, tg.q);
-          keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
-        }
-      }
-      if (!keep) {
-        S.pairs.recOff[p] = 0;
-        S.pairs.recCnt[p] = 0;
-      }
-    }
-    const unsigned long long m = ballot64(keep);
-    if (m) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(workCount, __popcll(m));
-      base = readlaneI(base, 0);
-      if (keep) work[base + __popcll(m & lt)] = code;
-    }
-  }
+    st.v[k + 1] = st.v[k] + (k < a.nsets ? min(*a.s[k].pairs.total, a.s[k].pairs.capacity) : 0);
+  return st;
 }
 
-// One wave per kept pair of the work list, WAVES waves per workgroup (one
-// wave per workgroup frees a finished wave's slot at once; more waves per
-// workgroup launch fewer workgroups).
+// The pre-cull and the sweep in one pass: one wave per candidate pair (WAVES
+// waves per workgroup). The wave takes the pre-cull's test itself (its
+// values are wave-uniform) and sweeps the pair only when it is kept; a culled
+// pair gets empty record ranges. No work list: the candidate index is the
+// work item.
 template <int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) kNarrowSweep(NarrowMulti a, NarrowScratch x,
-                                                           const int* work, const int* workCount) {
+__global__ void __launch_bounds__(64 * WAVES) kCullSweep(NarrowMulti a, NarrowScratch x) {
   if (gateOff(a.gate)) return;
   __shared__ HitSlots slots[WAVES];
   const int lane = laneId();
-  // wave-uniform (scalar), so the pair's data loads stay scalar loads
   const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = *workCount;
-  for (int w = blockIdx.x * WAVES + wave; w < total; w += gridDim.x * WAVES) {
-    const int code = work[w];
-    const NarrowSet& s = a.s[code & 7];
-    const int p = code >> 3;
-    const int pool = w % FDM_REC_POOLS;
-    if (s.kind == FDM_SET_PARTICLE) sweepPair<FDM_SET_PARTICLE>(a.c, s, x, w, p, lane, slots[wave], pool);
-    else if (s.kind == FDM_SET_WALL) sweepPair<FDM_SET_WALL>(a.c, s, x, w, p, lane, slots[wave], pool);
-    else sweepPair<FDM_SET_GHOST>(a.c, s, x, w, p, lane, slots[wave], pool);
+  const SetStarts st = setStarts(a);
+  const int total = st.v[kMaxSets];
+  for (int g = blockIdx.x * WAVES + wave; g < total; g += gridDim.x * WAVES) {
+    int k = 0, base = 0;
+#pragma unroll
+    for (int q = 1; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
+      if (g >= st.v[q]) {
+        k = q;
+        base = st.v[q];
+      }
+    const NarrowSet& s = a.s[k];
+    const int p = g - base;
+    if (s.pairs.pairA[p] < 0) {  // an empty slot of an overflowing list
+      if (lane == 0) {
+        s.pairs.recOff[p] = 0;
+        s.pairs.recCnt[p] = 0;
+      }
+      continue;
+    }
+    const int pool = g % FDM_REC_POOLS;
+    if (s.kind == FDM_SET_PARTICLE)
+      sweepPair<FDM_SET_PARTICLE, true>(a.c, s, x, g, p, lane, slots[wave], pool);
+    else if (s.kind == FDM_SET_WALL)
+      sweepPair<FDM_SET_WALL, true>(a.c, s, x, g, p, lane, slots[wave], pool);
+    else
+      sweepPair<FDM_SET_GHOST, true>(a.c, s, x, g, p, lane, slots[wave], pool);
   }
 }
 
-// One thread per kept pair.
-__global__ void __launch_bounds__(256) kContactLaw(NarrowMulti a, NarrowScratch x, const int* work,
-                                                   const int* workCount) {
+// The contact law over the candidate index space of kCullSweep (one thread
+// per candidate; culled pairs and pairs without contacts return at once).
+__global__ void __launch_bounds__(256) kContactLawAll(NarrowMulti a, NarrowScratch x) {
   if (gateOff(a.gate)) return;
-  const int total = *workCount;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < total; w += gridDim.x * blockDim.x) {
-    const int code = work[w];
-    const NarrowSet& s = a.s[code & 7];
-    const int p = code >> 3;
-    if (s.kind == FDM_SET_PARTICLE) lawPair<FDM_SET_PARTICLE>(a.c, s, x, w, p);
-    else if (s.kind == FDM_SET_WALL) lawPair<FDM_SET_WALL>(a.c, s, x, w, p);
-    else lawPair<FDM_SET_GHOST>(a.c, s, x, w, p);
+  const SetStarts st = setStarts(a);
+  const int total = st.v[kMaxSets];
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    int k = 0, base = 0;
+#pragma unroll
+    for (int q = 1; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
+      if (g >= st.v[q]) {
+        k = q;
+        base = st.v[q];
+      }
+    const NarrowSet& s = a.s[k];
+    const int p = g - base;
+    if (s.kind == FDM_SET_PARTICLE) lawPair<FDM_SET_PARTICLE>(a.c, s, x, g, p);
+    else if (s.kind == FDM_SET_WALL) lawPair<FDM_SET_WALL>(a.c, s, x, g, p);
+    else lawPair<FDM_SET_GHOST>(a.c, s, x, g, p);
   }
 }
 
@@ -1316,8 +1311,8 @@ extern "C" int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream
 
 extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets,
                                          const fdm_narrow_workspace* ws, hipStream_t stream) {
-  if (!sets || nsets < 1 || nsets > kMaxSets || !ws || !ws->workList || !ws->workCount ||
-      !ws->workHit || !ws->hitGeom || !ws->hitCount || ws->hitCapacity < FDM_REC_POOLS)
+  if (!sets || nsets < 1 || nsets > kMaxSets || !ws || !ws->workHit || !ws->hitGeom ||
+      !ws->hitCount || ws->hitCapacity < FDM_REC_POOLS)
     return FDM_EINVAL;
   NarrowMulti k;
   fillCommon(k.c, sets[0]);
@@ -1336,21 +1331,16 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   x.hitGeom = ws->hitGeom;
   x.hitCapacity = ws->hitCapacity;
   x.hitCount = ws->hitCount;
-  const unsigned cullBlocks = std::max<unsigned>(1, std::min<unsigned>(blocksFor(cap, 256), 2048));
-  hipLaunchKernelGGL(kPreCull, dim3(cullBlocks), dim3(256), 0, stream, k, ws->workList,
-                     ws->workCount);
   const fdm_tuning& tu = tuning();
   const int sweepBlocks = std::max(1, persistentBlocks(tu.sweepWaves) * tu.sweepWavesPerCU / 32);
-  const int* wl = ws->workList;
-  const int* wc = ws->workCount;
   if (tu.sweepWaves == 4)
-    hipLaunchKernelGGL(kNarrowSweep<4>, dim3(sweepBlocks), dim3(256), 0, stream, k, x, wl, wc);
+    hipLaunchKernelGGL(kCullSweep<4>, dim3(sweepBlocks), dim3(256), 0, stream, k, x);
   else if (tu.sweepWaves == 2)
-    hipLaunchKernelGGL(kNarrowSweep<2>, dim3(sweepBlocks), dim3(128), 0, stream, k, x, wl, wc);
+    hipLaunchKernelGGL(kCullSweep<2>, dim3(sweepBlocks), dim3(128), 0, stream, k, x);
   else
-    hipLaunchKernelGGL(kNarrowSweep<1>, dim3(sweepBlocks), dim3(64), 0, stream, k, x, wl, wc);
-  hipLaunchKernelGGL(kContactLaw, dim3(persistentBlocks(tu.lawBlock / 64)), dim3(tu.lawBlock), 0,
-                     stream, k, x, wl, wc);
+    hipLaunchKernelGGL(kCullSweep<1>, dim3(sweepBlocks), dim3(64), 0, stream, k, x);
+  hipLaunchKernelGGL(kContactLawAll, dim3(persistentBlocks(tu.lawBlock / 64)), dim3(tu.lawBlock), 0,
+                     stream, k, x);
   return launchStatus();
 }
 

@@ -325,8 +325,6 @@ class LSSolver {
       for (int s = 0; s < 3; ++s)
         if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
-      workList_.allocate((size_t)cap, false);
-      workCount_.allocate(FDM_REC_COUNTERS);
       workHit_.allocate((size_t)cap, false);
       // staged contacts of one step, all sets together
       hitCount_.allocate(FDM_REC_COUNTERS);
@@ -1096,7 +1094,6 @@ class LSSolver {
     if (walls_.num()) a.clearCounters[q++] = pw_.nextRecordCounter();
     for (int s = 0; s < 3; ++s)
       if (periodic_.active(s)) a.clearCounters[q++] = periodic_.set(s).interaction.nextRecordCounter();
-    a.clearCounters[q++] = workCount_.get();
     a.clearCounters[q++] = hitCount_.get();
     const fdm_skin sk = skinView();
     const size_t wsBytes = workspace_.size() * sizeof(double);
@@ -1236,9 +1233,9 @@ class LSSolver {
     {
       Timer t(this, "narrow");
       fdm_narrow_workspace w;
-      w.workList = workList_.get();
-      w.workCapacity = (int)workList_.size();
-      w.workCount = workCount_.get();
+      w.workList = nullptr;
+      w.workCapacity = (int)workHit_.size();
+      w.workCount = nullptr;
       w.workHit = workHit_.get();
       w.hitGeom = hitGeom_.get();
       w.hitCapacity = hitCapacity_;
@@ -1436,7 +1433,7 @@ class LSSolver {
   DeviceBuffer<fdm_grid> instPGrid_, instWGrid_;
   DeviceBuffer<long long> instPOff_, instWOff_;
   DeviceBuffer<fdm_periodic> instPeriodic_;
-  DeviceBuffer<int> workList_, workCount_, workHit_, hitCount_;
+  DeviceBuffer<int> workHit_, hitCount_;
   DeviceBuffer<double> hitGeom_;
   int hitCapacity_ = 0;
   DeviceBuffer<double> disp_;

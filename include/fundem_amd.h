@@ -519,28 +519,28 @@ typedef struct {
 int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
 /* Scratch of fdm_launchNarrowPhaseSets. */
 typedef struct {
-  int* workList;               /* [workCapacity] kept pairs, (pair << 3) | set */
+  int* workList;               /* unused (may be NULL); kept for the layout */
   int workCapacity;            /* at least the sum of the sets' pair capacities */
-  int* workCount;              /* [FDM_REC_COUNTERS] kept-pair counter (entry 0) */
-  int* workHit;                /* [workCapacity] first staged contact of each kept pair */
+  int* workCount;              /* unused (may be NULL); kept for the layout */
+  int* workHit;                /* [workCapacity] first staged contact of each candidate pair */
   double* hitGeom;             /* [8 * hitCapacity] staged contacts */
   int hitCapacity;             /* staged contacts, split into FDM_REC_POOLS pools */
   int* hitCount;               /* [FDM_REC_COUNTERS] staged contacts per pool */
 } fdm_narrow_workspace;
 /* No counterpart in the reference: the narrow phase of up to five sets at
- * once, in three kernels. The first (one thread per pair) keeps the pairs
- * whose bodies can still touch at this step's positions (the broad-phase
- * tests without the skin) in workList and gives the others empty record
- * ranges; the second (one wave per kept pair) sweeps the surface nodes,
- * stages every contact's point, normal and overlap in hitGeom and reserves
- * the pair's records; the third (one thread per kept pair) applies the
- * contact law to the staged contacts in node order and writes the records
- * and the pair sums. Shared fields (bodies, geometry, materials, pgrid,
- * periodic, dt, updateSprings, status, dbg*) come from sets[0]. Unlike
+ * once, in two kernels over the candidate pairs of all sets (set k's pairs
+ * follow those of sets 0..k-1). The first (one wave per candidate pair)
+ * tests whether the pair's bodies can still touch at this step's positions
+ * (the broad-phase tests without the skin), gives a rejected pair empty
+ * record ranges, and otherwise sweeps the surface nodes, stages every
+ * contact's point, normal and overlap in hitGeom and reserves the pair's
+ * records; the second (one thread per candidate pair) applies the contact
+ * law to the staged contacts in node order and writes the records and the
+ * pair sums. Shared fields (bodies, geometry, materials, pgrid, periodic,
+ * dt, updateSprings, status, dbg*) come from sets[0]. Unlike
  * fdm_launchNarrowPhase it clears no counter: the sets' record counters
- * (pairs.recTotal), *workCount and the hitCount pools must be zero at launch.
- * Work item w reserves its records and staged contacts from pool
- * w % FDM_REC_POOLS. */
+ * (pairs.recTotal) and the hitCount pools must be zero at launch. Candidate
+ * g reserves its records and staged contacts from pool g % FDM_REC_POOLS. */
 int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int nsets,
                               const fdm_narrow_workspace* ws, hipStream_t stream);
 
