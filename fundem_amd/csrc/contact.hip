@@ -796,8 +796,9 @@ __device__ __forceinline__ int reservePooled(int* counters, int capacity, int po
 // processPair; the contacts' geometry is staged in node order (through LDS,
 // or recomputed chunk by chunk past kMaxHits) for the contact-law kernel,
 // and the pair's record range is reserved.
-// CULL: take the pre-cull's test (kPreCull) first, from the same values, and
-// leave empty record ranges when it rejects the pair.
+// CULL: first take the particle-level pre-cull test (sphere, SDF box,
+// Lipschitz bound at this step's positions) from the same values, and leave
+// empty record ranges when it rejects the pair.
 template <int SET, bool CULL = false>
 __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
                           int p, int lane, HitSlots& hs, int pool) {
