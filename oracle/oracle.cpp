@@ -1212,6 +1212,17 @@ int osim_set_spring(OracleSim* s, int set, int a, int tg, int k, const double* s
   return 0;
 }
 
+int osim_get_ghosts(const OracleSim* s, int set, double* pos, int* hash) {
+  if (set < 0 || set >= 3) return 0;
+  const GhostSet& G = s->gh[set];
+  const int n = (int)G.pos.size();
+  for (int i = 0; i < n; ++i) {
+    if (pos) st3(pos + 3 * i, G.pos[i]);
+    if (hash) hash[i] = ghostHashOf(G.pos[i], s->pgrid);
+  }
+  return n;
+}
+
 }  // extern "C"
 
 // The level-set sampling of the detection kernels on one grid, for the tests:

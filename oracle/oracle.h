@@ -141,6 +141,12 @@ void osim_get_bonds(const OracleSim* s, int* active, double* max_normal_stress,
 /* Set one entry of the tangential history of a contact set, looked up by key;
  * returns 1 if the key was present. Used by tests to seed histories. */
 int osim_set_spring(OracleSim* s, int set, int a, int target, int k, const double* spring3);
+/* Ghost set `set` (0 X, 1 Y, 2 XY; sector: R90, R180, R270) of the latest
+ * step, one entry per particle: the ghost position (the particle's own
+ * position where it has no image) and its ghost cell id
+ * (calculateGhostHashKernel; -1 where the ghost is invisible). Returns the
+ * entry count (0 before the first periodic step). */
+int osim_get_ghosts(const OracleSim* s, int set, double* pos, int* hash);
 
 /* Mesh level sets (meshsdf.cpp; LevelSetObjectGenerator.h:122-224,454-914).
  * v: [3*nv] vertices, f: [3*nf] triangles. Points: value and gradient at np

@@ -71,6 +71,7 @@ def _load():
         "osim_get_hash": (V, [P, I, _ip, _ip, _ip, _ip]),
         "osim_get_bonds": (V, [P, _ip, _dp, _dp]),
         "osim_set_spring": (I, [P, I, I, I, I, _dp]),
+        "osim_get_ghosts": (I, [P, I, _dp, _ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +292,14 @@ class OracleSim:
 
     def set_spring(self, set_id, a, target, k, spring):
         return lib().osim_set_spring(self.h, set_id, a, target, k, dptr(d3(spring)))
+
+    def ghosts(self, set_id):
+        """Ghost set 0 (X), 1 (Y) or 2 (XY) of the latest step, per particle:
+        positions (n, 3) and ghost cell ids (-1: invisible)."""
+        n = lib().osim_get_ghosts(self.h, set_id, None, None)
+        pos, h = np.zeros((n, 3)), np.zeros(n, np.int32)
+        lib().osim_get_ghosts(self.h, set_id, dptr(pos), iptr(h))
+        return pos, h
 
 
 def mesh_points(vertices, triangles, points, positive_inside=False):
