@@ -18,6 +18,17 @@ stream its kernels run on); their average per launch gives the roofline of
 the dominant kernel. The CPU baseline is the serial FP64 oracle
 on a 512-particle sample of the same workload.
 
+The same line carries, as extra keys (--extras 1, the default):
+  dense                 the pour's lattice compressed until its layers touch,
+                        restarted at rest and timed like the pour (the
+                        particle-particle narrow phase under load);
+  batch, batch_speedup  512 packings in one solver against the same
+                        packings run one after another in plain solvers;
+  adjoint, adjoint_over_forward
+                        one adjoint step of the 4096-particle column collapse
+                        (2000 steps, checkpoints every 100) against one
+                        forward step.
+
 Prints one JSON line.
 """
 import argparse
@@ -60,6 +71,14 @@ def parse():
                         "against forward step cost; meshsdf: level-set grid of a closed mesh")
     p.add_argument("--batch-sample", type=int, default=32,
                    help="packings timed one by one for the sequential reference")
+    p.add_argument("--extras", type=int, default=1,
+                   help="pour: also measure the contact-dense state, the batch against "
+                        "sequential plain runs and the adjoint against the forward step, as "
+                        "extra keys of the same line")
+    p.add_argument("--dense-closing", type=float, default=5e-3,
+                   help="dense state: time in which the lattice's layers close in (s)")
+    p.add_argument("--dense-max-steps", type=int, default=20000,
+                   help="dense state: most compression steps")
     return p.parse_args()
 
 
@@ -236,7 +255,7 @@ def batch_main(a):
                                  f"in a solver of its own, {a.steps} steps each, one after another"},
         "speedup": batched / sequential,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def column_main(a):
@@ -298,7 +317,7 @@ def column_main(a):
                    "forward_ms_per_step": 1e3 * fwd, "forward_with_checkpoints_ms_per_step":
                    1e3 * fwd_ckpt, "contacts_last_step": contacts},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def meshsdf_main(a):
@@ -345,32 +364,9 @@ def meshsdf_main(a):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    a = parse()
-    if a.workload == "meshsdf":
-        if a.gpus != 1:
-            print("bench.py measures one GPU", file=sys.stderr)
-            sys.exit(2)
-        meshsdf_main(a)
-        return
-    if a.workload == "column":
-        if a.gpus != 1:
-            print("bench.py measures one GPU", file=sys.stderr)
-            sys.exit(2)
-        column_main(a)
-        return
-    if a.workload == "batch":
-        if a.gpus != 1:
-            print("bench.py measures one GPU", file=sys.stderr)
-            sys.exit(2)
-        batch_main(a)
-        return
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the one-GPU build "
-              f"does not shard)", file=sys.stderr)
-        sys.exit(2)
-    if a.copy_probe:
-        copy_probe()
+def pour_main(a):
+    """The 1M-particle pour: the headline line (value, roofline of the
+    dominant stage, per-stage rooflines, whole-step HBM fraction)."""
     from fundem_amd import Solver
     from fundem_amd import workloads as W
 
@@ -385,12 +381,13 @@ def main():
     log(f"uploaded and initialised in {time.perf_counter() - t0:.1f} s")
     s.use_graph(bool(a.graph))
     s.step(1)
-    s.step(a.warmup)
-    s.sync()
     tuning = autotune(s, a)
+    # warm-up after tuning: the tuner leaves the solver at its saved point
+    # with a list build requested, which the warm-up consumes
+    s.step(max(a.warmup, 1))
+    s.sync()
     if s.status() != 0:
         raise RuntimeError(f"capacity exceeded during warm-up (status {s.status()})")
-    s.sync()
     b0, gb0, skin = s.list_builds()
     t0 = time.perf_counter()
     s.step(a.steps)
@@ -400,14 +397,7 @@ def main():
     status = s.status()
     if status != 0:
         raise RuntimeError(f"capacity exceeded during the timed steps (status {status})")
-    # stage timing (HIP events around every stage) in a separate eager pass,
-    # so that the timed steps above carry no instrumentation
-    s.use_graph(False)
-    s.set_timing(True)
-    s.step(a.stage_steps)
-    s.sync()
-    stages = s.stage_times()
-    s.set_timing(False)
+    stages, sb0, sb1 = stage_pass(s, a.stage_steps)
     contacts = {k: int(s.contacts(k)["A"].size) for k in range(5)}
     pairs = dict(enumerate(s.pair_totals()))
     n = scn.n
@@ -416,34 +406,13 @@ def main():
     log(f"{ms_step:.3f} ms/step, {value:.3e} particle-steps/s; contacts at the last step "
         f"{contacts}, candidate pairs {pairs}; skin {skin:.3e} m, {b1 - b0} list builds and "
         f"{gb1 - gb0} ghost-list builds in {a.steps} steps")
-    for k, (ms, calls) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
-        log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches  "
-            f"{ms / a.stage_steps * 1e3:8.1f} us/step (stage-timed pass)")
-
-    roof = None
-    modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}
-    # every modelled stage beside the dominant one: the streaming stages sit
-    # near the bandwidth roof, the narrow phase far below it (latency-bound)
-    stage_roof = []
-    for k, (ms, calls) in sorted(modeled.items(), key=lambda kv: -kv[1][0]):
-        nb = kernel_bytes(k, n, pairs, contacts)
-        avg_s = ms / max(calls, 1) * 1e-3
-        ach = nb / avg_s / 1e9 if avg_s > 0 else 0.0
-        stage_roof.append({"stage": k, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb,
-                           "achieved": ach, "frac": ach / HBM_PEAK_GBS,
-                           "traffic": measured_traffic(k)})
-    if modeled:
-        dom, (ms, calls) = max(modeled.items(), key=lambda kv: kv[1][0])
-        nb = kernel_bytes(dom, n, pairs, contacts)
-        avg_s = ms / max(calls, 1) * 1e-3
-        achieved = nb / avg_s / 1e9 if avg_s > 0 else 0.0
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
-                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb}
+    log_stages(stages, a.stage_steps)
+    roof, stage_roof = rooflines(stages, n, pairs, contacts)
     # algorithmic bytes of a whole step: state 224 B per particle and 20 B per
     # contact record read and written, plus the template tables
     c_total = sum(contacts.values())
     step_bytes = 224.0 * n + 2 * 20.0 * c_total + 16 * (32 ** 3 * 4 + 1000 * 12)
+    rebuild = rebuild_cost(stages, a.stage_steps, sb1 - sb0)
     out = {
         "metric": "particle-steps/s (1M level-set particle pour, one GPU)",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
@@ -458,9 +427,173 @@ def main():
         "step_hbm_fraction": step_bytes * a.steps / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
         "stage_roofline": stage_roof,
+        "list_rebuild": rebuild,
     }
+    s.close()
+    return out, scn.templates
+
+
+def stage_pass(s, steps):
+    """Stage timing (HIP events around every stage on the solver's stream)
+    in a separate eager pass, so that the timed steps carry no
+    instrumentation. Returns the stage times and the list-build counters
+    around the pass."""
+    s.use_graph(False)
+    s.set_timing(True)
+    b0, _, _ = s.list_builds()
+    s.step(steps)
+    s.sync()
+    b1, _, _ = s.list_builds()
+    stages = s.stage_times()
+    s.set_timing(False)
+    return stages, b0, b1
+
+
+def log_stages(stages, steps):
+    for k, (ms, calls) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
+        log(f"  {k:18s} {ms / max(calls, 1) * 1e3:9.1f} us/launch  {calls} launches  "
+            f"{ms / steps * 1e3:8.1f} us/step (stage-timed pass)")
+
+
+REBUILD_STAGES = ("sort_cells", "broad_pp", "broad_pw", "broad_ghost", "periodic_build")
+
+
+def rebuild_cost(stages, steps, builds):
+    """Time of the list-build stages in the stage-timed pass: per build and
+    amortised per step."""
+    ms = sum(stages[k][0] for k in REBUILD_STAGES if k in stages)
+    return {"builds": builds, "steps": steps, "us_per_build": 1e3 * ms / builds if builds else None,
+            "us_per_step_amortised": 1e3 * ms / steps}
+
+
+def rooflines(stages, n, pairs, contacts):
+    """Roofline of the dominant modelled stage (by total time) and of every
+    modelled stage: algorithmic bytes per launch over the average launch."""
+    modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}
+    stage_roof = []
+    for k, (ms, calls) in sorted(modeled.items(), key=lambda kv: -kv[1][0]):
+        nb = kernel_bytes(k, n, pairs, contacts)
+        avg_s = ms / max(calls, 1) * 1e-3
+        ach = nb / avg_s / 1e9 if avg_s > 0 else 0.0
+        stage_roof.append({"stage": k, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb,
+                           "achieved": ach, "frac": ach / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(k)})
+    roof = None
+    if modeled:
+        dom, (ms, calls) = max(modeled.items(), key=lambda kv: kv[1][0])
+        nb = kernel_bytes(dom, n, pairs, contacts)
+        avg_s = ms / max(calls, 1) * 1e-3
+        achieved = nb / avg_s / 1e9 if avg_s > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
+                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb}
+    return roof, stage_roof
+
+
+def dense_main(a, templates):
+    """The pour's lattice driven into a contact-dense state: every particle
+    starts falling at a speed proportional to its height above the lowest,
+    so the layers close in on one another within about --dense-closing
+    seconds and collide at about (lattice spacing) / closing; once the
+    particle-particle node contacts stop growing (checked every 200 steps),
+    that state is restarted at rest (v = w = 0) and timed like the pour."""
+    import dataclasses
+    import numpy as np
+    from fundem_amd import Solver
+    from fundem_amd import workloads as W
+    scn = W.pour((a.side, a.side, a.side), templates=templates)
+    v = scn.v.copy()
+    v[:, 2] = -(scn.x[:, 2] - scn.x[:, 2].min()) / a.dense_closing
+    s = Solver(0)
+    s.set_capacity(12, 64)
+    W.apply(dataclasses.replace(scn, v=v), s)
+    t0 = time.perf_counter()
+    best, done = 0, 0
+    while done < a.dense_max_steps:
+        s.step(200)
+        done += 200
+        c = int(s.contacts(0)["A"].size)
+        if c >= 4 * scn.n or (best > scn.n and c < best):
+            break
+        best = max(best, c)
+    if s.status() != 0:
+        raise RuntimeError(f"capacity exceeded while compressing (status {s.status()})")
+    st = s.state()
+    s.close()
+    log(f"dense: compressed for {done} steps in {time.perf_counter() - t0:.1f} s, {c} "
+        "particle-particle node contacts")
+    dense = dataclasses.replace(scn, x=st["x"], q=st["q"], v=np.zeros_like(st["v"]),
+                                w=np.zeros_like(st["w"]))
+    s = Solver(0)
+    s.set_capacity(12, 64)
+    s.set_skin(a.skin)
+    W.apply(dense, s)
+    s.step(1)
+    s.step(max(a.warmup, 1))
+    s.sync()
+    steps = max(a.steps, 50)
+    b0, gb0, skin = s.list_builds()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    b1, gb1, _ = s.list_builds()
+    if s.status() != 0:
+        raise RuntimeError(f"capacity exceeded in the dense state (status {s.status()})")
+    stages, sb0, sb1 = stage_pass(s, a.stage_steps)
+    contacts = {k: int(s.contacts(k)["A"].size) for k in range(5)}
+    pairs = dict(enumerate(s.pair_totals()))
+    n = dense.n
+    log(f"dense: {1e3 * elapsed / steps:.3f} ms/step, {n * steps / elapsed:.3e} particle-steps/s; "
+        f"contacts {contacts}, candidate pairs {pairs}, {b1 - b0} list builds in {steps} steps")
+    log_stages(stages, a.stage_steps)
+    roof, stage_roof = rooflines(stages, n, pairs, contacts)
+    s.close()
+    return {"metric": "particle-steps/s (1M pour lattice compressed until its layers touch)",
+            "value": n * steps / elapsed, "unit": "particle-steps/s", "steps": steps,
+            "ms_per_step": 1e3 * elapsed / steps,
+            "state": f"layers closed in at (spacing)/{a.dense_closing} s for {done} steps, then "
+                     "restarted at rest",
+            "contacts_last_step": contacts, "candidate_pairs": pairs, "skin_m": skin,
+            "list_builds": b1 - b0, "ghost_list_builds": gb1 - gb0,
+            "list_rebuild": rebuild_cost(stages, a.stage_steps, sb1 - sb0),
+            "roofline": roof, "stage_roofline": stage_roof}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the one-GPU build "
+              f"does not shard)", file=sys.stderr)
+        sys.exit(2)
+    if a.workload == "meshsdf":
+        meshsdf_main(a)
+        return
+    if a.workload == "column":
+        print(json.dumps(column_main(a)), flush=True)
+        return
+    if a.workload == "batch":
+        print(json.dumps(batch_main(a)), flush=True)
+        return
+    if a.copy_probe:
+        copy_probe()
+    out, templates = pour_main(a)
+    if a.extras:
+        # BASELINE's other two metrics and the contact-dense state, as extra
+        # keys of the same line
+        out["dense"] = dense_main(a, templates)
+        ab = argparse.Namespace(**vars(a))
+        ab.steps = max(a.steps, 100)
+        batch = batch_main(ab)
+        out["batch"] = batch
+        out["batch_speedup"] = batch["speedup"]
+        ac = argparse.Namespace(**vars(a))
+        ac.steps, ac.checkpoint, ac.warmup = 2000, 100, max(a.warmup, 1)
+        col = column_main(ac)
+        out["adjoint"] = col
+        out["adjoint_over_forward"] = col["value"]
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scn.templates, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(templates, a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

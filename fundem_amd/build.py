@@ -65,9 +65,31 @@ def build_solver() -> str:
     return SOLVER_LIB
 
 
+APP_SRC = os.path.join(ROOT, "tests", "apps", "tutorial_app.cpp")
+APP_BIN = os.path.join(ROOT, "tests", "apps", "build", "tutorial_app")
+
+
+def build_test_app() -> str:
+    """A tutorial-style application of the C++ host API (test program):
+    built like the binding, against the headers and the core library."""
+    os.makedirs(os.path.dirname(APP_BIN), exist_ok=True)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = APP_BIN + ".tmp"
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off",
+          "-D__HIP_PLATFORM_AMD__", "-Wno-unused-result",
+          "-I", os.path.join(PKG, "include"), "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(rocm, "include"), APP_SRC, "-L", LIB_DIR, "-lfundem_amd",
+          "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+          "-Wl,-rpath," + LIB_DIR, "-Wl,-rpath,$ORIGIN/../../../fundem_amd/lib",
+          "-Wl,-rpath," + os.path.join(rocm, "lib"), "-o", tmp])
+    os.replace(tmp, APP_BIN)
+    return APP_BIN
+
+
 def build_all() -> None:
     build_core()
     build_solver()
+    build_test_app()
 
 
 if __name__ == "__main__":

@@ -157,9 +157,9 @@ long long fdmx_num_bonds(void* s) {
   return (long long)static_cast<BoundSolver*>(s)->getLSParticleBondedInteraction().numPair();
 }
 
-// Autotuner (fundem::LSAutotuner): out7 = skin, cell size, sweep waves,
-// sweep waves per CU, law block, reduce block, ms per step; log receives
-// one line per timed candidate (truncated at logCap).
+// Autotuner (fundem::LSAutotuner): out8 = skin, cell size, sweep waves,
+// sweep waves per CU, law block, reduce block, ms per step, law mode; log
+// receives one line per timed candidate (truncated at logCap).
 int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log, int logCap) {
   return guarded([&] {
     fundem::LSAutotuner t(*static_cast<BoundSolver*>(s));
@@ -172,6 +172,7 @@ int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log
       out7[4] = c.launch.lawBlock;
       out7[5] = c.launch.reduceBlock;
       out7[6] = c.msPerStep;
+      out7[7] = c.launch.lawMode;
     }
     if (log && logCap > 0) {
       std::string all;
@@ -180,6 +181,27 @@ int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log
       log[logCap - 1] = 0;
     }
   });
+}
+
+// Launch shapes of one solver (zero fields: the process default).
+int fdmx_set_launch_tuning(void* s, const int* t5) {
+  return guarded([&] {
+    fdm_tuning t{t5[0], t5[1], t5[2], t5[3], t5[4]};
+    static_cast<BoundSolver*>(s)->setLaunchTuning(t);
+  });
+}
+int fdmx_get_launch_tuning(void* s, int* t5) {
+  return guarded([&] {
+    const fdm_tuning& t = static_cast<BoundSolver*>(s)->launchTuning();
+    t5[0] = t.sweepWaves;
+    t5[1] = t.sweepWavesPerCU;
+    t5[2] = t.lawBlock;
+    t5[3] = t.reduceBlock;
+    t5[4] = t.lawMode;
+  });
+}
+int fdmx_set_lookahead(void* s, int steps) {
+  return guarded([&] { static_cast<BoundSolver*>(s)->setLookahead(steps); });
 }
 
 // Adjoint (fundem::LSAdjoint) of a solver; the solver must outlive it.
@@ -379,7 +401,7 @@ int fdmx_status(void* s, unsigned* out) {
 int fdmx_get_state(void* s, double* x, double* v, double* w, double* q, double* f, double* t) {
   return guarded([&] {
     auto* S = static_cast<BoundSolver*>(s);
-    S->download();
+    S->downloadState();
     auto& P = S->getLSParticle();
     const size_t n = P.num();
     for (size_t i = 0; i < n; ++i) {

@@ -501,7 +501,10 @@ struct HitSlots {
 // contact law with one hit per lane and the pair sums folded in node order.
 // A pair with more than kMaxHits contacts recomputes its hits chunk by chunk
 // instead of staging them.
-template <int SET>
+// CULL: first take the particle-level pre-cull test of sweepPair (sphere,
+// SDF box, Lipschitz bound at this step's positions) and leave empty record
+// ranges when it rejects the pair.
+template <int SET, bool CULL = false>
 __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, int lane,
                            HitSlots& hs, int pool) {
   const int A = S.pairs.pairA[p];
@@ -522,6 +525,24 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   Target tg;
   V3 armB;
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
+  if (CULL) {
+    const double RA = tA.radius * kReach;
+    const double reachT = SET == FDM_SET_PARTICLE ? tg.t->radius : tg.t->boxReach;
+    const V3 d = vsub(xA, tg.x);
+    const double rs = RA + reachT * kReach;
+    bool keep = vlen2(d) <= rs * rs;
+    if (keep) {
+      const V3 local = qrotateT(d, tg.q);
+      keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
+    }
+    if (!keep) {
+      if (lane == 0) {
+        S.pairs.recOff[p] = 0;
+        S.pairs.recCnt[p] = 0;
+      }
+      return;
+    }
+  }
   const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
   const double RB = tg.t->radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
@@ -917,7 +938,8 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     base = reservePooled(S.pairs.recTotal, S.pairs.recCapacity, pool, total, lane);
     gbase = reservePooled(X.hitCount, X.hitCapacity, pool, total, lane);
     if (base < 0 || gbase < 0) {
-      if (lane == 0) atomicOr(C.status, FDM_STATUS_RECORD_OVERFLOW);
+      if (lane == 0)
+        atomicOr(C.status, base < 0 ? FDM_STATUS_RECORD_OVERFLOW : FDM_STATUS_STAGING_OVERFLOW);
       total = 0;
       base = gbase = 0;
     }
@@ -1108,6 +1130,38 @@ __global__ void __launch_bounds__(64 * WAVES) kCullSweep(NarrowMulti a, NarrowSc
   }
 }
 
+// The pre-cull, the sweep and the contact law in one pass: one wave per
+// candidate pair (WAVES waves per workgroup) as kCullSweep, whose wave then
+// applies the law to its pair's contacts (one contact per lane) and folds
+// the pair sums in node order (processPair); no staging, no second kernel.
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) kCullSweepLaw(NarrowMulti a) {
+  if (gateOff(a.gate)) return;
+  __shared__ HitSlots slots[WAVES];
+  const int lane = laneId();
+  const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const SetStarts st = setStarts(a);
+  const int total = st.v[kMaxSets];
+  for (int g = blockIdx.x * WAVES + wave; g < total; g += gridDim.x * WAVES) {
+    int k = 0, base = 0;
+#pragma unroll
+    for (int q = 1; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
+      if (g >= st.v[q]) {
+        k = q;
+        base = st.v[q];
+      }
+    const NarrowSet& s = a.s[k];
+    const int p = g - base;
+    const int pool = g % FDM_REC_POOLS;
+    if (s.kind == FDM_SET_PARTICLE)
+      processPair<FDM_SET_PARTICLE, true>(a.c, s, p, lane, slots[wave], pool);
+    else if (s.kind == FDM_SET_WALL)
+      processPair<FDM_SET_WALL, true>(a.c, s, p, lane, slots[wave], pool);
+    else
+      processPair<FDM_SET_GHOST, true>(a.c, s, p, lane, slots[wave], pool);
+  }
+}
+
 // The contact law over the candidate index space of kCullSweep (one thread
 // per candidate; culled pairs and pairs without contacts return at once).
 __global__ void __launch_bounds__(256) kContactLawAll(NarrowMulti a, NarrowScratch x) {
@@ -1138,10 +1192,28 @@ __global__ void kResetCounter(int* c) {
 
 namespace fdm {
 namespace {
-fdm_tuning g_tuning = {1, 32, 256, 256};
+fdm_tuning g_tuning = {1, 32, 256, 256, 1};
 }
 const fdm_tuning& tuning() { return g_tuning; }
 }  // namespace fdm
+
+// The shapes t asks for, zero fields taken from the process default; false
+// when a field is outside the documented shapes.
+static bool completeTuning(const fdm_tuning& t, fdm_tuning& n) {
+  n = fdm::tuning();
+  if (t.sweepWaves) n.sweepWaves = t.sweepWaves;
+  if (t.sweepWavesPerCU) n.sweepWavesPerCU = t.sweepWavesPerCU;
+  if (t.lawBlock) n.lawBlock = t.lawBlock;
+  if (t.reduceBlock) n.reduceBlock = t.reduceBlock;
+  if (t.lawMode) n.lawMode = t.lawMode;
+  if (!(n.sweepWaves == 1 || n.sweepWaves == 2 || n.sweepWaves == 4)) return false;
+  if (n.sweepWavesPerCU < 4 || n.sweepWavesPerCU > 128) return false;
+  if (!(n.lawBlock == 64 || n.lawBlock == 128 || n.lawBlock == 256)) return false;
+  if (n.reduceBlock < 64 || n.reduceBlock > 1024 || n.reduceBlock % 64) return false;
+  if (!(n.lawMode == 1 || n.lawMode == 2)) return false;
+  return true;
+}
+
 
 using namespace fdm;
 
@@ -1332,8 +1404,18 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   x.hitGeom = ws->hitGeom;
   x.hitCapacity = ws->hitCapacity;
   x.hitCount = ws->hitCount;
-  const fdm_tuning& tu = tuning();
+  fdm_tuning tu = tuning();
+  if (ws->tuning && !completeTuning(*ws->tuning, tu)) return FDM_EINVAL;
   const int sweepBlocks = std::max(1, persistentBlocks(tu.sweepWaves) * tu.sweepWavesPerCU / 32);
+  if (tu.lawMode == 2) {
+    if (tu.sweepWaves == 4)
+      hipLaunchKernelGGL(kCullSweepLaw<4>, dim3(sweepBlocks), dim3(256), 0, stream, k);
+    else if (tu.sweepWaves == 2)
+      hipLaunchKernelGGL(kCullSweepLaw<2>, dim3(sweepBlocks), dim3(128), 0, stream, k);
+    else
+      hipLaunchKernelGGL(kCullSweepLaw<1>, dim3(sweepBlocks), dim3(64), 0, stream, k);
+    return launchStatus();
+  }
   if (tu.sweepWaves == 4)
     hipLaunchKernelGGL(kCullSweep<4>, dim3(sweepBlocks), dim3(256), 0, stream, k, x);
   else if (tu.sweepWaves == 2)
@@ -1347,16 +1429,17 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
 
 extern "C" int fdm_setTuning(const fdm_tuning* t) {
   if (!t) return FDM_EINVAL;
-  fdm_tuning n = g_tuning;
-  if (t->sweepWaves) n.sweepWaves = t->sweepWaves;
-  if (t->sweepWavesPerCU) n.sweepWavesPerCU = t->sweepWavesPerCU;
-  if (t->lawBlock) n.lawBlock = t->lawBlock;
-  if (t->reduceBlock) n.reduceBlock = t->reduceBlock;
-  if (!(n.sweepWaves == 1 || n.sweepWaves == 2 || n.sweepWaves == 4)) return FDM_EINVAL;
-  if (n.sweepWavesPerCU < 4 || n.sweepWavesPerCU > 128) return FDM_EINVAL;
-  if (!(n.lawBlock == 64 || n.lawBlock == 128 || n.lawBlock == 256)) return FDM_EINVAL;
-  if (n.reduceBlock < 64 || n.reduceBlock > 1024 || n.reduceBlock % 64) return FDM_EINVAL;
+  fdm_tuning n;
+  if (!completeTuning(*t, n)) return FDM_EINVAL;
   g_tuning = n;
+  return FDM_OK;
+}
+
+extern "C" int fdm_checkTuning(fdm_tuning* t) {
+  if (!t) return FDM_EINVAL;
+  fdm_tuning n;
+  if (!completeTuning(*t, n)) return FDM_EINVAL;
+  *t = n;
   return FDM_OK;
 }
 

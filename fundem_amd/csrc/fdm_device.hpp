@@ -294,6 +294,32 @@ __device__ __forceinline__ V3 wrapPosition(V3 x, const fdm_grid& g, const fdm_pe
 // Sort key of a body of one instance among several: 1 + global cell, or 0
 // when the body is outside its instance's grid (ids rounded up past the last
 // cell are kept out too, as the table never holds them).
+// Periodic images a particle at x has (its visibility bits: set X 1, Y 2,
+// XY 4, or the sector's R90, R180, R270 images 1, 2, 4): an image exists
+// when its ghost cell id is valid (mySpatialGrid.cu:87-240). `sets` names the
+// sets that exist (any instance periodic on the axes), `per` the particle's
+// own instance's periodic axes. Bit 8 tells that the particle wraps.
+__device__ __forceinline__ int visibilityBits(V3 x, const fdm_grid& g, const fdm_periodic& sets,
+                                              const fdm_periodic& per) {
+  int bits = 0;
+  if (sets.sector) {
+    const SectorWrap sw = sectorWrap(x, g, sets);
+    for (int k = 0; k < 3; ++k) {
+      const V3 gp = sectorImageOn(sw.x, g, k) ? sectorImagePos(sw.x, g, sets, k) : sw.x;
+      bits |= (ghostCellHash(gp, g) >= 0 ? 1 : 0) << k;
+    }
+    if (sw.turns) bits |= 8;
+  } else if (sets.x || sets.y) {
+    const WrapStages w = wrapStages(x, g, per);
+    if (sets.x && per.x && ghostCellHash(ghostOf(w.afterX, g, 1), g) >= 0) bits |= 1;
+    if (sets.y && per.y && ghostCellHash(ghostOf(w.afterY, g, 2), g) >= 0) bits |= 2;
+    if (sets.x && sets.y && per.x && per.y && ghostCellHash(ghostOf(w.afterY, g, 3), g) >= 0)
+      bits |= 4;
+    if (!(w.afterY.x == x.x && w.afterY.y == x.y && w.afterY.z == x.z)) bits |= 8;
+  }
+  return bits;
+}
+
 __device__ __forceinline__ unsigned instanceKey(int h, const fdm_grid& g, long long cellOffset) {
   return (h < 0 || (long long)h >= g.num) ? 0u : (unsigned)(cellOffset + h + 1);
 }

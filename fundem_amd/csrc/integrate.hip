@@ -97,19 +97,31 @@ __global__ void __launch_bounds__(256) kKickDriftHash(fdm_bodies b, const fdm_te
   }
 }
 
-// Records each body's grid state after a list build.
-__global__ void kCommitState(const double* x, int n, fdm_grid grid, int* atBuild, const int* gate) {
-  if (gateOff(gate)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    atBuild[i] = bodyState(ldp(x, n, i), grid);
-}
-
-__global__ void kCommitStateInstances(const double* x, int n, fdm_instances I, int isWall,
-                                      int* atBuild, const int* gate) {
+// A list build's start: each body's grid state, and the unsorted (key,
+// index) lists the build sorts (key hash + 1, 0 outside the grid).
+__global__ void kCommitState(const double* x, int n, fdm_grid grid, int* atBuild,
+                             unsigned* sortKey, int* sortIndex, const int* gate) {
   if (gateOff(gate)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const fdm_grid& g = isWall ? I.wgrid[I.wallInstance[i]] : I.pgrid[I.particleInstance[i]];
-    atBuild[i] = bodyState(ldp(x, n, i), g);
+    const V3 p = ldp(x, n, i);
+    atBuild[i] = bodyState(p, grid);
+    sortKey[i] = (unsigned)(cellHash(p, grid) + 1);
+    sortIndex[i] = i;
+  }
+}
+
+// The same for bodies of several systems: keys in the global cell space.
+__global__ void kCommitStateInstances(const double* x, int n, fdm_instances I, int isWall,
+                                      int* atBuild, unsigned* sortKey, int* sortIndex,
+                                      const int* gate) {
+  if (gateOff(gate)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int in = isWall ? I.wallInstance[i] : I.particleInstance[i];
+    const fdm_grid& g = isWall ? I.wgrid[in] : I.pgrid[in];
+    const V3 p = ldp(x, n, i);
+    atBuild[i] = bodyState(p, g);
+    sortKey[i] = instanceKey(cellHash(p, g), g, isWall ? I.wcellOffset[in] : I.pcellOffset[in]);
+    sortIndex[i] = i;
   }
 }
 
@@ -155,7 +167,8 @@ __device__ void decideStep(const fdm_skin& s, float rateIn, int flagsIn) {
   const int seq = w[FDM_CTL_SEQ] + 1;
   c[FDM_CTL_SEQ] = seq;
   if (s.hostDecision)
-    *(volatile int*)s.hostDecision = (int)(((unsigned)seq << 2) | (rebuild ? 2u : 0u) | (ghost ? 1u : 0u));
+    *(volatile int*)(s.hostDecision + (seq & (FDM_DECISION_RING - 1))) =
+        (int)(((unsigned)seq << 2) | (rebuild ? 2u : 0u) | (ghost ? 1u : 0u));
 }
 
 __global__ void kSkinDecide(fdm_skin s) {
@@ -191,11 +204,7 @@ __device__ __forceinline__ void stepBeginBody(const fdm_step_begin_args& a, int 
   V3 x;
   if (a.drift) x = driftOne(b, j, a.dt);
   else x = ldp(b.x, b.n, j);
-  const int h = cellHash(x, grid);
-  t.hashValue[j] = h;
-  t.sortKey[j] = !many ? (unsigned)(h + 1)
-                       : instanceKey(h, grid, WALL ? a.inst.wcellOffset[in] : a.inst.pcellOffset[in]);
-  t.sortIndex[j] = j;
+  t.hashValue[j] = cellHash(x, grid);
   if (a.drift) {
     const fdm_template& tp = tmpl[b.geo[j]];
     const double rho = fmax(tp.radius, tp.boxReach);
@@ -204,45 +213,20 @@ __device__ __forceinline__ void stepBeginBody(const fdm_step_begin_args& a, int 
   }
   const int st = bodyState(x, grid);
   if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
-  if (!WALL && a.periodic.sector) {
-    // the sector's R90, R180, R270 images of the wrapped particle
-    const SectorWrap sw = sectorWrap(x, grid, a.periodic);
-    int bits = 0;
-    for (int k = 0; k < 3; ++k) {
-      const V3 gp = sectorImageOn(sw.x, grid, k) ? sectorImagePos(sw.x, grid, a.periodic, k) : sw.x;
-      const int f = ghostCellHash(gp, grid) >= 0 ? 1 : 0;
-      a.ghostFlag[k][j] = f;
-      bits |= f << k;
-    }
-    if (bits != a.ghostAtBuild[j]) flags |= 2;
-    if (sw.turns) flags |= 4;
-  } else if (!WALL && (a.periodic.x || a.periodic.y)) {
+  if (!WALL && anyPeriodic(a.periodic)) {
     // sets exist where any instance is periodic; an instance contributes
     // only to the sets of its own periodic axes
-    const WrapStages w = wrapStages(x, grid, perI);
-    int bits = 0;
-    if (a.periodic.x) {
-      const int f = perI.x && ghostCellHash(ghostOf(w.afterX, grid, 1), grid) >= 0 ? 1 : 0;
-      a.ghostFlag[0][j] = f;
-      bits |= f;
-    }
-    if (a.periodic.y) {
-      const int f = perI.y && ghostCellHash(ghostOf(w.afterY, grid, 2), grid) >= 0 ? 1 : 0;
-      a.ghostFlag[1][j] = f;
-      bits |= f << 1;
-    }
-    if (a.periodic.x && a.periodic.y) {
-      const int f =
-          perI.x && perI.y && ghostCellHash(ghostOf(w.afterY, grid, 3), grid) >= 0 ? 1 : 0;
-      a.ghostFlag[2][j] = f;
-      bits |= f << 2;
-    }
+    const int vb = visibilityBits(x, grid, a.periodic, perI);
+    const int bits = vb & 7;
+    for (int k = 0; k < 3; ++k)
+      if (a.ghostFlag[k]) a.ghostFlag[k][j] = (bits >> k) & 1;
     if (bits != a.ghostAtBuild[j]) flags |= 2;
-    if (!(w.afterY.x == x.x && w.afterY.y == x.y && w.afterY.z == x.z)) flags |= 4;
+    if (vb & 8) flags |= 4;
   }
 }
 
 __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
+  if (gateOff(k.a.gate)) return;
   const fdm_step_begin_args& a = k.a;
   __shared__ float waveMax[4];
   const int nP = a.particles.n, nW = a.walls.n;
@@ -268,6 +252,7 @@ __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
 // Clears the given counter arrays, folds the workgroup slots of kStepBegin
 // into the control words and takes the list decision (one workgroup).
 __global__ void __launch_bounds__(1024) kStepDecide(StepBeginK k, int nblocks) {
+  if (gateOff(k.a.gate)) return;
   __shared__ float rmax[16];
   __shared__ int fl[16];
   for (int q = 0; q < 8; ++q)
@@ -316,12 +301,25 @@ __global__ void kCarryPairs(fdm_pairs from, fdm_pairs to, int n, const int* gate
 }
 
 // addGlobalDampingForceTorque (globalDamping.cu:4-16) on FP32 forces
+__device__ __forceinline__ void dampOne(float* F, float* T, const float* v, const float* w, int n,
+                                        int i, double c) {
+  const V3 f = ld3(F, n, i), t = ld3(T, n, i);
+  st3(F, n, i, vsub(f, vmul(vnormalize(ld3(v, n, i)), c * vlen(f))));
+  st3(T, n, i, vsub(t, vmul(vnormalize(ld3(w, n, i)), c * vlen(t))));
+}
+
 __global__ void kGlobalDamping(fdm_bodies b, double c) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
-  const V3 f = ld3(b.F, b.n, i), t = ld3(b.T, b.n, i);
-  st3(b.F, b.n, i, vsub(f, vmul(vnormalize(ld3(b.v, b.n, i)), c * vlen(f))));
-  st3(b.T, b.n, i, vsub(t, vmul(vnormalize(ld3(b.w, b.n, i)), c * vlen(t))));
+  dampOne(b.F, b.T, b.v, b.w, b.n, i, c);
+}
+
+// The same on the caller's launch shape (grid-stride over the bodies).
+__global__ void kGlobalDampingArrays(float* F, float* T, const float* v, const float* w, int n,
+                                     double c) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dampOne(F, T, v, w, n, i, c);
 }
 
 // addGlobalConstantForceTorque (globalDamping.cu:38-49): both vectors go to
@@ -343,6 +341,20 @@ extern "C" int fdm_launchAddGlobalDampingForceTorque(fdm_bodies particles, doubl
   if (particles.n == 0) return FDM_OK;
   hipLaunchKernelGGL(kGlobalDamping, dim3(blocksFor(particles.n, 256)), dim3(256), 0, stream,
                      particles, c);
+  return launchStatus();
+}
+
+extern "C" int fdm_launchAddGlobalDampingForceTorqueArrays(float* force, float* torque,
+                                                           const float* velocity,
+                                                           const float* angularVelocity,
+                                                           double c, size_t num, size_t gridD,
+                                                           size_t blockD, hipStream_t stream) {
+  if (num == 0) return FDM_OK;
+  if (num > (size_t)INT32_MAX || !force || !torque || !velocity || !angularVelocity)
+    return FDM_EINVAL;
+  if (gridD == 0 || blockD == 0 || blockD > 1024 || gridD > (size_t)INT32_MAX) return FDM_EINVAL;
+  hipLaunchKernelGGL(kGlobalDampingArrays, dim3((unsigned)gridD), dim3((unsigned)blockD), 0, stream,
+                     force, torque, velocity, angularVelocity, (int)num, c);
   return launchStatus();
 }
 
@@ -405,8 +417,6 @@ int stepBeginSetup(const fdm_step_begin_args* args, const fdm_skin* skin, void* 
   if (anyPeriodic(a.periodic)) {
     if (!a.ghostAtBuild) return FDM_EINVAL;
     if (a.periodic.sector && (a.periodic.x || a.periodic.y || a.inst.count > 0)) return FDM_EINVAL;
-    for (int k = 0; k < 3; ++k)
-      if (ghostSetOn(a.periodic, k) && !a.ghostFlag[k]) return FDM_EINVAL;
   }
   const long long total = (long long)a.particles.n + a.walls.n;
   if (workspaceBytes < fdm_stepBeginWorkspaceBytes((size_t)total)) return FDM_EINVAL;
@@ -449,22 +459,27 @@ extern "C" int fdm_launchSkinDecide(const fdm_skin* skin, hipStream_t stream) {
 
 extern "C" int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
                                          const fdm_skin* skin, hipStream_t stream) {
-  if (bodies.n < 0 || !skin || !table.validAtBuild) return FDM_EINVAL;
+  if (bodies.n < 0 || !skin || !table.validAtBuild || !table.sortKey || !table.sortIndex)
+    return FDM_EINVAL;
   if (bodies.n == 0) return FDM_OK;
   const unsigned nb = std::min<unsigned>(blocksFor(bodies.n, 256), kIntegrateBlocks);
   hipLaunchKernelGGL(kCommitState, dim3(nb), dim3(256), 0, stream, bodies.x, bodies.n, grid,
-                     table.validAtBuild, skin->ctl + FDM_CTL_REBUILD);
+                     table.validAtBuild, table.sortKey, table.sortIndex,
+                     skin->ctl + FDM_CTL_REBUILD);
   return launchStatus();
 }
 
 extern "C" int fdm_launchCommitBodyStateInstances(fdm_bodies bodies, const fdm_instances* inst,
                                                   int isWall, fdm_hash_table table,
                                                   const fdm_skin* skin, hipStream_t stream) {
-  if (bodies.n < 0 || !inst || inst->count <= 0 || !skin || !table.validAtBuild) return FDM_EINVAL;
+  if (bodies.n < 0 || !inst || inst->count <= 0 || !skin || !table.validAtBuild ||
+      !table.sortKey || !table.sortIndex)
+    return FDM_EINVAL;
   if (bodies.n == 0) return FDM_OK;
   const unsigned nb = std::min<unsigned>(blocksFor(bodies.n, 256), kIntegrateBlocks);
   hipLaunchKernelGGL(kCommitStateInstances, dim3(nb), dim3(256), 0, stream, bodies.x, bodies.n,
-                     *inst, isWall, table.validAtBuild, skin->ctl + FDM_CTL_REBUILD);
+                     *inst, isWall, table.validAtBuild, table.sortKey, table.sortIndex,
+                     skin->ctl + FDM_CTL_REBUILD);
   return launchStatus();
 }
 

@@ -52,6 +52,23 @@ __global__ void kGhostPosition(double* gpos, const double* pos, int n, fdm_grid 
 
 __device__ __forceinline__ bool sameV3(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 
+// Visibility flags of every particle's images in the active sets (null
+// pointers: sets that do not exist), per particle with its own instance's
+// grid and periodic axes.
+__global__ void kGhostVisibility(const double* pos, int n, fdm_grid grid, fdm_periodic sets,
+                                 fdm_instances I, int* f0, int* f1, int* f2, const int* gate) {
+  if (gateOff(gate)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool many = I.count > 0;
+  const int in = many ? I.particleInstance[i] : 0;
+  const int bits = visibilityBits(ldp(pos, n, i), many ? I.pgrid[in] : grid, sets,
+                                  many ? I.periodic[in] : sets);
+  if (f0) f0[i] = bits & 1;
+  if (f1) f1[i] = (bits >> 1) & 1;
+  if (f2) f2[i] = (bits >> 2) & 1;
+}
+
 // Pass 1: visibility flag of each particle's ghost in each active set. With
 // list reuse (ctl != null) it flags a wrap and any visibility change since
 // the last ghost list build.
@@ -411,6 +428,9 @@ int buildGhosts(fdm_bodies particles, fdm_grid grid, fdm_periodic periodic, fdm_
   const int n = particles.n;
   hipLaunchKernelGGL(kResetBox, dim3(1), dim3(64), 0, stream, on[0] ? sets[0].cellBox : nullptr,
                      on[1] ? sets[1].cellBox : nullptr, on[2] ? sets[2].cellBox : nullptr, gate);
+  hipLaunchKernelGGL(kGhostVisibility, dim3(blocksFor(n, 256)), dim3(256), 0, stream, particles.x,
+                     n, grid, periodic, I, on[0] ? sets[0].flag : nullptr,
+                     on[1] ? sets[1].flag : nullptr, on[2] ? sets[2].flag : nullptr, gate);
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
     int r = scanInts(sets[s].flag, sets[s].flag, (size_t)n, false, sets[s].count, workspace,

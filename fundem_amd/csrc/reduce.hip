@@ -250,7 +250,12 @@ extern "C" int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStrea
     if (g.stepCounter) hipLaunchKernelGGL(kIncrement, dim3(1), dim3(64), 0, stream, g.stepCounter);
     return launchStatus();
   }
-  const int rb = tuning().reduceBlock;
+  fdm_tuning tu = tuning();
+  if (g.tuning) {
+    tu = *g.tuning;
+    if (fdm_checkTuning(&tu) != FDM_OK) return FDM_EINVAL;
+  }
+  const int rb = tu.reduceBlock;
   hipLaunchKernelGGL(kReduce, dim3(blocksFor(g.particles.n, rb)), dim3(rb), 0, stream, k);
   return launchStatus();
 }

@@ -7,9 +7,10 @@
 //     (only for one non-periodic system without bonds: there every candidate
 //     is at least the largest radius, so the 27-cell stencil finds every
 //     particle within reach and the contact set is the same);
-//   - the launch shapes of fdm_tuning: node-sweep waves (kept pairs in
-//     flight) per workgroup and sweep grid size, contact-law and reduction
-//     workgroup sizes.
+//   - the solver's launch shapes (fdm_tuning): node-sweep waves (kept pairs
+//     in flight) per workgroup and sweep grid size, where the contact law
+//     runs (a kernel of its own, or the sweep's wave), contact-law and
+//     reduction workgroup sizes.
 // Each candidate is timed from the same saved point (LSAdjoint::mark /
 // rollback) one knob at a time, keeping the best value of each knob before
 // the next; the solver is left at the saved point with the best settings.
@@ -46,7 +47,7 @@ class LSAutotuner {
     trialSteps = std::max<size_t>(trialSteps, 4);
     snap_.mark();
     TuneChoice best;
-    fdm_getTuning(&best.launch);
+    best.launch = s_.launchTuning();
     best.skin = s_.skin_;
     best.cellSize = s_.cellSize_;
     // the skin trades list builds for longer lists: time it over a window
@@ -86,6 +87,13 @@ class LSAutotuner {
       }
     bestCost = pick(group, trialSteps, best, "sweep");
     group.clear();
+    for (int m : {1, 2}) {
+      TuneChoice c = best;
+      c.launch.lawMode = m;
+      group.push_back(c);
+    }
+    bestCost = pick(group, trialSteps, best, "lawMode");
+    group.clear();
     for (int b : {64, 128, 256}) {
       TuneChoice c = best;
       c.launch.lawBlock = b;
@@ -108,8 +116,7 @@ class LSAutotuner {
 
  private:
   void apply(const TuneChoice& c) {
-    fdmCheck(fdm_setTuning(&c.launch), "tuning");
-    s_.destroyGraphs();
+    s_.setLaunchTuning(c.launch);
     s_.skinRequest_ = c.skin;
     s_.skin_ = c.skin;
     if (c.cellSize != s_.cellSize_) {
@@ -171,10 +178,11 @@ class LSAutotuner {
       total = (build + rest * double(steps - 1)) / double(steps);
     }
     std::snprintf(line, sizeof line,
-                  "%-12s skin %.3e cell %.3e sweep %d/%d law %d reduce %d: first %.3f ms, then %.3f "
-                  "ms/step, %ld builds, displacement bound %.2e -> %.4f ms/step",
+                  "%-12s skin %.3e cell %.3e sweep %d/%d law %d/%d reduce %d: first %.3f ms, then "
+                  "%.3f ms/step, %ld builds, displacement bound %.2e -> %.4f ms/step",
                   knob, c.skin, c.cellSize, c.launch.sweepWaves, c.launch.sweepWavesPerCU,
-                  c.launch.lawBlock, c.launch.reduceBlock, build, rest, (long)(b1 - b0), disp, total);
+                  c.launch.lawMode, c.launch.lawBlock, c.launch.reduceBlock, build, rest,
+                  (long)(b1 - b0), disp, total);
     log_.push_back(line);
     return total;
   }

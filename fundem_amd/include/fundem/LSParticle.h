@@ -413,6 +413,20 @@ struct LSParticle {
 
   size_t num() const { return position_.size(); }
   size_t num_device() const { return x_.size() ? num() : 0; }
+  // Launch shape of a kernel with one thread per body, for user hooks
+  // (LSParticle::gridDim/blockDim, LSParticle.h:765-766), set at
+  // initialisation by the rule of LSParticle.h:379-381: blockDim is
+  // maxGPUThread, or the body count when that is smaller; gridDim covers
+  // every body. Both stay 1 without bodies.
+  const size_t& gridDim() const { return gridDim_; }
+  const size_t& blockDim() const { return blockDim_; }
+  void setLaunchShape(size_t maxGPUThread) {
+    const size_t n = num();
+    if (n == 0) return;
+    if (maxGPUThread > 0) blockDim_ = maxGPUThread;
+    if (n < maxGPUThread) blockDim_ = n;
+    gridDim_ = (n + blockDim_ - 1) / blockDim_;
+  }
   size_t numGeometry() const { return templates_.size(); }
   size_t numMaterial() const { return materials_.size(); }
   const fdm_template& templateInfo(size_t g) const { return templates_[g]; }
@@ -474,6 +488,7 @@ struct LSParticle {
   DeviceBuffer<float> dsdf_, dnodes_, dchunks_;
   fdm_grid grid_{};
   HashTableBuffers table_;
+  size_t gridDim_ = 1, blockDim_ = 1;
 };
 
 }  // namespace fundem

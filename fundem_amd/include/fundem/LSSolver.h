@@ -40,7 +40,9 @@ class LSSolver {
  public:
   explicit LSSolver(const std::string dir = "Problem", hipStream_t stream = 0,
                     const size_t maxGPUThread = 256, const int device = 0)
-      : dir_(dir), stream_(stream), maxGPUThread_(maxGPUThread), device_(device) {}
+      : dir_(dir), stream_(stream), maxGPUThread_(maxGPUThread), device_(device) {
+    fdm_getTuning(&tuning_);
+  }
   virtual ~LSSolver() {
     destroyGraphs();
     if (hostDecision_) (void)hipHostFree(hostDecision_);
@@ -179,6 +181,16 @@ class LSSolver {
     recordsPerParticle_ = std::max(recordsPerParticle, 1);
   }
   void useGraph(bool on) { useGraph_ = on; }
+  // Launch shapes of this solver's narrow phase and reduction (results do
+  // not depend on them); zero fields take the process default. Starts at
+  // the process default of the time of construction.
+  void setLaunchTuning(const fdm_tuning& t) {
+    fdm_tuning n = t;
+    fdmCheck(fdm_checkTuning(&n), "launch tuning");
+    tuning_ = n;
+    destroyGraphs();
+  }
+  const fdm_tuning& launchTuning() const { return tuning_; }
   void setStageTiming(bool on) { timing_ = on; }
   // Extra reach of the candidate lists, which are then rebuilt only when a
   // body may have moved by half of it (decided on the device each step).
@@ -266,6 +278,10 @@ class LSSolver {
     } else {
       initializeSystems();
     }
+    particles_.setLaunchShape(maxGPUThread_);
+    walls_.setLaunchShape(maxGPUThread_);
+    pp_.setLaunchShape(maxGPUThread_);
+    pw_.setLaunchShape(maxGPUThread_);
     double maxR = 0.0;
     for (size_t g = 0; g < particles_.numGeometry(); ++g)
       maxR = std::max(maxR, particles_.templateInfo(g).radius);
@@ -326,10 +342,16 @@ class LSSolver {
         if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
       workHit_.allocate((size_t)cap, false);
-      // staged contacts of one step, all sets together
+      // staged contacts of one step, all sets together: room for every
+      // active set's records
+      int activeSets = 1 + (W ? 1 : 0);
+      for (int s = 0; s < 3; ++s)
+        if (periodic_.active(s)) activeSets += 1;
+      const long long hitCap = (long long)recCap * activeSets;
+      if (hitCap >= (1ll << 31)) throw std::invalid_argument("contact record capacity too large");
       hitCount_.allocate(FDM_REC_COUNTERS);
-      hitGeom_.allocate(8 * (size_t)recCap, false);
-      hitCapacity_ = recCap;
+      hitGeom_.allocate(8 * (size_t)hitCap, false);
+      hitCapacity_ = (int)hitCap;
     }
     status_.allocate(1);
     stepCounter_.allocate(1);
@@ -340,14 +362,14 @@ class LSSolver {
     workspace_.allocate(ws / sizeof(double) + 1, false);
     destroyGraphs();
     if (!hostDecision_) {
-      hipCheck(hipHostMalloc((void**)&hostDecision_, 4 * sizeof(int),
+      hipCheck(hipHostMalloc((void**)&hostDecision_, FDM_DECISION_RING * sizeof(int),
                              hipHostMallocMapped | hipHostMallocCoherent),
                "hipHostMalloc");
       hipCheck(hipHostGetDevicePointer((void**)&hostDecisionDev_, hostDecision_, 0),
                "hipHostGetDevicePointer");
     }
     hipCheck(hipStreamSynchronize(stream_), "initialize");
-    for (int k = 0; k < 4; ++k) hostDecision_[k] = 0;
+    for (int k = 0; k < FDM_DECISION_RING; ++k) hostDecision_[k] = 0;
     decisionSeq_ = 0;
     hookMode_ = HookUnknown;
     stepOnce(true, 0.0);
@@ -357,18 +379,50 @@ class LSSolver {
 
   // Advance nSteps loop iterations without output (the loop body of
   // compute*, LSSolver.h:1101-1110 and variants).
+  // Throws if a capacity overflowed during these steps (the device status
+  // word is read once, at the end).
   void advance(size_t nSteps, double time = 0.0) {
+    if (nSteps == 0) return;
+    // the first step after initialize runs alone (it finds out whether
+    // addExternalForceTorque acts); hooks and sector images need the
+    // host at every step; stage timing keeps one step in flight
     size_t done = 0;
-    if (useGraph_ && hookMode_ == HookNone && !timing_) {
-      for (; done < nSteps; ++done) graphStep();
-    }
-    for (; done < nSteps; ++done) {
+    if (hookMode_ == HookUnknown) {
       stepOnce(false, time);
       time += dt_;
+      done = 1;
     }
+    if (done < nSteps && hookMode_ == HookNone && !periodic_.isSector() && !timing_ &&
+        lookahead_ > 0) {
+      advanceAhead(nSteps - done, time);
+    } else {
+      const bool graphs = useGraph_ && hookMode_ == HookNone && !timing_;
+      for (; done < nSteps; ++done) {
+        if (graphs) graphStep();
+        else stepOnce(false, time);
+        time += dt_;
+      }
+    }
+    checkStatus();
+  }
+  // Steps queued ahead of the list decisions the host has read (advance);
+  // 0 waits for every step's decision before queueing the next. At most
+  // FDM_DECISION_RING - 1.
+  void setLookahead(int steps) {
+    lookahead_ = std::max(0, std::min(steps, FDM_DECISION_RING - 1));
   }
 
+  // LSSolver's finalize of the containers (copyDeviceToHost): the state,
+  // the bonds and the host copies of the particle and wall contacts.
   void download() {
+    downloadState();
+    if (particles_.num_device() > 0) {
+      fillHostContacts(FDM_SET_PARTICLE, pp_);
+      fillHostContacts(FDM_SET_WALL, pw_);
+    }
+  }
+  // Bodies and bonds only.
+  void downloadState() {
     particles_.finalize(stream_);
     walls_.finalize(stream_);
     bonds_.finalize(stream_);
@@ -428,8 +482,31 @@ class LSSolver {
   // contact (SolidInteraction contactPoint / contactNormal / overlap).
   // Synchronises.
   std::vector<ContactGeometry> contactGeometry(int set) {
-    download();
+    downloadState();
     return contactGeometryHost(set);
+  }
+  // The host copies of a set's contacts (SolidInteraction's *HostRef
+  // getters) from the state last downloaded.
+  void fillHostContacts(int set, SolidInteraction& si) {
+    const std::vector<ContactGeometry> cg = contactGeometryHost(set);
+    std::vector<long long> nodeBase(particles_.num() + 1, 0);
+    const auto& geo = particles_.geometryIDHostRef();
+    for (size_t i = 0; i < particles_.num(); ++i)
+      nodeBase[i + 1] = nodeBase[i] + particles_.templateInfo(geo[i]).nodeCount;
+    std::vector<double3> pt, nr;
+    std::vector<int> master, slave;
+    pt.reserve(cg.size());
+    nr.reserve(cg.size());
+    master.reserve(cg.size());
+    slave.reserve(cg.size());
+    for (const ContactGeometry& c : cg) {
+      pt.push_back(c.point);
+      nr.push_back(c.normal);
+      master.push_back((int)(nodeBase[c.A] + c.node));
+      slave.push_back(c.target);
+    }
+    si.setHostContacts(std::move(pt), std::move(nr), std::move(master), std::move(slave),
+                       maxGPUThread_);
   }
   // The same from the state last downloaded.
   std::vector<ContactGeometry> contactGeometryHost(int set) {
@@ -511,7 +588,7 @@ class LSSolver {
   // contacts of periodic sets are not written, as in the reference.
   // Synchronises.
   void writeVTU(const std::string& dir, size_t iFrame, size_t iStep, double time) {
-    download();
+    downloadState();
     vtu::makeDirs(dir);
     const std::string dp = dir + "/LSParticle", di = dir + "/LSParticleInteraction",
                       dw = dir + "/Wall", dpw = dir + "/LSParticle-WallInteraction";
@@ -1067,8 +1144,9 @@ class LSSolver {
   // waits for that word and launches phase B with only the work the step
   // needs: list builds when decided, then the narrow phase of every set in
   // one launch, bonds and the reduction with the second kick.
-  void phaseA(bool prime) {
+  void phaseA(bool prime, const int* gate = nullptr) {
     fdm_step_begin_args a{};
+    a.gate = gate;
     a.particles = particles_.bodies();
     a.pgeo = particles_.geometry();
     a.pgrid = particles_.grid();
@@ -1086,7 +1164,7 @@ class LSSolver {
     a.inst = inst_;
     if (periodic_.isActived()) {
       for (int s = 0; s < 3; ++s)
-        a.ghostFlag[s] = periodic_.active(s) ? periodic_.set(s).flag.get() : nullptr;
+        a.ghostFlag[s] = nullptr;  // the ghost list build computes them
       a.ghostAtBuild = periodic_.view(periodic_.active(0) ? 0 : 1).atBuild;
     }
     int q = 0;
@@ -1109,9 +1187,11 @@ class LSSolver {
     bool rebuild, ghost;
   };
   // Spin on the host-visible decision of the phase A just launched.
-  Decision waitDecision() {
-    const unsigned seq = (unsigned)++decisionSeq_ & 0x3fffffffu;
-    volatile int* h = hostDecision_;
+  Decision waitDecision() { return waitDecisionFor(++decisionSeq_); }
+  // Spin on the ring slot of decision number `number`.
+  Decision waitDecisionFor(int number) {
+    const unsigned seq = (unsigned)number & 0x3fffffffu;
+    volatile int* h = hostDecision_ + (number & (FDM_DECISION_RING - 1));
     unsigned v = 0;
     for (long spin = 1;; ++spin) {
       v = (unsigned)h[0];
@@ -1240,6 +1320,7 @@ class LSSolver {
       w.hitGeom = hitGeom_.get();
       w.hitCapacity = hitCapacity_;
       w.hitCount = hitCount_.get();
+      w.tuning = &tuning_;
       fdmCheck(fdm_launchNarrowPhaseSets(na, ns, &w, stream_), "narrow phase");
     }
     if (periodic_.isActived() && !gate) {
@@ -1281,6 +1362,7 @@ class LSSolver {
     r.h = 0.5 * dt_;
     r.stepCounter = stepCounter_.get();
     r.gate = gate;
+    r.tuning = &tuning_;
     {
       Timer t(this, "reduce_kick");
       if (hookMode_ == HookNone) {
@@ -1347,6 +1429,98 @@ class LSSolver {
     return 0;
   }
 
+  // ---------------------------------------------------------- lookahead
+  // advance() queues each step before the decisions of the steps ahead of
+  // it are known: phase A runs only when the control word FDM_CTL_KEEP is
+  // set, i.e. when the previous step kept its lists (its own decision sets
+  // the word) or the host completed it; the keep variant of phase B runs
+  // under the same word, now set by this step's decision. A step whose
+  // decision asks for a rebuild clears the word, so the device skips its
+  // keep variant and every step queued after it. When the host reads that
+  // decision it rolls its buffer generations back to the step, queues the
+  // rebuild variant, sets the word and queues the skipped steps again. So
+  // the device never waits for the host, and the host waits only when it is
+  // `lookahead_` steps ahead.
+  struct Queued {
+    int number;          // decision number of the step
+    GenState before;     // generations before the step's phase B
+    double time;
+  };
+  void queueStep(double time, std::vector<Queued>& q) {
+    const int* keep = ctl_.get() + FDM_CTL_KEEP;
+    Queued e;
+    e.time = time;
+    if (useGraph_) {
+      const int stepNext = pp_.nextStepIndex();
+      hipGraphExec_t& ga = graphsAGated_[stepNext];
+      if (!ga) ga = capture([&] { phaseA(false, keep); });
+      hipCheck(hipGraphLaunch(ga, stream_), "graph launch");
+      e.before = saveGenerations();
+      const Decision kd = {false, false};
+      hipGraphExec_t& gs = graphsKeep_[stepNext][pp_.listIndex()][ghostListIndex()];
+      if (!gs) {
+        gs = capture([&] {
+          beginSets(kd);
+          phaseBCommon(false, 0.0, kd, keep);
+        });
+      } else {
+        beginSets(kd);
+      }
+      hipCheck(hipGraphLaunch(gs, stream_), "graph launch");
+    } else {
+      phaseA(false, keep);
+      e.before = saveGenerations();
+      const Decision kd = {false, false};
+      beginSets(kd);
+      phaseBCommon(false, time, kd, keep);
+    }
+    e.number = ++decisionSeq_;
+    q.push_back(e);
+  }
+  // Completes the oldest queued step. Returns false when it needed a
+  // rebuild: then the steps queued after it were skipped on the device and
+  // are dropped from the queue.
+  bool resolveOldest(std::vector<Queued>& q) {
+    const Queued e = q.front();
+    q.erase(q.begin());
+    const Decision d = waitDecisionFor(e.number);
+    if (!(d.rebuild || d.ghost)) return true;
+    restoreGenerations(e.before);
+    if (useGraph_) {
+      const int stepNext = pp_.nextStepIndex();
+      const int mainNext = pp_.listIndex() ^ (d.rebuild ? 1 : 0);
+      const int ghostNext = ghostListIndex() ^ (d.ghost ? 1 : 0);
+      hipGraphExec_t& gb =
+          graphsB_[stepNext][mainNext][ghostNext][d.rebuild ? 1 : 0][d.ghost ? 1 : 0];
+      if (!gb) gb = capture([&] { phaseB(false, 0.0, d); });
+      else beginSets(d);
+      hipCheck(hipGraphLaunch(gb, stream_), "graph launch");
+    } else {
+      phaseB(false, e.time, d);
+    }
+    fdmCheck(fdm_fillInt(ctl_.get() + FDM_CTL_KEEP, 1, 1, stream_), "keep word");
+    decisionSeq_ = e.number;
+    q.clear();
+    return false;
+  }
+  void advanceAhead(size_t nSteps, double time) {
+    // the step before is complete: let the first queued phase A run
+    fdmCheck(fdm_fillInt(ctl_.get() + FDM_CTL_KEEP, 1, 1, stream_), "keep word");
+    std::vector<Queued> q;
+    size_t next = 0;       // next step to queue
+    size_t completed = 0;  // steps whose decision has been read
+    while (completed < nSteps) {
+      if (next < nSteps && (int)q.size() < lookahead_) {
+        queueStep(time + dt_ * double(next), q);
+        ++next;
+        continue;
+      }
+      const size_t first = completed;
+      if (!resolveOldest(q)) next = first + 1;  // queue the skipped steps again
+      completed = first + 1;
+    }
+  }
+
   // One step from captured graphs: phase A (keyed by the result generation
   // it clears) and the phase B variant for this step's decision and buffer
   // generations, each captured at first use.
@@ -1395,6 +1569,10 @@ class LSSolver {
       if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+    for (auto& g : graphsAGated_) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
     hipGraphExec_t* b = &graphsB_[0][0][0][0][0];
     for (size_t k = 0; k < sizeof(graphsB_) / sizeof(graphsB_[0][0][0][0][0]); ++k) {
       if (b[k]) (void)hipGraphExecDestroy(b[k]);
@@ -1436,6 +1614,7 @@ class LSSolver {
   DeviceBuffer<int> workHit_, hitCount_;
   DeviceBuffer<double> hitGeom_;
   int hitCapacity_ = 0;
+  fdm_tuning tuning_{};
   DeviceBuffer<double> disp_;
   double skinRequest_ = -1.0, skin_ = 0.0;
   double3 domainMin_{}, domainMax_{};
@@ -1444,6 +1623,8 @@ class LSSolver {
   bool baseHookCalled_ = false;
   HookMode hookMode_ = HookUnknown;
   hipGraphExec_t graphsA_[2] = {};
+  hipGraphExec_t graphsAGated_[2] = {};
+  int lookahead_ = 4;
   hipGraphExec_t graphsB_[2][2][2][2][2] = {};
   hipGraphExec_t graphsKeep_[2][2][2] = {};
   int* hostDecision_ = nullptr;     // pinned, host view

@@ -172,6 +172,70 @@ struct SolidInteraction {
     return list_[0].memoryGB() + list_[1].memoryGB() + step_[0].memoryGB() + step_[1].memoryGB();
   }
 
+  // ------------------------------------------------------------ getters
+  // The reference's getter set (SolidInteraction.h:323-372) on this
+  // layout. Device view of the latest step: the candidate pairs of the
+  // current list (master body, target body) and, per pair, the first
+  // history record and the number of its node contacts; a record is a
+  // float4 (sliding spring xyz, node id as int bits), a pair's records are
+  // contiguous in node order. The previous step's view resolves history.
+  int* masterID() { return list_[list_cur_].pairA.get(); }
+  int* slaveID() { return list_[list_cur_].pairT.get(); }
+  int* contactStart() { return step_[step_cur_].recOff.get(); }
+  int* contactCount() { return step_[step_cur_].recCnt.get(); }
+  float4* slidingSpring() { return step_[step_cur_].rec.get(); }
+  int* previousMasterID() { return list_[list_prev_].pairA.get(); }
+  int* previousSlaveID() { return list_[list_prev_].pairT.get(); }
+  float4* previousSlidingSpring() { return step_[step_cur_ ^ 1].rec.get(); }
+  // Candidate pairs of the current list (counts stay on the device between
+  // steps; this reads the count back and synchronises).
+  size_t numCandidatePair_device(hipStream_t s) { return (size_t)std::max(lastTotal(s), 0); }
+
+  // Node contacts of the step last downloaded (LSSolver::download, or an
+  // output frame), the length of the host copies below; launch shape over
+  // them and over the master bodies by the rules of SolidInteraction.h:
+  // 45-48 and 108-118.
+  size_t numPair_device() const { return masterIDHost_.size(); }
+  const size_t& pairGridDim() const { return pairGridDim_; }
+  const size_t& pairBlockDim() const { return pairBlockDim_; }
+  size_t numMaster() const { return nBodies_ == (size_t)-1 ? 0 : nBodies_; }
+  size_t numMaster_device() const { return numMaster(); }
+  const size_t& masterGridDim() const { return masterGridDim_; }
+  const size_t& masterBlockDim() const { return masterBlockDim_; }
+
+  // Host copies of the node contacts of the step last downloaded, in
+  // canonical order (master body, target, node): contact point and normal,
+  // the master's boundary node as a global index (nodes numbered particle by
+  // particle, as boundaryNodeInfo numbers them) and the target body.
+  const std::vector<double3> contactPointHostRef() const { return pointHost_; }
+  const std::vector<double3> contactNormalHostRef() const { return normalHost_; }
+  const std::vector<int> masterIDHostRef() const { return masterIDHost_; }
+  const std::vector<int> slaveIDHostRef() const { return slaveIDHost_; }
+
+  void setHostContacts(std::vector<double3> point, std::vector<double3> normal,
+                       std::vector<int> masterNode, std::vector<int> slave, size_t maxGPUThread) {
+    pointHost_ = std::move(point);
+    normalHost_ = std::move(normal);
+    masterIDHost_ = std::move(masterNode);
+    slaveIDHost_ = std::move(slave);
+    const size_t np = masterIDHost_.size();
+    if (np > 0) {
+      pairBlockDim_ = np;
+      if (maxGPUThread > 0 && maxGPUThread < np) pairBlockDim_ = maxGPUThread;
+      pairGridDim_ = (np + pairBlockDim_ - 1) / pairBlockDim_;
+    } else {
+      pairGridDim_ = 1;
+      pairBlockDim_ = 1;
+    }
+  }
+  void setLaunchShape(size_t maxGPUThread) {
+    const size_t n = numMaster();
+    if (n == 0) return;
+    if (maxGPUThread > 0) masterBlockDim_ = maxGPUThread;
+    if (n < maxGPUThread) masterBlockDim_ = n;
+    masterGridDim_ = (n + masterBlockDim_ - 1) / masterBlockDim_;
+  }
+
  private:
   size_t nBodies_ = (size_t)-1;
   ListBuffers list_[2];
@@ -179,6 +243,9 @@ struct SolidInteraction {
   int list_cur_ = 0, list_prev_ = 0, step_cur_ = 0;
   int mirrorCap_ = 0;
   DeviceBuffer<int> mirrorCount_, mirrorList_;
+  std::vector<double3> pointHost_, normalHost_;
+  std::vector<int> masterIDHost_, slaveIDHost_;
+  size_t pairGridDim_ = 1, pairBlockDim_ = 1, masterGridDim_ = 1, masterBlockDim_ = 1;
 };
 
 }  // namespace fundem

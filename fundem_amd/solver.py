@@ -199,13 +199,32 @@ class Solver:
     def autotune(self, trial_steps=20, tune_cell=True):
         """Pick skin, cell size and launch shapes for the fastest step on this
         system (results do not change); the solver is left where it was."""
-        out = np.zeros(7)
+        out = np.zeros(8)
         buf = C.create_string_buffer(16384)
         self._ok(self._lib.fdmx_autotune(self.h, int(trial_steps), int(bool(tune_cell)), _dp_(out), buf,
                                          len(buf)))
         return dict(skin=out[0], cell_size=out[1], sweep_waves=int(out[2]),
                     sweep_waves_per_cu=int(out[3]), law_block=int(out[4]), reduce_block=int(out[5]),
-                    ms_per_step=out[6], trials=buf.value.decode().splitlines())
+                    law_mode=int(out[7]), ms_per_step=out[6],
+                    trials=buf.value.decode().splitlines())
+
+    def set_launch_tuning(self, sweep_waves=0, sweep_waves_per_cu=0, law_block=0, reduce_block=0,
+                          law_mode=0):
+        """This solver's launch shapes (results do not depend on them); zero
+        keeps the process default of a field."""
+        t = np.array([sweep_waves, sweep_waves_per_cu, law_block, reduce_block, law_mode], np.int32)
+        self._ok(self._lib.fdmx_set_launch_tuning(self.h, _ip_(t)))
+
+    def launch_tuning(self):
+        t = np.zeros(5, np.int32)
+        self._ok(self._lib.fdmx_get_launch_tuning(self.h, _ip_(t)))
+        return dict(zip(("sweep_waves", "sweep_waves_per_cu", "law_block", "reduce_block",
+                         "law_mode"), (int(v) for v in t)))
+
+    def set_lookahead(self, steps):
+        """Steps queued ahead of the list decisions read back (0: one step
+        at a time)."""
+        self._ok(self._lib.fdmx_set_lookahead(self.h, int(steps)))
 
     def num_materials(self):
         a, b = C.c_int(0), C.c_int(0)

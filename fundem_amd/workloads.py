@@ -387,41 +387,68 @@ def cluster(n: int = 27, seed: int = 1, n_nodes: int = 256, n_grid: int = 16, n_
         periodic=tuple(periodic), cell_size=0.0, steps=steps)
 
 
+def fcc_sites(cells, L: float, origin):
+    """Sites of a face-centred cubic lattice of nearest-neighbour distance L
+    (cube side L sqrt 2, four sites per cube), cells[0] x cells[1] x cells[2]
+    cubes, ordered by cube (x fastest) and basis site."""
+    a = L * math.sqrt(2.0)
+    nx, ny, nz = cells
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    corner = np.stack([i, j, k], axis=-1).reshape(-1, 1, 3).astype(np.float64)
+    basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    return ((corner + basis[None]) * a).reshape(-1, 3) + np.asarray(origin, np.float64)
+
+
 def bonded_beam(counts=(20, 4, 4), radius: float = 0.5e-3, n_nodes: int = 400, n_grid: int = 16,
                 seed: int = 3, strength_mean: float = 5e6, strength_sd: float = 0.5e6,
-                tip_speed: float = 1.0, dt: float = 1e-7, steps: int = 50000) -> Scenario:
-    """Cantilever of bonded spheres (radius 0.5 mm, 16^3 SDF, 400 nodes) on a
-    simple cubic lattice of spacing 2r: bonds to the 6 lattice neighbours with
-    normal stiffness 1e9 Pa per area (E = 1e9 * L) and shear/normal 0.5
-    (bond radius L / sqrt(6)); tensile strength and cohesion ~ N(5, 0.5) MPa.
-    The x = 0 layer is fixed (zero mass, zero velocity) and the last layer
-    is driven at tip_speed along -z (zero mass)."""
+                tip_speed: float = 1.0, dt: float = 1e-7, steps: int = 50000,
+                lattice: str = "cubic") -> Scenario:
+    """Cantilever of bonded spheres (radius 0.5 mm, 16^3 SDF, 400 nodes) at
+    nearest-neighbour distance L = 2r, on a simple cubic lattice (`counts`
+    sites per axis, bonds to the 6 neighbours) or a face-centred cubic one
+    (`counts` cubes per axis, four sites each, bonds to the 12 neighbours:
+    about six bonds per site), with normal stiffness 1e9 Pa per area
+    (E = 1e9 * L) and shear/normal 0.5 (bond radius L / sqrt(6)); tensile
+    strength and cohesion ~ N(5, 0.5) MPa. The sites of the x = min plane are
+    fixed (zero mass, zero velocity) and those of the x = max plane are
+    driven at tip_speed along -z (zero mass)."""
     rng = np.random.default_rng(seed)
-    nx, ny, nz = counts
     L = 2.0 * radius
     sph = gen.sphere_template(radius, n_grid=n_grid, n_nodes=n_nodes)
-    pts = gen.jittered_lattice(counts, L, (radius, radius, radius), 0.0, rng)
+    if lattice == "cubic":
+        pts = gen.jittered_lattice(counts, L, (radius, radius, radius), 0.0, rng)
+        nx, ny, nz = counts
+        ii = np.arange(len(pts))
+        ix, iy, iz = ii % nx, (ii // nx) % ny, ii // (nx * ny)
+        pairs = []
+        for dx, dy, dz in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            ok = (ix + dx < nx) & (iy + dy < ny) & (iz + dz < nz)
+            i = ii[ok]
+            pairs.append(np.stack([i, i + dx + nx * (dy + ny * dz)], axis=1))
+        pairs = np.concatenate(pairs)
+        pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    elif lattice == "fcc":
+        from scipy.spatial import cKDTree
+        pts = fcc_sites(counts, L, (radius, radius, radius))
+        pairs = np.array(sorted(cKDTree(pts).query_pairs(1.001 * L)), dtype=np.int64)
+    else:
+        raise ValueError(f"unknown lattice {lattice!r}")
     n = len(pts)
-    ii = np.arange(n)
-    ix, iy, iz = ii % nx, (ii // nx) % ny, ii // (nx * ny)
+    tol = 1e-3 * L
     geo = np.zeros(n, np.int32)
-    geo[ix == 0] = 1       # clamp
-    geo[ix == nx - 1] = 2  # driven tip
+    geo[pts[:, 0] <= pts[:, 0].min() + tol] = 1  # clamp
+    tip = pts[:, 0] >= pts[:, 0].max() - tol
+    geo[tip] = 2                                  # driven tip
     v = np.zeros((n, 3))
-    v[ix == nx - 1, 2] = -tip_speed
+    v[tip, 2] = -tip_speed
     bonds = []
     E = 1e9 * L
     Rb = L / math.sqrt(6.0)
-    for i in range(n):
-        for dx, dy, dz in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
-            jx, jy, jz = ix[i] + dx, iy[i] + dy, iz[i] + dz
-            if jx >= nx or jy >= ny or jz >= nz:
-                continue
-            j = int(jx + nx * (jy + ny * jz))
-            c = 0.5 * (pts[i] + pts[j])
-            nrm = pts[i] - pts[j]
-            st, coh = rng.normal(strength_mean, strength_sd, size=2)
-            bonds.append((i, j, c, nrm, Rb, L, E, 0.3, float(st), float(coh), 0.0))
+    strengths = rng.normal(strength_mean, strength_sd, size=(len(pairs), 2))
+    for (i, j), (st, coh) in zip(pairs, strengths):
+        i, j = int(i), int(j)
+        bonds.append((i, j, 0.5 * (pts[i] + pts[j]), pts[i] - pts[j], Rb, L, E, 0.3, float(st),
+                      float(coh), 0.0))
     margin = 4 * L
     dmin = pts.min(axis=0) - margin
     dmax = pts.max(axis=0) + margin

@@ -46,6 +46,8 @@ extern "C" {
 #define FDM_STATUS_RECORD_OVERFLOW 2u   /* contact records exceeded capacity */
 #define FDM_STATUS_BUCKET_OVERFLOW 4u   /* per-particle mirror list exceeded capacity */
 #define FDM_STATUS_GHOST_OVERFLOW 8u    /* ghost pair list exceeded capacity */
+#define FDM_STATUS_STAGING_OVERFLOW 16u /* staged contact geometry of one step (all sets
+                                           together) exceeded its workspace */
 
 #define FDM_CHUNK 64         /* surface nodes per culling chunk (one wavefront) */
 /* History records are reserved from FDM_REC_POOLS pools, each with its own
@@ -155,15 +157,19 @@ typedef struct {
 #define FDM_CTL_SEQ 10           /* decisions taken so far */
 #define FDM_CTL_KEEP 11          /* 1 when this step keeps every list (no rebuild at all) */
 #define FDM_CTL_WORDS 16
+#define FDM_DECISION_RING 16     /* words of the host-visible decision ring */
 typedef struct {
   double skin;         /* extra reach of the candidate lists (0: rebuild every step) */
   double rhoMax;       /* largest lever arm of any body (FP32 orientation rounding) */
   double dt;
   int* ctl;            /* [FDM_CTL_WORDS] */
   double* disp;        /* [1] displacement bound accumulated since the last build */
-  int* hostDecision;   /* optional host-visible word: (decision number << 2) | rebuild << 1
-                          | ghost rebuild, written last by the decision (for a host that
-                          launches only the work a step needs); NULL to skip */
+  int* hostDecision;   /* optional host-visible ring of FDM_DECISION_RING words: decision
+                          number k writes word k % FDM_DECISION_RING with
+                          (k << 2) | rebuild << 1 | ghost rebuild, last (for a host that
+                          launches only the work a step needs, and may queue up to
+                          FDM_DECISION_RING - 1 steps ahead of the decisions it has read);
+                          NULL to skip */
 } fdm_skin;
 
 /* Candidate pairs and tangential history of one interaction set. */
@@ -289,6 +295,17 @@ int fdm_launchAddGlobalDampingForceTorque(fdm_bodies particles, double dampingCo
 int fdm_launchAddGlobalConstantForceTorque(fdm_bodies particles, fdm_double3 constantForce,
                                            fdm_double3 constantTorque, int index,
                                            hipStream_t stream);
+/* Global damping with the tutorials' argument list (launchAddGlobalDampingForceTorque,
+ * apps/tutorial/globalDamping.cuh:6-15) on this layout: force, torque, velocity and
+ * angular velocity of `num` bodies as FP32 planar SoA (LSParticle::force(), torque(),
+ * velocity(), angularVelocity()), on gridD workgroups of blockD threads
+ * (LSParticle::gridDim(), blockDim()) striding over the bodies. Same results as
+ * fdm_launchAddGlobalDampingForceTorque. */
+int fdm_launchAddGlobalDampingForceTorqueArrays(float* force, float* torque,
+                                               const float* velocity,
+                                               const float* angularVelocity,
+                                               double dampingCoefficient, size_t num,
+                                               size_t gridD, size_t blockD, hipStream_t stream);
 
 /* Replaces launchParticlePositionOrientationIntegration
  * (particleIntegrationKernel.cuh:23): drift of x and q over dt. */
@@ -312,10 +329,13 @@ int fdm_launchKickDriftHash(fdm_bodies bodies, fdm_geometry geo, fdm_double3 gra
 int fdm_launchSkinDecide(const fdm_skin* skin, hipStream_t stream);
 /* No counterpart in the reference: the first phase of a step in two
  * launches. fdm_launchStepBegin: kick (when `kick`), drift (when `drift`)
- * and cell id of every particle, drift and cell id of every wall (as
- * fdm_launchKickDriftHash), the periodic visibility flags of every particle
- * (as fdm_launchPeriodicPrepare), leaving per-workgroup partials in
- * `workspace` (fdm_stepBeginWorkspaceBytes(particles + walls) bytes).
+ * and cell id (table.hashValue) of every particle, drift and cell id of
+ * every wall, the periodic visibility of every particle against its state at
+ * the last ghost list build (the flags are stored only where ghostFlag[k] is
+ * not NULL: list builds compute their own), leaving per-workgroup partials in
+ * `workspace` (fdm_stepBeginWorkspaceBytes(particles + walls) bytes). The
+ * sort lists are left to fdm_launchCommitBodyState, which only steps that
+ * rebuild lists run.
  * fdm_launchStepDecide (same arguments, after it): clears the given counter
  * arrays and takes the list decision of fdm_launchSkinDecide from the
  * partials. */
@@ -336,6 +356,7 @@ typedef struct {
   int* ghostAtBuild;           /* [n] visibility bits at the last ghost build */
   fdm_instances inst;          /* count 0: one system (pgrid, wgrid, periodic above) */
   int* clearCounters[8];       /* arrays of FDM_REC_COUNTERS ints set to 0 (NULL: skipped) */
+  const int* gate;             /* both launches run only when *gate != 0 (NULL: always) */
 } fdm_step_begin_args;
 size_t fdm_stepBeginWorkspaceBytes(size_t bodies);
 int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
@@ -343,7 +364,9 @@ int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_skin* skin, v
 int fdm_launchStepDecide(const fdm_step_begin_args* args, const fdm_skin* skin, void* workspace,
                          size_t workspaceBytes, hipStream_t stream);
 /* No counterpart in the reference. When ctl[FDM_CTL_REBUILD] is set, records
- * every body's grid state in table.validAtBuild. */
+ * every body's grid state in table.validAtBuild and writes the table's
+ * unsorted lists for the build (sortKey = cell id + 1, sortIndex = body
+ * index; with instances, the key in the global cell space). */
 int fdm_launchCommitBodyState(fdm_bodies bodies, fdm_grid grid, fdm_hash_table table,
                               const fdm_skin* skin, hipStream_t stream);
 /* The same for bodies of several instances (isWall selects the wall grids). */
@@ -526,6 +549,7 @@ typedef struct {
   double* hitGeom;             /* [8 * hitCapacity] staged contacts */
   int hitCapacity;             /* staged contacts, split into FDM_REC_POOLS pools */
   int* hitCount;               /* [FDM_REC_COUNTERS] staged contacts per pool */
+  const struct fdm_tuning_s* tuning; /* launch shapes (NULL: the process default) */
 } fdm_narrow_workspace;
 /* No counterpart in the reference: the narrow phase of up to five sets at
  * once, in two kernels over the candidate pairs of all sets (set k's pairs
@@ -577,6 +601,7 @@ typedef struct {
   int kick;
   int* stepCounter;    /* incremented by one after the reduction (NULL: skip) */
   const int* gate;     /* run only when *gate != 0 (NULL: always) */
+  const struct fdm_tuning_s* tuning; /* launch shapes (NULL: the process default) */
 } fdm_reduce_args;
 int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream);
 
@@ -585,17 +610,25 @@ int fdm_launchReduceForceTorque(const fdm_reduce_args* args, hipStream_t stream)
 int fdm_fillInt(int* ptr, size_t n, int value, hipStream_t stream);
 
 /* No counterpart in the reference: launch shapes, picked by the autotuner
- * (fundem::LSAutotuner). Results do not depend on them. In fdm_setTuning a
- * zero field keeps the current value. The setting is process-wide; graphs
- * captured before a change keep the shapes they were captured with. */
-typedef struct {
+ * (fundem::LSAutotuner) for one solver. Results do not depend on them. A
+ * launcher uses the shapes its arguments point to (fdm_narrow_workspace.tuning,
+ * fdm_reduce_args.tuning), or the process default when that pointer is NULL.
+ * fdm_setTuning changes the process default (a zero field keeps the current
+ * value); fdm_checkTuning fills the zero fields of a caller's shapes from the
+ * default and validates them. Graphs captured before a change keep the shapes
+ * they were captured with. */
+typedef struct fdm_tuning_s {
   int sweepWaves;       /* node-sweep waves (kept pairs in flight) per workgroup: 1, 2 or 4 */
   int sweepWavesPerCU;  /* node-sweep grid size, in waves per CU: 4..128 (default 32) */
   int lawBlock;         /* contact-law threads per workgroup: 64, 128 or 256 */
   int reduceBlock;      /* reduction threads per workgroup: a multiple of 64 up to 1024 */
+  int lawMode;          /* contact law of the narrow phase: 1 in a kernel of its own (one
+                           thread per candidate pair, contacts staged in the workspace),
+                           2 in the sweep's wave (one contact per lane) */
 } fdm_tuning;
 int fdm_setTuning(const fdm_tuning* t);
 void fdm_getTuning(fdm_tuning* t);
+int fdm_checkTuning(fdm_tuning* t);
 
 /* Device properties used by the launchers (number of CUs); 0 on success. */
 int fdm_deviceInfo(int* numCU, int* warpSize);
