@@ -11,6 +11,7 @@
 // their input order: the sort is stable, as thrust's radix sort on int keys
 // is (buildHashStartEnd.cu:108-111).
 #include <algorithm>
+#include <mutex>
 
 #include "fdm_device.hpp"
 #include "fdm_launch.hpp"
@@ -426,6 +427,57 @@ extern "C" int fdm_buildHashStartEndGated(fdm_hash_table table, int n, fdm_grid 
                                           hipStream_t stream) {
   if (n < 0 || grid.num <= 0) return FDM_EINVAL;
   return buildStartEnd(table, n, nullptr, grid, workspace, workspaceBytes, stream, gate);
+}
+
+// ------------------------------------------------ fixed device scratch
+// Workspace of fdm_buildHashStartEndNoAlloc: one file-scope device array,
+// sized for FDM_NOALLOC_MAX_BODIES bodies (about 16.5 MiB), which every GPU
+// the library runs on holds until the process ends. The sort writes every
+// byte of it that it later reads (tile histograms, their scan, the
+// ping-pong key and index lists), so no call depends on what an earlier
+// call left there. The array's name is only a host-side stand-in: its
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device with that device current, and only that address reaches kernels.
+namespace {
+constexpr size_t kNoAllocTileBlocks =
+    (size_t(FDM_NOALLOC_MAX_BODIES) + size_t(kSortTile) - 1) / size_t(kSortTile);
+constexpr size_t kNoAllocHist = kNoAllocTileBlocks * kRadixBins;
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kNoAllocBytes =
+    2 * align256(size_t(FDM_NOALLOC_MAX_BODIES) * 4) + align256(kNoAllocHist * 4) +
+    align256(((kNoAllocHist + kScanTile - 1) / kScanTile + 1) * 4);
+constexpr int kNoAllocDevices = 64;
+
+std::mutex g_scratchMutex;
+void* g_scratchPtr[kNoAllocDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of this device's scratch array (looked up on first use).
+static int scratchOfDevice(int dev, void** out) {
+  if (dev < 0 || dev >= kNoAllocDevices) return FDM_ELAUNCH;
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FDM_ELAUNCH;
+    g_scratchPtr[dev] = p;
+  }
+  *out = g_scratchPtr[dev];
+  return FDM_OK;
+}
+
+extern "C" int fdm_buildHashStartEndNoAlloc(fdm_hash_table table, int n, fdm_grid grid,
+                                            hipStream_t stream) {
+  if (n < 0 || grid.num <= 0) return FDM_EINVAL;
+  if ((size_t)n > (size_t)FDM_NOALLOC_MAX_BODIES) return FDM_ECAPACITY;
+  if (radixWorkspace((size_t)n) > kNoAllocBytes) return FDM_ECAPACITY;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return FDM_ELAUNCH;
+  void* ws = nullptr;
+  const int r = scratchOfDevice(dev, &ws);
+  if (r != FDM_OK) return r;
+  return buildStartEnd(table, n, nullptr, grid, ws, kNoAllocBytes, stream);
 }
 
 extern "C" int fdm_deviceInfo(int* numCU, int* warpSize) {

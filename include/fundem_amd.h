@@ -400,6 +400,19 @@ int fdm_buildHashStartEnd(fdm_hash_table table, int n, fdm_grid grid, void* work
 int fdm_buildHashStartEndGated(fdm_hash_table table, int n, fdm_grid grid, const int* gate,
                                void* workspace, size_t workspaceBytes, hipStream_t stream);
 
+/* No counterpart in the reference. fdm_buildHashStartEnd without a workspace
+ * argument, for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the sort's workspace is a fixed device array inside the library, one per
+ * GPU, held until the process ends. The call allocates nothing. The caller
+ * still passes the table (inputs and outputs). Returns FDM_ECAPACITY for
+ * n > FDM_NOALLOC_MAX_BODIES. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race;
+ * calls ordered on one stream do not. Results equal fdm_buildHashStartEnd's
+ * bit for bit. */
+#define FDM_NOALLOC_MAX_BODIES (1 << 21)
+int fdm_buildHashStartEndNoAlloc(fdm_hash_table table, int n, fdm_grid grid, hipStream_t stream);
+
 /* Replaces launchUpdateSpatialGridHashStartEnd (myUtility/mySpatialGrid.h:33):
  * cell id of every body (calculateHashKernel) then buildHashStartEnd. */
 int fdm_launchUpdateSpatialGridHashStartEnd(fdm_hash_table table, const double* position, int n,
