@@ -65,7 +65,8 @@ def parse():
     p.add_argument("--settle", type=int, default=20000,
                    help="column: plain steps that pack the column before the timed runs")
     p.add_argument("--checkpoint", type=int, default=100, help="column: steps per checkpoint")
-    p.add_argument("--workload", choices=("pour", "batch", "column", "meshsdf"), default="pour",
+    p.add_argument("--workload", choices=("pour", "dense", "batch", "column", "meshsdf"),
+                   default="pour",
                    help="pour: the 1M-particle pour; batch: 512 packings in one solver against "
                         "plain runs of the same packings one after another; column: adjoint "
                         "against forward step cost; meshsdf: level-set grid of a closed mesh")
@@ -96,8 +97,9 @@ def log(*a):
 #   reduce_kick  per particle: pair counts read (particle set 8, wall 8,
 #                three ghost sets 24, mirror counts 16), F T write 24,
 #                template id 4, v w read and write 48, q read 16 (= 148);
-#                per contact record pair side: sums read 48 (bounded by 96
-#                per contact)
+#                per candidate pair with contacts, read from each side: its
+#                record count 4 and sums 48 (bounded by 104 per pair: every
+#                candidate pair counted)
 #   narrow       per candidate pair: ids 8, both bodies' x q template id 88,
 #                empty record range or the kept pair's record range and pair
 #                sums 80, history ids 16 (= 192); per contact: velocities of
@@ -106,7 +108,7 @@ def kernel_bytes(stage, n, pairs, contacts):
     if stage == "step_begin":
         return 188.0 * n
     if stage == "reduce_kick":
-        return 148.0 * n + 96.0 * sum(contacts.values())
+        return 148.0 * n + 104.0 * sum(pairs.values())
     if stage == "narrow":
         return 192.0 * sum(pairs.values()) + 80.0 * sum(contacts.values())
     return None
@@ -169,7 +171,8 @@ def autotune(s, a):
         log("  tune " + line)
     log(f"autotuned in {time.perf_counter() - t0:.1f} s: skin {c['skin']:.3e} m, cell "
         f"{c['cell_size']:.3e} m (0: the cell rule), sweep {c['sweep_waves']} waves/workgroup, "
-        f"{c['sweep_waves_per_cu']} waves/CU, law block {c['law_block']}, reduce block "
+        f"{c['sweep_waves_per_cu']} waves/CU, {c['nodes_per_lane']} node(s) per lane, law mode "
+        f"{c['law_mode']}, law block {c['law_block']}, reduce block "
         f"{c['reduce_block']} ({c['ms_per_step']:.3f} ms/step in the trials)")
     return {k: (float(v) if not isinstance(v, int) else v) for k, v in c.items() if k != "trials"}
 
@@ -282,6 +285,10 @@ def column_main(a):
     fwd = run_steps(f, steps, a.warmup, False) / steps
     contacts = [int(f.contacts(k)["A"].size) for k in (0, 1)]
     del f
+    f = Solver(0)
+    W.apply(sc, f)
+    fwd_graph = run_steps(f, steps, a.warmup, True) / steps
+    del f
     g = Solver(0)
     W.apply(sc, g)
     adj = Adjoint(g, contacts_per_particle=16)
@@ -301,7 +308,8 @@ def column_main(a):
     t3 = time.perf_counter()
     fwd_ckpt = (t1 - t0) / steps
     back = (t3 - t2) / steps
-    log(f"forward {1e3 * fwd:.3f} ms/step, forward with checkpoints {1e3 * fwd_ckpt:.3f} ms/step, "
+    log(f"forward {1e3 * fwd:.3f} ms/step (captured graphs {1e3 * fwd_graph:.3f}), forward with "
+        f"checkpoints {1e3 * fwd_ckpt:.3f} ms/step, "
         f"adjoint {1e3 * back:.3f} ms/step (ratio {back / fwd:.2f}); contacts at the last step "
         f"{contacts}; dJ/dkn {grad['kn'][0]:.4e} dJ/dzeta {grad['zeta'][0]:.4e} dJ/dmu {grad['mu'][0]:.4e}")
     out = {
@@ -314,7 +322,8 @@ def column_main(a):
                                f"({len([1 for m in masses if m == 0.0])} fixed bodies), packed by "
                                f"{a.settle} settling steps, dt 1e-6 s, kn 1e4, zeta 0.3, mu 0.5; "
                                f"checkpoint every {a.checkpoint} steps; J = sum of |v|^2 / 2 over the moving bodies",
-                   "forward_ms_per_step": 1e3 * fwd, "forward_with_checkpoints_ms_per_step":
+                   "forward_ms_per_step": 1e3 * fwd, "forward_graph_ms_per_step": 1e3 * fwd_graph,
+                   "forward_with_checkpoints_ms_per_step":
                    1e3 * fwd_ckpt, "contacts_last_step": contacts},
     }
     return out
@@ -529,6 +538,7 @@ def dense_main(a, templates):
     s.set_skin(a.skin)
     W.apply(dense, s)
     s.step(1)
+    tuning = autotune(s, a)
     s.step(max(a.warmup, 1))
     s.sync()
     steps = max(a.steps, 50)
@@ -555,7 +565,7 @@ def dense_main(a, templates):
             "state": f"layers closed in at (spacing)/{a.dense_closing} s for {done} steps, then "
                      "restarted at rest",
             "contacts_last_step": contacts, "candidate_pairs": pairs, "skin_m": skin,
-            "list_builds": b1 - b0, "ghost_list_builds": gb1 - gb0,
+            "list_builds": b1 - b0, "ghost_list_builds": gb1 - gb0, "tuning": tuning,
             "list_rebuild": rebuild_cost(stages, a.stage_steps, sb1 - sb0),
             "roofline": roof, "stage_roofline": stage_roof}
 
@@ -574,6 +584,9 @@ def main():
         return
     if a.workload == "batch":
         print(json.dumps(batch_main(a)), flush=True)
+        return
+    if a.workload == "dense":
+        print(json.dumps(dense_main(a, None)), flush=True)
         return
     if a.copy_probe:
         copy_probe()

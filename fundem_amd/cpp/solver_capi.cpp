@@ -157,8 +157,9 @@ long long fdmx_num_bonds(void* s) {
   return (long long)static_cast<BoundSolver*>(s)->getLSParticleBondedInteraction().numPair();
 }
 
-// Autotuner (fundem::LSAutotuner): out8 = skin, cell size, sweep waves,
-// sweep waves per CU, law block, reduce block, ms per step, law mode; log
+// Autotuner (fundem::LSAutotuner): out9 = skin, cell size, sweep waves,
+// sweep waves per CU, law block, reduce block, ms per step, law mode,
+// nodes per lane; log
 // receives one line per timed candidate (truncated at logCap).
 int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log, int logCap) {
   return guarded([&] {
@@ -173,6 +174,7 @@ int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log
       out7[5] = c.launch.reduceBlock;
       out7[6] = c.msPerStep;
       out7[7] = c.launch.lawMode;
+      out7[8] = c.launch.nodesPerLane;
     }
     if (log && logCap > 0) {
       std::string all;
@@ -183,10 +185,13 @@ int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log
   });
 }
 
+// Bytes per stored state value of this build (4: FP32, 8: FP64 storage).
+int fdmx_state_bytes() { return (int)sizeof(fdm_real); }
+
 // Launch shapes of one solver (zero fields: the process default).
 int fdmx_set_launch_tuning(void* s, const int* t5) {
   return guarded([&] {
-    fdm_tuning t{t5[0], t5[1], t5[2], t5[3], t5[4]};
+    fdm_tuning t{t5[0], t5[1], t5[2], t5[3], t5[4], t5[5]};
     static_cast<BoundSolver*>(s)->setLaunchTuning(t);
   });
 }
@@ -198,6 +203,7 @@ int fdmx_get_launch_tuning(void* s, int* t5) {
     t5[2] = t.lawBlock;
     t5[3] = t.reduceBlock;
     t5[4] = t.lawMode;
+    t5[5] = t.nodesPerLane;
   });
 }
 int fdmx_set_lookahead(void* s, int steps) {

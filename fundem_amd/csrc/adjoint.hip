@@ -45,7 +45,10 @@ __device__ __forceinline__ void stQ(double* p, int n, int i, adj::Q q) {
   p[2 * n + i] = q.q2;
   p[3 * n + i] = q.q3;
 }
-__device__ __forceinline__ adj::V f32(adj::V v) { return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z}; }
+// a vector as the state storage holds it
+__device__ __forceinline__ adj::V stateRound(adj::V v) {
+  return {roundState(v.x), roundState(v.y), roundState(v.z)};
+}
 
 // ------------------------------------------------------------ tape append
 // The sets of a step in tape order: particle, wall, ghost X, Y, XY. An
@@ -133,8 +136,8 @@ __global__ void kTapeScatter(TapeSets S, const int* off, const int* total, fdm_c
     if (n <= 0) continue;
     const int i0 = off[S.capOff[set] + p], A = s.pairA[p], T = s.pairT[p], r0 = s.recOff[p];
     for (int r = 0; r < n; ++r) {
-      const float4 rec = s.rec[r0 + r];
-      const int c = start + i0 + r, k = __float_as_int(rec.w);
+      const fdm_record rec = s.rec[r0 + r];
+      const int c = start + i0 + r, k = rec.node;
       t.A[c] = A;
       t.T[c] = T;
       t.node[c] = k;
@@ -167,8 +170,8 @@ __global__ void kRestoreScatter(fdm_contact_tape t, int block, int set, const in
   }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const int c = start + i;
-    s.rec[i] = make_float4(t.spring[3 * c], t.spring[3 * c + 1], t.spring[3 * c + 2],
-                           __int_as_float(t.node[c]));
+    s.rec[i] = makeRecord({(double)t.spring[3 * c], (double)t.spring[3 * c + 1],
+                           (double)t.spring[3 * c + 2]}, t.node[c]);
     if (!flag[i]) continue;
     int j = i + 1;
     while (j < count && t.A[start + j] == t.A[c] && t.T[start + j] == t.T[c]) ++j;
@@ -223,8 +226,8 @@ __device__ __forceinline__ void kickPrimal(const fdm_adjoint_step_args& a, const
   const adj::V F = toV(ld3(b.F, n, i)), T = toV(ld3(b.T, n, i));
   const adj::V g = {a.gravity.x, a.gravity.y, a.gravity.z};
   const double h = 0.5 * a.dt;
-  v = f32(adj::add(v, adj::mul(adj::add(adj::mul(F, t.invMass), g), h)));
-  w = f32(adj::spin(w, toQ(ld4(b.q, n, i)), T, toS(t.inertia), toS(t.invInertia), h));
+  v = stateRound(adj::add(v, adj::mul(adj::add(adj::mul(F, t.invMass), g), h)));
+  w = stateRound(adj::spin(w, toQ(ld4(b.q, n, i)), T, toS(t.inertia), toS(t.invInertia), h));
 }
 
 // reverse of the second half-kick, v' = vh + h (F' invM + g), w' = spin(wh, q', T')
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(256) kAdjContacts(fdm_adjoint_step_args a, Adj
     }
     // the node's SDF cell at the linearisation point
     adj::GeoConst g;
-    const float* nd = a.pgeo.nodes + 3 * ((size_t)tA.nodeStart + node);
+    const fdm_real* nd = a.pgeo.nodes + 3 * ((size_t)tA.nodeStart + node);
     g.s = {(double)nd[0], (double)nd[1], (double)nd[2]};
     g.origin = {tB->origin[0], tB->origin[1], tB->origin[2]};
     g.ih = tB->invSpacing;

@@ -407,6 +407,26 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
   const I3 c = clampedCell(p, g);
   return abs(tcell.x - c.x) <= 1 && abs(tcell.y - c.y) <= 1 && abs(tcell.z - c.z) <= 1;
 }
+// nodeHit in two halves, so that the SDF fetches of several nodes can be in
+// flight together: the tests before the fetch (target cell id valid, sphere,
+// SDF box) with the cell's location, then the tests after it (phi <= 0, the
+// node's cell in the target's stencil).
+template <int SET>
+__device__ __forceinline__ bool nodeCell(V3 p, const Target& tg, const fdm_grid& g, double RB,
+                                         Sample& s, int& base) {
+  if (tg.hash < 0 || (long long)tg.hash >= g.num) return false;
+  const V3 rel = vsub(p, tg.x);
+  if (SET == FDM_SET_PARTICLE) {
+    if (vlen2(rel) > RB * RB) return false;
+  }
+  return sdfLocate(qrotateT(rel, tg.q), *tg.t, s, base);
+}
+__device__ __forceinline__ bool nodeHitAt(V3 p, const Sample& s, I3 tcell, const fdm_grid& g) {
+  const double ov = -trilinear(s.x, s.y, s.z, s.c);
+  if (!(ov >= 0.)) return false;
+  const I3 c = clampedCell(p, g);
+  return abs(tcell.x - c.x) <= 1 && abs(tcell.y - c.y) <= 1 && abs(tcell.z - c.z) <= 1;
+}
 
 __device__ __forceinline__ int findPrevPair(const NarrowSet& a, int p, int A, int T, int& off) {
   // lists kept from the previous step hold the pair at the same index
@@ -433,9 +453,9 @@ __device__ __forceinline__ V3 findPrevSpring(const NarrowSet& a, int off, int cn
   int lo = off, hi = off + cnt - 1;
   while (lo <= hi) {
     const int mid = (lo + hi) >> 1;
-    const float4 r = a.prev.rec[mid];
-    const int kk = __float_as_int(r.w);
-    if (kk == k) return {(double)r.x, (double)r.y, (double)r.z};
+    const fdm_record r = a.prev.rec[mid];
+    const int kk = r.node;
+    if (kk == k) return recordSpring(r);
     if (kk < k) lo = mid + 1;
     else hi = mid - 1;
   }
@@ -495,6 +515,17 @@ struct HitSlots {
   int node[kMaxHits];
 };
 
+// Nodes of one pair that passed the tests before the SDF fetch (sphere, SDF
+// box), per wave in LDS, in node order: position, fractional cell position,
+// first corner of the cell, node id; and the hit mask of each chunk.
+struct CandSlots {
+  double px[kMaxHits], py[kMaxHits], pz[kMaxHits];
+  double fx[kMaxHits], fy[kMaxHits], fz[kMaxHits];
+  int base[kMaxHits];
+  int node[kMaxHits];
+  unsigned long long chunkHits[FDM_MAX_CHUNKS];
+};
+
 // One wave per candidate pair: chunk culls (one chunk per lane), the node
 // sweep over surviving chunks (one node per lane) evaluating the reference's
 // predicate, the contact geometry of every hit staged in LDS, then the
@@ -546,7 +577,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
   const double RB = tg.t->radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
-  const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
+  const fdm_real* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
   const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
   const int nChunks = tA.chunkCount;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -584,7 +615,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   const int prevCnt = liveChunks ? findPrevPair(S, p, A, T, prevOff) : 0;
   // node coordinates of the next surviving chunk are loaded one iteration
   // ahead, so their latency overlaps the current chunk's SDF fetches
-  float nx = 0.f, ny = 0.f, nz = 0.f;
+  fdm_real nx = 0, ny = 0, nz = 0;
   if (liveChunks) {
     const int k0 = __builtin_ctzll(liveChunks) * FDM_CHUNK + lane;
     if (k0 < tA.nodeCount) {
@@ -689,8 +720,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
       }
       TA = vcross(vsub(pc, armA), F);
       TB = vcross(vsub(pc, armB), vneg(F));
-      S.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z,
-                                   __int_as_float(k));
+      S.pairs.rec[r] = makeRecord(spring, k);
       if (C.dbgPoint) {
         C.dbgPoint[3 * r] = pc.x;
         C.dbgPoint[3 * r + 1] = pc.y;
@@ -820,9 +850,15 @@ __device__ __forceinline__ int reservePooled(int* counters, int capacity, int po
 // CULL: first take the particle-level pre-cull test (sphere, SDF box,
 // Lipschitz bound at this step's positions) from the same values, and leave
 // empty record ranges when it rejects the pair.
-template <int SET, bool CULL = false>
+// NPL (nodes per lane and iteration): 1 sweeps one surviving chunk at a
+// time; 2 two chunks at a time; 3 first collects, chunk by chunk, the nodes
+// that pass the tests before the SDF fetch into LDS (in node order) and then
+// fetches the SDF cells of up to 64 of them at once, so that a pair whose
+// few candidate nodes are spread over several chunks waits for one round of
+// fetches instead of one per chunk.
+template <int SET, bool CULL = false, int NPL = 1>
 __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
-                          int p, int lane, HitSlots& hs, int pool) {
+                          int p, int lane, HitSlots& hs, int pool, CandSlots* cs = nullptr) {
   const int A = S.pairs.pairA[p];
   const int T = S.pairs.pairT[p];
   const int n = C.P.n;
@@ -854,7 +890,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   }
   const double RB = tg.t->radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
-  const float* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
+  const fdm_real* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
   const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
   const int nChunks = tA.chunkCount;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -883,8 +919,140 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
 
   unsigned long long myMask = 0ull;
   int total = 0;
-  float nx = 0.f, ny = 0.f, nz = 0.f;
-  if (liveChunks) {
+  // a hit's contact geometry, into LDS slot `slot` (node order)
+  auto stage = [&](int slot, int k, V3 pn, const Sample& sm) {
+    const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+    const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+    const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+    hs.pc[0][slot] = pc.x;
+    hs.pc[1][slot] = pc.y;
+    hs.pc[2][slot] = pc.z;
+    hs.nrm[0][slot] = nrm.x;
+    hs.nrm[1][slot] = nrm.y;
+    hs.nrm[2][slot] = nrm.z;
+    hs.ov[slot] = ov;
+    hs.node[slot] = k;
+  };
+  if (NPL == 3) {
+    if (lane < nChunks) cs->chunkHits[lane] = 0ull;
+    int ncand = 0;
+    // SDF fetch and predicate of the collected candidates, their hits staged
+    // in node order
+    auto flush = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      bool hit = false;
+      Sample sm;
+      V3 pn = {0., 0., 0.};
+      int k = 0;
+      if (lane < ncand) {
+        pn = {cs->px[lane], cs->py[lane], cs->pz[lane]};
+        sm.x = cs->fx[lane];
+        sm.y = cs->fy[lane];
+        sm.z = cs->fz[lane];
+        k = cs->node[lane];
+        sdfCorners(tg.sdf + cs->base[lane], *tg.t, sm);
+        hit = nodeHitAt(pn, sm, tcell, tgrid);
+      }
+      const unsigned long long m = ballot64(hit);
+      if (hit) {
+        const int slot = total + __popcll(m & lt);
+        if (slot < kMaxHits) stage(slot, k, pn, sm);
+        atomicOr(&cs->chunkHits[k / FDM_CHUNK], 1ull << (k % FDM_CHUNK));
+      }
+      total += __popcll(m);
+      ncand = 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    };
+    fdm_real cx = 0, cy = 0, cz = 0;
+    if (liveChunks) {
+      const int k0 = __builtin_ctzll(liveChunks) * FDM_CHUNK + lane;
+      if (k0 < tA.nodeCount) {
+        cx = nodes[3 * k0];
+        cy = nodes[3 * k0 + 1];
+        cz = nodes[3 * k0 + 2];
+      }
+    }
+    for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
+      const int c = __builtin_ctzll(lcm);
+      const int k = c * FDM_CHUNK + lane;
+      const V3 sn = {(double)cx, (double)cy, (double)cz};
+      const unsigned long long rest = lcm & (lcm - 1ull);
+      if (rest) {
+        const int k2 = __builtin_ctzll(rest) * FDM_CHUNK + lane;
+        if (k2 < tA.nodeCount) {
+          cx = nodes[3 * k2];
+          cy = nodes[3 * k2 + 1];
+          cz = nodes[3 * k2 + 2];
+        }
+      }
+      bool ok = false;
+      Sample sm;
+      int b = 0;
+      V3 pn = {0., 0., 0.};
+      if (k < tA.nodeCount) {
+        pn = vadd(qrotate(qA, sn), xA);
+        ok = nodeCell<SET>(pn, tg, tgrid, RB, sm, b);
+      }
+      const unsigned long long m = ballot64(ok);
+      if (ncand + __popcll(m) > kMaxHits) flush();
+      if (ok) {
+        const int slot = ncand + __popcll(m & lt);
+        cs->px[slot] = pn.x;
+        cs->py[slot] = pn.y;
+        cs->pz[slot] = pn.z;
+        cs->fx[slot] = sm.x;
+        cs->fy[slot] = sm.y;
+        cs->fz[slot] = sm.z;
+        cs->base[slot] = b;
+        cs->node[slot] = k;
+      }
+      ncand += __popcll(m);
+    }
+    if (ncand > 0) flush();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < nChunks) myMask = cs->chunkHits[lane];
+  }
+  if (NPL == 2) {
+    // two surviving chunks per iteration (128 nodes in flight per wave): each
+    // lane tests node `lane` of both, the second chunk's SDF fetches issued
+    // before the first chunk's values are used; hits staged in node order
+    for (unsigned long long lcm = liveChunks; lcm;) {
+      const int c = __builtin_ctzll(lcm);
+      lcm &= lcm - 1ull;
+      const int c2 = lcm ? __builtin_ctzll(lcm) : -1;
+      if (lcm) lcm &= lcm - 1ull;
+      const int k = c * FDM_CHUNK + lane, k2 = c2 * FDM_CHUNK + lane;
+      const bool has = k < tA.nodeCount, has2 = c2 >= 0 && k2 < tA.nodeCount;
+      V3 pn = {0., 0., 0.}, pn2 = {0., 0., 0.};
+      if (has) pn = vadd(qrotate(qA, {(double)nodes[3 * k], (double)nodes[3 * k + 1],
+                                      (double)nodes[3 * k + 2]}), xA);
+      if (has2) pn2 = vadd(qrotate(qA, {(double)nodes[3 * k2], (double)nodes[3 * k2 + 1],
+                                        (double)nodes[3 * k2 + 2]}), xA);
+      Sample sm, sm2;
+      int b1 = 0, b2 = 0;
+      bool in1 = has && nodeCell<SET>(pn, tg, tgrid, RB, sm, b1);
+      bool in2 = has2 && nodeCell<SET>(pn2, tg, tgrid, RB, sm2, b2);
+      if (in1) sdfCorners(tg.sdf + b1, *tg.t, sm);
+      if (in2) sdfCorners(tg.sdf + b2, *tg.t, sm2);
+      const bool hit = in1 && nodeHitAt(pn, sm, tcell, tgrid);
+      const bool hit2 = in2 && nodeHitAt(pn2, sm2, tcell, tgrid);
+      const unsigned long long m = ballot64(hit), m2 = ballot64(hit2);
+      if (lane == c) myMask = m;
+      if (c2 >= 0 && lane == c2) myMask = m2;
+      if (hit) {
+        const int slot = total + __popcll(m & lt);
+        if (slot < kMaxHits) stage(slot, k, pn, sm);
+      }
+      total += __popcll(m);
+      if (hit2) {
+        const int slot = total + __popcll(m2 & lt);
+        if (slot < kMaxHits) stage(slot, k2, pn2, sm2);
+      }
+      total += __popcll(m2);
+    }
+  }
+  fdm_real nx = 0, ny = 0, nz = 0;
+  if (NPL == 1 && liveChunks) {
     const int k0 = __builtin_ctzll(liveChunks) * FDM_CHUNK + lane;
     if (k0 < tA.nodeCount) {
       nx = nodes[3 * k0];
@@ -892,7 +1060,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       nz = nodes[3 * k0 + 2];
     }
   }
-  for (unsigned long long lcm = liveChunks; lcm; lcm &= lcm - 1ull) {
+  for (unsigned long long lcm = NPL == 1 ? liveChunks : 0ull; lcm; lcm &= lcm - 1ull) {
     const int c = __builtin_ctzll(lcm);
     const int k = c * FDM_CHUNK + lane;
     const V3 s = {(double)nx, (double)ny, (double)nz};
@@ -1053,7 +1221,7 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
     STA = vadd(STA, TA);
     STB = vadd(STB, TB);
     const int r = base + h;
-    S.pairs.rec[r] = make_float4((float)spring.x, (float)spring.y, (float)spring.z, __int_as_float(k));
+    S.pairs.rec[r] = makeRecord(spring, k);
     if (C.dbgPoint) {
       C.dbgPoint[3 * r] = pc.x;
       C.dbgPoint[3 * r + 1] = pc.y;
@@ -1095,10 +1263,15 @@ __device__ __forceinline__ SetStarts setStarts(const NarrowMulti& a) {
 // values are wave-uniform) and sweeps the pair only when it is kept; a culled
 // pair gets empty record ranges. No work list: the candidate index is the
 // work item.
-template <int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) kCullSweep(NarrowMulti a, NarrowScratch x) {
+// Four waves per SIMD (at most 128 VGPRs): the sweep waits on memory for
+// most of its life, so the occupancy that hides that wait matters more than
+// the few registers the collecting mode then spills.
+template <int WAVES, int NPL>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4, 4)))
+kCullSweep(NarrowMulti a, NarrowScratch x) {
   if (gateOff(a.gate)) return;
   __shared__ HitSlots slots[WAVES];
+  __shared__ CandSlots cands[NPL == 3 ? WAVES : 1];
   const int lane = laneId();
   const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const SetStarts st = setStarts(a);
@@ -1121,12 +1294,13 @@ __global__ void __launch_bounds__(64 * WAVES) kCullSweep(NarrowMulti a, NarrowSc
       continue;
     }
     const int pool = g % FDM_REC_POOLS;
+    CandSlots* cs = NPL == 3 ? &cands[wave] : nullptr;
     if (s.kind == FDM_SET_PARTICLE)
-      sweepPair<FDM_SET_PARTICLE, true>(a.c, s, x, g, p, lane, slots[wave], pool);
+      sweepPair<FDM_SET_PARTICLE, true, NPL>(a.c, s, x, g, p, lane, slots[wave], pool, cs);
     else if (s.kind == FDM_SET_WALL)
-      sweepPair<FDM_SET_WALL, true>(a.c, s, x, g, p, lane, slots[wave], pool);
+      sweepPair<FDM_SET_WALL, true, NPL>(a.c, s, x, g, p, lane, slots[wave], pool, cs);
     else
-      sweepPair<FDM_SET_GHOST, true>(a.c, s, x, g, p, lane, slots[wave], pool);
+      sweepPair<FDM_SET_GHOST, true, NPL>(a.c, s, x, g, p, lane, slots[wave], pool, cs);
   }
 }
 
@@ -1192,7 +1366,7 @@ __global__ void kResetCounter(int* c) {
 
 namespace fdm {
 namespace {
-fdm_tuning g_tuning = {1, 32, 256, 256, 1};
+fdm_tuning g_tuning = {1, 32, 256, 256, 1, 1};
 }
 const fdm_tuning& tuning() { return g_tuning; }
 }  // namespace fdm
@@ -1206,11 +1380,13 @@ static bool completeTuning(const fdm_tuning& t, fdm_tuning& n) {
   if (t.lawBlock) n.lawBlock = t.lawBlock;
   if (t.reduceBlock) n.reduceBlock = t.reduceBlock;
   if (t.lawMode) n.lawMode = t.lawMode;
+  if (t.nodesPerLane) n.nodesPerLane = t.nodesPerLane;
   if (!(n.sweepWaves == 1 || n.sweepWaves == 2 || n.sweepWaves == 4)) return false;
   if (n.sweepWavesPerCU < 4 || n.sweepWavesPerCU > 128) return false;
   if (!(n.lawBlock == 64 || n.lawBlock == 128 || n.lawBlock == 256)) return false;
   if (n.reduceBlock < 64 || n.reduceBlock > 1024 || n.reduceBlock % 64) return false;
   if (!(n.lawMode == 1 || n.lawMode == 2)) return false;
+  if (n.nodesPerLane < 1 || n.nodesPerLane > 3) return false;
   return true;
 }
 
@@ -1416,12 +1592,27 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
       hipLaunchKernelGGL(kCullSweepLaw<1>, dim3(sweepBlocks), dim3(64), 0, stream, k);
     return launchStatus();
   }
-  if (tu.sweepWaves == 4)
-    hipLaunchKernelGGL(kCullSweep<4>, dim3(sweepBlocks), dim3(256), 0, stream, k, x);
-  else if (tu.sweepWaves == 2)
-    hipLaunchKernelGGL(kCullSweep<2>, dim3(sweepBlocks), dim3(128), 0, stream, k, x);
-  else
-    hipLaunchKernelGGL(kCullSweep<1>, dim3(sweepBlocks), dim3(64), 0, stream, k, x);
+  if (tu.nodesPerLane == 3) {
+    if (tu.sweepWaves == 4)
+      hipLaunchKernelGGL((kCullSweep<4, 3>), dim3(sweepBlocks), dim3(256), 0, stream, k, x);
+    else if (tu.sweepWaves == 2)
+      hipLaunchKernelGGL((kCullSweep<2, 3>), dim3(sweepBlocks), dim3(128), 0, stream, k, x);
+    else
+      hipLaunchKernelGGL((kCullSweep<1, 3>), dim3(sweepBlocks), dim3(64), 0, stream, k, x);
+  } else if (tu.nodesPerLane == 2) {
+    if (tu.sweepWaves == 4)
+      hipLaunchKernelGGL((kCullSweep<4, 2>), dim3(sweepBlocks), dim3(256), 0, stream, k, x);
+    else if (tu.sweepWaves == 2)
+      hipLaunchKernelGGL((kCullSweep<2, 2>), dim3(sweepBlocks), dim3(128), 0, stream, k, x);
+    else
+      hipLaunchKernelGGL((kCullSweep<1, 2>), dim3(sweepBlocks), dim3(64), 0, stream, k, x);
+  } else if (tu.sweepWaves == 4) {
+    hipLaunchKernelGGL((kCullSweep<4, 1>), dim3(sweepBlocks), dim3(256), 0, stream, k, x);
+  } else if (tu.sweepWaves == 2) {
+    hipLaunchKernelGGL((kCullSweep<2, 1>), dim3(sweepBlocks), dim3(128), 0, stream, k, x);
+  } else {
+    hipLaunchKernelGGL((kCullSweep<1, 1>), dim3(sweepBlocks), dim3(64), 0, stream, k, x);
+  }
   hipLaunchKernelGGL(kContactLawAll, dim3(persistentBlocks(tu.lawBlock / 64)), dim3(tu.lawBlock), 0,
                      stream, k, x);
   return launchStatus();

@@ -102,22 +102,25 @@ __device__ __forceinline__ S6 srotate(Q4 q, S6 I) {
 }
 
 // ---------------------------------------------------------------- state I/O
-__device__ __forceinline__ V3 ld3(const float* p, int n, int i) {
+// State is stored as fdm_real (FP32, or FP64 in the FP64-storage build) and
+// computed with in FP64; a store rounds to the storage type.
+__device__ __forceinline__ double roundState(double x) { return (double)(fdm_real)x; }
+__device__ __forceinline__ V3 ld3(const fdm_real* p, int n, int i) {
   return {(double)p[i], (double)p[n + i], (double)p[2 * n + i]};
 }
-__device__ __forceinline__ void st3(float* p, int n, int i, V3 v) {
-  p[i] = (float)v.x;
-  p[n + i] = (float)v.y;
-  p[2 * n + i] = (float)v.z;
+__device__ __forceinline__ void st3(fdm_real* p, int n, int i, V3 v) {
+  p[i] = (fdm_real)v.x;
+  p[n + i] = (fdm_real)v.y;
+  p[2 * n + i] = (fdm_real)v.z;
 }
-__device__ __forceinline__ Q4 ld4(const float* p, int n, int i) {
+__device__ __forceinline__ Q4 ld4(const fdm_real* p, int n, int i) {
   return {(double)p[i], (double)p[n + i], (double)p[2 * n + i], (double)p[3 * n + i]};
 }
-__device__ __forceinline__ void st4(float* p, int n, int i, Q4 q) {
-  p[i] = (float)q.q0;
-  p[n + i] = (float)q.q1;
-  p[2 * n + i] = (float)q.q2;
-  p[3 * n + i] = (float)q.q3;
+__device__ __forceinline__ void st4(fdm_real* p, int n, int i, Q4 q) {
+  p[i] = (fdm_real)q.q0;
+  p[n + i] = (fdm_real)q.q1;
+  p[2 * n + i] = (fdm_real)q.q2;
+  p[3 * n + i] = (fdm_real)q.q3;
 }
 // positions: planar FP64 (a per-step displacement can be far below one FP32
 // ulp of the coordinate)
@@ -128,8 +131,24 @@ __device__ __forceinline__ void stp(double* p, int n, int i, V3 v) {
   p[2 * n + i] = v.z;
 }
 __device__ __forceinline__ V3 ldd3(const double* p) { return {p[0], p[1], p[2]}; }
-__device__ __forceinline__ V3 round32(V3 v) {
-  return {(double)(float)v.x, (double)(float)v.y, (double)(float)v.z};
+// a vector as the state storage holds it
+__device__ __forceinline__ V3 roundState(V3 v) {
+  return {roundState(v.x), roundState(v.y), roundState(v.z)};
+}
+__device__ __forceinline__ Q4 roundState(Q4 q) {
+  return {roundState(q.q0), roundState(q.q1), roundState(q.q2), roundState(q.q3)};
+}
+// one history record: the spring in storage precision and the node id
+__device__ __forceinline__ fdm_record makeRecord(V3 spring, int node) {
+  fdm_record r;
+  r.x = (fdm_real)spring.x;
+  r.y = (fdm_real)spring.y;
+  r.z = (fdm_real)spring.z;
+  r.node = node;
+  return r;
+}
+__device__ __forceinline__ V3 recordSpring(const fdm_record& r) {
+  return {(double)r.x, (double)r.y, (double)r.z};
 }
 __device__ __forceinline__ S6 lds6(const double* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
 __host__ __device__ __forceinline__ V3 fromD3(fdm_double3 d) { return {d.x, d.y, d.z}; }
@@ -269,13 +288,13 @@ __device__ __forceinline__ V3 sectorTurnV(V3 v, int turns, const fdm_periodic& p
   if (!turns) return v;
   if (turns & 1) v = sectorRotV(v, p, 2);
   if (turns & 2) v = sectorRotV(v, p, 0);
-  return round32(v);
+  return roundState(v);
 }
 __device__ __forceinline__ Q4 sectorTurnQ(Q4 q, int turns, const fdm_periodic& p) {
   if (!turns) return q;
   if (turns & 1) q = sectorRotQ(q, p, 2);
   if (turns & 2) q = sectorRotQ(q, p, 0);
-  return {(double)(float)q.q0, (double)(float)q.q1, (double)(float)q.q2, (double)(float)q.q3};
+  return roundState(q);
 }
 // calculateSectorGhostVelocityPositionOrientationKernel (mySpatialGrid.cu:154-212):
 // image k of a wrapped particle is turned when it lies in the first y row

@@ -301,7 +301,7 @@ __global__ void kCarryPairs(fdm_pairs from, fdm_pairs to, int n, const int* gate
 }
 
 // addGlobalDampingForceTorque (globalDamping.cu:4-16) on FP32 forces
-__device__ __forceinline__ void dampOne(float* F, float* T, const float* v, const float* w, int n,
+__device__ __forceinline__ void dampOne(fdm_real* F, fdm_real* T, const fdm_real* v, const fdm_real* w, int n,
                                         int i, double c) {
   const V3 f = ld3(F, n, i), t = ld3(T, n, i);
   st3(F, n, i, vsub(f, vmul(vnormalize(ld3(v, n, i)), c * vlen(f))));
@@ -315,7 +315,7 @@ __global__ void kGlobalDamping(fdm_bodies b, double c) {
 }
 
 // The same on the caller's launch shape (grid-stride over the bodies).
-__global__ void kGlobalDampingArrays(float* F, float* T, const float* v, const float* w, int n,
+__global__ void kGlobalDampingArrays(fdm_real* F, fdm_real* T, const fdm_real* v, const fdm_real* w, int n,
                                      double c) {
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -327,7 +327,7 @@ __global__ void kGlobalDampingArrays(float* F, float* T, const float* v, const f
 __global__ void kConstantForce(fdm_bodies b, V3 F, V3 T, int index) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   V3 f = ld3(b.F, b.n, index);
-  f = round32(vadd(f, F));
+  f = roundState(vadd(f, F));
   st3(b.F, b.n, index, vadd(f, T));
 }
 
@@ -344,9 +344,9 @@ extern "C" int fdm_launchAddGlobalDampingForceTorque(fdm_bodies particles, doubl
   return launchStatus();
 }
 
-extern "C" int fdm_launchAddGlobalDampingForceTorqueArrays(float* force, float* torque,
-                                                           const float* velocity,
-                                                           const float* angularVelocity,
+extern "C" int fdm_launchAddGlobalDampingForceTorqueArrays(fdm_real* force, fdm_real* torque,
+                                                           const fdm_real* velocity,
+                                                           const fdm_real* angularVelocity,
                                                            double c, size_t num, size_t gridD,
                                                            size_t blockD, hipStream_t stream) {
   if (num == 0) return FDM_OK;

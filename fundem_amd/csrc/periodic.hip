@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kBoxThreads) kGhostBox(const double* gpos, con
 // wall contacts before the wrap (LSSolver.h:1168-1171).
 // With several instances (I.count > 0) each particle uses its instance's
 // grid and periodic flags; `per` then only says which sets exist.
-__global__ void kGhostScatter(const double* pos, const float* vel, const float* quat, int n,
+__global__ void kGhostScatter(const double* pos, const fdm_real* vel, const fdm_real* quat, int n,
                               fdm_grid g0, fdm_periodic per, GhostOut ox, GhostOut oy, GhostOut oxy,
                               int* atBuild, fdm_instances I) {
   if (gateOff(ox.gate)) return;
@@ -280,7 +280,7 @@ __global__ void kGhostScatter(const double* pos, const float* vel, const float* 
 // launchUpdatePositionOutOfBoundaryXD / YD, PeriodicBoundary.h:225-231), run
 // after the step's detection; gated by *gate (a ghost rebuild, which every
 // wrap causes).
-__global__ void kCommitWrap(double* pos, float* vel, float* quat, int n, fdm_grid g0,
+__global__ void kCommitWrap(double* pos, fdm_real* vel, fdm_real* quat, int n, fdm_grid g0,
                             fdm_periodic per, fdm_instances I, const int* gate) {
   if (gateOff(gate)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {

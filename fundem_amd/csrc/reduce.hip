@@ -57,13 +57,13 @@ __global__ void kBonds(fdm_bodies P, fdm_bonds b) {
   const int n = P.n;
   const int i = b.master[k], j = b.slave[k];
   const Q4 qi = ld4(P.q, n, i), qj = ld4(P.q, n, j);
-  const float* L = b.local + 24 * (size_t)k;
+  const fdm_real* L = b.local + 24 * (size_t)k;
   auto lv = [&](int o) { return V3{(double)L[o], (double)L[o + 1], (double)L[o + 2]}; };
   const V3 n1i = qrotate(qi, lv(0)), n2i = qrotate(qi, lv(3)), n3i = qrotate(qi, lv(6));
   const V3 n1j = qrotate(qj, lv(12)), n2j = qrotate(qj, lv(15)), n3j = qrotate(qj, lv(18));
   const V3 ri = ldp(P.x, n, i), rj = ldp(P.x, n, j);
   const V3 rbi = vadd(qrotate(qi, lv(9)), ri), rbj = vadd(qrotate(qj, lv(21)), rj);
-  const float* c = b.coef + 9 * (size_t)k;
+  const fdm_real* c = b.coef + 9 * (size_t)k;
   V3 F = {0., 0., 0.}, Mij = {0., 0., 0.}, Mji = {0., 0., 0.};
   double sig = b.maxNormalStress[k], tau = b.maxShearStress[k];
   const int on = vbond(F, Mij, Mji, sig, tau, rbi, rbj, n1i, n2i, n3i, n1j, n2j, n3j, c[0], c[1],
@@ -192,7 +192,7 @@ __global__ void kReduce(ReduceK a) {
   const fdm_template& t = a.pgeo.info[a.P.geo[i]];
   const double invM = t.invMass;
   if (nearZero(invM)) return;
-  const V3 Fs = round32(F), Ts = round32(T);
+  const V3 Fs = roundState(F), Ts = roundState(T);
   const V3 v = ld3(a.P.v, n, i);
   st3(a.P.v, n, i, vadd(v, vmul(vadd(vmul(Fs, invM), a.gravity), a.h)));
   const V3 w = ld3(a.P.w, n, i);

@@ -207,12 +207,12 @@ class LSAdjoint {
  private:
   struct StateBuf {
     DeviceBuffer<double> x, wx;
-    DeviceBuffer<float> q, v, w, F, T, wq, wv, ww, wF, wT;
+    DeviceBuffer<fdm_real> q, v, w, F, T, wq, wv, ww, wF, wT;
   };
   struct TapeBuf {
     int capacity = 0, maxBlocks = 0;
     DeviceBuffer<int> blockStart, blockPart, blockCount, A, T, node, prev;
-    DeviceBuffer<float> spring;
+    DeviceBuffer<fdm_real> spring;
     void allocate(int cap, int blocks) {
       capacity = std::max(cap, 1);
       maxBlocks = std::max(blocks, 1);
@@ -246,7 +246,7 @@ class LSAdjoint {
     int part[6] = {0, 0, 0, 0, 0, 0};
     int count = 0;
     DeviceBuffer<int> A, T, node;
-    DeviceBuffer<float> spring;
+    DeviceBuffer<fdm_real> spring;
   };
 
   static void zero(DeviceBuffer<double>& d, hipStream_t s) {

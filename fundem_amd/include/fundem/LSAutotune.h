@@ -8,9 +8,10 @@
 //     is at least the largest radius, so the 27-cell stencil finds every
 //     particle within reach and the contact set is the same);
 //   - the solver's launch shapes (fdm_tuning): node-sweep waves (kept pairs
-//     in flight) per workgroup and sweep grid size, where the contact law
-//     runs (a kernel of its own, or the sweep's wave), contact-law and
-//     reduction workgroup sizes.
+//     in flight) per workgroup and sweep grid size, nodes per lane (64 or
+//     128 nodes in flight per sweep wave), where the contact law runs (a
+//     kernel of its own, or the sweep's wave), contact-law and reduction
+//     workgroup sizes.
 // Each candidate is timed from the same saved point (LSAdjoint::mark /
 // rollback) one knob at a time, keeping the best value of each knob before
 // the next; the solver is left at the saved point with the best settings.
@@ -94,6 +95,15 @@ class LSAutotuner {
     }
     bestCost = pick(group, trialSteps, best, "lawMode");
     group.clear();
+    if (best.launch.lawMode == 1) {
+      for (int npl : {1, 2, 3}) {
+        TuneChoice c = best;
+        c.launch.nodesPerLane = npl;
+        group.push_back(c);
+      }
+      bestCost = pick(group, trialSteps, best, "nodesPerLane");
+      group.clear();
+    }
     for (int b : {64, 128, 256}) {
       TuneChoice c = best;
       c.launch.lawBlock = b;
@@ -178,11 +188,11 @@ class LSAutotuner {
       total = (build + rest * double(steps - 1)) / double(steps);
     }
     std::snprintf(line, sizeof line,
-                  "%-12s skin %.3e cell %.3e sweep %d/%d law %d/%d reduce %d: first %.3f ms, then "
-                  "%.3f ms/step, %ld builds, displacement bound %.2e -> %.4f ms/step",
+                  "%-12s skin %.3e cell %.3e sweep %d/%d x%d law %d/%d reduce %d: first %.3f ms, "
+                  "then %.3f ms/step, %ld builds, displacement bound %.2e -> %.4f ms/step",
                   knob, c.skin, c.cellSize, c.launch.sweepWaves, c.launch.sweepWavesPerCU,
-                  c.launch.lawMode, c.launch.lawBlock, c.launch.reduceBlock, build, rest,
-                  (long)(b1 - b0), disp, total);
+                  c.launch.nodesPerLane, c.launch.lawMode, c.launch.lawBlock, c.launch.reduceBlock,
+                  build, rest, (long)(b1 - b0), disp, total);
     log_.push_back(line);
     return total;
   }

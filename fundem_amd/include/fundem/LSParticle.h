@@ -106,8 +106,9 @@ struct LSParticle {
     materials_.push_back(m);
   }
 
-  // LSParticle::addGeometry (LSParticle.h:242-290). SDF values and nodes are
-  // stored FP32 on the device; the template constants stay FP64.
+  // LSParticle::addGeometry (LSParticle.h:242-290). SDF values are stored
+  // FP32 on the device, node coordinates at state precision (fdm_real); the
+  // template constants stay FP64.
   void addGeometry(double mass, symMatrix inertia, double3 origin, double spacing, int3 size,
                    const std::vector<double>& sdf, const std::vector<double3>& nodes,
                    const std::vector<int3>& connectivity) {
@@ -156,7 +157,7 @@ struct LSParticle {
     double radius = 0.;
     std::vector<double3> fn(nodes.size());
     for (size_t k = 0; k < nodes.size(); ++k) {
-      const float x = (float)nodes[k].x, y = (float)nodes[k].y, z = (float)nodes[k].z;
+      const fdm_real x = (fdm_real)nodes[k].x, y = (fdm_real)nodes[k].y, z = (fdm_real)nodes[k].z;
       nodes_.push_back(x);
       nodes_.push_back(y);
       nodes_.push_back(z);
@@ -313,13 +314,13 @@ struct LSParticle {
   void upload(hipStream_t s) {
     const size_t n = num();
     const size_t m = std::max<size_t>(n, 1);
-    std::vector<float> buf;
-    auto put3 = [&](DeviceBuffer<float>& d, const std::vector<double3>& h) {
-      buf.assign(3 * m, 0.f);
+    std::vector<fdm_real> buf;
+    auto put3 = [&](DeviceBuffer<fdm_real>& d, const std::vector<double3>& h) {
+      buf.assign(3 * m, fdm_real(0));
       for (size_t i = 0; i < n; ++i) {
-        buf[i] = (float)h[i].x;
-        buf[n + i] = (float)h[i].y;
-        buf[2 * n + i] = (float)h[i].z;
+        buf[i] = (fdm_real)h[i].x;
+        buf[n + i] = (fdm_real)h[i].y;
+        buf[2 * n + i] = (fdm_real)h[i].z;
       }
       d.allocate(3 * m, false);
       d.upload(buf, s);
@@ -340,12 +341,12 @@ struct LSParticle {
     put3(w_, angularVelocity_);
     put3(F_, force_);
     put3(T_, torque_);
-    buf.assign(4 * m, 0.f);
+    buf.assign(4 * m, fdm_real(0));
     for (size_t i = 0; i < n; ++i) {
-      buf[i] = (float)orientation_[i].q0;
-      buf[n + i] = (float)orientation_[i].q1;
-      buf[2 * n + i] = (float)orientation_[i].q2;
-      buf[3 * n + i] = (float)orientation_[i].q3;
+      buf[i] = (fdm_real)orientation_[i].q0;
+      buf[n + i] = (fdm_real)orientation_[i].q1;
+      buf[2 * n + i] = (fdm_real)orientation_[i].q2;
+      buf[3 * n + i] = (fdm_real)orientation_[i].q3;
     }
     q_.allocate(4 * m, false);
     q_.upload(buf, s);
@@ -363,8 +364,8 @@ struct LSParticle {
   void finalize(hipStream_t s) {
     const size_t n = num();
     if (n == 0 || !x_.get()) return;
-    std::vector<float> buf(4 * n);
-    auto get3 = [&](const DeviceBuffer<float>& d, std::vector<double3>& h) {
+    std::vector<fdm_real> buf(4 * n);
+    auto get3 = [&](const DeviceBuffer<fdm_real>& d, std::vector<double3>& h) {
       d.download(buf.data(), 3 * n, s);
       hipCheck(hipStreamSynchronize(s), "download");
       for (size_t i = 0; i < n; ++i) h[i] = make_double3(buf[i], buf[n + i], buf[2 * n + i]);
@@ -435,7 +436,7 @@ struct LSParticle {
   const fdm_material& materialInfo(size_t m) const { return materials_[m]; }
   // concatenated FP32 SDF values and node triples, as uploaded
   const std::vector<float>& sdfHost() const { return sdf_; }
-  const std::vector<float>& nodesHost() const { return nodes_; }
+  const std::vector<fdm_real>& nodesHost() const { return nodes_; }
   long long totalNodes() const {
     long long s = 0;
     for (int g : geometryID_) s += templates_[g].nodeCount;
@@ -445,11 +446,11 @@ struct LSParticle {
   // Device state, planar SoA (n values per component): positions FP64, the
   // rest FP32.
   double* position() { return x_.get(); }
-  float* velocity() { return v_.get(); }
-  float* angularVelocity() { return w_.get(); }
-  float* force() { return F_.get(); }
-  float* torque() { return T_.get(); }
-  float* orientation() { return q_.get(); }
+  fdm_real* velocity() { return v_.get(); }
+  fdm_real* angularVelocity() { return w_.get(); }
+  fdm_real* force() { return F_.get(); }
+  fdm_real* torque() { return T_.get(); }
+  fdm_real* orientation() { return q_.get(); }
   int* materialID() { return mat_.get(); }
   int* geometryID() { return geo_.get(); }
   int* hashValue() { return table_.hashValue.get(); }
@@ -472,7 +473,8 @@ struct LSParticle {
  private:
   std::vector<fdm_material> materials_;
   std::vector<fdm_template> templates_;
-  std::vector<float> sdf_, nodes_, chunks_;
+  std::vector<float> sdf_, chunks_;
+  std::vector<fdm_real> nodes_;
   std::vector<std::vector<double3>> nodeHost_;
   std::vector<std::vector<int3>> connectivity_;
   std::vector<double3> position_, velocity_, angularVelocity_, force_, torque_;
@@ -481,11 +483,12 @@ struct LSParticle {
   int currentSystem_ = 0;
 
   DeviceBuffer<double> x_;
-  DeviceBuffer<float> v_, w_, q_, F_, T_;
+  DeviceBuffer<fdm_real> v_, w_, q_, F_, T_;
   DeviceBuffer<int> mat_, geo_;
   DeviceBuffer<fdm_material> dmat_;
   DeviceBuffer<fdm_template> dtmpl_;
-  DeviceBuffer<float> dsdf_, dnodes_, dchunks_;
+  DeviceBuffer<float> dsdf_, dchunks_;
+  DeviceBuffer<fdm_real> dnodes_;
   fdm_grid grid_{};
   HashTableBuffers table_;
   size_t gridDim_ = 1, blockDim_ = 1;

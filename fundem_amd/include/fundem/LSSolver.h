@@ -131,7 +131,7 @@ class LSSolver {
     if (masterObjectID >= (int)particles_.num() || slaveObjectID >= (int)particles_.num()) return;
     auto r3 = [](double3 v) { return v; };
     auto r4 = [](quaternion q) {
-      return make_quaternion((float)q.q0, (float)q.q1, (float)q.q2, (float)q.q3);
+      return make_quaternion((fdm_real)q.q0, (fdm_real)q.q1, (fdm_real)q.q2, (fdm_real)q.q3);
     };
     bonds_.add(masterObjectID, slaveObjectID, r3(particles_.positionHostRef()[masterObjectID]),
                r3(particles_.positionHostRef()[slaveObjectID]),
@@ -407,7 +407,7 @@ class LSSolver {
   }
   // Steps queued ahead of the list decisions the host has read (advance);
   // 0 waits for every step's decision before queueing the next. At most
-  // FDM_DECISION_RING - 1.
+  // FDM_DECISION_RING - 1; default 8 (two graphs of four steps).
   void setLookahead(int steps) {
     lookahead_ = std::max(0, std::min(steps, FDM_DECISION_RING - 1));
   }
@@ -520,7 +520,7 @@ class LSSolver {
     LSParticle& tb = wall ? walls_ : particles_;
     for (const ContactRecord& r : recs) {
       const fdm_template& tA = particles_.templateInfo(geo[r.A]);
-      const float* nd = particles_.nodesHost().data() + 3 * ((size_t)tA.nodeStart + r.node);
+      const fdm_real* nd = particles_.nodesHost().data() + 3 * ((size_t)tA.nodeStart + r.node);
       const double3 s = make_double3(nd[0], nd[1], nd[2]);
       const double3 p = vec::add(vec::rotate(pq[r.A], s), px[r.A]);
       double3 xB = tb.positionHostRef()[r.target];
@@ -851,8 +851,8 @@ class LSSolver {
     std::vector<uint8_t> types(N, 1);
     for (size_t k = 0; k < N; ++k) {
       const int i = bonds_.masterObjectIDHostRef()[k], j = bonds_.slaveObjectIDHostRef()[k];
-      const float* L = bonds_.localHost().data() + 24 * k;
-      const float* C = bonds_.coefHost().data() + 9 * k;
+      const fdm_real* L = bonds_.localHost().data() + 24 * k;
+      const fdm_real* C = bonds_.coefHost().data() + 9 * k;
       auto lv = [&](int o) { return make_double3(L[o], L[o + 1], L[o + 2]); };
       const double3 n1i = vec::rotate(q[i], lv(0)), n2i = vec::rotate(q[i], lv(3)),
                     n3i = vec::rotate(q[i], lv(6));
@@ -1446,36 +1446,48 @@ class LSSolver {
     GenState before;     // generations before the step's phase B
     double time;
   };
-  void queueStep(double time, std::vector<Queued>& q) {
+  // Queues `count` steps (1, or kGraphSteps from a captured graph of that
+  // many steps when graphs are on).
+  void queueSteps(int count, double time, std::vector<Queued>& q) {
     const int* keep = ctl_.get() + FDM_CTL_KEEP;
-    Queued e;
-    e.time = time;
+    const Decision kd = {false, false};
+    GenState before[kGraphSteps];
     if (useGraph_) {
       const int stepNext = pp_.nextStepIndex();
-      hipGraphExec_t& ga = graphsAGated_[stepNext];
-      if (!ga) ga = capture([&] { phaseA(false, keep); });
-      hipCheck(hipGraphLaunch(ga, stream_), "graph launch");
-      e.before = saveGenerations();
-      const Decision kd = {false, false};
-      hipGraphExec_t& gs = graphsKeep_[stepNext][pp_.listIndex()][ghostListIndex()];
-      if (!gs) {
-        gs = capture([&] {
+      hipGraphExec_t& ge = (count == 1 ? graphsQueued1_ : graphsQueuedK_)[stepNext][pp_.listIndex()]
+                                                                        [ghostListIndex()];
+      auto body = [&] {
+        for (int j = 0; j < count; ++j) {
+          phaseA(false, keep);
+          before[j] = saveGenerations();
           beginSets(kd);
           phaseBCommon(false, 0.0, kd, keep);
-        });
+        }
+      };
+      if (!ge) {
+        ge = capture(body);
       } else {
-        beginSets(kd);
+        for (int j = 0; j < count; ++j) {
+          before[j] = saveGenerations();
+          beginSets(kd);
+        }
       }
-      hipCheck(hipGraphLaunch(gs, stream_), "graph launch");
+      hipCheck(hipGraphLaunch(ge, stream_), "graph launch");
     } else {
-      phaseA(false, keep);
-      e.before = saveGenerations();
-      const Decision kd = {false, false};
-      beginSets(kd);
-      phaseBCommon(false, time, kd, keep);
+      for (int j = 0; j < count; ++j) {
+        phaseA(false, keep);
+        before[j] = saveGenerations();
+        beginSets(kd);
+        phaseBCommon(false, time + dt_ * double(j), kd, keep);
+      }
     }
-    e.number = ++decisionSeq_;
-    q.push_back(e);
+    for (int j = 0; j < count; ++j) {
+      Queued e;
+      e.time = time + dt_ * double(j);
+      e.before = before[j];
+      e.number = ++decisionSeq_;
+      q.push_back(e);
+    }
   }
   // Completes the oldest queued step. Returns false when it needed a
   // rebuild: then the steps queued after it were skipped on the device and
@@ -1510,8 +1522,14 @@ class LSSolver {
     size_t next = 0;       // next step to queue
     size_t completed = 0;  // steps whose decision has been read
     while (completed < nSteps) {
+      // with graphs, kGraphSteps steps per launch where they fit
+      if (useGraph_ && next + kGraphSteps <= nSteps && (int)q.size() + kGraphSteps <= lookahead_) {
+        queueSteps(kGraphSteps, time + dt_ * double(next), q);
+        next += kGraphSteps;
+        continue;
+      }
       if (next < nSteps && (int)q.size() < lookahead_) {
-        queueStep(time + dt_ * double(next), q);
+        queueSteps(1, time + dt_ * double(next), q);
         ++next;
         continue;
       }
@@ -1569,10 +1587,11 @@ class LSSolver {
       if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
-    for (auto& g : graphsAGated_) {
-      if (g) (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+    for (auto* set : {&graphsQueued1_[0][0][0], &graphsQueuedK_[0][0][0]})
+      for (int k = 0; k < 8; ++k) {
+        if (set[k]) (void)hipGraphExecDestroy(set[k]);
+        set[k] = nullptr;
+      }
     hipGraphExec_t* b = &graphsB_[0][0][0][0][0];
     for (size_t k = 0; k < sizeof(graphsB_) / sizeof(graphsB_[0][0][0][0][0]); ++k) {
       if (b[k]) (void)hipGraphExecDestroy(b[k]);
@@ -1623,8 +1642,13 @@ class LSSolver {
   bool baseHookCalled_ = false;
   HookMode hookMode_ = HookUnknown;
   hipGraphExec_t graphsA_[2] = {};
-  hipGraphExec_t graphsAGated_[2] = {};
-  int lookahead_ = 4;
+  // queued steps as graphs: one step, or kGraphSteps steps (an even number,
+  // so a graph starts and ends at the same generation parity), keyed by the
+  // result generation, list generation and ghost list generation
+  static constexpr int kGraphSteps = 4;
+  hipGraphExec_t graphsQueued1_[2][2][2] = {};
+  hipGraphExec_t graphsQueuedK_[2][2][2] = {};
+  int lookahead_ = 2 * kGraphSteps;
   hipGraphExec_t graphsB_[2][2][2][2][2] = {};
   hipGraphExec_t graphsKeep_[2][2][2] = {};
   int* hostDecision_ = nullptr;     // pinned, host view

@@ -36,7 +36,7 @@ struct StepBuffers {
   int recCapacity = 0;
   DeviceBuffer<int> recOff, recCnt, recTotal;
   DeviceBuffer<double> sumF, sumTA, sumTB;
-  DeviceBuffer<float4> rec;
+  DeviceBuffer<fdm_record> rec;
   void allocate(int pairCap, int recCap) {
     const int cap = std::max(pairCap, 1);
     recCapacity = std::max(recCap, 1);
@@ -158,13 +158,11 @@ struct SolidInteraction {
     for (int p = 0; p < np; ++p)
       if (cnt[p] > 0) nr = std::max(nr, off[p] + cnt[p]);
     nr = std::min(nr, g.recCapacity);
-    std::vector<float4> rec = g.rec.toHost(s, std::max(nr, 0));
+    std::vector<fdm_record> rec = g.rec.toHost(s, std::max(nr, 0));
     for (int p = 0; p < np; ++p)
       for (int r = off[p]; r < off[p] + cnt[p] && r < nr; ++r) {
-        float4 v = rec[r];
-        int k;
-        std::memcpy(&k, &v.w, sizeof(int));
-        out.push_back({A[p], T[p], k, make_double3(v.x, v.y, v.z)});
+        const fdm_record& v = rec[r];
+        out.push_back({A[p], T[p], v.node, make_double3(v.x, v.y, v.z)});
       }
     return out;
   }
@@ -177,16 +175,16 @@ struct SolidInteraction {
   // layout. Device view of the latest step: the candidate pairs of the
   // current list (master body, target body) and, per pair, the first
   // history record and the number of its node contacts; a record is a
-  // float4 (sliding spring xyz, node id as int bits), a pair's records are
+  // fdm_record (sliding spring xyz, node id), a pair's records are
   // contiguous in node order. The previous step's view resolves history.
   int* masterID() { return list_[list_cur_].pairA.get(); }
   int* slaveID() { return list_[list_cur_].pairT.get(); }
   int* contactStart() { return step_[step_cur_].recOff.get(); }
   int* contactCount() { return step_[step_cur_].recCnt.get(); }
-  float4* slidingSpring() { return step_[step_cur_].rec.get(); }
+  fdm_record* slidingSpring() { return step_[step_cur_].rec.get(); }
   int* previousMasterID() { return list_[list_prev_].pairA.get(); }
   int* previousSlaveID() { return list_[list_prev_].pairT.get(); }
-  float4* previousSlidingSpring() { return step_[step_cur_ ^ 1].rec.get(); }
+  fdm_record* previousSlidingSpring() { return step_[step_cur_ ^ 1].rec.get(); }
   // Candidate pairs of the current list (counts stay on the device between
   // steps; this reads the count back and synchronises).
   size_t numCandidatePair_device(hipStream_t s) { return (size_t)std::max(lastTotal(s), 0); }

@@ -63,15 +63,15 @@ struct VBondedInteraction {
     const double3 N3i = vec::unit(vec::cross(N1i, N2i));
     const double3 N1j = vec::neg(N1i), N2j = N2i, N3j = N3i;
     const double coef[9] = {B1, B2, B3, B4, Rb, Lb, tensile, cohesion, friction};
-    for (double c : coef) coef_.push_back((float)c);
+    for (double c : coef) coef_.push_back((fdm_real)c);
     const double3 loc[8] = {vec::unit(vec::rotateT(N1i, qi)), vec::unit(vec::rotateT(N2i, qi)), vec::unit(vec::rotateT(N3i, qi)),
                             lpi,
                             vec::unit(vec::rotateT(N1j, qj)), vec::unit(vec::rotateT(N2j, qj)), vec::unit(vec::rotateT(N3j, qj)),
                             lpj};
     for (const double3& v : loc) {
-      local_.push_back((float)v.x);
-      local_.push_back((float)v.y);
-      local_.push_back((float)v.z);
+      local_.push_back((fdm_real)v.x);
+      local_.push_back((fdm_real)v.y);
+      local_.push_back((fdm_real)v.z);
     }
     master_.push_back(master);
     slave_.push_back(slave);
@@ -150,8 +150,8 @@ struct VBondedInteraction {
   const std::vector<int>& slaveObjectIDHostRef() const { return slave_; }
   // per bond: B1 B2 B3 B4 R L0 tensile cohesion friction; local frames n1 n2 n3 p of
   // master then slave (FP32, as uploaded)
-  const std::vector<float>& coefHost() const { return coef_; }
-  const std::vector<float>& localHost() const { return local_; }
+  const std::vector<fdm_real>& coefHost() const { return coef_; }
+  const std::vector<fdm_real>& localHost() const { return local_; }
   std::vector<double> maxNormalStress(hipStream_t s) { return sigma_.toHost(s, master_.size()); }
   std::vector<double> maxShearStress(hipStream_t s) { return tau_.toHost(s, master_.size()); }
   double deviceMemoryGB() const {
@@ -160,10 +160,10 @@ struct VBondedInteraction {
 
  private:
   std::vector<int> master_, slave_, active_, breakStep_;
-  std::vector<float> coef_, local_;
+  std::vector<fdm_real> coef_, local_;
   std::vector<double3> center_;
   DeviceBuffer<int> dmaster_, dslave_, dactive_, dbreak_, applied_, partStart_, partList_;
-  DeviceBuffer<float> dcoef_, dlocal_;
+  DeviceBuffer<fdm_real> dcoef_, dlocal_;
   DeviceBuffer<double> outF_, outTi_, outTj_, sigma_, tau_;
   const int* stepCounter_ = nullptr;
 };

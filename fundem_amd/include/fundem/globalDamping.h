@@ -8,8 +8,8 @@
 //         getLSParticle().num_device(), getLSParticle().gridDim(), getLSParticle().blockDim(), 0);
 //   }
 //
-// compiles unchanged against this header. The arrays are this build's FP32
-// planar SoA device state. The tutorial's constant-force launcher has no
+// compiles unchanged against this header. The arrays are this build's
+// planar SoA device state (fdm_real: FP32, or FP64 in the FP64-storage build). The tutorial's constant-force launcher has no
 // body count in its arguments, which planar SoA needs to address a body's
 // y and z components; fdm_launchAddGlobalConstantForceTorque takes the bodies.
 #pragma once
@@ -18,8 +18,9 @@
 
 #include "../../../include/fundem_amd.h"
 
-inline void launchAddGlobalDampingForceTorque(float* force, float* torque, const float* velocity,
-                                              const float* angularVelocity,
+inline void launchAddGlobalDampingForceTorque(fdm_real* force, fdm_real* torque,
+                                              const fdm_real* velocity,
+                                              const fdm_real* angularVelocity,
                                               const double dampingCoefficient, const size_t num,
                                               const size_t gridD, const size_t blockD,
                                               hipStream_t stream) {

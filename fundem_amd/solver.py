@@ -17,23 +17,30 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _up = C.POINTER(C.c_uint)
 
-_LIB = None
+_LIBS = {}
 
 
 class NativeLibraryMissing(RuntimeError):
     pass
 
 
-def solver_lib():
-    """Load libfundem_solver.so (and through it libfundem_amd.so)."""
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    if not (os.path.exists(_build.SOLVER_LIB) and os.path.exists(_build.CORE_LIB)):
+def solver_lib(precision: str = "fp32"):
+    """Load libfundem_solver.so and through it libfundem_amd.so (FP32 state
+    storage), or their FP64-storage builds (precision "fp64"). Both builds
+    export the same names: the FP64 pair is loaded with its own symbols
+    bound ahead of any already loaded (RTLD_DEEPBIND), so one process can use
+    both."""
+    if precision in _LIBS:
+        return _LIBS[precision]
+    core, solver = _build.libs(precision)
+    if not (os.path.exists(solver) and os.path.exists(core)):
         raise NativeLibraryMissing(
-            f"native libraries not built ({_build.SOLVER_LIB}); run fundem_amd.build.build_all()")
-    C.CDLL(_build.CORE_LIB, mode=C.RTLD_GLOBAL)
-    lib = C.CDLL(_build.SOLVER_LIB)
+            f"native libraries not built ({solver}); run fundem_amd.build.build_all()")
+    if precision == "fp32":
+        C.CDLL(core, mode=C.RTLD_GLOBAL)
+        lib = C.CDLL(solver)
+    else:
+        lib = C.CDLL(solver, mode=C.RTLD_LOCAL | os.RTLD_DEEPBIND)
     I, LL, D, P = C.c_int, C.c_longlong, C.c_double, C.c_void_p
     sig = {
         "fdmx_last_error": (C.c_char_p, []),
@@ -101,11 +108,17 @@ def solver_lib():
         "fdmx_get_geometry": (I, [P, I, _dp, _dp, _dp, _dp, _dp]),
         "fdmx_stage_times": (I, [P, C.c_char_p, I, _dp, C.POINTER(C.c_longlong), I]),
     }
+    sig["fdmx_set_launch_tuning"] = (I, [P, _ip])
+    sig["fdmx_get_launch_tuning"] = (I, [P, _ip])
+    sig["fdmx_set_lookahead"] = (I, [P, I])
+    sig["fdmx_state_bytes"] = (I, [])
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _LIB = lib
+    if lib.fdmx_state_bytes() != (8 if precision == "fp64" else 4):
+        raise RuntimeError(f"{solver} does not store state in {precision}")
+    _LIBS[precision] = lib
     return lib
 
 
@@ -134,8 +147,13 @@ def _tris(t):
 class Solver:
     """LSSolver on the GPU (device `device`)."""
 
-    def __init__(self, device: int = 0):
-        lib = solver_lib()
+    def __init__(self, device: int = 0, precision: str = "fp32"):
+        """precision: storage of the state a step carries over (velocities,
+        spins, orientations, forces, torques, springs, bond constants):
+        "fp32" (the production build) or "fp64" (the reference's all-FP64
+        arithmetic)."""
+        lib = solver_lib(precision)
+        self.precision = precision
         self._lib = lib
         self.h = lib.fdmx_create(int(device))
         if not self.h:
@@ -199,27 +217,28 @@ class Solver:
     def autotune(self, trial_steps=20, tune_cell=True):
         """Pick skin, cell size and launch shapes for the fastest step on this
         system (results do not change); the solver is left where it was."""
-        out = np.zeros(8)
+        out = np.zeros(9)
         buf = C.create_string_buffer(16384)
         self._ok(self._lib.fdmx_autotune(self.h, int(trial_steps), int(bool(tune_cell)), _dp_(out), buf,
                                          len(buf)))
         return dict(skin=out[0], cell_size=out[1], sweep_waves=int(out[2]),
                     sweep_waves_per_cu=int(out[3]), law_block=int(out[4]), reduce_block=int(out[5]),
-                    law_mode=int(out[7]), ms_per_step=out[6],
+                    law_mode=int(out[7]), nodes_per_lane=int(out[8]), ms_per_step=out[6],
                     trials=buf.value.decode().splitlines())
 
     def set_launch_tuning(self, sweep_waves=0, sweep_waves_per_cu=0, law_block=0, reduce_block=0,
-                          law_mode=0):
+                          law_mode=0, nodes_per_lane=0):
         """This solver's launch shapes (results do not depend on them); zero
         keeps the process default of a field."""
-        t = np.array([sweep_waves, sweep_waves_per_cu, law_block, reduce_block, law_mode], np.int32)
+        t = np.array([sweep_waves, sweep_waves_per_cu, law_block, reduce_block, law_mode,
+                      nodes_per_lane], np.int32)
         self._ok(self._lib.fdmx_set_launch_tuning(self.h, _ip_(t)))
 
     def launch_tuning(self):
-        t = np.zeros(5, np.int32)
+        t = np.zeros(6, np.int32)
         self._ok(self._lib.fdmx_get_launch_tuning(self.h, _ip_(t)))
         return dict(zip(("sweep_waves", "sweep_waves_per_cu", "law_block", "reduce_block",
-                         "law_mode"), (int(v) for v in t)))
+                         "law_mode", "nodes_per_lane"), (int(v) for v in t)))
 
     def set_lookahead(self, steps):
         """Steps queued ahead of the list decisions read back (0: one step

@@ -99,8 +99,13 @@ def apply_batch(scenarios, t) -> None:
     time step and gravity; materials, walls and wall templates are per system."""
     base = scenarios[0]
     dens = base.density if isinstance(base.density, (list, tuple)) else [base.density] * len(base.templates)
-    for d, tp in zip(dens, base.templates):
-        t.add_geometry_density(d, tp)
+    for g, (d, tp) in enumerate(zip(dens, base.templates)):
+        if g in base.fixed_templates:
+            t.add_geometry(0.0, np.zeros(6), tp)
+        elif base.mass_properties is not None:
+            t.add_geometry(base.mass_properties[g][0], base.mass_properties[g][1], tp)
+        else:
+            t.add_geometry_density(d, tp)
     mat_off = wmat_off = wt_off = 0
     for s in scenarios:
         if s.templates is not base.templates and len(s.templates) != len(base.templates):
@@ -109,11 +114,13 @@ def apply_batch(scenarios, t) -> None:
             raise ValueError("bonded systems are not batched")
         if s.dt != base.dt or not np.array_equal(s.gravity, base.gravity):
             raise ValueError("systems of one batch share dt and gravity")
+        if s.damping_ratio != base.damping_ratio:
+            raise ValueError("systems of one batch give their materials the same way")
         t.add_system(s.domain_min, s.domain_max, s.periodic[0], s.periodic[1])
         for m in s.materials:
-            t.add_material(*m)
+            (t.add_material_damping if s.damping_ratio else t.add_material)(*m)
         for m in s.wall_materials:
-            t.add_wall_material(*m)
+            (t.add_wall_material_damping if s.damping_ratio else t.add_wall_material)(*m)
         for tp in s.wall_templates:
             t.add_wall_geometry(tp)
         t.add_particles(s.x, s.v, s.w, s.q, np.asarray(s.mat, np.int32) + mat_off, s.geo)

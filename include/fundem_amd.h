@@ -61,6 +61,31 @@ extern "C" {
 typedef struct { double x, y, z; } fdm_double3;
 typedef struct { int x, y, z; } fdm_int3;
 
+/* Storage precision of the state a step carries over: velocities, angular
+ * velocities, orientations, forces, torques, tangential springs and bond
+ * constants, and the templates' surface-node coordinates (positions are
+ * always FP64, arithmetic always FP64, SDF values always FP32). The
+ * library is built twice from the same sources: libfundem_amd.so stores
+ * them in FP32 (the production path, bit for bit the oracle with FP32
+ * storage), libfundem_amd_fp64.so (built with FDM_STATE_FP64) in FP64, so
+ * that its steps are the reference's all-FP64 arithmetic (bit for bit the
+ * oracle with FP64 storage and canonical sums). One history record is the
+ * sliding spring of a node-contact and the node's id. */
+#ifdef FDM_STATE_FP64
+typedef double fdm_real;
+typedef struct __attribute__((aligned(16))) {
+  double x, y, z;
+  int node;
+  int pad_;
+} fdm_record;
+#else
+typedef float fdm_real;
+typedef struct __attribute__((aligned(16))) {
+  float x, y, z;
+  int node;
+} fdm_record;
+#endif
+
 /* One particle or wall template (geometryInfo, LSParticle.h:53-174), FP64. */
 typedef struct {
   double radius;         /* max |node| after the centroid shift */
@@ -91,15 +116,15 @@ typedef struct {
 
 /* Device view of a set of bodies (particles or walls), planar SoA: positions
  * FP64 (a step's displacement, v*dt, can be below one FP32 ulp of the
- * coordinate), everything else FP32. */
+ * coordinate), everything else fdm_real. */
 typedef struct {
   int n;
   double* x;
-  float* v;
-  float* w;
-  float* q;
-  float* F;
-  float* T;
+  fdm_real* v;
+  fdm_real* w;
+  fdm_real* q;
+  fdm_real* F;
+  fdm_real* T;
   const int* mat;
   const int* geo;
 } fdm_bodies;
@@ -109,7 +134,7 @@ typedef struct {
   int num;
   const fdm_template* info;
   const float* sdf;
-  const float* nodes;    /* xyz triples */
+  const fdm_real* nodes; /* xyz triples after the centroid shift */
   const float* chunks;   /* float4 per chunk */
 } fdm_geometry;
 
@@ -188,7 +213,7 @@ typedef struct {
   int recCapacity;     /* records, split into FDM_REC_POOLS equal pools */
   int* recTotal;       /* [FDM_REC_COUNTERS] records used per pool, pool j at
                           recTotal[j * FDM_REC_STRIDE] */
-  float4* rec;         /* [recCapacity] (spring xyz, node id as int bits), per pair by node */
+  fdm_record* rec;     /* [recCapacity] (spring xyz, node id), per pair by node */
 } fdm_pairs;
 
 /* Per-particle mirror lists: pairs in which the particle is the target. */
@@ -215,15 +240,15 @@ typedef struct {
                           last ghost list build; NULL when rebuilt every step */
 } fdm_ghosts;
 
-/* Bonds (VBondedInteraction, VBondedInteraction.h:61-566), FP32 constants. */
+/* Bonds (VBondedInteraction, VBondedInteraction.h:61-566), fdm_real constants. */
 typedef struct {
   int n;
   const int* master;
   const int* slave;
   int* active;
   int* breakStep;      /* step index at which the bond failed, -1 while intact */
-  const float* coef;   /* [n*9]: B1 B2 B3 B4 R L0 tensile cohesion friction */
-  const float* local;  /* [n*24]: n1i n2i n3i pi n1j n2j n3j pj (xyz each) */
+  const fdm_real* coef;   /* [n*9]: B1 B2 B3 B4 R L0 tensile cohesion friction */
+  const fdm_real* local;  /* [n*24]: n1i n2i n3i pi n1j n2j n3j pj (xyz each) */
   double* outF;        /* [3n] bond force on master */
   double* outTi;       /* [3n] torque on master */
   double* outTj;       /* [3n] torque on slave */
@@ -297,13 +322,13 @@ int fdm_launchAddGlobalConstantForceTorque(fdm_bodies particles, fdm_double3 con
                                            hipStream_t stream);
 /* Global damping with the tutorials' argument list (launchAddGlobalDampingForceTorque,
  * apps/tutorial/globalDamping.cuh:6-15) on this layout: force, torque, velocity and
- * angular velocity of `num` bodies as FP32 planar SoA (LSParticle::force(), torque(),
+ * angular velocity of `num` bodies as fdm_real planar SoA (LSParticle::force(), torque(),
  * velocity(), angularVelocity()), on gridD workgroups of blockD threads
  * (LSParticle::gridDim(), blockDim()) striding over the bodies. Same results as
  * fdm_launchAddGlobalDampingForceTorque. */
-int fdm_launchAddGlobalDampingForceTorqueArrays(float* force, float* torque,
-                                               const float* velocity,
-                                               const float* angularVelocity,
+int fdm_launchAddGlobalDampingForceTorqueArrays(fdm_real* force, fdm_real* torque,
+                                               const fdm_real* velocity,
+                                               const fdm_real* angularVelocity,
                                                double dampingCoefficient, size_t num,
                                                size_t gridD, size_t blockD, hipStream_t stream);
 
@@ -638,6 +663,11 @@ typedef struct fdm_tuning_s {
   int lawMode;          /* contact law of the narrow phase: 1 in a kernel of its own (one
                            thread per candidate pair, contacts staged in the workspace),
                            2 in the sweep's wave (one contact per lane) */
+  int nodesPerLane;     /* how a sweep wave walks a pair's surviving 64-node chunks: 1 one
+                           chunk per iteration (one node per lane), 2 two chunks per
+                           iteration (128 nodes in flight), 3 the nodes that pass the tests
+                           before the SDF fetch collected first, then up to 64 of their SDF
+                           cells fetched at once; lawMode 1 only */
 } fdm_tuning;
 int fdm_setTuning(const fdm_tuning* t);
 void fdm_getTuning(fdm_tuning* t);
@@ -666,7 +696,7 @@ typedef struct {
   int* A;              /* [capacity] */
   int* T;              /* [capacity] target particle or wall */
   int* node;           /* [capacity] node of A */
-  float* spring;       /* [3 * capacity] tangential spring after the step, xyz per contact */
+  fdm_real* spring;    /* [3 * capacity] tangential spring after the step, xyz per contact */
   int* prev;           /* [capacity] the same (A, target, node) in the previous block, or -1 */
   unsigned* status;    /* FDM_STATUS_RECORD_OVERFLOW when a block does not fit */
 } fdm_contact_tape;

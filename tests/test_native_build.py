@@ -24,11 +24,24 @@ def test_header_declares_launchers():
         assert s in syms
 
 
-@pytest.mark.skipif(not os.path.exists(build.CORE_LIB), reason="core library not built")
-def test_core_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(build.CORE_LIB)
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_core_library_exports_every_declared_symbol(precision):
+    core, _ = build.libs(precision)
+    if not os.path.exists(core):
+        pytest.skip("core library not built")
+    lib = ctypes.CDLL(core, mode=ctypes.RTLD_LOCAL | os.RTLD_DEEPBIND)
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
+
+
+@pytest.mark.parametrize("precision,nbytes", [("fp32", 4), ("fp64", 8)])
+def test_solver_binding_stores_state_in_its_precision(precision, nbytes):
+    _, solver_path = build.libs(precision)
+    if not os.path.exists(solver_path):
+        pytest.skip("solver library not built")
+    from fundem_amd import solver
+    lib = solver.solver_lib(precision)
+    assert lib.fdmx_state_bytes() == nbytes
 
 
 @pytest.mark.skipif(not os.path.exists(build.SOLVER_LIB), reason="solver library not built")
