@@ -3,10 +3,20 @@
 //   - the candidate-list skin (rebuild distance), as a fraction of the
 //     largest particle radius (not with the sector periodic boundary, whose
 //     images are rebuilt every step);
-//   - the particle grid's cell size, as a multiple of the configured one
-//     (only for one non-periodic system without bonds: there every candidate
-//     is at least the largest radius, so the 27-cell stencil finds every
-//     particle within reach and the contact set is the same);
+//   - the particle grid's cell size, as a multiple (1.25, 1.5, 2) of the
+//     configured one or of the rule's 2 x the largest radius. A node reaches
+//     a particle target within the target's radius, so any cell of at least
+//     the largest radius puts every target within reach in the node's
+//     27-cell stencil. With planar periodic faces an image exists for the
+//     bodies of the first cell column (row) only, and two bodies touch
+//     across a face only when their centres are within the sum of their
+//     radii, at most the rule's cell; every candidate cell is at least the
+//     rule's, so every body that can touch across a face still has its
+//     image, and a clamped ghost cell stays in the stencil of every node
+//     within reach. Batched systems share the candidate on every instance
+//     grid. The contact set is therefore the same; with canonical sums the
+//     results are the same bits. Not with the sector boundary (its images
+//     are rebuilt each step by their own cell rule);
 //   - the solver's launch shapes (fdm_tuning): node-sweep waves (kept pairs
 //     in flight) per workgroup and sweep grid size, nodes per lane (64 or
 //     128 nodes in flight per sweep wave), where the contact law runs (a
@@ -65,11 +75,9 @@ class LSAutotuner {
       }
       bestCost = pick(group, window, best, "skin");
     }
-    const bool cellOk = tuneCellSize && s_.systems_.empty() && !s_.periodic_.isActived() &&
-                        s_.bonds_.view().n == 0;
+    const bool cellOk = tuneCellSize && !s_.periodic_.isSector();
     if (cellOk) {
-      const fdm_grid g = s_.particles_.grid();
-      const double base = s_.cellSize_ > 0.0 ? s_.cellSize_ : 1.0 / g.invCell.x;
+      const double base = s_.particles_.cellSizeRule(s_.cellSize_);
       group.clear();
       for (double f : {1.25, 1.5, 2.0}) {
         TuneChoice c = best;

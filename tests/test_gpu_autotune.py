@@ -1,11 +1,16 @@
 """The autotuner changes speed, not results: a run after autotune() (which
 times candidate skins, cell sizes and launch shapes from a saved point and
 returns the solver to it) equals a plain run bit for bit, for a
-non-periodic system (where the cell size is tuned too) and a periodic one."""
+non-periodic system and a periodic one; cells coarser than the grid rule's
+keep periodic contact sets and states equal to the oracle's; every launch
+shape gives the same bits."""
 import numpy as np
 import pytest
 
+import pyoracle as po
 from fundem_amd import workloads as W
+
+SETS = (po.SET_PP, po.SET_PW, po.SET_GX, po.SET_GY, po.SET_GXY)
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +30,7 @@ def test_autotune_keeps_results(periodic):
         print(line)
     print({k: v for k, v in choice.items() if k != "trials"})
     assert len(choice["trials"]) >= 16
+    assert sum(line.startswith("cell") for line in choice["trials"]) >= 3, "cell size not tuned"
     assert choice["law_block"] in (64, 128, 256) and choice["sweep_waves"] in (1, 2, 4)
     tuned.step(150)
     got = tuned.state()
@@ -73,3 +79,35 @@ def test_launch_shapes_keep_results(name):
     for shape, st, _ in runs[1:]:
         for key in ("x", "v", "w", "q", "F", "T"):
             assert np.array_equal(st[key], runs[0][1][key]), (shape, key)
+
+
+@pytest.mark.parametrize("periodic,factor", [((True, True), 1.25), ((True, True), 2.0),
+                                             ((True, False), 1.5), ((False, True), 1.5)])
+def test_coarser_cells_keep_periodic_results(periodic, factor):
+    """A periodic cluster gridded with cells `factor` times the rule's (2 x
+    the largest radius), which the autotuner may pick, against the oracle
+    gridded by the rule: states, contact sets and springs bit for bit."""
+    from fundem_amd import Solver
+    scn = W.cluster(n=125, periodic=periodic, speed=1.0)
+    o = po.OracleSim(True, True)
+    W.apply(scn, o)
+    rule = 2.0 * max(o.geometry(k)["radius"] for k in range(len(scn.templates)))
+    g = Solver()
+    g.set_cell_size(factor * rule)
+    W.apply(scn, g)
+    assert g.grid()["size"][0] < o.grid()["size"][0], (g.grid()["size"], o.grid()["size"])
+    seen = np.zeros(len(SETS), int)
+    for i in range(1, 301):
+        g.step(1)
+        o.step(1)
+        if i % 50 == 0:
+            gs, os_ = g.state(), o.state()
+            for key in ("x", "v", "w", "q", "F", "T"):
+                assert np.array_equal(gs[key], os_[key]), (i, key)
+            for j, s in enumerate(SETS):
+                gc, oc = g.contacts(s), o.contacts(s)
+                for key in ("A", "B", "k", "spring"):
+                    assert np.array_equal(gc[key], oc[key]), (i, s, key)
+                seen[j] += oc["A"].size
+    print(f"{periodic} x{factor}: contacts seen per set {seen.tolist()}", flush=True)
+    assert g.status() == 0 and seen[0] > 0 and seen[2:].sum() > 0

@@ -30,19 +30,29 @@ def split_contacts(c, a_off, a_n, b_off):
             "spring": c["spring"][sel]}
 
 
-@pytest.mark.parametrize("skin", [None, 0.0])
-def test_batch_systems_equal_their_plain_runs(skin):
+def rule_cell(oracle, n_templates):
+    """The grid rule's cell: twice the largest template radius."""
+    return 2.0 * max(oracle.geometry(k)["radius"] for k in range(n_templates))
+
+
+@pytest.mark.parametrize("skin,cells", [(None, None), (0.0, None), (None, 1.5)])
+def test_batch_systems_equal_their_plain_runs(skin, cells):
+    """With cells=1.5 every instance grid of the batch has cells 1.5 times
+    the rule's (as the autotuner may choose) while each oracle grids by the
+    rule."""
     from fundem_amd import Solver
     scns = systems()
-    g = Solver()
-    if skin is not None:
-        g.set_skin(skin)
-    W.apply_batch(scns, g)
     oracles = []
     for s in scns:
         o = po.OracleSim(True, True)
         W.apply(s, o)
         oracles.append(o)
+    g = Solver()
+    if skin is not None:
+        g.set_skin(skin)
+    if cells is not None:
+        g.set_cell_size(cells * rule_cell(oracles[0], len(scns[0].templates)))
+    W.apply_batch(scns, g)
     p_off = np.cumsum([0] + [s.n for s in scns])
     w_off = np.cumsum([0] + [len(s.walls) for s in scns])
     total = 0
