@@ -45,14 +45,31 @@ constexpr double kReach = 1.0 + 1e-5;
 struct Target {
   V3 x;       // detection position
   Q4 q;
-  const fdm_template* t;
+  fdm_template t;
   const float* sdf;
   int hash;   // hash value in the target grid
 };
 
-__device__ __forceinline__ const fdm_template& tmplOf(const fdm_geometry& g, int geo) {
-  return g.info[geo];
+// Every kernel of this file only reads body state, templates, materials,
+// candidate lists and the previous step's records: they are read through the
+// constant address space (ldu), so that the wave-uniform reads of the sweep
+// (one pair per wave) become scalar loads into SGPRs, issued together,
+// instead of vector loads that each hold VGPRs and wait in turn.
+__device__ __forceinline__ fdm_template tmplOf(const fdm_geometry& g, int geo) {
+  return ldu(g.info + geo);
 }
+__device__ __forceinline__ fdm_material matOf(const fdm_material* m, int k) { return ldu(m + k); }
+__device__ __forceinline__ V3 ldpR(const double* p, int n, int i) {
+  return {ldu(p + i), ldu(p + n + i), ldu(p + 2 * n + i)};
+}
+__device__ __forceinline__ V3 ld3R(const fdm_real* p, int n, int i) {
+  return {(double)ldu(p + i), (double)ldu(p + n + i), (double)ldu(p + 2 * n + i)};
+}
+__device__ __forceinline__ Q4 ld4R(const fdm_real* p, int n, int i) {
+  return {(double)ldu(p + i), (double)ldu(p + n + i), (double)ldu(p + 2 * n + i),
+          (double)ldu(p + 3 * n + i)};
+}
+__device__ __forceinline__ V3 ldd3R(const double* p) { return {ldu(p), ldu(p + 1), ldu(p + 2)}; }
 
 // True when no point within `reach` of `local` (target frame) can have
 // phi <= 0: the point is inside the SDF box and phi - Lip*reach is positive
@@ -121,28 +138,18 @@ struct BroadK {
 template <int SET>
 __device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, double RA, int T,
                                                 int G, V3 xT) {
-  double reachT;
-  Q4 qT;
-  const fdm_template* tT;
-  const float* sdf;
-  if (SET == FDM_SET_WALL) {
-    tT = &tmplOf(a.wgeo, a.W.geo[T]);
-    qT = ld4(a.W.q, a.W.n, T);
-    sdf = a.wgeo.sdf;
-    reachT = tT->boxReach;
-  } else {
-    tT = &tmplOf(a.pgeo, a.P.geo[T]);
-    qT = ld4(a.P.q, a.P.n, T);
-    sdf = a.pgeo.sdf;
-    reachT = SET == FDM_SET_PARTICLE ? tT->radius : tT->boxReach;
-  }
+  const fdm_template tT = SET == FDM_SET_WALL ? tmplOf(a.wgeo, ldu(a.W.geo + T))
+                                              : tmplOf(a.pgeo, ldu(a.P.geo + T));
+  const Q4 qT = SET == FDM_SET_WALL ? ld4R(a.W.q, a.W.n, T) : ld4R(a.P.q, a.P.n, T);
+  const float* sdf = SET == FDM_SET_WALL ? a.wgeo.sdf : a.pgeo.sdf;
+  const double reachT = SET == FDM_SET_PARTICLE ? tT.radius : tT.boxReach;
   const V3 d = vsub(xA, xT);
   const double rs = (RA + reachT) * kReach + a.skin;
   if (vlen2(d) > rs * rs) return false;
   const V3 local = qrotateT(d, qT);
   const double reach = RA * kReach + a.skin;
   // no node can enter the SDF box, or none can reach phi <= 0 inside it
-  return !outsideBox(local, reach, *tT) && !lipschitzClear(local, reach, *tT, sdf);
+  return !outsideBox(local, reach, tT) && !lipschitzClear(local, reach, tT, sdf);
 }
 
 // Visit the candidate targets of particle A; `emit(T, G)` for each.
@@ -163,15 +170,15 @@ __device__ __forceinline__ bool acceptCandidate(const BroadK& a, int A, V3 xA, d
 template <int SET, class Emit>
 __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emit) {
   const int n = a.P.n;
-  const double RA = tmplOf(a.pgeo, a.P.geo[A]).radius;
+  const double RA = tmplOf(a.pgeo, ldu(a.P.geo + A)).radius;
   // with several instances: A's own grid, and its instance's cells and walls
   const bool many = a.inst.count > 0;
   const int in = many ? a.inst.particleInstance[A] : 0;
   // ghost contacts are detected after the periodic wrap
   const V3 xA = SET == FDM_SET_GHOST
-                    ? wrapPosition(ldp(a.P.x, n, A), many ? a.inst.pgrid[in] : a.grid,
+                    ? wrapPosition(ldpR(a.P.x, n, A), many ? a.inst.pgrid[in] : a.grid,
                                    many ? a.inst.periodic[in] : a.periodic)
-                    : ldp(a.P.x, n, A);
+                    : ldpR(a.P.x, n, A);
   if (SET == FDM_SET_WALL) {
     const fdm_grid& wg = many ? a.inst.wgrid[in] : a.grid;
     const int w0 = many ? a.inst.wallStart[in] : 0;
@@ -179,7 +186,7 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
     for (int w = w0; w < w1; ++w) {
       const int h = a.table.hashValue[w];
       if (h < 0 || (long long)h >= wg.num) continue;
-      if (acceptCandidate<SET>(a, A, xA, RA, w, w, ldp(a.W.x, a.W.n, w))) emit(w, w);
+      if (acceptCandidate<SET>(a, A, xA, RA, w, w, ldpR(a.W.x, a.W.n, w))) emit(w, w);
     }
     return;
   }
@@ -220,10 +227,10 @@ __device__ __forceinline__ void visitCandidates(const BroadK& a, int A, Emit emi
         const int idx = a.table.sortIndex[m];
         if (SET == FDM_SET_PARTICLE) {
           if (idx <= A) continue;
-          if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ldp(a.P.x, n, idx))) emit(idx, idx);
+          if (acceptCandidate<SET>(a, A, xA, RA, idx, idx, ldpR(a.P.x, n, idx))) emit(idx, idx);
         } else {
           const int j = a.ghostReal[idx];
-          if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3(a.ghostPos + 3 * idx)))
+          if (acceptCandidate<SET>(a, A, xA, RA, j, idx, ldd3R(a.ghostPos + 3 * idx)))
             emit(j, idx);
         }
       }
@@ -347,7 +354,7 @@ template <int SET>
 __device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const NarrowSet& S, int A) {
   PairFrame f;
   if (C.inst.count > 0) {
-    const int in = C.inst.particleInstance[A];
+    const int in = ldu(C.inst.particleInstance + A);
     f.pgrid = &C.inst.pgrid[in];
     f.tgrid = SET == FDM_SET_WALL ? &C.inst.wgrid[in] : f.pgrid;
     f.periodic = C.inst.periodic[in];
@@ -364,24 +371,24 @@ __device__ __forceinline__ PairFrame pairFrame(const NarrowCommon& C, const Narr
 // the reference detects after updateGhostSpatialGrid (LSSolver.h:1168-1171).
 template <int SET>
 __device__ __forceinline__ V3 detectPosA(const NarrowCommon& C, const PairFrame& f, int A) {
-  const V3 x = ldp(C.P.x, C.P.n, A);
+  const V3 x = ldpR(C.P.x, C.P.n, A);
   return SET == FDM_SET_GHOST ? wrappedArm(x, *f.pgrid, f.periodic) : x;
 }
 // A's orientation as the set's detection sees it: a sector turn of this
 // step's wrap rotates it for ghost pairs.
 template <int SET>
 __device__ __forceinline__ Q4 detectQA(const NarrowCommon& C, const PairFrame& f, int A) {
-  const Q4 q = ld4(C.P.q, C.P.n, A);
+  const Q4 q = ld4R(C.P.q, C.P.n, A);
   if (SET != FDM_SET_GHOST || !f.periodic.sector) return q;
-  return sectorTurnQ(q, sectorWrap(ldp(C.P.x, C.P.n, A), *f.pgrid, f.periodic).turns, f.periodic);
+  return sectorTurnQ(q, sectorWrap(ldpR(C.P.x, C.P.n, A), *f.pgrid, f.periodic).turns, f.periodic);
 }
 // A particle's velocity as the contact forces see it: after this step's wrap,
 // which turns it in a sector (the force kernels run after
 // updateGhostSpatialGrid, LSSolver.h:1170-1175).
 __device__ __forceinline__ V3 forceVel(const NarrowCommon& C, const PairFrame& f, int i) {
-  const V3 v = ld3(C.P.v, C.P.n, i);
+  const V3 v = ld3R(C.P.v, C.P.n, i);
   if (!f.periodic.sector) return v;
-  return sectorTurnV(v, sectorWrap(ldp(C.P.x, C.P.n, i), *f.pgrid, f.periodic).turns, f.periodic);
+  return sectorTurnV(v, sectorWrap(ldpR(C.P.x, C.P.n, i), *f.pgrid, f.periodic).turns, f.periodic);
 }
 
 // The reference's node predicate for one (node, target) candidate: the
@@ -400,8 +407,8 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
     if (vlen2(rel) > RB * RB) return false;
   }
   int base;
-  if (!sdfLocate(qrotateT(rel, tg.q), *tg.t, s, base)) return false;
-  sdfCorners(tg.sdf + base, *tg.t, s);
+  if (!sdfLocate(qrotateT(rel, tg.q), tg.t, s, base)) return false;
+  sdfCorners(tg.sdf + base, tg.t, s);
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
   if (!(ov >= 0.)) return false;
   const I3 c = clampedCell(p, g);
@@ -419,7 +426,7 @@ __device__ __forceinline__ bool nodeCell(V3 p, const Target& tg, const fdm_grid&
   if (SET == FDM_SET_PARTICLE) {
     if (vlen2(rel) > RB * RB) return false;
   }
-  return sdfLocate(qrotateT(rel, tg.q), *tg.t, s, base);
+  return sdfLocate(qrotateT(rel, tg.q), tg.t, s, base);
 }
 __device__ __forceinline__ bool nodeHitAt(V3 p, const Sample& s, I3 tcell, const fdm_grid& g) {
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
@@ -430,17 +437,19 @@ __device__ __forceinline__ bool nodeHitAt(V3 p, const Sample& s, I3 tcell, const
 
 __device__ __forceinline__ int findPrevPair(const NarrowSet& a, int p, int A, int T, int& off) {
   // lists kept from the previous step hold the pair at the same index
-  if (p < min(*a.prev.total, a.prev.capacity) && a.prev.pairA[p] == A && a.prev.pairT[p] == T) {
-    off = a.prev.recOff[p];
-    return a.prev.recCnt[p];
+  if (p < min(ldu(a.prev.total), a.prev.capacity) && ldu(a.prev.pairA + p) == A &&
+      ldu(a.prev.pairT + p) == T) {
+    off = ldu(a.prev.recOff + p);
+    return ldu(a.prev.recCnt + p);
   }
-  int lo = min(a.prev.count[A], a.prev.capacity), hi = min(a.prev.count[A + 1], a.prev.capacity) - 1;
+  int lo = min(ldu(a.prev.count + A), a.prev.capacity);
+  int hi = min(ldu(a.prev.count + A + 1), a.prev.capacity) - 1;
   while (lo <= hi) {
     const int mid = (lo + hi) >> 1;
-    const int t = a.prev.pairT[mid];
+    const int t = ldu(a.prev.pairT + mid);
     if (t == T) {
-      off = a.prev.recOff[mid];
-      return a.prev.recCnt[mid];
+      off = ldu(a.prev.recOff + mid);
+      return ldu(a.prev.recCnt + mid);
     }
     if (t < T) lo = mid + 1;
     else hi = mid - 1;
@@ -469,38 +478,38 @@ __device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const Narro
                                               const PairFrame& f, Target& tg, V3& armB) {
   const int n = C.P.n;
   if (SET == FDM_SET_WALL) {
-    tg.x = ldp(C.W.x, C.W.n, T);
-    tg.q = ld4(C.W.q, C.W.n, T);
-    tg.t = &tmplOf(C.wgeo, C.W.geo[T]);
+    tg.x = ldpR(C.W.x, C.W.n, T);
+    tg.q = ld4R(C.W.q, C.W.n, T);
+    tg.t = tmplOf(C.wgeo, ldu(C.W.geo + T));
     tg.sdf = C.wgeo.sdf;
-    tg.hash = S.targetHash[T];
+    tg.hash = ldu(S.targetHash + T);
     armB = tg.x;
   } else if (SET == FDM_SET_PARTICLE) {
-    tg.x = ldp(C.P.x, n, T);
-    tg.q = ld4(C.P.q, n, T);
-    tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
+    tg.x = ldpR(C.P.x, n, T);
+    tg.q = ld4R(C.P.q, n, T);
+    tg.t = tmplOf(C.pgeo, ldu(C.P.geo + T));
     tg.sdf = C.pgeo.sdf;
-    tg.hash = S.targetHash[T];
+    tg.hash = ldu(S.targetHash + T);
     armB = wrappedArm(tg.x, *f.pgrid, f.periodic);
   } else {
     if (S.ghostFromReal) {
       // no wrap happened this step: the ghost is the real particle shifted
       // by the periods, as the ghost stage computes it
-      const WrapStages w = wrapStages(ldp(C.P.x, n, T), *f.pgrid, f.periodic);
+      const WrapStages w = wrapStages(ldpR(C.P.x, n, T), *f.pgrid, f.periodic);
       tg.x = ghostOf(S.ghostMask == 1 ? w.afterX : w.afterY, *f.pgrid, S.ghostMask);
       tg.hash = ghostCellHash(tg.x, *f.pgrid);
     } else {
-      const int G = S.pairs.pairG[p];
-      tg.x = ldd3(S.ghostPos + 3 * G);
-      tg.hash = S.ghostHash[G];
+      const int G = ldu(S.pairs.pairG + p);
+      tg.x = ldd3R(S.ghostPos + 3 * G);
+      tg.hash = ldu(S.ghostHash + G);
     }
     if (S.ghostQ) {
       const double* gq = S.ghostQ + 4 * (size_t)S.pairs.pairG[p];
       tg.q = {gq[0], gq[1], gq[2], gq[3]};
     } else {
-      tg.q = ld4(C.P.q, n, T);
+      tg.q = ld4R(C.P.q, n, T);
     }
-    tg.t = &tmplOf(C.pgeo, C.P.geo[T]);
+    tg.t = tmplOf(C.pgeo, ldu(C.P.geo + T));
     tg.sdf = C.pgeo.sdf;
     armB = tg.x;
   }
@@ -538,7 +547,7 @@ struct CandSlots {
 template <int SET, bool CULL = false>
 __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, int lane,
                            HitSlots& hs, int pool) {
-  const int A = S.pairs.pairA[p];
+  const int A = ldu(S.pairs.pairA + p);
   if (A < 0) {  // slot left empty by a capacity overflow
     if (lane == 0) {
       S.pairs.recOff[p] = 0;
@@ -546,9 +555,9 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     }
     return;
   }
-  const int T = S.pairs.pairT[p];
+  const int T = ldu(S.pairs.pairT + p);
   const int n = C.P.n;
-  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const fdm_template& tA = tmplOf(C.pgeo, ldu(C.P.geo + A));
   const PairFrame fr = pairFrame<SET>(C, S, A);
   const Q4 qA = detectQA<SET>(C, fr, A);
   const V3 xA = detectPosA<SET>(C, fr, A);
@@ -558,13 +567,13 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
   if (CULL) {
     const double RA = tA.radius * kReach;
-    const double reachT = SET == FDM_SET_PARTICLE ? tg.t->radius : tg.t->boxReach;
+    const double reachT = SET == FDM_SET_PARTICLE ? tg.t.radius : tg.t.boxReach;
     const V3 d = vsub(xA, tg.x);
     const double rs = RA + reachT * kReach;
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
       const V3 local = qrotateT(d, tg.q);
-      keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
+      keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
     }
     if (!keep) {
       if (lane == 0) {
@@ -575,7 +584,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     }
   }
   const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
-  const double RB = tg.t->radius;
+  const double RB = tg.t.radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
   const fdm_real* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
   const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
@@ -601,7 +610,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     }
     if (live) {
       const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
-      if (outsideBox(lc, rc, *tg.t) || lipschitzClear(lc, rc, *tg.t, tg.sdf)) live = false;
+      if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
   const unsigned long long liveChunks = ballot64(live);
@@ -681,35 +690,35 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   if (total > 0) {
     // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
-    const int mi = C.P.mat[A];
-    const fdm_material& Mi = C.pmat[mi];
+    const int mi = ldu(C.P.mat + A);
+    const fdm_material Mi = matOf(C.pmat, mi);
     const double invMi = tA.invMass;
     double kn = 0., ks = 0., effM = 0., mu, diss;
     V3 vB, wB;
     if (SET == FDM_SET_WALL) {
-      const fdm_material& Mw = C.wmat[C.W.mat[T]];
+      const fdm_material Mw = matOf(C.wmat, ldu(C.W.mat + T));
       if (invMi > 0.) effM = 1. / invMi;
       mu = fmin(Mi.mu, Mw.mu);
       diss = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
       kn = Mi.kn;
       ks = Mi.ks;
-      vB = ld3(C.W.v, C.W.n, T);
-      wB = ld3(C.W.w, C.W.n, T);
+      vB = ld3R(C.W.v, C.W.n, T);
+      wB = ld3R(C.W.w, C.W.n, T);
     } else {
-      const fdm_material& Mj = C.pmat[C.P.mat[T]];
-      const double invMj = tg.t->invMass;
+      const fdm_material Mj = matOf(C.pmat, ldu(C.P.mat + T));
+      const double invMj = tg.t.invMass;
       if (Mi.kn > 0. && Mj.kn > 0.) kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
       if (Mi.ks > 0. && Mj.ks > 0.) ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
       if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
       diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
       mu = fmin(Mi.mu, Mj.mu);
       vB = SET == FDM_SET_GHOST && S.ghostVel
-               ? ldd3(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
+               ? ldd3R(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
                : forceVel(C, fr, T);
-      wB = ld3(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
+      wB = ld3R(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
     }
     const V3 vA = forceVel(C, fr, A);
-    const V3 wA = ld3(C.P.w, n, A);
+    const V3 wA = ld3R(C.P.w, n, A);
     // contact law of one contact: its spring is updated in place and stored
     // with its node id as record r
     auto law = [&](V3 pc, V3 nrm, double ov, int k, int r, V3& F, V3& TA, V3& TB) {
@@ -859,10 +868,10 @@ __device__ __forceinline__ int reservePooled(int* counters, int capacity, int po
 template <int SET, bool CULL = false, int NPL = 1>
 __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
                           int p, int lane, HitSlots& hs, int pool, CandSlots* cs = nullptr) {
-  const int A = S.pairs.pairA[p];
-  const int T = S.pairs.pairT[p];
+  const int A = ldu(S.pairs.pairA + p);
+  const int T = ldu(S.pairs.pairT + p);
   const int n = C.P.n;
-  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const fdm_template& tA = tmplOf(C.pgeo, ldu(C.P.geo + A));
   const PairFrame fr = pairFrame<SET>(C, S, A);
   const Q4 qA = detectQA<SET>(C, fr, A);
   const V3 xA = detectPosA<SET>(C, fr, A);
@@ -872,13 +881,13 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
   if (CULL) {
     const double RA = tA.radius * kReach;
-    const double reachT = SET == FDM_SET_PARTICLE ? tg.t->radius : tg.t->boxReach;
+    const double reachT = SET == FDM_SET_PARTICLE ? tg.t.radius : tg.t.boxReach;
     const V3 d = vsub(xA, tg.x);
     const double rs = RA + reachT * kReach;
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
       const V3 local = qrotateT(d, tg.q);
-      keep = !outsideBox(local, RA, *tg.t) && !lipschitzClear(local, RA, *tg.t, tg.sdf);
+      keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
     }
     if (!keep) {
       if (lane == 0) {
@@ -888,7 +897,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       return;
     }
   }
-  const double RB = tg.t->radius;
+  const double RB = tg.t.radius;
   const I3 tcell = decodeHash(tg.hash, tgrid);
   const fdm_real* nodes = C.pgeo.nodes + 3 * (size_t)tA.nodeStart;
   const float4* chunks = reinterpret_cast<const float4*>(C.pgeo.chunks) + tA.chunkStart;
@@ -912,7 +921,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     }
     if (live) {
       const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
-      if (outsideBox(lc, rc, *tg.t) || lipschitzClear(lc, rc, *tg.t, tg.sdf)) live = false;
+      if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
   const unsigned long long liveChunks = ballot64(live);
@@ -950,7 +959,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
         sm.y = cs->fy[lane];
         sm.z = cs->fz[lane];
         k = cs->node[lane];
-        sdfCorners(tg.sdf + cs->base[lane], *tg.t, sm);
+        sdfCorners(tg.sdf + cs->base[lane], tg.t, sm);
         hit = nodeHitAt(pn, sm, tcell, tgrid);
       }
       const unsigned long long m = ballot64(hit);
@@ -1032,8 +1041,8 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       int b1 = 0, b2 = 0;
       bool in1 = has && nodeCell<SET>(pn, tg, tgrid, RB, sm, b1);
       bool in2 = has2 && nodeCell<SET>(pn2, tg, tgrid, RB, sm2, b2);
-      if (in1) sdfCorners(tg.sdf + b1, *tg.t, sm);
-      if (in2) sdfCorners(tg.sdf + b2, *tg.t, sm2);
+      if (in1) sdfCorners(tg.sdf + b1, tg.t, sm);
+      if (in2) sdfCorners(tg.sdf + b2, tg.t, sm2);
       const bool hit = in1 && nodeHitAt(pn, sm, tcell, tgrid);
       const bool hit2 = in2 && nodeHitAt(pn2, sm2, tcell, tgrid);
       const unsigned long long m = ballot64(hit), m2 = ballot64(hit2);
@@ -1162,10 +1171,10 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
   if (cnt <= 0) return;
   const int base = S.pairs.recOff[p];
   const int gbase = X.workHit[w];
-  const int A = S.pairs.pairA[p];
-  const int T = S.pairs.pairT[p];
+  const int A = ldu(S.pairs.pairA + p);
+  const int T = ldu(S.pairs.pairT + p);
   const int n = C.P.n;
-  const fdm_template& tA = tmplOf(C.pgeo, C.P.geo[A]);
+  const fdm_template& tA = tmplOf(C.pgeo, ldu(C.P.geo + A));
   const PairFrame fr = pairFrame<SET>(C, S, A);
   const V3 xA = detectPosA<SET>(C, fr, A);
   Target tg;
@@ -1175,33 +1184,33 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
   int prevOff = 0;
   const int prevCnt = findPrevPair(S, p, A, T, prevOff);
   // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
-  const fdm_material& Mi = C.pmat[C.P.mat[A]];
+  const fdm_material Mi = matOf(C.pmat, ldu(C.P.mat + A));
   const double invMi = tA.invMass;
   double kn = 0., ks = 0., effM = 0., mu, diss;
   V3 vB, wB;
   if (SET == FDM_SET_WALL) {
-    const fdm_material& Mw = C.wmat[C.W.mat[T]];
+    const fdm_material Mw = matOf(C.wmat, ldu(C.W.mat + T));
     if (invMi > 0.) effM = 1. / invMi;
     mu = fmin(Mi.mu, Mw.mu);
     diss = Mi.e <= Mw.e ? Mi.dissipation : Mw.dissipation;
     kn = Mi.kn;
     ks = Mi.ks;
-    vB = ld3(C.W.v, C.W.n, T);
-    wB = ld3(C.W.w, C.W.n, T);
+    vB = ld3R(C.W.v, C.W.n, T);
+    wB = ld3R(C.W.w, C.W.n, T);
   } else {
-    const fdm_material& Mj = C.pmat[C.P.mat[T]];
-    const double invMj = tg.t->invMass;
+    const fdm_material Mj = matOf(C.pmat, ldu(C.P.mat + T));
+    const double invMj = tg.t.invMass;
     if (Mi.kn > 0. && Mj.kn > 0.) kn = Mi.kn * Mj.kn / (Mi.kn + Mj.kn);
     if (Mi.ks > 0. && Mj.ks > 0.) ks = Mi.ks * Mj.ks / (Mi.ks + Mj.ks);
     if (invMi > 0. || invMj > 0.) effM = 1. / (invMi + invMj);
     diss = Mi.e <= Mj.e ? Mi.dissipation : Mj.dissipation;
     mu = fmin(Mi.mu, Mj.mu);
-    vB = SET == FDM_SET_GHOST && S.ghostVel ? ldd3(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
+    vB = SET == FDM_SET_GHOST && S.ghostVel ? ldd3R(S.ghostVel + 3 * (size_t)S.pairs.pairG[p])
                                             : forceVel(C, fr, T);
-    wB = ld3(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
+    wB = ld3R(C.P.w, n, T);  // a sector image keeps the real spin (contactKernel.cu:258)
   }
   const V3 vA = forceVel(C, fr, A);
-  const V3 wA = ld3(C.P.w, n, A);
+  const V3 wA = ld3R(C.P.w, n, A);
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
   for (int h = 0; h < cnt; ++h) {
     const double* g = X.hitGeom + 8 * (size_t)(gbase + h);

@@ -6,6 +6,7 @@
 // compiled with -ffp-contract=off, so no multiply-add is fused and each FP64
 // operation rounds exactly once, as on the host.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdint>
@@ -99,6 +100,26 @@ __device__ __forceinline__ S6 srotate(Q4 q, S6 I) {
                r2z = I.xz * g + I.yz * h + I.zz * i;
   return {r0x * a + r0y * b + r0z * c, r1x * d + r1y * e + r1z * f, r2x * g + r2y * h + r2z * i,
           r0x * d + r0y * e + r0z * f, r0x * g + r0y * h + r0z * i, r1x * g + r1y * h + r1z * i};
+}
+
+// A load through the constant address space, for data that no lane writes
+// while the kernel runs: with a wave-uniform address the compiler issues it
+// as a scalar load (into SGPRs), otherwise as an ordinary vector load.
+// Structures are read word by word (the words no field in use needs are
+// dropped by the compiler).
+template <class T>
+__device__ __forceinline__ T ldu(const T* p) {
+  if constexpr (std::is_arithmetic<T>::value) {
+    return *(const __attribute__((address_space(4))) T*)p;
+  } else {
+    static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "word-sized POD");
+    const __attribute__((address_space(4))) unsigned* w = (const __attribute__((address_space(4))) unsigned*)p;
+    T r;
+    unsigned* d = reinterpret_cast<unsigned*>(&r);
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) d[k] = w[k];
+    return r;
+  }
 }
 
 // ---------------------------------------------------------------- state I/O

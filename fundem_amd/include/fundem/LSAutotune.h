@@ -56,6 +56,14 @@ class LSAutotuner {
   TuneChoice run(size_t trialSteps = 20, bool tuneCellSize = true) {
     if (s_.particles_.num_device() == 0) throw std::logic_error("initialise the solver first");
     trialSteps = std::max<size_t>(trialSteps, 4);
+    // candidates are timed on eager launches: a cell-size candidate
+    // re-initialises the solver, and every candidate re-captures its graphs
+    struct GraphsBack {
+      LSSolver& s;
+      bool on;
+      ~GraphsBack() { s.useGraph_ = on; }
+    } graphsBack{s_, s_.useGraph_};
+    s_.useGraph_ = false;
     snap_.mark();
     TuneChoice best;
     best.launch = s_.launchTuning();
