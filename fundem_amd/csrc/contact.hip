@@ -45,10 +45,17 @@ constexpr double kReach = 1.0 + 1e-5;
 struct Target {
   V3 x;       // detection position
   Q4 q;
+  bool ident;  // q is exactly (1, 0, 0, 0): the rotations return their input
   fdm_template t;
   const float* sdf;
   int hash;   // hash value in the target grid
 };
+// Into and out of the target's frame. A rotation by the exact identity
+// quaternion returns its input (up to the sign of a zero component), so it
+// is skipped; the test is wave-uniform in the sweep.
+__device__ __forceinline__ V3 toLocal(V3 v, const Target& tg) { return tg.ident ? v : qrotateT(v, tg.q); }
+__device__ __forceinline__ V3 toWorld(V3 v, const Target& tg) { return tg.ident ? v : qrotate(tg.q, v); }
+
 
 // Every kernel of this file only reads body state, templates, materials,
 // candidate lists and the previous step's records: they are read through the
@@ -407,7 +414,7 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
     if (vlen2(rel) > RB * RB) return false;
   }
   int base;
-  if (!sdfLocate(qrotateT(rel, tg.q), tg.t, s, base)) return false;
+  if (!sdfLocate(toLocal(rel, tg), tg.t, s, base)) return false;
   sdfCorners(tg.sdf + base, tg.t, s);
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
   if (!(ov >= 0.)) return false;
@@ -426,7 +433,7 @@ __device__ __forceinline__ bool nodeCell(V3 p, const Target& tg, const fdm_grid&
   if (SET == FDM_SET_PARTICLE) {
     if (vlen2(rel) > RB * RB) return false;
   }
-  return sdfLocate(qrotateT(rel, tg.q), tg.t, s, base);
+  return sdfLocate(toLocal(rel, tg), tg.t, s, base);
 }
 __device__ __forceinline__ bool nodeHitAt(V3 p, const Sample& s, I3 tcell, const fdm_grid& g) {
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
@@ -513,6 +520,7 @@ __device__ __forceinline__ void resolveTarget(const NarrowCommon& C, const Narro
     tg.sdf = C.pgeo.sdf;
     armB = tg.x;
   }
+  tg.ident = tg.q.q0 == 1.0 && tg.q.q1 == 0.0 && tg.q.q2 == 0.0 && tg.q.q3 == 0.0;
 }
 
 // Contacts staged per wave in LDS between the node sweep and the contact law.
@@ -572,7 +580,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     const double rs = RA + reachT * kReach;
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
-      const V3 local = qrotateT(d, tg.q);
+      const V3 local = toLocal(d, tg);
       keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
     }
     if (!keep) {
@@ -609,7 +617,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
       if (!cellNear(tcell, lo, hi)) live = false;
     }
     if (live) {
-      const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
+      const V3 lc = toLocal(vsub(cw, tg.x), tg);
       if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
@@ -659,7 +667,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
       const int slot = total + __popcll(m & lt);
       if (slot < kMaxHits) {
         const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
         const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
         hs.pc[0][slot] = pc.x;
         hs.pc[1][slot] = pc.y;
@@ -773,7 +781,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
           Sample sm;
           nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
           const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-          const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+          const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
           const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
           law(pc, nrm, ov, k, base + running + __popcll(m & lt), F, TA, TB);
         }
@@ -886,7 +894,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     const double rs = RA + reachT * kReach;
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
-      const V3 local = qrotateT(d, tg.q);
+      const V3 local = toLocal(d, tg);
       keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
     }
     if (!keep) {
@@ -920,7 +928,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       if (!cellNear(tcell, lo, hi)) live = false;
     }
     if (live) {
-      const V3 lc = qrotateT(vsub(cw, tg.x), tg.q);
+      const V3 lc = toLocal(vsub(cw, tg.x), tg);
       if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
@@ -931,7 +939,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   // a hit's contact geometry, into LDS slot `slot` (node order)
   auto stage = [&](int slot, int k, V3 pn, const Sample& sm) {
     const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-    const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+    const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
     const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
     hs.pc[0][slot] = pc.x;
     hs.pc[1][slot] = pc.y;
@@ -1095,7 +1103,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       const int slot = total + __popcll(m & lt);
       if (slot < kMaxHits) {
         const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
         const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
         hs.pc[0][slot] = pc.x;
         hs.pc[1][slot] = pc.y;
@@ -1142,7 +1150,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
         Sample sm;
         nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
         const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
-        const V3 nrm = vnormalize(qrotate(tg.q, gradientIntX(sm.x, sm.y, sm.z, sm.c)));
+        const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
         const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
         stageHit(X.hitGeom + 8 * (size_t)(gbase + running + __popcll(m & lt)), pc, nrm, ov, k);
       }
