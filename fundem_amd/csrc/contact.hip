@@ -832,6 +832,11 @@ __global__ void __launch_bounds__(256) kNarrow(NarrowK a) {
 
 struct NarrowScratch {
   int* workHit;      // [work item] first staged contact
+  // [8 * work item] or null: what the law kernel needs first, written by the
+  // sweep so that the law's thread starts with one load instead of a chain:
+  // record count, first record, first staged contact, A, target, and the
+  // previous step's record range of the pair
+  int* lawHead;
   double* hitGeom;   // [8 * hitCapacity]: contact point, normal, overlap, node id
   int hitCapacity;
   int* hitCount;     // [FDM_REC_COUNTERS] pool counters of the staged contacts
@@ -895,12 +900,16 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
       const V3 local = toLocal(d, tg);
-      keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
+      // (no Lipschitz test at the pair level here: it would put an SDF fetch
+      // before the chunk culls, whose own Lipschitz tests reject the same
+      // pairs with their fetches in flight together)
+      keep = !outsideBox(local, RA, tg.t);
     }
     if (!keep) {
       if (lane == 0) {
         S.pairs.recOff[p] = 0;
         S.pairs.recCnt[p] = 0;
+        if (X.lawHead) *reinterpret_cast<int4*>(X.lawHead + 8 * (size_t)w) = make_int4(0, 0, 0, 0);
       }
       return;
     }
@@ -933,6 +942,10 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     }
   }
   const unsigned long long liveChunks = ballot64(live);
+  // the previous step's record range of the pair, for the law's header (its
+  // scalar loads complete while the nodes are swept)
+  int prevOff = 0;
+  const int prevCnt = X.lawHead && liveChunks ? findPrevPair(S, p, A, T, prevOff) : 0;
 
   unsigned long long myMask = 0ull;
   int total = 0;
@@ -1161,6 +1174,11 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     S.pairs.recOff[p] = base;
     S.pairs.recCnt[p] = total;
     X.workHit[w] = gbase;
+    if (X.lawHead) {
+      int4* h = reinterpret_cast<int4*>(X.lawHead + 8 * (size_t)w);
+      h[0] = make_int4(total, base, gbase, A);
+      if (total > 0) h[1] = make_int4(T, prevOff, prevCnt, 0);
+    }
     if (SET != FDM_SET_WALL && total > 0 && T != A) {
       const int pos = atomicAdd(&S.mirror.count[T], 1);
       if (pos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + pos] = p;
@@ -1175,12 +1193,27 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
 template <int SET>
 __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X, int w,
                         int p) {
-  const int cnt = S.pairs.recCnt[p];
-  if (cnt <= 0) return;
-  const int base = S.pairs.recOff[p];
-  const int gbase = X.workHit[w];
-  const int A = ldu(S.pairs.pairA + p);
-  const int T = ldu(S.pairs.pairT + p);
+  int cnt, base, gbase, A, T, prevOff = 0, prevCnt;
+  if (X.lawHead) {
+    const int4 h0 = ldu(reinterpret_cast<const int4*>(X.lawHead + 8 * (size_t)w));
+    cnt = h0.x;
+    if (cnt <= 0) return;
+    const int4 h1 = ldu(reinterpret_cast<const int4*>(X.lawHead + 8 * (size_t)w + 4));
+    base = h0.y;
+    gbase = h0.z;
+    A = h0.w;
+    T = h1.x;
+    prevOff = h1.y;
+    prevCnt = h1.z;
+  } else {
+    cnt = S.pairs.recCnt[p];
+    if (cnt <= 0) return;
+    base = S.pairs.recOff[p];
+    gbase = X.workHit[w];
+    A = ldu(S.pairs.pairA + p);
+    T = ldu(S.pairs.pairT + p);
+    prevCnt = findPrevPair(S, p, A, T, prevOff);
+  }
   const int n = C.P.n;
   const fdm_template& tA = tmplOf(C.pgeo, ldu(C.P.geo + A));
   const PairFrame fr = pairFrame<SET>(C, S, A);
@@ -1189,8 +1222,6 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
   V3 armB;
   resolveTarget<SET>(C, S, p, T, fr, tg, armB);
   const V3 armA = SET == FDM_SET_GHOST ? xA : wrappedArm(xA, *fr.pgrid, fr.periodic);
-  int prevOff = 0;
-  const int prevCnt = findPrevPair(S, p, A, T, prevOff);
   // material constants of the pair (contactKernel.cu:79-110,167-193,243-274)
   const fdm_material Mi = matOf(C.pmat, ldu(C.P.mat + A));
   const double invMi = tA.invMass;
@@ -1594,6 +1625,7 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   for (int i = nsets; i < kMaxSets; ++i) k.s[i] = k.s[0];
   NarrowScratch x;
   x.workHit = ws->workHit;
+  x.lawHead = ws->workList;
   x.hitGeom = ws->hitGeom;
   x.hitCapacity = ws->hitCapacity;
   x.hitCount = ws->hitCount;

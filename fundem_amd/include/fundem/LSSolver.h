@@ -342,6 +342,7 @@ class LSSolver {
         if (periodic_.active(s)) cap += std::max<long long>(4 * (long long)N, 1024);
       if (cap >= (1ll << 28)) throw std::invalid_argument("contact capacities too large");
       workHit_.allocate((size_t)cap, false);
+      lawHead_.allocate(8 * (size_t)cap, false);
       // staged contacts of one step, all sets together: room for every
       // active set's records
       int activeSets = 1 + (W ? 1 : 0);
@@ -1313,7 +1314,7 @@ class LSSolver {
     {
       Timer t(this, "narrow");
       fdm_narrow_workspace w;
-      w.workList = nullptr;
+      w.workList = lawHead_.get();
       w.workCapacity = (int)workHit_.size();
       w.workCount = nullptr;
       w.workHit = workHit_.get();
@@ -1630,7 +1631,7 @@ class LSSolver {
   DeviceBuffer<fdm_grid> instPGrid_, instWGrid_;
   DeviceBuffer<long long> instPOff_, instWOff_;
   DeviceBuffer<fdm_periodic> instPeriodic_;
-  DeviceBuffer<int> workHit_, hitCount_;
+  DeviceBuffer<int> workHit_, hitCount_, lawHead_;
   DeviceBuffer<double> hitGeom_;
   int hitCapacity_ = 0;
   fdm_tuning tuning_{};

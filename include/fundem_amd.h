@@ -580,7 +580,10 @@ typedef struct {
 int fdm_launchNarrowPhase(const fdm_narrowphase_args* args, hipStream_t stream);
 /* Scratch of fdm_launchNarrowPhaseSets. */
 typedef struct {
-  int* workList;               /* unused (may be NULL); kept for the layout */
+  int* workList;               /* [8 * workCapacity] or NULL: per candidate pair, the header
+                                  the sweep writes for the contact law (record count, first
+                                  record, first staged contact, A, target, previous record
+                                  range), so that the law starts from one load */
   int workCapacity;            /* at least the sum of the sets' pair capacities */
   int* workCount;              /* unused (may be NULL); kept for the layout */
   int* workHit;                /* [workCapacity] first staged contact of each candidate pair */
