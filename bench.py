@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--settle", type=int, default=20000,
                    help="column: plain steps that pack the column before the timed runs")
     p.add_argument("--checkpoint", type=int, default=100, help="column: steps per checkpoint")
+    p.add_argument("--graph-compare", type=int, default=1,
+                   help="column: also time the forward step replayed from captured graphs")
     p.add_argument("--workload", choices=("pour", "dense", "batch", "column", "meshsdf"),
                    default="pour",
                    help="pour: the 1M-particle pour; batch: 512 packings in one solver against "
@@ -92,8 +94,9 @@ def log(*a):
 #   step_begin   (kStepBegin alone; the decision kernel is timed as "decide")
 #                per particle: read x 24, v 12, w 12, q 16, F 12, T 12,
 #                template id 4, grid state at build 4, visibility at build 4
-#                (= 100); write x 24, v 12, w 12, q 16, cell id 4, sort key
-#                4, index 4, three visibility flags 12 (= 88)
+#                (= 100); write x 24, v 12, w 12, q 16, cell id 4 (= 68);
+#                sort lists and visibility flags are written by list builds
+#                only
 #   reduce_kick  per particle: pair counts read (particle set 8, wall 8,
 #                three ghost sets 24, mirror counts 16), F T write 24,
 #                template id 4, v w read and write 48, q read 16 (= 148);
@@ -106,7 +109,7 @@ def log(*a):
 #                both bodies 48, history read 16 and write 16 (= 80)
 def kernel_bytes(stage, n, pairs, contacts):
     if stage == "step_begin":
-        return 188.0 * n
+        return 168.0 * n
     if stage == "reduce_kick":
         return 148.0 * n + 104.0 * sum(pairs.values())
     if stage == "narrow":
@@ -285,10 +288,12 @@ def column_main(a):
     fwd = run_steps(f, steps, a.warmup, False) / steps
     contacts = [int(f.contacts(k)["A"].size) for k in (0, 1)]
     del f
-    f = Solver(0)
-    W.apply(sc, f)
-    fwd_graph = run_steps(f, steps, a.warmup, True) / steps
-    del f
+    fwd_graph = None
+    if a.graph_compare:
+        f = Solver(0)
+        W.apply(sc, f)
+        fwd_graph = run_steps(f, steps, a.warmup, True) / steps
+        del f
     g = Solver(0)
     W.apply(sc, g)
     adj = Adjoint(g, contacts_per_particle=16)
@@ -308,7 +313,8 @@ def column_main(a):
     t3 = time.perf_counter()
     fwd_ckpt = (t1 - t0) / steps
     back = (t3 - t2) / steps
-    log(f"forward {1e3 * fwd:.3f} ms/step (captured graphs {1e3 * fwd_graph:.3f}), forward with "
+    graph_note = "" if fwd_graph is None else f" (captured graphs {1e3 * fwd_graph:.3f})"
+    log(f"forward {1e3 * fwd:.3f} ms/step{graph_note}, forward with "
         f"checkpoints {1e3 * fwd_ckpt:.3f} ms/step, "
         f"adjoint {1e3 * back:.3f} ms/step (ratio {back / fwd:.2f}); contacts at the last step "
         f"{contacts}; dJ/dkn {grad['kn'][0]:.4e} dJ/dzeta {grad['zeta'][0]:.4e} dJ/dmu {grad['mu'][0]:.4e}")
@@ -322,7 +328,8 @@ def column_main(a):
                                f"({len([1 for m in masses if m == 0.0])} fixed bodies), packed by "
                                f"{a.settle} settling steps, dt 1e-6 s, kn 1e4, zeta 0.3, mu 0.5; "
                                f"checkpoint every {a.checkpoint} steps; J = sum of |v|^2 / 2 over the moving bodies",
-                   "forward_ms_per_step": 1e3 * fwd, "forward_graph_ms_per_step": 1e3 * fwd_graph,
+                   "forward_ms_per_step": 1e3 * fwd,
+                   "forward_graph_ms_per_step": None if fwd_graph is None else 1e3 * fwd_graph,
                    "forward_with_checkpoints_ms_per_step":
                    1e3 * fwd_ckpt, "contacts_last_step": contacts},
     }
@@ -592,8 +599,8 @@ def main():
         copy_probe()
     out, templates = pour_main(a)
     if a.extras:
-        # BASELINE's other two metrics and the contact-dense state, as extra
-        # keys of the same line
+        # the batch speedup, the adjoint's cost and the contact-dense state,
+        # as extra keys of the same line
         out["dense"] = dense_main(a, templates)
         ab = argparse.Namespace(**vars(a))
         ab.steps = max(a.steps, 100)
