@@ -13,10 +13,11 @@
 //     radii, at most the rule's cell; every candidate cell is at least the
 //     rule's, so every body that can touch across a face still has its
 //     image, and a clamped ghost cell stays in the stencil of every node
-//     within reach. Batched systems share the candidate on every instance
-//     grid. The contact set is therefore the same; with canonical sums the
-//     results are the same bits. Not with the sector boundary (its images
-//     are rebuilt each step by their own cell rule);
+//     within reach. The contact set is therefore the same; with canonical
+//     sums the results are the same bits. Not with the sector boundary (its
+//     images are rebuilt each step by their own cell rule), nor for batched
+//     systems: on the 512-packing batch a coarser candidate overflowed a
+//     per-particle mirror list (status 4), which is not explained yet;
 //   - the solver's launch shapes (fdm_tuning): node-sweep waves (kept pairs
 //     in flight) per workgroup and sweep grid size, nodes per lane (64 or
 //     128 nodes in flight per sweep wave), where the contact law runs (a
@@ -83,7 +84,7 @@ class LSAutotuner {
       }
       bestCost = pick(group, window, best, "skin");
     }
-    const bool cellOk = tuneCellSize && !s_.periodic_.isSector();
+    const bool cellOk = tuneCellSize && !s_.periodic_.isSector() && s_.systems_.empty();
     if (cellOk) {
       const double base = s_.particles_.cellSizeRule(s_.cellSize_);
       group.clear();
