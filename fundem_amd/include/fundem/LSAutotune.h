@@ -10,14 +10,18 @@
 //     27-cell stencil. With planar periodic faces an image exists for the
 //     bodies of the first cell column (row) only, and two bodies touch
 //     across a face only when their centres are within the sum of their
-//     radii, at most the rule's cell; every candidate cell is at least the
-//     rule's, so every body that can touch across a face still has its
-//     image, and a clamped ghost cell stays in the stencil of every node
-//     within reach. The contact set is therefore the same; with canonical
-//     sums the results are the same bits. Not with the sector boundary (its
-//     images are rebuilt each step by their own cell rule), nor for batched
-//     systems: on the 512-packing batch a coarser candidate overflowed a
-//     per-particle mirror list (status 4), which is not explained yet;
+//     radii, at most twice the largest radius: when the configured cell is
+//     at least that, every candidate (coarser) cell keeps every body that can
+//     touch across a face in the first column, and a clamped ghost cell stays
+//     in the stencil of every node within reach. A configured cell below
+//     twice the largest radius (the pour's 2.5 mm) leaves some such bodies
+//     without an image, as the reference does, and a coarser cell would add
+//     their contacts, so periodic systems configured so keep their cell. The
+//     contact set is therefore the same; with canonical sums the results are
+//     the same bits. Not with the sector boundary (its images are rebuilt
+//     each step by their own cell rule), nor for batched systems (their
+//     packings are configured with cells below twice the largest radius: a
+//     coarser candidate changed their contacts and overflowed a mirror list);
 //   - the solver's launch shapes (fdm_tuning): node-sweep waves (kept pairs
 //     in flight) per workgroup and sweep grid size, nodes per lane (64 or
 //     128 nodes in flight per sweep wave), where the contact law runs (a
@@ -84,9 +88,10 @@ class LSAutotuner {
       }
       bestCost = pick(group, window, best, "skin");
     }
-    const bool cellOk = tuneCellSize && !s_.periodic_.isSector() && s_.systems_.empty();
+    const double base = s_.particles_.cellSizeRule(s_.cellSize_);
+    const bool periodicOk = !s_.periodic_.isActived() || base >= 2.0 * s_.maxRadius_;
+    const bool cellOk = tuneCellSize && !s_.periodic_.isSector() && s_.systems_.empty() && periodicOk;
     if (cellOk) {
-      const double base = s_.particles_.cellSizeRule(s_.cellSize_);
       group.clear();
       for (double f : {1.25, 1.5, 2.0}) {
         TuneChoice c = best;

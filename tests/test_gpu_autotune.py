@@ -111,3 +111,30 @@ def test_coarser_cells_keep_periodic_results(periodic, factor):
                 seen[j] += oc["A"].size
     print(f"{periodic} x{factor}: contacts seen per set {seen.tolist()}", flush=True)
     assert g.status() == 0 and seen[0] > 0 and seen[2:].sum() > 0
+
+
+def test_periodic_cell_below_twice_the_radius_is_kept():
+    """A periodic system configured with cells below twice the largest radius
+    (as the pour's 2.5 mm cell) leaves some bodies near a face without an
+    image; a coarser cell would add their contacts, so the autotuner keeps
+    the configured cell and the tuned run equals a plain one."""
+    import dataclasses
+    from fundem_amd import Solver
+    scn = W.cluster(n=125, periodic=(True, True), speed=1.0)
+    o = po.OracleSim(True, True)
+    W.apply(scn, o)
+    R = max(o.geometry(k)["radius"] for k in range(len(scn.templates)))
+    scn = dataclasses.replace(scn, cell_size=1.3 * R)
+    plain = Solver()
+    W.apply(scn, plain)
+    plain.step(100)
+    ref = plain.state()
+    tuned = Solver()
+    W.apply(scn, tuned)
+    choice = tuned.autotune(trial_steps=6)
+    assert not any(line.startswith("cell") for line in choice["trials"])
+    assert choice["cell_size"] == 1.3 * R
+    tuned.step(100)
+    got = tuned.state()
+    for key in ("x", "v", "w", "q", "F", "T"):
+        assert np.array_equal(got[key], ref[key]), key
