@@ -1251,14 +1251,34 @@ __device__ void lawPair(const NarrowCommon& C, const NarrowSet& S, const NarrowS
   const V3 vA = forceVel(C, fr, A);
   const V3 wA = ld3R(C.P.w, n, A);
   V3 SF = {0., 0., 0.}, STA = {0., 0., 0.}, STB = {0., 0., 0.};
+  // The pair's previous records are in node order, as its staged contacts
+  // are: walk both in step rather than search per contact; and load the next
+  // contact's staged geometry one iteration ahead.
+  int j = prevOff;
+  const int jEnd = prevOff + prevCnt;
+  fdm_record pr{};
+  if (j < jEnd) pr = ldu(S.prev.rec + j);
+  double gn[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) gn[c] = ldu(X.hitGeom + 8 * (size_t)gbase + c);
   for (int h = 0; h < cnt; ++h) {
-    const double* g = X.hitGeom + 8 * (size_t)(gbase + h);
+    double g[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) g[c] = gn[c];
+    if (h + 1 < cnt) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) gn[c] = ldu(X.hitGeom + 8 * (size_t)(gbase + h + 1) + c);
+    }
     const V3 pc = {g[0], g[1], g[2]};
     const V3 nrm = {g[3], g[4], g[5]};
     const double ov = g[6];
     const int k = (int)g[7];
     V3 F = {0., 0., 0.};
-    V3 spring = findPrevSpring(S, prevOff, prevCnt, k);
+    while (j < jEnd && pr.node < k) {
+      ++j;
+      if (j < jEnd) pr = ldu(S.prev.rec + j);
+    }
+    V3 spring = j < jEnd && pr.node == k ? recordSpring(pr) : V3{0., 0., 0.};
     if (C.updateSprings) {
       const V3 vc = vsub(vadd(vA, vcross(wA, vsub(pc, armA))), vadd(vB, vcross(wB, vsub(pc, armB))));
       linearContact(F, spring, vc, nrm, ov, C.dt, kn, ks, mu, diss, effM);
