@@ -102,6 +102,15 @@ __device__ __forceinline__ S6 srotate(Q4 q, S6 I) {
           r0x * d + r0y * e + r0z * f, r0x * g + r0y * h + r0z * i, r1x * g + r1y * h + r1z * i};
 }
 
+// A template whose inverse inertia is exactly zero (the reference's inverse
+// returns zero below |det I| = 1e-30, e.g. for millimetre particles): the
+// angular half-kick then adds an exact zero to the spin, so it is skipped
+// (the spin keeps its bits, up to the sign of a zero component).
+__device__ __forceinline__ bool spinFixed(const fdm_template& t) {
+  return t.invInertia[0] == 0.0 && t.invInertia[1] == 0.0 && t.invInertia[2] == 0.0 &&
+         t.invInertia[3] == 0.0 && t.invInertia[4] == 0.0 && t.invInertia[5] == 0.0;
+}
+
 // A load through the constant address space, for data that no lane writes
 // while the kernel runs: with a wave-uniform address the compiler issues it
 // as a scalar load (into SGPRs), otherwise as an ordinary vector load.

@@ -18,6 +18,7 @@ __device__ __forceinline__ void kickOne(const fdm_bodies& b, const fdm_template*
   const V3 F = ld3(b.F, n, i);
   const V3 v = ld3(b.v, n, i);
   st3(b.v, n, i, vadd(v, vmul(vadd(vmul(F, invM), g), h)));
+  if (spinFixed(t)) return;
   const V3 w = ld3(b.w, n, i);
   const Q4 q = ld4(b.q, n, i);
   const V3 Iw = smul(srotate(q, lds6(t.inertia)), w);

@@ -195,6 +195,7 @@ __global__ void kReduce(ReduceK a) {
   const V3 Fs = roundState(F), Ts = roundState(T);
   const V3 v = ld3(a.P.v, n, i);
   st3(a.P.v, n, i, vadd(v, vmul(vadd(vmul(Fs, invM), a.gravity), a.h)));
+  if (spinFixed(t)) return;
   const V3 w = ld3(a.P.w, n, i);
   const Q4 q = ld4(a.P.q, n, i);
   const V3 Iw = smul(srotate(q, lds6(t.inertia)), w);
