@@ -90,6 +90,15 @@ __device__ __forceinline__ bool lipschitzClear(V3 local, double reach, const fdm
   return phi - lr > 1e-6 * (fabs(phi) + lr);
 }
 
+// True when phi > 0 at `local` for certain on an affine (plane-wall) SDF:
+// the plane's value exceeds the fit's bound on the residual and rounding.
+// Such a node cannot touch, so its SDF fetch is skipped.
+__device__ __forceinline__ bool planeClear(V3 local, const fdm_template& t) {
+  if (!t.affine) return false;
+  const double v = t.plane[3] + t.plane[0] * local.x + t.plane[1] * local.y + t.plane[2] * local.z;
+  return v > t.planeSlack;
+}
+
 // True when the ball (local, reach) misses the SDF box of t (target frame).
 __device__ __forceinline__ bool outsideBox(V3 local, double reach, const fdm_template& t) {
   const double h = 1.0 / t.invSpacing;
@@ -414,7 +423,9 @@ __device__ __forceinline__ bool nodeHit(V3 p, const Target& tg, I3 tcell, const 
     if (vlen2(rel) > RB * RB) return false;
   }
   int base;
-  if (!sdfLocate(toLocal(rel, tg), tg.t, s, base)) return false;
+  const V3 local = toLocal(rel, tg);
+  if (!sdfLocate(local, tg.t, s, base)) return false;
+  if (SET == FDM_SET_WALL && planeClear(local, tg.t)) return false;
   sdfCorners(tg.sdf + base, tg.t, s);
   const double ov = -trilinear(s.x, s.y, s.z, s.c);
   if (!(ov >= 0.)) return false;
@@ -433,7 +444,9 @@ __device__ __forceinline__ bool nodeCell(V3 p, const Target& tg, const fdm_grid&
   if (SET == FDM_SET_PARTICLE) {
     if (vlen2(rel) > RB * RB) return false;
   }
-  return sdfLocate(toLocal(rel, tg), tg.t, s, base);
+  const V3 local = toLocal(rel, tg);
+  if (!sdfLocate(local, tg.t, s, base)) return false;
+  return !(SET == FDM_SET_WALL && planeClear(local, tg.t));
 }
 __device__ __forceinline__ bool nodeHitAt(V3 p, const Sample& s, I3 tcell, const fdm_grid& g) {
   const double ov = -trilinear(s.x, s.y, s.z, s.c);

@@ -221,6 +221,49 @@ struct LSParticle {
       reach = std::max(reach, std::sqrt(px * px + py * py + pz * pz));
     }
     t.boxReach = reach * (1.0 + 1e-9);
+    // Affine fit of the stored SDF over the grid nodes (least squares; the
+    // centred node coordinates are orthogonal on a regular grid). The
+    // trilinear interpolant of the residuals is a convex combination of
+    // them, so phi >= fit - max |residual| everywhere in the box; a small
+    // relative slack covers the rounding of both evaluations. Only a fit
+    // within a thousandth of the spacing is kept (plane walls).
+    {
+      const size_t nn = size_t(size.x) * size.y * size.z;
+      const double ci = 0.5 * (size.x - 1), cj = 0.5 * (size.y - 1), ck = 0.5 * (size.z - 1);
+      double mean = 0.0, sI = 0.0, sJ = 0.0, sK = 0.0, dI = 0.0, dJ = 0.0, dK = 0.0, vmax = 0.0;
+      for (int z = 0; z < size.z; ++z)
+        for (int y = 0; y < size.y; ++y)
+          for (int x = 0; x < size.x; ++x) {
+            const double v = s[size_t(z) * sz + size_t(y) * sy + x];
+            mean += v;
+            sI += (x - ci) * v;
+            sJ += (y - cj) * v;
+            sK += (z - ck) * v;
+            dI += (x - ci) * (x - ci);
+            dJ += (y - cj) * (y - cj);
+            dK += (z - ck) * (z - ck);
+            vmax = std::max(vmax, std::fabs(v));
+          }
+      mean /= double(nn);
+      const double b = dI > 0 ? sI / dI : 0.0, c = dJ > 0 ? sJ / dJ : 0.0, d = dK > 0 ? sK / dK : 0.0;
+      double res = 0.0;
+      for (int z = 0; z < size.z; ++z)
+        for (int y = 0; y < size.y; ++y)
+          for (int x = 0; x < size.x; ++x) {
+            const double fit = mean + b * (x - ci) + c * (y - cj) + d * (z - ck);
+            res = std::max(res, std::fabs((double)s[size_t(z) * sz + size_t(y) * sy + x] - fit));
+          }
+      const double h = spacing;
+      // phi(l) = mean + b (i - ci) + ... with i = (l.x - origin.x) / h
+      t.plane[0] = b / h;
+      t.plane[1] = c / h;
+      t.plane[2] = d / h;
+      t.plane[3] = mean - b * (origin.x / h + ci) - c * (origin.y / h + cj) - d * (origin.z / h + ck);
+      const double scale = vmax + (std::fabs(t.plane[3]) + (std::fabs(b) + std::fabs(c) + std::fabs(d)) *
+                                                              (reach / h + ci + cj + ck));
+      t.planeSlack = res + 1e-9 * scale;
+      t.affine = res <= 1e-3 * h ? 1 : 0;
+    }
     templates_.push_back(t);
     nodeHost_.push_back(fn);
     connectivity_.push_back(connectivity);

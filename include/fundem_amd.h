@@ -102,7 +102,11 @@ typedef struct {
   int nodeCount;
   int chunkStart;        /* offset into the chunk table (float4: centre xyz, radius) */
   int chunkCount;
-  int pad_;
+  int affine;            /* 1: the stored SDF is affine in the box (a plane wall): the
+                            trilinear phi at a template-frame point l is at least
+                            plane[3] + plane[0..2] . l - planeSlack (culling only) */
+  double plane[4];
+  double planeSlack;
 } fdm_template;
 
 /* One material (materialInfo, LSParticle.h:12-51) after the clamps of
