@@ -513,33 +513,19 @@ def dense_main(a, templates):
     seconds and collide at about (lattice spacing) / closing; once the
     particle-particle node contacts stop growing (checked every 200 steps),
     that state is restarted at rest (v = w = 0) and timed like the pour."""
-    import dataclasses
-    import numpy as np
     from fundem_amd import Solver
     from fundem_amd import workloads as W
     scn = W.pour((a.side, a.side, a.side), templates=templates)
-    v = scn.v.copy()
-    v[:, 2] = -(scn.x[:, 2] - scn.x[:, 2].min()) / a.dense_closing
-    s = Solver(0)
-    s.set_capacity(12, 64)
-    W.apply(dataclasses.replace(scn, v=v), s)
+
+    def make_solver():
+        s = Solver(0)
+        s.set_capacity(12, 64)
+        return s
+
     t0 = time.perf_counter()
-    best, done = 0, 0
-    while done < a.dense_max_steps:
-        s.step(200)
-        done += 200
-        c = int(s.contacts(0)["A"].size)
-        if c >= 4 * scn.n or (best > scn.n and c < best):
-            break
-        best = max(best, c)
-    if s.status() != 0:
-        raise RuntimeError(f"capacity exceeded while compressing (status {s.status()})")
-    st = s.state()
-    s.close()
+    dense, done, c = W.compressed_pour(scn, make_solver, a.dense_closing, a.dense_max_steps)
     log(f"dense: compressed for {done} steps in {time.perf_counter() - t0:.1f} s, {c} "
         "particle-particle node contacts")
-    dense = dataclasses.replace(scn, x=st["x"], q=st["q"], v=np.zeros_like(st["v"]),
-                                w=np.zeros_like(st["w"]))
     s = Solver(0)
     s.set_capacity(12, 64)
     s.set_skin(a.skin)

@@ -137,6 +137,37 @@ def apply_batch(scenarios, t) -> None:
 IDENT = np.array([1.0, 0.0, 0.0, 0.0])
 
 
+def compressed_pour(scn: Scenario, make_solver, closing: float = 5e-3, max_steps: int = 20000,
+                    check_every: int = 200):
+    """The pour's lattice driven into a contact-dense state: every particle
+    starts falling at a speed proportional to its height above the lowest,
+    so the layers close in on one another within about `closing` seconds;
+    once the particle-particle node contacts stop growing (checked every
+    `check_every` steps) the state is returned at rest (v = w = 0), with the
+    number of steps run and of particle node contacts. `make_solver()` gives
+    an empty solver with the capacities the dense state needs."""
+    import dataclasses
+    v = scn.v.copy()
+    v[:, 2] = -(scn.x[:, 2] - scn.x[:, 2].min()) / closing
+    s = make_solver()
+    apply(dataclasses.replace(scn, v=v), s)
+    best, done, c = 0, 0, 0
+    while done < max_steps:
+        s.step(check_every)
+        done += check_every
+        c = int(s.contacts(0)["A"].size)
+        if c >= 4 * scn.n or (best > scn.n and c < best):
+            break
+        best = max(best, c)
+    if s.status() != 0:
+        raise RuntimeError(f"capacity exceeded while compressing (status {s.status()})")
+    st = s.state()
+    s.close()
+    dense = dataclasses.replace(scn, x=st["x"], q=st["q"], v=np.zeros_like(st["v"]),
+                                w=np.zeros_like(st["w"]))
+    return dense, done, c
+
+
 def floor_wall(xy_min, xy_max, z0: float = 0.0):
     """A plane wall at height z0 covering the given xy range: the reference's
     plane construction (apps/test/LSSphereBouncing.cpp:86-127) built around

@@ -8,7 +8,11 @@ test_gpu_parity.py compare every bit with it.
   for many steps, and the states equal those of a run that rebuilds every
   list every step (skin 0), bit for bit.
 * Launches are deterministic: a second run, and a run replaying captured
-  graphs, give the same bits."""
+  graphs, give the same bits.
+* The same reuse property in the contact-dense state (the lattice compressed
+  until its layers touch: about a million particle pairs and several million
+  particle node contacts, so the particle-particle cull and sweep, the contact
+  law and the mirror reduction run at size)."""
 import numpy as np
 import pytest
 
@@ -67,3 +71,32 @@ def test_full_pour_repeat_and_graph_replay(pour):
     replay, _ = run(pour, graph=True)
     assert_same(again, first, "repeat")
     assert_same(replay, first, "graph replay")
+
+
+def test_full_dense_pour_list_reuse_equals_rebuilding_every_step(pour):
+    from fundem_amd import Solver
+
+    def make_solver():
+        g = Solver()
+        g.set_capacity(12, 64)
+        return g
+
+    dense, done, contacts = W.compressed_pour(pour, make_solver)
+    print(f"compressed for {done} steps: {contacts} particle node contacts", flush=True)
+    runs = []
+    for skin in (None, 0.0):
+        g = make_solver()
+        if skin is not None:
+            g.set_skin(skin)
+        W.apply(dense, g)
+        g.step(1)
+        g.step(19)
+        assert g.status() == 0
+        pp = g.contacts(0)["A"].size
+        runs.append((g.state(), g.list_builds(), pp))
+        g.close()
+    (a, (builds, _, skin), pp), (b, (builds0, _, _), _) = runs
+    print(f"skin {skin:.3g} m: {builds} list builds in 20 steps, skin 0: {builds0}; "
+          f"{pp} particle node contacts at the end", flush=True)
+    assert pp > 1_000_000 and builds0 >= 20 and builds < builds0
+    assert_same(a, b, "dense: kept lists vs rebuilds")
