@@ -96,22 +96,26 @@ def log(*a):
 #                template id 4, grid state at build 4, visibility at build 4
 #                (= 100); write x 24, v 12, w 12, q 16, cell id 4 (= 68);
 #                sort lists and visibility flags are written by list builds
-#                only
+#                only; a body whose template's inverse inertia is exactly
+#                zero (its spin never changes) skips T read and w write
+#                (168 - 24 = 144)
 #   reduce_kick  per particle: pair counts read (particle set 8, wall 8,
 #                three ghost sets 24, mirror counts 16), F T write 24,
-#                template id 4, v w read and write 48, q read 16 (= 148);
-#                per candidate pair with contacts, read from each side: its
-#                record count 4 and sums 48 (bounded by 104 per pair: every
-#                candidate pair counted)
+#                template id 4, v w read and write 48, q read 16 (= 148;
+#                148 - 40 = 108 for a body whose spin never changes: no w,
+#                no q); per candidate pair with contacts, read from each
+#                side: its record count 4 and sums 48 (bounded by 104 per
+#                pair: every candidate pair counted)
 #   narrow       per candidate pair: ids 8, both bodies' x q template id 88,
 #                empty record range or the kept pair's record range and pair
 #                sums 80, history ids 16 (= 192); per contact: velocities of
 #                both bodies 48, history read 16 and write 16 (= 80)
-def kernel_bytes(stage, n, pairs, contacts):
+def kernel_bytes(stage, n, pairs, contacts, fixed=0):
+    """`fixed`: bodies whose template has an exactly zero inverse inertia."""
     if stage == "step_begin":
-        return 168.0 * n
+        return 168.0 * n - 24.0 * fixed
     if stage == "reduce_kick":
-        return 148.0 * n + 104.0 * sum(pairs.values())
+        return 148.0 * n - 40.0 * fixed + 104.0 * sum(pairs.values())
     if stage == "narrow":
         return 192.0 * sum(pairs.values()) + 80.0 * sum(contacts.values())
     return None
@@ -423,7 +427,7 @@ def pour_main(a):
         f"{contacts}, candidate pairs {pairs}; skin {skin:.3e} m, {b1 - b0} list builds and "
         f"{gb1 - gb0} ghost-list builds in {a.steps} steps")
     log_stages(stages, a.stage_steps)
-    roof, stage_roof = rooflines(stages, n, pairs, contacts)
+    roof, stage_roof = rooflines(stages, n, pairs, contacts, spin_fixed(s, scn.geo))
     # algorithmic bytes of a whole step: state 224 B per particle and 20 B per
     # contact record read and written, plus the template tables
     c_total = sum(contacts.values())
@@ -482,26 +486,40 @@ def rebuild_cost(stages, steps, builds):
             "us_per_step_amortised": 1e3 * ms / steps}
 
 
-def rooflines(stages, n, pairs, contacts):
+def spin_fixed(s, geo):
+    """Bodies whose template's inverse inertia is exactly zero (the solver's
+    own mass properties: a millimetre template's inertia determinant falls
+    below the inversion threshold)."""
+    import numpy as np
+    geo = np.asarray(geo)
+    fixed = [g for g in np.unique(geo) if not np.any(s.geometry(int(g))["inv_inertia"])]
+    return int(np.isin(geo, fixed).sum())
+
+
+def rooflines(stages, n, pairs, contacts, fixed=0, counters=True):
     """Roofline of the dominant modelled stage (by total time) and of every
-    modelled stage: algorithmic bytes per launch over the average launch."""
-    modeled = {k: v for k, v in stages.items() if kernel_bytes(k, n, pairs, contacts) is not None}
+    modelled stage: algorithmic bytes per launch over the average launch.
+    `counters`: attach the pour's measured traffic (profiles/traffic.json);
+    other workloads get null."""
+    kb = lambda k: kernel_bytes(k, n, pairs, contacts, fixed)
+    traffic = measured_traffic if counters else (lambda k: None)
+    modeled = {k: v for k, v in stages.items() if kb(k) is not None}
     stage_roof = []
     for k, (ms, calls) in sorted(modeled.items(), key=lambda kv: -kv[1][0]):
-        nb = kernel_bytes(k, n, pairs, contacts)
+        nb = kb(k)
         avg_s = ms / max(calls, 1) * 1e-3
         ach = nb / avg_s / 1e9 if avg_s > 0 else 0.0
         stage_roof.append({"stage": k, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb,
                            "achieved": ach, "frac": ach / HBM_PEAK_GBS,
-                           "traffic": measured_traffic(k)})
+                           "traffic": traffic(k)})
     roof = None
     if modeled:
         dom, (ms, calls) = max(modeled.items(), key=lambda kv: kv[1][0])
-        nb = kernel_bytes(dom, n, pairs, contacts)
+        nb = kb(dom)
         avg_s = ms / max(calls, 1) * 1e-3
         achieved = nb / avg_s / 1e9 if avg_s > 0 else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic(dom), "kernel": dom,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": nb}
     return roof, stage_roof
 
@@ -550,7 +568,7 @@ def dense_main(a, templates):
     log(f"dense: {1e3 * elapsed / steps:.3f} ms/step, {n * steps / elapsed:.3e} particle-steps/s; "
         f"contacts {contacts}, candidate pairs {pairs}, {b1 - b0} list builds in {steps} steps")
     log_stages(stages, a.stage_steps)
-    roof, stage_roof = rooflines(stages, n, pairs, contacts)
+    roof, stage_roof = rooflines(stages, n, pairs, contacts, spin_fixed(s, dense.geo), counters=False)
     s.close()
     return {"metric": "particle-steps/s (1M pour lattice compressed until its layers touch)",
             "value": n * steps / elapsed, "unit": "particle-steps/s", "steps": steps,

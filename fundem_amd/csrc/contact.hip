@@ -98,6 +98,19 @@ __device__ __forceinline__ bool planeClear(V3 local, const fdm_template& t) {
   const double v = t.plane[3] + t.plane[0] * local.x + t.plane[1] * local.y + t.plane[2] * local.z;
   return v > t.planeSlack;
 }
+// The same for every point within `reach` of `local`: the plane changes by at
+// most |a|_1 |d| over an offset d (|a . d| <= sum |a_i| |d_i| <= |a|_1 |d|).
+// For an affine SDF this replaces the Lipschitz test without an SDF fetch.
+__device__ __forceinline__ bool planeClearBall(V3 local, double reach, const fdm_template& t) {
+  const double v = t.plane[3] + t.plane[0] * local.x + t.plane[1] * local.y + t.plane[2] * local.z;
+  const double a1 = fabs(t.plane[0]) + fabs(t.plane[1]) + fabs(t.plane[2]);
+  return v - a1 * reach > t.planeSlack;
+}
+// Ball test against the zero level: the plane bound for affine SDFs, the
+// Lipschitz bound otherwise.
+__device__ __forceinline__ bool levelClear(V3 local, double reach, const fdm_template& t, const float* sdf) {
+  return t.affine ? planeClearBall(local, reach, t) : lipschitzClear(local, reach, t, sdf);
+}
 
 // True when the ball (local, reach) misses the SDF box of t (target frame).
 __device__ __forceinline__ bool outsideBox(V3 local, double reach, const fdm_template& t) {
@@ -594,7 +607,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     bool keep = vlen2(d) <= rs * rs;
     if (keep) {
       const V3 local = toLocal(d, tg);
-      keep = !outsideBox(local, RA, tg.t) && !lipschitzClear(local, RA, tg.t, tg.sdf);
+      keep = !outsideBox(local, RA, tg.t) && !levelClear(local, RA, tg.t, tg.sdf);
     }
     if (!keep) {
       if (lane == 0) {
@@ -631,7 +644,7 @@ __device__ void processPair(const NarrowCommon& C, const NarrowSet& S, int p, in
     }
     if (live) {
       const V3 lc = toLocal(vsub(cw, tg.x), tg);
-      if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
+      if (outsideBox(lc, rc, tg.t) || levelClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
   const unsigned long long liveChunks = ballot64(live);
@@ -866,18 +879,6 @@ __device__ __forceinline__ void stageHit(double* g, V3 pc, V3 nrm, double ov, in
   g[7] = (double)k;
 }
 
-// Reserve n entries from pool `pool` of a pooled buffer of `capacity`
-// entries; returns the first index, or -1 when the pool is full (the whole
-// wave gets the same answer).
-__device__ __forceinline__ int reservePooled(int* counters, int capacity, int pool, int n, int lane) {
-  const int poolCap = capacity / FDM_REC_POOLS;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(counters + pool * FDM_REC_STRIDE, n);
-  base = readlaneI(base, 0);
-  if (base + n > poolCap) return -1;
-  return pool * poolCap + base;
-}
-
 // Node sweep of one kept pair (one wave): the culls and exact predicate of
 // processPair; the contacts' geometry is staged in node order (through LDS,
 // or recomputed chunk by chunk past kMaxHits) for the contact-law kernel,
@@ -951,10 +952,28 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     }
     if (live) {
       const V3 lc = toLocal(vsub(cw, tg.x), tg);
-      if (outsideBox(lc, rc, tg.t) || lipschitzClear(lc, rc, tg.t, tg.sdf)) live = false;
+      if (outsideBox(lc, rc, tg.t) || levelClear(lc, rc, tg.t, tg.sdf)) live = false;
     }
   }
   const unsigned long long liveChunks = ballot64(live);
+  // An affine (plane-wall) target: the plane's value at node s of A is
+  // affine in s, plane(toLocal(R_A s + x_A - x_T)) = c0 + u . s with
+  // g = R_T a (a: the plane's gradient), u = R_A^T g, c0 = plane_3 +
+  // g . (x_A - x_T) (the rotation formula's transpose is the conjugate's,
+  // for any quaternion). A node with c0 + u . s above the plane's slack has
+  // phi > 0 (the slack covers this evaluation's rounding as well), so its
+  // transform and SDF fetch are skipped.
+  const bool planeNodes = SET == FDM_SET_WALL && tg.t.affine;
+  V3 pu = {0., 0., 0.};
+  double pc0 = 0.;
+  if (planeNodes && liveChunks) {
+    const V3 ga = toWorld({tg.t.plane[0], tg.t.plane[1], tg.t.plane[2]}, tg);
+    pu = qrotateT(ga, qA);
+    pc0 = tg.t.plane[3] + vdot(ga, vsub(xA, tg.x));
+  }
+  auto planeSkip = [&](V3 sn) {
+    return planeNodes && pc0 + pu.x * sn.x + pu.y * sn.y + pu.z * sn.z > tg.t.planeSlack;
+  };
   // the previous step's record range of the pair, for the law's header (its
   // scalar loads complete while the nodes are swept)
   int prevOff = 0;
@@ -1032,7 +1051,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       Sample sm;
       int b = 0;
       V3 pn = {0., 0., 0.};
-      if (k < tA.nodeCount) {
+      if (k < tA.nodeCount && !planeSkip(sn)) {
         pn = vadd(qrotate(qA, sn), xA);
         ok = nodeCell<SET>(pn, tg, tgrid, RB, sm, b);
       }
@@ -1065,12 +1084,15 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       const int c2 = lcm ? __builtin_ctzll(lcm) : -1;
       if (lcm) lcm &= lcm - 1ull;
       const int k = c * FDM_CHUNK + lane, k2 = c2 * FDM_CHUNK + lane;
-      const bool has = k < tA.nodeCount, has2 = c2 >= 0 && k2 < tA.nodeCount;
+      V3 s1 = {0., 0., 0.}, s2 = {0., 0., 0.};
+      if (k < tA.nodeCount) s1 = {(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+      if (c2 >= 0 && k2 < tA.nodeCount)
+        s2 = {(double)nodes[3 * k2], (double)nodes[3 * k2 + 1], (double)nodes[3 * k2 + 2]};
+      const bool has = k < tA.nodeCount && !planeSkip(s1);
+      const bool has2 = c2 >= 0 && k2 < tA.nodeCount && !planeSkip(s2);
       V3 pn = {0., 0., 0.}, pn2 = {0., 0., 0.};
-      if (has) pn = vadd(qrotate(qA, {(double)nodes[3 * k], (double)nodes[3 * k + 1],
-                                      (double)nodes[3 * k + 2]}), xA);
-      if (has2) pn2 = vadd(qrotate(qA, {(double)nodes[3 * k2], (double)nodes[3 * k2 + 1],
-                                        (double)nodes[3 * k2 + 2]}), xA);
+      if (has) pn = vadd(qrotate(qA, s1), xA);
+      if (has2) pn2 = vadd(qrotate(qA, s2), xA);
       Sample sm, sm2;
       int b1 = 0, b2 = 0;
       bool in1 = has && nodeCell<SET>(pn, tg, tgrid, RB, sm, b1);
@@ -1119,7 +1141,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     bool hit = false;
     Sample sm;
     V3 pn;
-    if (k < tA.nodeCount) {
+    if (k < tA.nodeCount && !planeSkip(s)) {
       pn = vadd(qrotate(qA, s), xA);
       hit = nodeHit<SET>(pn, tg, tcell, tgrid, RB, sm);
     }
@@ -1144,15 +1166,27 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
     total += __popcll(m);
   }
 
-  int base = 0, gbase = 0;
+  // the record and staging reservations and the mirror slot, their atomics
+  // issued together by lane 0 (one round trip instead of three)
+  int base = 0, gbase = 0, mpos = 0;
+  const bool mirrored = SET != FDM_SET_WALL && total > 0 && T != A;
   if (total > 0) {
-    base = reservePooled(S.pairs.recTotal, S.pairs.recCapacity, pool, total, lane);
-    gbase = reservePooled(X.hitCount, X.hitCapacity, pool, total, lane);
-    if (base < 0 || gbase < 0) {
+    int rb = 0, gb = 0;
+    if (lane == 0) {
+      rb = atomicAdd(S.pairs.recTotal + pool * FDM_REC_STRIDE, total);
+      gb = atomicAdd(X.hitCount + pool * FDM_REC_STRIDE, total);
+      if (mirrored) mpos = atomicAdd(&S.mirror.count[T], 1);
+    }
+    rb = readlaneI(rb, 0);
+    gb = readlaneI(gb, 0);
+    const int rCap = S.pairs.recCapacity / FDM_REC_POOLS, gCap = X.hitCapacity / FDM_REC_POOLS;
+    if (rb + total > rCap || gb + total > gCap) {
       if (lane == 0)
-        atomicOr(C.status, base < 0 ? FDM_STATUS_RECORD_OVERFLOW : FDM_STATUS_STAGING_OVERFLOW);
+        atomicOr(C.status, rb + total > rCap ? FDM_STATUS_RECORD_OVERFLOW : FDM_STATUS_STAGING_OVERFLOW);
       total = 0;
-      base = gbase = 0;
+    } else {
+      base = pool * rCap + rb;
+      gbase = pool * gCap + gb;
     }
   }
   if (total > 0 && total <= kMaxHits) {
@@ -1192,9 +1226,10 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       h[0] = make_int4(total, base, gbase, A);
       if (total > 0) h[1] = make_int4(T, prevOff, prevCnt, 0);
     }
-    if (SET != FDM_SET_WALL && total > 0 && T != A) {
-      const int pos = atomicAdd(&S.mirror.count[T], 1);
-      if (pos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + pos] = p;
+    // (a pair whose reservation overflowed keeps its mirror slot, with no
+    // records: the slot count and the list agree)
+    if (mirrored) {
+      if (mpos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + mpos] = p;
       else atomicOr(C.status, FDM_STATUS_BUCKET_OVERFLOW);
     }
   }
