@@ -162,6 +162,25 @@ __device__ __forceinline__ void stp(double* p, int n, int i, V3 v) {
 }
 __device__ __forceinline__ V3 ldd3(const double* p) { return {p[0], p[1], p[2]}; }
 // a vector as the state storage holds it
+// stored-precision triples and quadruples, widened to FP64 where used
+struct R3 {
+  fdm_real x, y, z;
+};
+struct R4 {
+  fdm_real q0, q1, q2, q3;
+};
+__device__ __forceinline__ R3 ld3s(const fdm_real* p, int n, int i) { return {p[i], p[n + i], p[2 * n + i]}; }
+__device__ __forceinline__ R4 ld4s(const fdm_real* p, int n, int i) {
+  return {p[i], p[n + i], p[2 * n + i], p[3 * n + i]};
+}
+__device__ __forceinline__ void st3s(fdm_real* p, int n, int i, R3 v) {
+  p[i] = v.x;
+  p[n + i] = v.y;
+  p[2 * n + i] = v.z;
+}
+__device__ __forceinline__ V3 wide(R3 v) { return {(double)v.x, (double)v.y, (double)v.z}; }
+__device__ __forceinline__ Q4 wide(R4 q) { return {(double)q.q0, (double)q.q1, (double)q.q2, (double)q.q3}; }
+__device__ __forceinline__ R3 narrow3(V3 v) { return {(fdm_real)v.x, (fdm_real)v.y, (fdm_real)v.z}; }
 __device__ __forceinline__ V3 roundState(V3 v) {
   return {roundState(v.x), roundState(v.y), roundState(v.z)};
 }

@@ -190,42 +190,82 @@ struct StepBeginK {
 // One body of phase A. Everything it reads from the launch arguments is
 // copied into registers first (selecting between argument structs by
 // reference would make the compiler copy them to per-lane scratch).
-template <bool WALL>
+// MANY: bodies of several systems (per-body grids and periodic flags); the
+// single-system kernel holds its grid in scalar registers.
+template <bool WALL, bool MANY>
 __device__ __forceinline__ void stepBeginBody(const fdm_step_begin_args& a, int j, float& rate,
                                               int& flags) {
   const fdm_bodies b = WALL ? a.walls : a.particles;
   const fdm_template* tmpl = WALL ? a.wgeo.info : a.pgeo.info;
   const fdm_hash_table t = WALL ? a.wtable : a.ptable;
-  const bool many = a.inst.count > 0;
+  const bool many = MANY;
   const int in = !many ? 0 : (WALL ? a.inst.wallInstance[j] : a.inst.particleInstance[j]);
   const fdm_grid grid =
       !many ? (WALL ? a.wgrid : a.pgrid) : (WALL ? a.inst.wgrid[in] : a.inst.pgrid[in]);
   const fdm_periodic perI = many ? a.inst.periodic[in] : a.periodic;
-  if (!WALL && a.kick) kickOne(b, tmpl, j, fromD3(a.gravity), 0.5 * a.dt);
-  V3 x;
-  if (a.drift) x = driftOne(b, j, a.dt);
-  else x = ldp(b.x, b.n, j);
+  const bool ghosts = !WALL && anyPeriodic(a.periodic);
+  const bool kick = !WALL && a.kick;
+  const int n = b.n;
+  // Every input is loaded before anything is stored: the arrays may alias as
+  // far as the compiler knows, so a load placed after a store could not be
+  // issued before it. One round of loads (the torque only where the spin
+  // changes), then the half-kick and the drift in registers with the
+  // roundings of the stored state (kickOne, driftOne), then the stores.
+  // (the stored FP32 values are held as loaded and widened where used)
+  const int geo = b.geo[j];
+  const V3 x0 = ldp(b.x, n, j);
+  R3 v = ld3s(b.v, n, j), w = ld3s(b.w, n, j);
+  const R4 q0 = ld4s(b.q, n, j);
+  R3 F = {0, 0, 0};
+  if (kick) F = ld3s(b.F, n, j);
+  const int atBuild = t.validAtBuild[j];
+  const int ghostAtBuild = ghosts ? a.ghostAtBuild[j] : 0;
+  const fdm_template& tp = tmpl[geo];
+  bool vNew = false, wNew = false;
+  if (kick && !nearZero(tp.invMass)) {
+    // particleVelocityAngularVelocityIntegrationKernel (particleIntegrationKernel.cu:3-32)
+    const double h = 0.5 * a.dt;
+    v = narrow3(vadd(wide(v), vmul(vadd(vmul(wide(F), tp.invMass), fromD3(a.gravity)), h)));
+    vNew = true;
+    if (!spinFixed(tp)) {
+      const V3 wd = wide(w);
+      const Q4 qd = wide(q0);
+      const V3 Iw = smul(srotate(qd, lds6(tp.inertia)), wd);
+      const V3 T = ld3(b.T, n, j);
+      w = narrow3(vadd(wd, vmul(smul(srotate(qd, lds6(tp.invInertia)), vsub(T, vcross(wd, Iw))), h)));
+      wNew = true;
+    }
+  }
+  V3 x = x0;
+  if (vNew) st3s(b.v, n, j, v);
+  if (wNew) st3s(b.w, n, j, w);
+  if (a.drift) {
+    // particlePositionOrientationIntegrationKernel (particleIntegrationKernel.cu:34-48)
+    x = vadd(x0, vmul(wide(v), a.dt));
+    stp(b.x, n, j, x);
+    st4(b.q, n, j, qintegrate(wide(q0), wide(w), a.dt));
+  }
   t.hashValue[j] = cellHash(x, grid);
   if (a.drift) {
-    const fdm_template& tp = tmpl[b.geo[j]];
     const double rho = fmax(tp.radius, tp.boxReach);
-    const double r = vlen(ld3(b.v, b.n, j)) + vlen(ld3(b.w, b.n, j)) * rho;
+    const double r = vlen(wide(v)) + vlen(wide(w)) * rho;
     rate = (float)r * (1.0f + 1e-6f);
   }
   const int st = bodyState(x, grid);
-  if (st == 2 || st != t.validAtBuild[j]) flags |= 1;
-  if (!WALL && anyPeriodic(a.periodic)) {
+  if (st == 2 || st != atBuild) flags |= 1;
+  if (ghosts) {
     // sets exist where any instance is periodic; an instance contributes
     // only to the sets of its own periodic axes
     const int vb = visibilityBits(x, grid, a.periodic, perI);
     const int bits = vb & 7;
     for (int k = 0; k < 3; ++k)
       if (a.ghostFlag[k]) a.ghostFlag[k][j] = (bits >> k) & 1;
-    if (bits != a.ghostAtBuild[j]) flags |= 2;
+    if (bits != ghostAtBuild) flags |= 2;
     if (vb & 8) flags |= 4;
   }
 }
 
+template <bool MANY>
 __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
   if (gateOff(k.a.gate)) return;
   const fdm_step_begin_args& a = k.a;
@@ -234,8 +274,8 @@ __global__ void __launch_bounds__(256) kStepBegin(StepBeginK k) {
   float rate = 0.f;
   int flags = 0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nP) stepBeginBody<false>(a, i, rate, flags);
-  else if (i < nP + nW) stepBeginBody<true>(a, i - nP, rate, flags);
+  if (i < nP) stepBeginBody<false, MANY>(a, i, rate, flags);
+  else if (i < nP + nW) stepBeginBody<true, MANY>(a, i - nP, rate, flags);
   const int f1 = __syncthreads_or(flags & 1), f2 = __syncthreads_or(flags & 2),
             f4 = __syncthreads_or(flags & 4);
 #pragma unroll
@@ -438,7 +478,8 @@ extern "C" int fdm_launchStepBegin(const fdm_step_begin_args* args, const fdm_sk
   unsigned nb = 0;
   const int r = stepBeginSetup(args, skin, workspace, workspaceBytes, k, nb);
   if (r != FDM_OK) return r;
-  hipLaunchKernelGGL(kStepBegin, dim3(nb), dim3(256), 0, stream, k);
+  if (args->inst.count > 0) hipLaunchKernelGGL(kStepBegin<true>, dim3(nb), dim3(256), 0, stream, k);
+  else hipLaunchKernelGGL(kStepBegin<false>, dim3(nb), dim3(256), 0, stream, k);
   return launchStatus();
 }
 
