@@ -1392,7 +1392,11 @@ kCullSweep(NarrowMulti a, NarrowScratch x) {
   const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const SetStarts st = setStarts(a);
   const int total = st.v[kMaxSets];
-  for (int g = blockIdx.x * WAVES + wave; g < total; g += gridDim.x * WAVES) {
+  // candidates are taken from the end of the index space: the ghost sets
+  // (image pairs, whose spheres hold many nodes) before the wall set, so
+  // that the cheap wall pairs fill the last round of waves
+  for (int r = blockIdx.x * WAVES + wave; r < total; r += gridDim.x * WAVES) {
+    const int g = total - 1 - r;
     int k = 0, base = 0;
 #pragma unroll
     for (int q = 1; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
@@ -1406,6 +1410,7 @@ kCullSweep(NarrowMulti a, NarrowScratch x) {
       if (lane == 0) {
         s.pairs.recOff[p] = 0;
         s.pairs.recCnt[p] = 0;
+        if (x.lawHead) *reinterpret_cast<int4*>(x.lawHead + 8 * (size_t)g) = make_int4(0, 0, 0, 0);
       }
       continue;
     }
