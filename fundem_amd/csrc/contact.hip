@@ -359,6 +359,7 @@ struct NarrowK {
 };
 
 constexpr int kMaxSets = 5;
+constexpr int kXcds = 8;  // accelerator complex dies of the MI355X, each with its own L2
 struct NarrowMulti {
   NarrowCommon c;
   const int* gate;
@@ -1392,10 +1393,19 @@ kCullSweep(NarrowMulti a, NarrowScratch x) {
   const int wave = WAVES == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const SetStarts st = setStarts(a);
   const int total = st.v[kMaxSets];
-  // candidates are taken from the end of the index space: the ghost sets
-  // (image pairs, whose spheres hold many nodes) before the wall set, so
-  // that the cheap wall pairs fill the last round of waves
-  for (int r = blockIdx.x * WAVES + wave; r < total; r += gridDim.x * WAVES) {
+  // Work assignment. Workgroups are dealt to the eight XCDs in turn
+  // (workgroup b runs on XCD b mod 8), each XCD with an L2 of its own, and
+  // neighbouring candidates share bodies, templates and list lines: so each
+  // XCD takes runs of 64 consecutive candidates (run q on XCD q mod 8), its
+  // waves sharing out its runs. Candidates are taken from the end of the
+  // index space: the ghost sets (image pairs, whose spheres hold many nodes)
+  // before the wall set, so that the cheap wall pairs fill the last round.
+  const bool byXcd = gridDim.x % kXcds == 0;
+  const int xcd = byXcd ? blockIdx.x % kXcds : 0;
+  const int stride = byXcd ? gridDim.x / kXcds * WAVES : gridDim.x * WAVES;
+  for (int i = (byXcd ? blockIdx.x / kXcds : blockIdx.x) * WAVES + wave;; i += stride) {
+    const int r = byXcd ? ((i >> 6) * kXcds + xcd) * 64 + (i & 63) : i;
+    if (r >= total) break;
     const int g = total - 1 - r;
     int k = 0, base = 0;
 #pragma unroll
