@@ -1228,10 +1228,16 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
       if (total > 0) h[1] = make_int4(T, prevOff, prevCnt, 0);
     }
     // (a pair whose reservation overflowed keeps its mirror slot, with no
-    // records: the slot count and the list agree)
+    // records and zero sums: the slot count and the list agree, and the
+    // target's fold adds nothing stale)
     if (mirrored) {
       if (mpos < S.mirror.capacity) S.mirror.list[(size_t)T * S.mirror.capacity + mpos] = p;
       else atomicOr(C.status, FDM_STATUS_BUCKET_OVERFLOW);
+      if (total == 0)
+        for (int c = 0; c < 3; ++c) {
+          S.pairs.sumF[3 * (size_t)p + c] = 0.;
+          S.pairs.sumTB[3 * (size_t)p + c] = 0.;
+        }
     }
   }
 }
