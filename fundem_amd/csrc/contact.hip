@@ -364,6 +364,7 @@ struct NarrowMulti {
   NarrowCommon c;
   const int* gate;
   int nsets;
+  int wallSet;  // set swept by kWallGroups (kCullSweep leaves it out), or -1
   NarrowSet s[kMaxSets];
 };
 
@@ -1242,6 +1243,256 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
   }
 }
 
+// Four wall pairs per wave, sixteen lanes each (pair 4q + j on lanes
+// 16j..16j+15): the pre-cull, chunk culls and exact node predicate of
+// sweepPair<FDM_SET_WALL, true>, with the per-pair values held per lane. A
+// group first collects, in node order, the nodes of its surviving chunks
+// that pass the plane test of an affine target (every node of another
+// target) into LDS, then evaluates the predicate on them sixteen at a time,
+// so that the few nodes near a plane wall are fetched in one round for four
+// pairs. Contacts are staged in LDS in node order (past kGroupHits they are
+// recomputed straight into the workspace once the range is reserved).
+constexpr int kGroupHits = 16;
+constexpr int kGroupCands = 64;
+struct WallGroupSlots {
+  double pc[3][4][kGroupHits];
+  double nrm[3][4][kGroupHits];
+  double ov[4][kGroupHits];
+  int node[4][kGroupHits];
+  int cand[4][kGroupCands];
+};
+
+__device__ void sweepWallGroup(const NarrowCommon& C, const NarrowSet& S, const NarrowScratch& X,
+                               int g0, int nPairs, int q, unsigned gmask, int lane,
+                               WallGroupSlots& ws) {
+  const int grp = lane >> 4, gl = lane & 15, sh = 16 * grp;
+  const int p = 4 * q + grp;
+  const int w = g0 + p;
+  const unsigned below = gl == 0 ? 0u : (0xFFFFu >> (16 - gl));  // lanes of the group below this one
+  auto groupBits = [&](unsigned long long m) __attribute__((always_inline)) {
+    return (unsigned)((m >> sh) & 0xFFFFull);
+  };
+  bool active = p < nPairs && ((gmask >> grp) & 1u);
+  const int A = active ? ldu(S.pairs.pairA + p) : 0;
+  const int T = active ? ldu(S.pairs.pairT + p) : 0;
+  auto emptyRange = [&]() __attribute__((always_inline)) {
+    if (gl == 0) {
+      S.pairs.recOff[p] = 0;
+      S.pairs.recCnt[p] = 0;
+      *reinterpret_cast<int4*>(X.lawHead + 8 * (size_t)w) = make_int4(0, 0, 0, 0);
+    }
+  };
+  if (active && A < 0) {  // slot left empty by a capacity overflow
+    emptyRange();
+    active = false;
+  }
+  // the caller gives groups whose pairs share the target (and system)
+  const unsigned long long act = ballot64(active);
+  if (!act) return;
+  const int ref = __builtin_ctzll(act);
+  const int Tu = readlaneI(T, ref), Au = readlaneI(A, ref);
+  const int inst = C.inst.count > 0 ? ldu(C.inst.particleInstance + Au) : 0;
+  // lanes of an inactive group read body Au and use nothing
+  const int Ai = active ? A : Au;
+  const int geoA = ldu(C.P.geo + Ai);
+  const double radiusA = ldu(&C.pgeo.info[geoA].radius);
+  const PairFrame fr = pairFrame<FDM_SET_WALL>(C, S, Au);
+  const Q4 qA = detectQA<FDM_SET_WALL>(C, fr, Ai);
+  const V3 xA = detectPosA<FDM_SET_WALL>(C, fr, Ai);
+  // the wall grid by value (wave-uniform; a pointer that may point into
+  // the kernel arguments would make the compiler copy them to scratch)
+  const fdm_grid tgrid = C.inst.count > 0 ? ldu(C.inst.wgrid + inst) : S.grid;
+  Target tg;
+  V3 armB;
+  resolveTarget<FDM_SET_WALL>(C, S, p, Tu, fr, tg, armB);
+  if (active) {
+    const double RA = radiusA * kReach;
+    const double rs = RA + tg.t.boxReach * kReach;
+    const V3 d = vsub(xA, tg.x);
+    bool keep = vlen2(d) <= rs * rs;
+    if (keep) keep = !outsideBox(toLocal(d, tg), RA, tg.t);
+    if (!keep) {
+      emptyRange();
+      active = false;
+    }
+  }
+  const double RB = tg.t.radius;
+  const I3 tcell = decodeHash(tg.hash, tgrid);
+  const int nChunks = active ? ldu(&C.pgeo.info[geoA].chunkCount) : 0;
+  const int nodeCount = active ? ldu(&C.pgeo.info[geoA].nodeCount) : 0;
+  const float4* chunks =
+      reinterpret_cast<const float4*>(C.pgeo.chunks) + ldu(&C.pgeo.info[geoA].chunkStart);
+  const fdm_real* nodes = C.pgeo.nodes + 3 * (size_t)ldu(&C.pgeo.info[geoA].nodeStart);
+
+  // chunk culls, sixteen chunks of the group's pair at a time
+  unsigned long long live = 0ull;
+  for (int cb = 0; cb < FDM_MAX_CHUNKS && ballot64(cb < nChunks); cb += 16) {
+    const int c = cb + gl;
+    bool lv = false;
+    if (c < nChunks) {
+      const float4 ch = chunks[c];
+      const V3 cw = vadd(qrotate(qA, {(double)ch.x, (double)ch.y, (double)ch.z}), xA);
+      const double rc = (double)ch.w * kReach;
+      I3 lo, hi;
+      cellRange(cw, rc, tgrid, lo, hi);
+      lv = cellNear(tcell, lo, hi);
+      if (lv) {
+        const V3 lc = toLocal(vsub(cw, tg.x), tg);
+        if (outsideBox(lc, rc, tg.t) || levelClear(lc, rc, tg.t, tg.sdf)) lv = false;
+      }
+    }
+    live |= (unsigned long long)groupBits(ballot64(lv)) << cb;
+  }
+  int prevOff = 0;
+  const int prevCnt = active && live ? findPrevPair(S, p, A, T, prevOff) : 0;
+  const bool planeNodes = active && tg.t.affine;
+  V3 pu = {0., 0., 0.};
+  double pc0 = 0.;
+  if (planeNodes && live) {
+    const V3 ga = toWorld({tg.t.plane[0], tg.t.plane[1], tg.t.plane[2]}, tg);
+    pu = qrotateT(ga, qA);
+    pc0 = tg.t.plane[3] + vdot(ga, vsub(xA, tg.x));
+  }
+  auto loadNode = [&](int k) __attribute__((always_inline)) {
+    return V3{(double)nodes[3 * k], (double)nodes[3 * k + 1], (double)nodes[3 * k + 2]};
+  };
+  // a node worth the exact predicate: in range and, on an affine target,
+  // not above the plane by more than its slack (as in sweepPair)
+  auto candidate = [&](int k, V3 sn) __attribute__((always_inline)) {
+    return k < nodeCount &&
+           !(planeNodes && pc0 + pu.x * sn.x + pu.y * sn.y + pu.z * sn.z > tg.t.planeSlack);
+  };
+  auto stageLds = [&](int slot, int k, V3 pn, const Sample& sm) __attribute__((always_inline)) {
+    const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+    const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
+    const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+    ws.pc[0][grp][slot] = pc.x;
+    ws.pc[1][grp][slot] = pc.y;
+    ws.pc[2][grp][slot] = pc.z;
+    ws.nrm[0][grp][slot] = nrm.x;
+    ws.nrm[1][grp][slot] = nrm.y;
+    ws.nrm[2][grp][slot] = nrm.z;
+    ws.ov[grp][slot] = ov;
+    ws.node[grp][slot] = k;
+  };
+
+  int ncand = 0, total = 0;
+  // the exact predicate of the collected candidates, sixteen per round, hits
+  // staged in node order
+  auto flush = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int r = 0; ballot64(r < ncand); r += 16) {
+      bool hit = false;
+      Sample sm;
+      V3 pn = {0., 0., 0.};
+      int k = 0;
+      if (r + gl < ncand) {
+        k = ws.cand[grp][r + gl];
+        pn = vadd(qrotate(qA, loadNode(k)), xA);
+        hit = nodeHit<FDM_SET_WALL>(pn, tg, tcell, tgrid, RB, sm);
+      }
+      const unsigned m = groupBits(ballot64(hit));
+      if (hit) {
+        const int slot = total + __popc(m & below);
+        if (slot < kGroupHits) stageLds(slot, k, pn, sm);
+      }
+      total += __popc(m);
+    }
+    ncand = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
+  // collection: the i-th surviving chunk of every group per iteration, its
+  // 64 nodes four per lane (node 64c + 16u + gl, so u-major is node order)
+  for (unsigned long long rem = live; ballot64(rem != 0ull);) {
+    const int c = rem ? __builtin_ctzll(rem) : -1;
+    if (rem) rem &= rem - 1ull;
+    bool pass[4];
+    unsigned m[4];
+    int add = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = c * FDM_CHUNK + 16 * u + gl;
+      pass[u] = c >= 0 && k < nodeCount && candidate(k, loadNode(k));
+      m[u] = groupBits(ballot64(pass[u]));
+      add += __popc(m[u]);
+    }
+    if (ballot64(ncand + add > kGroupCands)) flush();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (pass[u]) ws.cand[grp][ncand + __popc(m[u] & below)] = c * FDM_CHUNK + 16 * u + gl;
+      ncand += __popc(m[u]);
+    }
+  }
+  if (ballot64(ncand > 0)) flush();
+
+  // record and staging reservations (one lane of each group)
+  const int pool = w % FDM_REC_POOLS;
+  int rb = 0, gb = 0;
+  if (active && gl == 0 && total > 0) {
+    rb = atomicAdd(S.pairs.recTotal + pool * FDM_REC_STRIDE, total);
+    gb = atomicAdd(X.hitCount + pool * FDM_REC_STRIDE, total);
+  }
+  rb = __shfl(rb, sh);
+  gb = __shfl(gb, sh);
+  int base = 0, gbase = 0;
+  if (active && total > 0) {
+    const int rCap = S.pairs.recCapacity / FDM_REC_POOLS, gCap = X.hitCapacity / FDM_REC_POOLS;
+    if (rb + total > rCap || gb + total > gCap) {
+      if (gl == 0)
+        atomicOr(C.status, rb + total > rCap ? FDM_STATUS_RECORD_OVERFLOW : FDM_STATUS_STAGING_OVERFLOW);
+      total = 0;
+    } else {
+      base = pool * rCap + rb;
+      gbase = pool * gCap + gb;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (active && total > 0 && total <= kGroupHits)
+    for (int h = gl; h < total; h += 16)
+      stageHit(X.hitGeom + 8 * (size_t)(gbase + h), {ws.pc[0][grp][h], ws.pc[1][grp][h], ws.pc[2][grp][h]},
+               {ws.nrm[0][grp][h], ws.nrm[1][grp][h], ws.nrm[2][grp][h]}, ws.ov[grp][h], ws.node[grp][h]);
+  // more contacts than slots: the group's surviving chunks again, hits
+  // written in node order at the reserved range
+  const bool again = active && total > kGroupHits;
+  if (ballot64(again)) {
+    int running = 0;
+    for (unsigned long long rem = again ? live : 0ull; ballot64(rem != 0ull);) {
+      const int c = rem ? __builtin_ctzll(rem) : -1;
+      if (rem) rem &= rem - 1ull;
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        const int k = c * FDM_CHUNK + 16 * u + gl;
+        bool hit = false;
+        Sample sm;
+        V3 pn = {0., 0., 0.};
+        if (c >= 0 && k < nodeCount) {
+          const V3 sn = loadNode(k);
+          if (candidate(k, sn)) {
+            pn = vadd(qrotate(qA, sn), xA);
+            hit = nodeHit<FDM_SET_WALL>(pn, tg, tcell, tgrid, RB, sm);
+          }
+        }
+        const unsigned mh = groupBits(ballot64(hit));
+        if (hit) {
+          const double ov = -trilinear(sm.x, sm.y, sm.z, sm.c);
+          const V3 nrm = vnormalize(toWorld(gradientIntX(sm.x, sm.y, sm.z, sm.c), tg));
+          const V3 pc = vadd(pn, vmul(nrm, 0.5 * ov));
+          stageHit(X.hitGeom + 8 * (size_t)(gbase + running + __popc(mh & below)), pc, nrm, ov, k);
+        }
+        running += __popc(mh);
+      }
+    }
+  }
+  if (active && gl == 0) {
+    S.pairs.recOff[p] = base;
+    S.pairs.recCnt[p] = total;
+    X.workHit[w] = gbase;
+    int4* h = reinterpret_cast<int4*>(X.lawHead + 8 * (size_t)w);
+    h[0] = make_int4(total, base, gbase, A);
+    if (total > 0) h[1] = make_int4(T, prevOff, prevCnt, 0);
+  }
+}
+
 // Contact law of one kept pair with contacts (one thread): the previous
 // springs by (A, target, node), the linear spring-dashpot law per contact in
 // node order, and the pair's force and torque sums in that order.
@@ -1406,13 +1657,23 @@ kCullSweep(NarrowMulti a, NarrowScratch x) {
   // waves sharing out its runs. Candidates are taken from the end of the
   // index space: the ghost sets (image pairs, whose spheres hold many nodes)
   // before the wall set, so that the cheap wall pairs fill the last round.
+  // (a set swept by kWallGroups is left out of the index space)
+  int ws0 = 0, nWall = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxSets; ++q)
+    if (q == a.wallSet) {
+      ws0 = st.v[q];
+      nWall = st.v[q + 1] - st.v[q];
+    }
+  const int nOther = total - nWall;
   const bool byXcd = gridDim.x % kXcds == 0;
   const int xcd = byXcd ? blockIdx.x % kXcds : 0;
   const int stride = byXcd ? gridDim.x / kXcds * WAVES : gridDim.x * WAVES;
   for (int i = (byXcd ? blockIdx.x / kXcds : blockIdx.x) * WAVES + wave;; i += stride) {
     const int r = byXcd ? ((i >> 6) * kXcds + xcd) * 64 + (i & 63) : i;
-    if (r >= total) break;
-    const int g = total - 1 - r;
+    if (r >= nOther) break;
+    const int go = nOther - 1 - r;
+    const int g = go < ws0 ? go : go + nWall;
     int k = 0, base = 0;
 #pragma unroll
     for (int q = 1; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
@@ -1438,6 +1699,52 @@ kCullSweep(NarrowMulti a, NarrowScratch x) {
       sweepPair<FDM_SET_WALL, true, NPL>(a.c, s, x, g, p, lane, slots[wave], pool, cs);
     else
       sweepPair<FDM_SET_GHOST, true, NPL>(a.c, s, x, g, p, lane, slots[wave], pool, cs);
+  }
+}
+
+// The wall set's pairs four per wave (sweepWallGroup) for the one-node-per-
+// lane sweep, launched before kCullSweep, which then leaves the wall set
+// out. One wave per workgroup; runs of 16 groups (64 consecutive pairs) per
+// XCD as in kCullSweep. A group of four whose pairs do not share the target
+// and system is swept one pair at a time (the same code with one group of
+// lanes taking part).
+__global__ void __launch_bounds__(64) kWallGroups(NarrowMulti a, NarrowScratch x) {
+  if (gateOff(a.gate)) return;
+  __shared__ WallGroupSlots groups;
+  const int lane = laneId();
+  const SetStarts st = setStarts(a);
+  int k = 0, ws0 = 0, nWall = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxSets; ++q)  // constant indices: the starts stay scalar
+    if (q == a.wallSet) {
+      k = q;
+      ws0 = st.v[q];
+      nWall = st.v[q + 1] - st.v[q];
+    }
+  const NarrowSet& S = a.s[k];
+  const NarrowCommon& C = a.c;
+  const int items = (nWall + 3) / 4;
+  const bool byXcd = gridDim.x % kXcds == 0;
+  const int xcd = byXcd ? blockIdx.x % kXcds : 0;
+  const int stride = byXcd ? gridDim.x / kXcds : gridDim.x;
+  for (int i = byXcd ? blockIdx.x / kXcds : blockIdx.x;; i += stride) {
+    const int r = byXcd ? ((i >> 4) * kXcds + xcd) * 16 + (i & 15) : i;
+    if (r >= items) break;
+    const int p = 4 * r + (lane >> 4);
+    const bool in = p < nWall;
+    const int A = in ? ldu(S.pairs.pairA + p) : -1;
+    const int T = in ? ldu(S.pairs.pairT + p) : 0;
+    const bool live = A >= 0;
+    const int inst = live && C.inst.count > 0 ? ldu(C.inst.particleInstance + A) : 0;
+    const unsigned long long act = ballot64(live);
+    bool uniform = true;
+    if (act) {
+      const int ref = __builtin_ctzll(act);
+      uniform = !ballot64(live && (T != readlaneI(T, ref) || inst != readlaneI(inst, ref)));
+    }
+    const int passes = uniform ? 1 : 4;
+    for (int it = 0; it < passes; ++it)
+      sweepWallGroup(C, S, x, ws0, nWall, r, uniform ? 0xFu : (1u << it), lane, groups);
   }
 }
 
@@ -1720,7 +2027,15 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   x.hitCount = ws->hitCount;
   fdm_tuning tu = tuning();
   if (ws->tuning && !completeTuning(*ws->tuning, tu)) return FDM_EINVAL;
+  // the wall set four pairs per wave (one node per lane, law in its own kernel)
+  k.wallSet = -1;
+  if (tu.lawMode == 1 && tu.nodesPerLane == 1 && x.lawHead)
+    for (int i = 0; i < nsets && k.wallSet < 0; ++i)
+      if (sets[i].set == FDM_SET_WALL) k.wallSet = i;
   const int sweepBlocks = std::max(1, persistentBlocks(tu.sweepWaves) * tu.sweepWavesPerCU / 32);
+  if (k.wallSet >= 0)
+    hipLaunchKernelGGL(kWallGroups, dim3(std::max(kXcds, persistentBlocks(1) * 12 / 32)), dim3(64), 0,
+                       stream, k, x);
   if (tu.lawMode == 2) {
     if (tu.sweepWaves == 4)
       hipLaunchKernelGGL(kCullSweepLaw<4>, dim3(sweepBlocks), dim3(256), 0, stream, k);
