@@ -188,10 +188,11 @@ int fdmx_autotune(void* s, int trialSteps, int tuneCell, double* out7, char* log
 // Bytes per stored state value of this build (4: FP32, 8: FP64 storage).
 int fdmx_state_bytes() { return (int)sizeof(fdm_real); }
 
-// Launch shapes of one solver (zero fields: the process default).
+// Launch shapes of one solver, seven fields in fdm_tuning order (zero
+// fields: the process default).
 int fdmx_set_launch_tuning(void* s, const int* t5) {
   return guarded([&] {
-    fdm_tuning t{t5[0], t5[1], t5[2], t5[3], t5[4], t5[5]};
+    fdm_tuning t{t5[0], t5[1], t5[2], t5[3], t5[4], t5[5], t5[6]};
     static_cast<BoundSolver*>(s)->setLaunchTuning(t);
   });
 }
@@ -204,6 +205,7 @@ int fdmx_get_launch_tuning(void* s, int* t5) {
     t5[3] = t.reduceBlock;
     t5[4] = t.lawMode;
     t5[5] = t.nodesPerLane;
+    t5[6] = t.wallGroups;
   });
 }
 int fdmx_set_lookahead(void* s, int steps) {

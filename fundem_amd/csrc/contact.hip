@@ -1253,6 +1253,7 @@ __device__ void sweepPair(const NarrowCommon& C, const NarrowSet& S, const Narro
 // pairs. Contacts are staged in LDS in node order (past kGroupHits they are
 // recomputed straight into the workspace once the range is reserved).
 constexpr int kGroupHits = 16;
+constexpr int kGroupedWallPairs = 65536;
 constexpr int kGroupCands = 64;
 struct WallGroupSlots {
   double pc[3][4][kGroupHits];
@@ -1810,7 +1811,7 @@ __global__ void kResetCounter(int* c) {
 
 namespace fdm {
 namespace {
-fdm_tuning g_tuning = {1, 32, 256, 256, 1, 1};
+fdm_tuning g_tuning = {1, 32, 256, 256, 1, 1, 0};
 }
 const fdm_tuning& tuning() { return g_tuning; }
 }  // namespace fdm
@@ -1825,12 +1826,14 @@ static bool completeTuning(const fdm_tuning& t, fdm_tuning& n) {
   if (t.reduceBlock) n.reduceBlock = t.reduceBlock;
   if (t.lawMode) n.lawMode = t.lawMode;
   if (t.nodesPerLane) n.nodesPerLane = t.nodesPerLane;
+  if (t.wallGroups) n.wallGroups = t.wallGroups;
   if (!(n.sweepWaves == 1 || n.sweepWaves == 2 || n.sweepWaves == 4)) return false;
   if (n.sweepWavesPerCU < 4 || n.sweepWavesPerCU > 128) return false;
   if (!(n.lawBlock == 64 || n.lawBlock == 128 || n.lawBlock == 256)) return false;
   if (n.reduceBlock < 64 || n.reduceBlock > 1024 || n.reduceBlock % 64) return false;
   if (!(n.lawMode == 1 || n.lawMode == 2)) return false;
   if (n.nodesPerLane < 1 || n.nodesPerLane > 3) return false;
+  if (n.wallGroups < 0 || n.wallGroups > 2) return false;
   return true;
 }
 
@@ -2027,15 +2030,26 @@ extern "C" int fdm_launchNarrowPhaseSets(const fdm_narrowphase_args* sets, int n
   x.hitCount = ws->hitCount;
   fdm_tuning tu = tuning();
   if (ws->tuning && !completeTuning(*ws->tuning, tu)) return FDM_EINVAL;
-  // the wall set four pairs per wave (one node per lane, law in its own kernel)
+  // The wall set four pairs per wave (one node per lane, law in its own
+  // kernel): by default for wall sets sized for at least kGroupedWallPairs
+  // pairs, as a small system's step is bound by launch latency, which a
+  // second sweep kernel would add to. The grid covers the set's capacity in
+  // groups.
   k.wallSet = -1;
+  int wallGroups = 0;
   if (tu.lawMode == 1 && tu.nodesPerLane == 1 && x.lawHead)
     for (int i = 0; i < nsets && k.wallSet < 0; ++i)
-      if (sets[i].set == FDM_SET_WALL) k.wallSet = i;
+      if (sets[i].set == FDM_SET_WALL && tu.wallGroups != 2 &&
+          (tu.wallGroups == 1 || sets[i].pairs.capacity >= kGroupedWallPairs)) {
+        k.wallSet = i;
+        wallGroups = (sets[i].pairs.capacity + 3) / 4;
+      }
   const int sweepBlocks = std::max(1, persistentBlocks(tu.sweepWaves) * tu.sweepWavesPerCU / 32);
-  if (k.wallSet >= 0)
-    hipLaunchKernelGGL(kWallGroups, dim3(std::max(kXcds, persistentBlocks(1) * 12 / 32)), dim3(64), 0,
-                       stream, k, x);
+  if (k.wallSet >= 0) {
+    const int blocks = std::min(persistentBlocks(1) * 12 / 32, wallGroups);
+    hipLaunchKernelGGL(kWallGroups, dim3(std::max(kXcds, blocks / kXcds * kXcds)), dim3(64), 0, stream,
+                       k, x);
+  }
   if (tu.lawMode == 2) {
     if (tu.sweepWaves == 4)
       hipLaunchKernelGGL(kCullSweepLaw<4>, dim3(sweepBlocks), dim3(256), 0, stream, k);

@@ -227,18 +227,20 @@ class Solver:
                     trials=buf.value.decode().splitlines())
 
     def set_launch_tuning(self, sweep_waves=0, sweep_waves_per_cu=0, law_block=0, reduce_block=0,
-                          law_mode=0, nodes_per_lane=0):
+                          law_mode=0, nodes_per_lane=0, wall_groups=0):
         """This solver's launch shapes (results do not depend on them); zero
-        keeps the process default of a field."""
+        keeps the process default of a field. wall_groups: 1 sweeps wall
+        pairs four per wave whatever the system's size, 2 never (0: for wall
+        sets sized for at least 65536 pairs)."""
         t = np.array([sweep_waves, sweep_waves_per_cu, law_block, reduce_block, law_mode,
-                      nodes_per_lane], np.int32)
+                      nodes_per_lane, wall_groups], np.int32)
         self._ok(self._lib.fdmx_set_launch_tuning(self.h, _ip_(t)))
 
     def launch_tuning(self):
-        t = np.zeros(6, np.int32)
+        t = np.zeros(7, np.int32)
         self._ok(self._lib.fdmx_get_launch_tuning(self.h, _ip_(t)))
         return dict(zip(("sweep_waves", "sweep_waves_per_cu", "law_block", "reduce_block",
-                         "law_mode", "nodes_per_lane"), (int(v) for v in t)))
+                         "law_mode", "nodes_per_lane", "wall_groups"), (int(v) for v in t)))
 
     def set_lookahead(self, steps):
         """Steps queued ahead of the list decisions read back (0: one step

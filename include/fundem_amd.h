@@ -675,6 +675,9 @@ typedef struct fdm_tuning_s {
                            iteration (128 nodes in flight), 3 the nodes that pass the tests
                            before the SDF fetch collected first, then up to 64 of their SDF
                            cells fetched at once; lawMode 1 only */
+  int wallGroups;       /* wall pairs four per wave in a kernel of their own (nodesPerLane 1,
+                           lawMode 1): 0 automatic (wall sets sized for at least 65536
+                           pairs), 1 always, 2 never */
 } fdm_tuning;
 int fdm_setTuning(const fdm_tuning* t);
 void fdm_getTuning(fdm_tuning* t);

@@ -18,7 +18,8 @@ EINVAL = -1
 
 class Tuning(C.Structure):
     _fields_ = [("sweepWaves", C.c_int), ("sweepWavesPerCU", C.c_int), ("lawBlock", C.c_int),
-                ("reduceBlock", C.c_int), ("lawMode", C.c_int), ("nodesPerLane", C.c_int)]
+                ("reduceBlock", C.c_int), ("lawMode", C.c_int), ("nodesPerLane", C.c_int),
+                ("wallGroups", C.c_int)]
 
 
 class Periodic(C.Structure):
@@ -71,7 +72,8 @@ def test_tuning_accepts_only_documented_shapes(lib):
     try:
         for bad in ((3, 0, 0, 0, 0), (0, 2, 0, 0, 0), (0, 200, 0, 0, 0), (0, 0, 96, 0, 0),
                     (0, 0, 0, 100, 0), (0, 0, 0, 2048, 0), (0, 0, 0, 0, 3), (0, 0, 0, 0, -1),
-                    (0, 0, 0, 0, 0, 4), (0, 0, 0, 0, 0, -2)):
+                    (0, 0, 0, 0, 0, 4), (0, 0, 0, 0, 0, -2), (0, 0, 0, 0, 0, 0, 3),
+                    (0, 0, 0, 0, 0, 0, -1)):
             assert lib.fdm_setTuning(C.byref(Tuning(*bad))) == EINVAL, bad
         got = Tuning()
         lib.fdm_getTuning(C.byref(got))
@@ -83,10 +85,10 @@ def test_tuning_accepts_only_documented_shapes(lib):
             (2, before.sweepWavesPerCU, 128, before.reduceBlock, before.lawMode)
         # a caller's own shapes: zero fields from the default, invalid ones refused
         lib.fdm_checkTuning.argtypes = [C.POINTER(Tuning)]
-        mine = Tuning(0, 64, 0, 0, 2, 2)
+        mine = Tuning(0, 64, 0, 0, 2, 2, 1)
         assert lib.fdm_checkTuning(C.byref(mine)) == 0
         assert (mine.sweepWaves, mine.sweepWavesPerCU, mine.lawBlock, mine.lawMode,
-                mine.nodesPerLane) == (2, 64, 128, 2, 2)
+                mine.nodesPerLane, mine.wallGroups) == (2, 64, 128, 2, 2, 1)
         assert lib.fdm_checkTuning(C.byref(Tuning(0, 0, 0, 0, 5))) == EINVAL
         lib.fdm_getTuning(C.byref(got))
         assert got.sweepWavesPerCU == before.sweepWavesPerCU and got.lawMode == before.lawMode
